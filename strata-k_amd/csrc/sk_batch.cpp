@@ -1,0 +1,278 @@
+// sk_batch.cpp — many independent programs evaluated together.
+//
+// Programs whose Core-IR is identical up to predicate names form a group.  A
+// group is evaluated as ONE fused program: every atom gets a leading
+// program-id column bound to a shared variable, so the fused relations are
+// the disjoint union of the programs' relations keyed by program id, every
+// round is one set of launches over all programs of the group, and the
+// rounds end when the last program converges.  Per program the engine keeps
+// the reference's own bookkeeping: its round count (rounds in which its delta
+// was non-empty), its Bellman-Ford cap (its own universe + 2) and its errors;
+// a program that fails stops contributing without disturbing the others.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "sk_engine.h"
+#include "strata_k.h"
+
+using namespace sk;
+
+namespace {
+
+struct BProgram {
+  Program prog;  // own copy
+  std::vector<std::string> edb_preds;
+  std::vector<u64> edb_n;
+  std::vector<std::vector<std::vector<u32>>> edb_cols;
+  std::vector<std::vector<i64>> edb_w;
+  int group = -1;
+  int member = -1;  // index within the group
+  EvalErr err;
+  std::vector<u64> iters;
+  u64 universe = 0;
+};
+
+// Structural signature of a program up to predicate names.
+std::string signature(const Program& p) {
+  std::string s = "S" + std::to_string(p.num_strata) + ";";
+  for (auto& d : p.preds) s += "p" + std::to_string(d.arity) + "," + std::to_string(d.sem) + "," + std::to_string(d.stratum) + ";";
+  auto atom = [&](const Atom& a) {
+    std::string t = (a.neg ? "!" : "") + std::to_string(a.rel) + "(";
+    for (auto& x : a.args) t += std::to_string(x.kind) + ":" + std::to_string(x.a) + ":" + std::to_string(x.b) + ",";
+    return t + ")";
+  };
+  for (auto& r : p.rules) {
+    s += "r" + std::to_string(r.stratum) + "," + std::to_string(r.var_count) + ":" + atom(r.head) + ":-";
+    for (auto& b : r.body) s += atom(b);
+    s += ";";
+  }
+  return s;
+}
+
+}  // namespace
+
+struct sk_batch {
+  sk_ctx* ctx = nullptr;
+  std::vector<BProgram> progs;
+  struct Group {
+    std::vector<int> members;
+    std::unique_ptr<Engine> eng;
+    Program fused;
+    EvalErr err;  // group-wide error (shape, device, budget)
+  };
+  std::vector<Group> groups;
+  bool loaded = false;
+  double exec_ms = 0;
+  EvalErr err;
+};
+
+namespace {
+
+// The fused program: slot `var_count` of every rule is the program id and
+// becomes column 0 of every atom; arities grow by one.
+Program fuse(const Program& p) {
+  Program f = p;
+  for (auto& d : f.preds) d.arity += 1;
+  for (auto& r : f.rules) {
+    u32 pid = r.var_count;
+    r.var_count += 1;
+    auto add = [&](Atom& a) { a.args.insert(a.args.begin(), Term{SK_T_VAR, pid, 0}); };
+    add(r.head);
+    for (auto& b : r.body) add(b);
+  }
+  return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+sk_status sk_batch_create(sk_ctx* ctx, sk_batch** out) {
+  if (!ctx || !out) return SK_ERR_INVALID_ARGUMENT;
+  *out = new sk_batch();
+  (*out)->ctx = ctx;
+  return SK_OK;
+}
+
+void sk_batch_free(sk_batch* b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->ctx->c.stream);
+  delete b;
+}
+
+sk_status sk_batch_add(sk_batch* b, const sk_program* prog, uint32_t n_edb, const char* const* edb_preds,
+                       const uint64_t* edb_n, const uint32_t* const* const* edb_cols, const int64_t* const* edb_w,
+                       uint32_t* out_index) {
+  if (!b || !prog || b->loaded) return SK_ERR_INVALID_ARGUMENT;
+  BProgram bp;
+  bp.prog = Program::from_c(prog);
+  for (uint32_t i = 0; i < n_edb; i++) {
+    int r = bp.prog.rel_of(edb_preds[i]);
+    if (r < 0) return SK_ERR_UNKNOWN_PRED;
+    u32 ar = bp.prog.preds[r].arity;
+    bp.edb_preds.push_back(edb_preds[i]);
+    bp.edb_n.push_back(edb_n[i]);
+    std::vector<std::vector<u32>> cols(ar);
+    for (u32 j = 0; j < ar; j++) cols[j].assign(edb_cols[i][j], edb_cols[i][j] + edb_n[i]);
+    bp.edb_cols.push_back(std::move(cols));
+    if (edb_w && edb_w[i]) bp.edb_w.emplace_back(edb_w[i], edb_w[i] + edb_n[i]);
+    else bp.edb_w.emplace_back();
+  }
+  if (out_index) *out_index = (uint32_t)b->progs.size();
+  b->progs.push_back(std::move(bp));
+  return SK_OK;
+}
+
+sk_status sk_batch_load(sk_batch* b) {
+  if (!b || b->loaded) return SK_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(b->ctx->c.device);
+  std::map<std::string, int> gid;
+  for (size_t i = 0; i < b->progs.size(); i++) {
+    std::string sig = signature(b->progs[i].prog);
+    auto it = gid.find(sig);
+    if (it == gid.end()) {
+      it = gid.emplace(sig, (int)b->groups.size()).first;
+      b->groups.emplace_back();
+    }
+    b->progs[i].group = it->second;
+    b->progs[i].member = (int)b->groups[it->second].members.size();
+    b->groups[it->second].members.push_back((int)i);
+  }
+  for (auto& g : b->groups) {
+    const Program& p0 = b->progs[g.members[0]].prog;
+    g.fused = fuse(p0);
+    try {
+      g.eng = std::make_unique<Engine>(&b->ctx->c, g.fused);
+      g.eng->validate();
+      g.eng->set_partitions((u32)g.members.size());
+      // stage every member's facts with its program id as column 0
+      for (int m = 0; m < (int)g.members.size(); m++) {
+        BProgram& bp = b->progs[g.members[m]];
+        // per-program universe: distinct Sym values of its EDB
+        std::set<u32> syms;
+        for (auto& cols : bp.edb_cols)
+          for (auto& c : cols) syms.insert(c.begin(), c.end());
+        bp.universe = syms.size();
+        g.eng->set_partition_universe((u32)m, bp.universe);
+        for (size_t e = 0; e < bp.edb_preds.size(); e++) {
+          const std::string& pname = p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name;
+          u64 n = bp.edb_n[e];
+          std::vector<u32> pid(n, (u32)m);
+          std::vector<const u32*> cols;
+          cols.push_back(pid.data());
+          for (auto& c : bp.edb_cols[e]) cols.push_back(c.data());
+          g.eng->stage_sym(pname, n, (u32)cols.size(), cols.data(), bp.edb_w[e].empty() ? nullptr : bp.edb_w[e].data(),
+                           nullptr);
+        }
+        std::vector<std::vector<std::vector<u32>>>().swap(bp.edb_cols);
+        std::vector<std::vector<i64>>().swap(bp.edb_w);
+      }
+      g.eng->load();
+    } catch (const EvalErr& e) {
+      g.err = e;
+    } catch (const DeviceError& d) {
+      g.err.code = SK_ERR_DEVICE;
+      g.err.msg = d.where;
+    } catch (const BudgetError&) {
+      g.err.code = SK_ERR_OUT_OF_BUDGET;
+    }
+  }
+  b->loaded = true;
+  return SK_OK;
+}
+
+sk_status sk_batch_execute(sk_batch* b) {
+  if (!b || !b->loaded) return SK_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(b->ctx->c.device);
+  auto t0 = std::chrono::steady_clock::now();
+  for (auto& g : b->groups) {
+    if (g.err.code) continue;
+    try {
+      g.eng->execute();
+    } catch (const EvalErr& e) {
+      g.err = e;
+    } catch (const DeviceError& d) {
+      g.err.code = SK_ERR_DEVICE;
+      g.err.msg = d.where;
+    } catch (const BudgetError&) {
+      g.err.code = SK_ERR_OUT_OF_BUDGET;
+    }
+  }
+  (void)hipStreamSynchronize(b->ctx->c.stream);
+  b->exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (auto& bp : b->progs) {
+    auto& g = b->groups[bp.group];
+    if (g.err.code) {
+      bp.err = g.err;
+      continue;
+    }
+    bp.err = g.eng->partition_error((u32)bp.member);
+    bp.iters.assign(bp.prog.num_strata, 0);
+    for (u32 k = 0; k < bp.prog.num_strata; k++) bp.iters[k] = g.eng->partition_iterations((u32)bp.member, k);
+  }
+  return SK_OK;
+}
+
+sk_status sk_batch_status(const sk_batch* b, uint32_t index, sk_error_info* out) {
+  if (!b || index >= b->progs.size()) return SK_ERR_INVALID_ARGUMENT;
+  const EvalErr& e = b->progs[index].err;
+  if (out) {
+    std::memset(out, 0, sizeof *out);
+    out->code = e.code;
+    out->slot = e.slot;
+    out->lhs = e.lhs;
+    out->rhs = e.rhs;
+    std::snprintf(out->pred, sizeof out->pred, "%s", e.pred.c_str());
+    std::snprintf(out->message, sizeof out->message, "%s", e.msg.c_str());
+  }
+  return (sk_status)e.code;
+}
+
+uint64_t sk_batch_iterations(const sk_batch* b, uint32_t index, uint32_t stratum) {
+  if (!b || index >= b->progs.size()) return 0;
+  const auto& it = b->progs[index].iters;
+  return stratum < it.size() ? it[stratum] : 0;
+}
+
+int64_t sk_batch_result_size(sk_batch* b, uint32_t index, const char* pred) {
+  if (!b || index >= b->progs.size() || !pred) return -1;
+  BProgram& bp = b->progs[index];
+  auto& g = b->groups[bp.group];
+  if (bp.err.code || !g.eng) return -1;
+  int r = bp.prog.rel_of(pred);
+  if (r < 0) return -1;
+  const std::string& fname = g.fused.preds[r].name;
+  try {
+    return (int64_t)g.eng->partition_size(fname, (u32)bp.member);
+  } catch (...) {
+    return -1;
+  }
+}
+
+sk_status sk_batch_result_read_sym(sk_batch* b, uint32_t index, const char* pred, uint32_t* const* cols,
+                                   int64_t* w, uint8_t* inf) {
+  if (!b || index >= b->progs.size() || !pred) return SK_ERR_INVALID_ARGUMENT;
+  BProgram& bp = b->progs[index];
+  auto& g = b->groups[bp.group];
+  if (bp.err.code) return (sk_status)bp.err.code;
+  int r = bp.prog.rel_of(pred);
+  if (r < 0) return SK_ERR_UNKNOWN_PRED;
+  try {
+    g.eng->partition_read_sym(g.fused.preds[r].name, (u32)bp.member, cols, w, inf);
+  } catch (const EvalErr& e) {
+    return (sk_status)e.code;
+  } catch (const DeviceError&) {
+    return SK_ERR_DEVICE;
+  }
+  return SK_OK;
+}
+
+double sk_batch_execute_ms(const sk_batch* b) { return b ? b->exec_ms : 0; }
+
+}  // extern "C"

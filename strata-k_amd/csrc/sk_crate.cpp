@@ -1,0 +1,232 @@
+// sk_crate.cpp — the reference GPU crate's public functions
+// (crates/strata-gpu/src/lib.rs:60-149) on top of the device engine.
+//
+// transitive_closure_bool / shortest_paths_trop run the closure program
+// through the same semi-naive engine as sk_run_semi_naive (interpreter
+// semantics: checked ⊗, negative-cycle cap universe+2).  count_triangles
+// orients each undirected edge from lower to higher id, drops self-loops and
+// duplicates (wcoj.rs:99-131) and runs the leapfrog operator; count_4cliques
+// extends it by one more intersection level (wcoj.rs:49-77).
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "sk_engine.h"
+#include "strata_k.h"
+
+using namespace sk;
+
+namespace sk {
+u64 k4_count(const Exec& x, const u64* keys, u64 ne, const u64* csr, const u32* adj, int bits);
+void intern_lookup(const Exec& x, const u64* table, u64 nt, const u64* q, u64 nq, u32* ids);
+}  // namespace sk
+
+namespace {
+
+// Closure program: path(X,Y) :- edge(X,Y).  path(X,Z) :- edge(X,Y), path(Y,Z).
+struct TcProg {
+  sk_pred preds[2];
+  sk_term terms[10];
+  sk_atom atoms[5];
+  sk_rule rules[2];
+  sk_program p;
+  explicit TcProg(uint32_t sem) {
+    preds[0] = {"edge", 2, sem, 0, 0};
+    preds[1] = {"path", 2, sem, 0, 0};
+    auto V = [](uint32_t s) { return sk_term{SK_T_VAR, s, 0}; };
+    // rule 0: head path(0,1), body edge(0,1)
+    terms[0] = V(0); terms[1] = V(1);
+    terms[2] = V(0); terms[3] = V(1);
+    // rule 1: head path(0,2), body edge(0,1), path(1,2)
+    terms[4] = V(0); terms[5] = V(2);
+    terms[6] = V(0); terms[7] = V(1);
+    terms[8] = V(1); terms[9] = V(2);
+    atoms[0] = {"path", 0, 0, 2, 0};
+    atoms[1] = {"edge", 0, 2, 2, 0};
+    atoms[2] = {"path", 0, 4, 2, 0};
+    atoms[3] = {"edge", 0, 6, 2, 0};
+    atoms[4] = {"path", 0, 8, 2, 0};
+    rules[0] = {0, 1, 1, 0, 2, 0};
+    rules[1] = {2, 3, 2, 0, 3, 0};
+    p = {preds, 2, rules, 2, atoms, 5, terms, 10, 1};
+  }
+};
+
+template <class T>
+T* dup_out(const std::vector<T>& v) {
+  T* p = (T*)std::malloc(std::max<size_t>(1, v.size() * sizeof(T)));
+  if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+sk_status sk_transitive_closure_bool(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst, uint64_t n,
+                                     uint32_t** out_src, uint32_t** out_dst, uint64_t* out_n) {
+  if (!ctx || !out_src || !out_dst || !out_n || (n && (!src || !dst))) return SK_ERR_INVALID_ARGUMENT;
+  TcProg tp(SK_BOOL);
+  sk_run* run = nullptr;
+  sk_status s = sk_prepare(ctx, &tp.p, &run);
+  if (s) return s;
+  const uint32_t* cols[2] = {src, dst};
+  if (!(s = sk_run_add_facts_sym(run, "edge", n, 2, cols, nullptr, nullptr)) && !(s = sk_run_load(run)) &&
+      !(s = sk_run_execute(run))) {
+    int64_t m = sk_result_size(run, "path");
+    std::vector<uint32_t> a(m), b(m);
+    uint32_t* oc[2] = {a.data(), b.data()};
+    s = sk_result_read_sym(run, "path", oc, nullptr, nullptr);
+    if (!s) {
+      *out_src = dup_out(a);
+      *out_dst = dup_out(b);
+      *out_n = (uint64_t)m;
+    }
+  }
+  sk_run_free(run);
+  return s;
+}
+
+sk_status sk_shortest_paths_trop(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst, const int64_t* w, uint64_t n,
+                                 uint32_t** out_src, uint32_t** out_dst, int64_t** out_w, uint64_t* out_n) {
+  if (!ctx || !out_src || !out_dst || !out_w || !out_n || (n && (!src || !dst || !w))) return SK_ERR_INVALID_ARGUMENT;
+  TcProg tp(SK_TROP);
+  sk_run* run = nullptr;
+  sk_status s = sk_prepare(ctx, &tp.p, &run);
+  if (s) return s;
+  const uint32_t* cols[2] = {src, dst};
+  int64_t dummy = 0;
+  if (!(s = sk_run_add_facts_sym(run, "edge", n, 2, cols, n ? w : &dummy, nullptr)) && !(s = sk_run_load(run)) &&
+      !(s = sk_run_execute(run))) {
+    int64_t m = sk_result_size(run, "path");
+    std::vector<uint32_t> a(m), b(m);
+    std::vector<int64_t> ww(m);
+    uint32_t* oc[2] = {a.data(), b.data()};
+    s = sk_result_read_sym(run, "path", oc, ww.data(), nullptr);
+    if (!s) {
+      *out_src = dup_out(a);
+      *out_dst = dup_out(b);
+      *out_w = dup_out(ww);
+      *out_n = (uint64_t)m;
+    }
+  }
+  sk_run_free(run);
+  return s;
+}
+
+// Orient u<v, drop self-loops, dedup (wcoj.rs:99-131).
+static std::vector<uint64_t> orient_by_id(const uint32_t* a, const uint32_t* b, uint64_t n) {
+  std::vector<uint64_t> k;
+  k.reserve(n);
+  for (uint64_t i = 0; i < n; i++) {
+    uint32_t x = a[i], y = b[i];
+    if (x == y) continue;
+    if (y < x) std::swap(x, y);
+    k.push_back(((uint64_t)x << 32) | y);
+  }
+  std::sort(k.begin(), k.end());
+  k.erase(std::unique(k.begin(), k.end()), k.end());
+  return k;
+}
+
+sk_status sk_count_triangles(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* out) {
+  if (!ctx || !out || (n && (!a || !b))) return SK_ERR_INVALID_ARGUMENT;
+  // tri(X,Y,Z) :- e(X,Y), e(Y,Z), e(X,Z).  ntri(count<X>) :- tri(X,_Y,_Z).
+  std::vector<uint64_t> k = orient_by_id(a, b, n);
+  std::vector<uint32_t> u(k.size()), v(k.size());
+  for (size_t i = 0; i < k.size(); i++) {
+    u[i] = (uint32_t)(k[i] >> 32);
+    v[i] = (uint32_t)k[i];
+  }
+  sk_pred preds[2] = {{"e", 2, SK_BOOL, 0, 0}, {"tri", 3, SK_BOOL, 0, 0}};
+  auto V = [](uint32_t s) { return sk_term{SK_T_VAR, s, 0}; };
+  sk_term terms[9] = {V(0), V(1), V(2), V(0), V(1), V(1), V(2), V(0), V(2)};
+  sk_atom atoms[4] = {{"tri", 0, 0, 3, 0}, {"e", 0, 3, 2, 0}, {"e", 0, 5, 2, 0}, {"e", 0, 7, 2, 0}};
+  sk_rule rules[1] = {{0, 1, 3, 0, 3, 0}};
+  sk_program p = {preds, 2, rules, 1, atoms, 4, terms, 9, 1};
+  sk_run* run = nullptr;
+  sk_status s = sk_prepare(ctx, &p, &run);
+  if (s) return s;
+  const uint32_t* cols[2] = {u.data(), v.data()};
+  if (!(s = sk_run_add_facts_sym(run, "e", u.size(), 2, cols, nullptr, nullptr)) && !(s = sk_run_load(run)) &&
+      !(s = sk_run_execute(run)))
+    *out = (uint64_t)sk_result_size(run, "tri");
+  sk_run_free(run);
+  return s;
+}
+
+sk_status sk_count_4cliques(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* out) {
+  if (!ctx || !out || (n && (!a || !b))) return SK_ERR_INVALID_ARGUMENT;
+  std::vector<uint64_t> k = orient_by_id(a, b, n);
+  *out = 0;
+  if (k.empty()) return SK_OK;
+  EvalErr e;
+  try {
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    Exec x = c.exec();
+    uint64_t ne = k.size();
+    uint32_t maxv = 0;
+    for (auto kk : k) maxv = std::max(maxv, (uint32_t)kk);
+    uint64_t nv = (uint64_t)maxv + 1;
+    int bits = 1;
+    while ((1ull << bits) < nv) bits++;
+    std::vector<uint64_t> packed(ne);
+    for (uint64_t i = 0; i < ne; i++) packed[i] = ((k[i] >> 32) << bits) | (k[i] & 0xffffffffull);
+    DBuf<u64> dk(c.pool.get(), ne);
+    DBuf<u64> csr(c.pool.get(), nv + 1);
+    DBuf<u32> adj(c.pool.get(), ne), first(c.pool.get(), ne);
+    SK_HIP(hipMemcpyAsync(dk.get(), packed.data(), ne * 8, hipMemcpyHostToDevice, c.stream));
+    build_csr<u64>(x, dk.get(), ne, bits, nv, csr.get());
+    u32* cp[2] = {first.get(), adj.get()};
+    unpack_cols<u64>(x, dk.get(), ne, 2, bits, cp);
+    *out = k4_count(x, dk.get(), ne, csr.get(), adj.get(), bits);
+  } catch (const DeviceError& d) {
+    return SK_ERR_DEVICE;
+  } catch (const BudgetError&) {
+    return SK_ERR_OUT_OF_BUDGET;
+  }
+  return SK_OK;
+}
+
+sk_status sk_intern_terms(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_t* out_ids) {
+  if (!ctx || (n && (!keys || !out_ids))) return SK_ERR_INVALID_ARGUMENT;
+  if (n == 0) return SK_OK;
+  try {
+    Ctx& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    Exec x = c.exec();
+    DBuf<u64> a(c.pool.get(), n), b(c.pool.get(), n), u(c.pool.get(), n), q(c.pool.get(), n);
+    DBuf<u32> ids(c.pool.get(), n);
+    SK_HIP(hipMemcpyAsync(a.get(), keys, n * 8, hipMemcpyHostToDevice, c.stream));
+    SK_HIP(hipMemcpyAsync(q.get(), keys, n * 8, hipMemcpyHostToDevice, c.stream));
+    u64* sk;
+    radix_sort<u64>(x, a.get(), b.get(), nullptr, nullptr, nullptr, nullptr, n, 64, &sk, nullptr, nullptr);
+    u64 m = unique_runs<u64>(x, sk, nullptr, nullptr, n, u.get(), nullptr, nullptr);
+    intern_lookup(x, u.get(), m, q.get(), n, ids.get());
+    SK_HIP(hipMemcpyAsync(out_ids, ids.get(), n * 4, hipMemcpyDeviceToHost, c.stream));
+    SK_HIP(hipStreamSynchronize(c.stream));
+  } catch (const DeviceError&) {
+    return SK_ERR_DEVICE;
+  } catch (const BudgetError&) {
+    return SK_ERR_OUT_OF_BUDGET;
+  }
+  return SK_OK;
+}
+
+uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint64_t n) {
+  std::vector<uint64_t> k = orient_by_id(a, b, n);
+  uint64_t s = 0, run = 0;
+  for (size_t i = 0; i < k.size(); i++) {
+    if (i > 0 && (k[i] >> 32) != (k[i - 1] >> 32)) {
+      s += run * run;
+      run = 0;
+    }
+    run++;
+  }
+  s += run * run;
+  return k.empty() ? 0 : s;
+}
+
+}  // extern "C"

@@ -1,0 +1,1957 @@
+// sk_engine.cpp — semi-naive evaluation driver for gfx950.
+//
+// Restates the reference interpreter's evaluation order on device-resident
+// relations (crates/strata-eval/src/seminaive.rs):
+//   run_semi_naive (seminaive.rs:29-67)    EDB seeding with ⊕, universe, strata in order
+//   saturate_stratum (seminaive.rs:70-106) aggregates once, bootstrap round, delta
+//                                          rounds while any delta is non-empty, the
+//                                          Trop cap universe+2 checked after each round
+//   apply_round (seminaive.rs:114-210)     per rule in declaration order, per IDB
+//                                          positive literal with a non-empty delta;
+//                                          Jacobi: all derivations combined at the end
+// A relation is a sorted, unique array of packed keys (+ Trop weight and
+// +inf flag); the round's derivations per head are sorted, collapsed (Bool
+// dedup / Trop min) and merged into the full relation, which yields the next
+// delta (new tuples, or strictly lower weights) in the same pass.
+#include "sk_engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <set>
+#include <unordered_set>
+
+namespace sk {
+
+// ================================================================ program
+
+Program Program::from_c(const sk_program* p) {
+  Program P;
+  P.num_strata = p->num_strata;
+  for (u32 i = 0; i < p->n_preds; i++) {
+    const sk_pred& d = p->preds[i];
+    std::string nm = d.name ? d.name : "";
+    auto it = P.by_name.find(nm);
+    PredInfo pi{nm, d.arity, d.semiring, d.stratum};
+    if (it == P.by_name.end()) {
+      P.by_name[nm] = (int)P.preds.size();
+      P.preds.push_back(pi);
+    } else {
+      P.preds[it->second] = pi;  // a repeated declaration keeps the last one (BTreeMap insert)
+    }
+  }
+  auto mk_atom = [&](u32 ai) {
+    Atom a;
+    const sk_atom& s = p->atoms[ai];
+    a.pred = s.pred ? s.pred : "";
+    a.neg = s.negated != 0;
+    for (u32 t = 0; t < s.n_terms; t++) {
+      const sk_term& tm = p->terms[s.first_term + t];
+      a.args.push_back(Term{tm.kind, tm.a, tm.b});
+    }
+    a.rel = P.rel_of(a.pred);
+    return a;
+  };
+  for (u32 i = 0; i < p->n_rules; i++) {
+    const sk_rule& r = p->rules[i];
+    Rule R;
+    R.head = mk_atom(r.head);
+    for (u32 l = 0; l < r.n_lits; l++) R.body.push_back(mk_atom(r.first_lit + l));
+    R.stratum = r.stratum;
+    R.var_count = r.var_count;
+    for (auto& t : R.head.args)
+      if (t.kind == SK_T_AGG) R.agg = true;
+    P.rules.push_back(std::move(R));
+  }
+  return P;
+}
+
+// ============================================================= dictionary
+
+bool Dict::int_code(i64 v, u32* c) const {
+  auto it = std::lower_bound(ints.begin(), ints.end(), v);
+  if (it == ints.end() || *it != v) return false;
+  *c = (u32)(sym_span + (it - ints.begin()));
+  return true;
+}
+
+u32 Dict::encode(u32 kind, i64 v, bool* ok) const {
+  *ok = true;
+  if (kind == SK_VAL_SYM) {
+    if (v < 0 || (u64)v >= sym_span) {
+      *ok = false;
+      return 0;
+    }
+    return (u32)v;
+  }
+  u32 c = 0;
+  *ok = int_code(v, &c);
+  return c;
+}
+
+void Dict::decode(u32 code, u32* kind, i64* v) const {
+  if (code < sym_span) {
+    *kind = SK_VAL_SYM;
+    *v = code;
+  } else {
+    *kind = SK_VAL_INT;
+    *v = ints[code - sym_span];
+  }
+}
+
+// ================================================================= helpers
+
+static EvalErr mkerr(int code, const std::string& pred = "", const std::string& msg = "") {
+  EvalErr e;
+  e.code = code;
+  e.pred = pred;
+  e.msg = msg;
+  return e;
+}
+
+static int key_bits_of(int arity, int bits) { return arity * bits; }
+
+template <class F>
+static void with_key(bool wide, F f) {
+  if (wide) f((u128*)nullptr);
+  else f((u64*)nullptr);
+}
+
+// Owners of tables / head chunks, used by the kernel wrappers' allocators.
+struct TableOwner {
+  Pool* pool = nullptr;
+  Table t;
+  std::vector<DBuf<u32>> cols;
+  DBuf<i64> w;
+  DBuf<u8> inf;
+  bool trop = false, has_inf = false;
+};
+
+static void table_alloc(void* ctx, Table* t, u64 n) {
+  TableOwner* o = (TableOwner*)ctx;
+  o->cols.resize(t->ncols);
+  for (int c = 0; c < t->ncols; c++) {
+    o->cols[c].reset(o->pool, n ? n : 1);
+    t->col[c] = o->cols[c].get();
+  }
+  if (o->trop) {
+    o->w.reset(o->pool, n ? n : 1);
+    t->w = o->w.get();
+    if (o->has_inf) {
+      o->inf.reset(o->pool, n ? n : 1);
+      t->inf = o->inf.get();
+    } else {
+      t->inf = nullptr;
+    }
+  } else {
+    t->w = nullptr;
+    t->inf = nullptr;
+  }
+  t->n = n;
+}
+
+struct HeadOwner {
+  Pool* pool = nullptr;
+  Chunk* c = nullptr;
+  bool wide = false, trop = false, has_inf = false;
+};
+
+static void head_alloc(void* ctx, OutDesc* o, u64 n) {
+  HeadOwner* h = (HeadOwner*)ctx;
+  h->c->k.ensure(h->pool, n, h->wide);
+  o->keys = h->c->k.ptr();
+  if (h->trop) {
+    h->c->w.ensure(h->pool, n ? n : 1);
+    o->w = h->c->w.get();
+    if (h->has_inf) {
+      h->c->inf.ensure(h->pool, n ? n : 1);
+      o->inf = h->c->inf.get();
+    } else {
+      o->inf = nullptr;
+    }
+  } else {
+    o->w = nullptr;
+    o->inf = nullptr;
+  }
+  h->c->n = n;
+}
+
+// ============================================================ validation
+
+void Engine::validate() {
+  // compound terms (@terms) are not evaluated on the device
+  auto no_compound = [&](const Atom& a) {
+    for (auto& t : a.args)
+      if (t.kind == SK_T_COMPOUND)
+        throw mkerr(SK_ERR_UNSUPPORTED, a.pred, "compound terms (@terms) are not evaluated by the device engine");
+  };
+  for (auto& r : P_.rules) {
+    no_compound(r.head);
+    for (auto& b : r.body) no_compound(b);
+    int hr = r.head.rel;
+    // the reference panics on an annotation mismatch in ⊗ (crates/strata-eval/src/value.rs:56-62);
+    // the engine refuses such rules up front
+    u32 hs = hr >= 0 ? P_.preds[hr].sem : SK_BOOL;
+    if (r.agg) {
+      if (hr >= 0 && hs != SK_BOOL)
+        throw mkerr(SK_ERR_SEMIRING_MISMATCH, r.head.pred, "aggregate head must be Bool (the result is inserted as Unit)");
+      for (auto& b : r.body)
+        if (!b.neg && b.rel >= 0 && P_.preds[b.rel].sem != SK_BOOL)
+          throw mkerr(SK_ERR_SEMIRING_MISMATCH, b.pred, "aggregate body over a Trop relation");
+    } else {
+      for (auto& b : r.body)
+        if (!b.neg && b.rel >= 0 && hr >= 0 && P_.preds[b.rel].sem != hs)
+          throw mkerr(SK_ERR_SEMIRING_MISMATCH, b.pred, "body literal semiring differs from the head's");
+    }
+    if (r.head.args.size() > (size_t)kMaxCols)
+      throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, r.head.pred, "head arity above 8");
+    for (auto& b : r.body)
+      if (b.args.size() > (size_t)kMaxCols) throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, b.pred, "atom arity above 8");
+  }
+  for (auto& p : P_.preds)
+    if (p.arity > (u32)kMaxCols) throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, p.name, "predicate arity above 8");
+  R_.resize(P_.preds.size());
+  for (size_t i = 0; i < P_.preds.size(); i++) {
+    R_[i].arity = (int)P_.preds[i].arity;
+    R_[i].sem = P_.preds[i].sem;
+  }
+}
+
+// ================================================================ staging
+
+void Engine::stage_rows(const std::string& pred, u64 n, const sk_val* rows, const i64* w, const u8* inf) {
+  if (pending.code) return;
+  int r = P_.rel_of(pred);
+  if (r < 0) {
+    pending = mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+    return;
+  }
+  DRel& R = R_[r];
+  bool weighted = w != nullptr;
+  if (weighted != (R.sem == SK_TROP)) {
+    pending = mkerr(SK_ERR_SEMIRING_MISMATCH, pred, "fact annotation differs from the predicate's semiring");
+    return;
+  }
+  u64 a = (u64)R.arity;
+  for (u64 i = 0; i < n * a; i++) {
+    R.st_kind.push_back(rows[i].kind);
+    R.st_val.push_back(rows[i].v);
+  }
+  // keep the generic rows aligned with their weights: generic rows are
+  // appended after any all-Sym stage of the same relation at load time
+  if (weighted)
+    for (u64 i = 0; i < n; i++) {
+      R.st_w.push_back(w[i]);
+      u8 f = inf ? inf[i] : 0;
+      R.st_inf.push_back(f);
+      if (f) has_inf_ = true;
+    }
+  R.st_weighted = weighted;
+  R.st_rows += n;
+}
+
+void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* const* cols, const i64* w,
+                       const u8* inf) {
+  if (pending.code) return;
+  int r = P_.rel_of(pred);
+  if (r < 0) {
+    pending = mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+    return;
+  }
+  DRel& R = R_[r];
+  if ((int)arity != R.arity) {
+    pending = mkerr(SK_ERR_INVALID_ARGUMENT, pred, "fact arity differs from the declaration");
+    return;
+  }
+  bool weighted = w != nullptr;
+  if (weighted != (R.sem == SK_TROP)) {
+    pending = mkerr(SK_ERR_SEMIRING_MISMATCH, pred, "fact annotation differs from the predicate's semiring");
+    return;
+  }
+  if (!R.st_kind.empty()) {
+    // mixed staging: convert to generic rows
+    for (u64 i = 0; i < n; i++)
+      for (u32 j = 0; j < arity; j++) {
+        R.st_kind.push_back(SK_VAL_SYM);
+        R.st_val.push_back(cols[j][i]);
+      }
+  } else {
+    R.st_sym.resize(arity);
+    for (u32 j = 0; j < arity; j++) R.st_sym[j].insert(R.st_sym[j].end(), cols[j], cols[j] + n);
+  }
+  if (weighted)
+    for (u64 i = 0; i < n; i++) {
+      R.st_w.push_back(w[i]);
+      u8 f = inf ? inf[i] : 0;
+      R.st_inf.push_back(f);
+      if (f) has_inf_ = true;
+    }
+  R.st_weighted = weighted;
+  R.st_rows += n;
+}
+
+// =================================================================== load
+
+void Engine::build_dict() {
+  u64 maxsym = 0;
+  std::set<i64> ints;
+  for (auto& R : R_) {
+    for (auto& c : R.st_sym)
+      for (u32 v : c) maxsym = std::max<u64>(maxsym, v);
+    for (size_t i = 0; i < R.st_kind.size(); i++) {
+      if (R.st_kind[i] == SK_VAL_SYM) {
+        if (R.st_val[i] < 0 || R.st_val[i] > 0xffffffffll)
+          throw mkerr(SK_ERR_INVALID_ARGUMENT, "", "symbol id outside u32");
+        maxsym = std::max<u64>(maxsym, (u64)R.st_val[i]);
+      } else {
+        ints.insert(R.st_val[i]);
+      }
+    }
+  }
+  auto scan_terms = [&](const Atom& a) {
+    for (auto& t : a.args) {
+      if (t.kind == SK_T_CONST) maxsym = std::max<u64>(maxsym, t.a);
+      if (t.kind == SK_T_INT) ints.insert(t.b);
+    }
+  };
+  for (auto& r : P_.rules) {
+    scan_terms(r.head);
+    for (auto& b : r.body) scan_terms(b);
+  }
+  maxsym = std::max<u64>(maxsym, attached_max_sym_);
+  D_.sym_span = maxsym + 1;
+  D_.ints.assign(ints.begin(), ints.end());
+  if (D_.nvals() > 0xffffffffull) throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, "", "more than 2^32 distinct values");
+}
+
+void Engine::upload_rel(int r) {
+  DRel& R = R_[r];
+  if (!attached_.empty() && attached_[r]) return;
+  Exec x = ctx_->exec();
+  const int bits = D_.bits();
+  R.bits = bits;
+  R.wide = key_bits_of(R.arity, bits) > 64;
+  if (key_bits_of(R.arity, bits) > 128)
+    throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, P_.preds[r].name, "packed tuple wider than 128 bits");
+  u64 n = R.st_rows;
+  if (n == 0) {
+    R.full.n = 0;
+    return;
+  }
+  bool trop = R.sem == SK_TROP;
+  // encode to codes, one column at a time
+  std::vector<DBuf<u32>> dcols(R.arity);
+  Cols cols{};
+  u64 nsym = R.st_sym.empty() ? 0 : R.st_sym[0].size();
+  for (int j = 0; j < R.arity; j++) {
+    dcols[j].reset(x.pool, n);
+    cols.c[j] = dcols[j].get();
+    if (nsym)
+      SK_HIP(hipMemcpyAsync(dcols[j].get(), R.st_sym[j].data(), nsym * 4, hipMemcpyHostToDevice, x.stream));
+    u64 ng = n - nsym;
+    if (ng) {
+      std::vector<u32> enc(ng);
+      for (u64 i = 0; i < ng; i++) {
+        bool ok;
+        enc[i] = D_.encode(R.st_kind[i * R.arity + j], R.st_val[i * R.arity + j], &ok);
+      }
+      SK_HIP(hipMemcpyAsync(dcols[j].get() + nsym, enc.data(), ng * 4, hipMemcpyHostToDevice, x.stream));
+      SK_HIP(hipStreamSynchronize(x.stream));
+    }
+  }
+  Keys k1, k2;
+  k1.ensure(x.pool, n, R.wide);
+  k2.ensure(x.pool, n, R.wide);
+  DBuf<i64> w1, w2;
+  DBuf<u8> f1, f2;
+  if (trop) {
+    w1.reset(x.pool, n);
+    w2.reset(x.pool, n);
+    SK_HIP(hipMemcpyAsync(w1.get(), R.st_w.data(), n * 8, hipMemcpyHostToDevice, x.stream));
+    if (has_inf_) {
+      f1.reset(x.pool, n);
+      f2.reset(x.pool, n);
+      SK_HIP(hipMemcpyAsync(f1.get(), R.st_inf.data(), n, hipMemcpyHostToDevice, x.stream));
+    }
+  }
+  with_key(R.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    K* kp = (K*)k1.ptr();
+    pack_cols<K>(x, cols, R.arity, bits, n, kp);
+    K* sk;
+    i64* sw;
+    u8* sf;
+    radix_sort<K>(x, kp, (K*)k2.ptr(), trop ? w1.get() : nullptr, trop ? w2.get() : nullptr,
+                  (trop && has_inf_) ? f1.get() : nullptr, (trop && has_inf_) ? f2.get() : nullptr, n,
+                  key_bits_of(R.arity, bits), &sk, &sw, &sf);
+    R.full.k.ensure(x.pool, n, R.wide);
+    if (trop) {
+      R.full.w.reset(x.pool, n);
+      if (has_inf_) R.full.inf.reset(x.pool, n);
+    }
+    u64 m = unique_runs<K>(x, sk, trop ? sw : nullptr, (trop && has_inf_) ? sf : nullptr, n, (K*)R.full.k.ptr(),
+                           trop ? R.full.w.get() : nullptr, (trop && has_inf_) ? R.full.inf.get() : nullptr);
+    R.full.n = m;
+  });
+  SK_HIP(hipStreamSynchronize(x.stream));
+  R.version++;
+  // release staging
+  std::vector<u32>().swap(R.st_kind);
+  std::vector<i64>().swap(R.st_val);
+  std::vector<std::vector<u32>>().swap(R.st_sym);
+  std::vector<i64>().swap(R.st_w);
+  std::vector<u8>().swap(R.st_inf);
+  R.st_rows = 0;
+}
+
+void Engine::compute_universe() {
+  Exec x = ctx_->exec();
+  u64 words = (D_.sym_span + 31) / 32;
+  DBuf<u32> bm(x.pool, words ? words : 1);
+  memset_async(x, bm.get(), 0, words * 4);
+  for (auto& R : R_) {
+    const int bits = R.bits;
+    if (R.full.n == 0 || R.arity == 0) continue;
+    std::vector<DBuf<u32>> cols(R.arity);
+    std::vector<u32*> cp(R.arity);
+    for (int j = 0; j < R.arity; j++) {
+      cols[j].reset(x.pool, R.full.n);
+      cp[j] = cols[j].get();
+    }
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      unpack_cols<K>(x, (const K*)R.full.k.ptr(), R.full.n, R.arity, bits, cp.data());
+    });
+    for (int j = 0; j < R.arity; j++) mark_codes(x, cp[j], R.full.n, (u32)std::min<u64>(D_.sym_span, 0xffffffffull), bm.get());
+    SK_HIP(hipStreamSynchronize(x.stream));
+  }
+  universe_ = popcount_bits(x, bm.get(), words);
+}
+
+void Engine::load() {
+  if (pending.code) throw pending;
+  build_dict();
+  for (size_t r = 0; r < R_.size(); r++) upload_rel((int)r);
+  // distinct Sym values after the EDB load (crates/strata-eval/src/naive.rs:143-155)
+  if (universe_override_ != ~0ull) universe_ = universe_override_;
+  else compute_universe();
+  for (size_t r = 0; r < R_.size(); r++)
+    if (!attached_.empty() && attached_[r] && R_[r].bits < D_.bits())
+      throw mkerr(SK_ERR_INVALID_ARGUMENT, P_.preds[r].name, "attached relation packed narrower than the dictionary");
+  Exec x = ctx_->exec();
+  if (!D_.ints.empty()) {
+    int_vals_.reset(x.pool, D_.ints.size());
+    SK_HIP(hipMemcpyAsync(int_vals_.get(), D_.ints.data(), D_.ints.size() * 8, hipMemcpyHostToDevice, x.stream));
+  }
+  SK_HIP(hipStreamSynchronize(x.stream));
+  loaded = true;
+}
+
+// Grow the dictionary with new Int values (aggregate results).  New codes
+// are order-preserving.  When existing codes keep their values (every new
+// Int sorts after the old ones) only relations that can still change (stratum
+// >= the current one) are re-packed, and only if their column width must
+// grow; otherwise every relation is re-encoded through the monotone code map,
+// so packed keys stay sorted.
+void Engine::grow_dict(const Dict& nd, u32 stratum) {
+  Exec x = ctx_->exec();
+  bool shifted = false;
+  std::vector<u32> map(D_.nvals());
+  for (u64 c = 0; c < D_.nvals(); c++) {
+    if (c < D_.sym_span) map[c] = (u32)c;
+    else {
+      u32 nc = 0;
+      nd.int_code(D_.ints[c - D_.sym_span], &nc);
+      map[c] = nc;
+      if (nc != c) shifted = true;
+    }
+  }
+  DBuf<u32> dmap(x.pool, map.size() ? map.size() : 1);
+  if (!map.empty()) SK_HIP(hipMemcpyAsync(dmap.get(), map.data(), map.size() * 4, hipMemcpyHostToDevice, x.stream));
+  const int nb = nd.bits();
+  std::vector<bool> live(R_.size(), false);
+  for (size_t i = 0; i < R_.size(); i++) live[i] = P_.preds[i].stratum >= stratum;
+  for (auto& r : P_.rules)
+    if (r.stratum >= stratum && r.head.rel >= 0) live[r.head.rel] = true;
+  for (size_t i = 0; i < R_.size(); i++) {
+    DRel& R = R_[i];
+    int ob = R.bits;
+    int tb = shifted ? std::max(ob, nb) : ((live[i] && ob < nb) ? nb : ob);
+    if (!shifted && tb == ob) continue;
+    bool nw = key_bits_of(R.arity, tb) > 64;
+    if (key_bits_of(R.arity, tb) > 128) throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, "", "packed tuple wider than 128 bits");
+    if (R.full.n) {
+      Keys nk;
+      nk.ensure(x.pool, R.full.n, nw);
+      with_key(R.wide, [&](auto* t1) {
+        using K = std::remove_pointer_t<decltype(t1)>;
+        with_key(nw, [&](auto* t2) {
+          using KO = std::remove_pointer_t<decltype(t2)>;
+          remap_keys<K, KO>(x, (const K*)R.full.k.ptr(), R.full.n, R.arity, ob, tb, dmap.get(), (KO*)nk.ptr());
+        });
+      });
+      SK_HIP(hipStreamSynchronize(x.stream));
+      R.full.k = std::move(nk);
+    }
+    R.bits = tb;
+    R.wide = nw;
+    R.delta.release();
+    R.idx.clear();
+    R.version++;
+    R.dversion++;
+  }
+  D_ = nd;
+  int_vals_.release();
+  if (!D_.ints.empty()) {
+    int_vals_.reset(x.pool, D_.ints.size());
+    SK_HIP(hipMemcpyAsync(int_vals_.get(), D_.ints.data(), D_.ints.size() * 8, hipMemcpyHostToDevice, x.stream));
+  }
+  SK_HIP(hipStreamSynchronize(x.stream));
+}
+
+// ================================================================ indexes
+
+Index& Engine::get_index(int rel, bool delta, const std::vector<int>& perm, bool want_csr, bool want_adj) {
+  DRel& R = R_[rel];
+  std::string key = delta ? "d" : "f";
+  for (int p : perm) key += char('0' + p);
+  Index& ix = R.idx[key];
+  u64 ver = delta ? R.dversion : R.version;
+  Exec x = ctx_->exec();
+  const int bits = R.bits;
+  bool identity = true;
+  for (size_t j = 0; j < perm.size(); j++)
+    if (perm[j] != (int)j) identity = false;
+  const RelData& src = source(rel, delta);
+  if (ix.version != ver) {
+    ix.perm = perm;
+    ix.csr_version = ~0ull;
+    ix.adj_version = ~0ull;
+    if (!identity) {
+      bool trop = R.sem == SK_TROP;
+      u64 n = src.n;
+      ix.data.n = n;
+      ix.data.k.ensure(x.pool, n, R.wide);
+      Keys alt;
+      alt.ensure(x.pool, n, R.wide);
+      DBuf<i64> w2;
+      DBuf<u8> f2;
+      if (trop) {
+        ix.data.w.ensure(x.pool, n ? n : 1);
+        w2.reset(x.pool, n ? n : 1);
+        if (has_inf_) {
+          ix.data.inf.ensure(x.pool, n ? n : 1);
+          f2.reset(x.pool, n ? n : 1);
+        }
+        if (n) {
+          SK_HIP(hipMemcpyAsync(ix.data.w.get(), src.w.get(), n * 8, hipMemcpyDeviceToDevice, x.stream));
+          if (has_inf_)
+            SK_HIP(hipMemcpyAsync(ix.data.inf.get(), src.inf.get(), n, hipMemcpyDeviceToDevice, x.stream));
+        }
+      }
+      with_key(R.wide, [&](auto* tag) {
+        using K = std::remove_pointer_t<decltype(tag)>;
+        repack<K, K>(x, (const K*)src.k.ptr(), n, R.arity, bits, perm.data(), (K*)ix.data.k.ptr());
+        K* sk;
+        i64* sw;
+        u8* sf;
+        radix_sort<K>(x, (K*)ix.data.k.ptr(), (K*)alt.ptr(), trop ? ix.data.w.get() : nullptr,
+                      trop ? w2.get() : nullptr, (trop && has_inf_) ? ix.data.inf.get() : nullptr,
+                      (trop && has_inf_) ? f2.get() : nullptr, n, key_bits_of(R.arity, bits), &sk, &sw, &sf);
+        if (sk != (K*)ix.data.k.ptr()) {
+          std::swap(ix.data.k, alt);
+          if (trop) {
+            std::swap(ix.data.w, w2);
+            if (has_inf_) std::swap(ix.data.inf, f2);
+          }
+        }
+      });
+    }
+    ix.version = ver;
+  }
+  const RelData& d = identity ? src : ix.data;
+  if (want_csr && ix.csr_version != ver) {
+    u64 nv = D_.nvals();
+    ix.csr.ensure(x.pool, nv + 1);
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      build_csr<K>(x, (const K*)d.k.ptr(), d.n, bits * (R.arity - 1), nv, ix.csr.get());
+    });
+    ix.csr_version = ver;
+  }
+  if (want_adj && ix.adj_version != ver) {
+    ix.adj.ensure(x.pool, d.n ? d.n : 1);
+    DBuf<u32> first(x.pool, d.n ? d.n : 1);
+    u32* cp[2] = {first.get(), ix.adj.get()};
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      unpack_cols<K>(x, (const K*)d.k.ptr(), d.n, 2, bits, cp);
+    });
+    SK_HIP(hipStreamSynchronize(x.stream));
+    ix.adj_version = ver;
+  }
+  return ix;
+}
+
+// ================================================================= plans
+
+u32 Engine::code_or_never(u32 kind, i64 v, bool* never) const {
+  bool ok;
+  u32 c = D_.encode(kind == SK_T_CONST ? SK_VAL_SYM : SK_VAL_INT, v, &ok);
+  if (!ok) *never = true;
+  return c;
+}
+
+RulePlan Engine::plan_rule(int ri, int dpos) {
+  const Rule& r = P_.rules[ri];
+  RulePlan pl;
+  pl.rule = ri;
+  pl.dpos = dpos;
+  int hr = r.head.rel;
+  pl.trop = hr >= 0 && P_.preds[hr].sem == SK_TROP;
+  const int n = (int)r.body.size();
+  const u32 cur = r.stratum;
+
+  // literals that raise a static error when reached keep body order
+  bool issue = false;
+  {
+    std::set<u32> bound;
+    for (int j = 0; j < n; j++) {
+      const Atom& a = r.body[j];
+      if (a.rel < 0) issue = true;
+      if (a.neg) {
+        if (a.rel >= 0 && P_.preds[a.rel].stratum >= cur) issue = true;
+        for (auto& t : a.args)
+          if ((t.kind == SK_T_VAR && !bound.count(t.a)) || t.kind == SK_T_AGG) issue = true;
+      } else {
+        for (auto& t : a.args)
+          if (t.kind == SK_T_VAR) bound.insert(t.a);
+      }
+    }
+  }
+
+  // leapfrog triangle: Bool, three positive binary atoms over three vars,
+  // each pair of atoms sharing one var, no literal changing in this stratum
+  if (!pl.trop && !r.agg && dpos < 0 && n == 3 && !issue) {
+    bool ok = true;
+    for (auto& a : r.body) {
+      if (a.neg || a.args.size() != 2 || a.rel < 0 || P_.preds[a.rel].arity != 2 || in_idb_[a.rel]) ok = false;
+      for (auto& t : a.args)
+        if (t.kind != SK_T_VAR) ok = false;
+      if (ok && a.args[0].a == a.args[1].a) ok = false;
+    }
+    if (ok) {
+      u32 u = r.body[0].args[0].a, v = r.body[0].args[1].a;
+      u32 w = ~0u;
+      int lv = -1, lu = -1;
+      for (int j = 1; j < 3; j++) {
+        const Atom& a = r.body[j];
+        u32 x0 = a.args[0].a, x1 = a.args[1].a;
+        bool hasv = x0 == v || x1 == v, hasu = x0 == u || x1 == u;
+        u32 other = (x0 == v || x0 == u) ? x1 : x0;
+        if (hasv && !hasu) {
+          lv = j;
+          if (w == ~0u) w = other;
+          else if (w != other) ok = false;
+        } else if (hasu && !hasv) {
+          lu = j;
+          if (w == ~0u) w = other;
+          else if (w != other) ok = false;
+        } else {
+          ok = false;
+        }
+      }
+      if (ok && lv > 0 && lu > 0 && w != u && w != v) {
+        pl.tri = true;
+        pl.tri_lit[0] = 0;
+        pl.tri_lit[1] = lv;
+        pl.tri_lit[2] = lu;
+        pl.tri_slot[0] = (int)u;
+        pl.tri_slot[1] = (int)v;
+        pl.tri_slot[2] = (int)w;
+        pl.tri_col_v = r.body[lv].args[0].a == v ? 0 : 1;
+        pl.tri_col_w2 = 1 - pl.tri_col_v;
+        pl.tri_col_u = r.body[lu].args[0].a == u ? 0 : 1;
+        pl.tri_col_w3 = 1 - pl.tri_col_u;
+        // head (u, v, w) in order: the emitted rows are already sorted and unique
+        const Atom& h = r.head;
+        pl.sorted_out = h.args.size() == 3 && h.args[0].kind == SK_T_VAR && h.args[0].a == u &&
+                        h.args[1].kind == SK_T_VAR && h.args[1].a == v && h.args[2].kind == SK_T_VAR &&
+                        h.args[2].a == w;
+        for (auto& t : h.args)
+          if (t.kind == SK_T_VAR && t.a != u && t.a != v && t.a != w) {
+            pl.head_err = SK_ERR_UNSAFE_HEAD_VAR;
+            pl.head_err_slot = t.a;
+            break;
+          }
+        if (!pl.head_err) return pl;
+        pl.tri = false;
+      }
+    }
+  }
+
+  // evaluation order
+  std::vector<int> order;
+  bool reorder = !pl.trop && !issue;
+  if (reorder) {
+    std::vector<bool> used(n, false);
+    std::set<u32> bound;
+    auto add_negs = [&]() {
+      for (int j = 0; j < n; j++) {
+        if (used[j] || !r.body[j].neg) continue;
+        bool all = true;
+        for (auto& t : r.body[j].args)
+          if (t.kind == SK_T_VAR && !bound.count(t.a)) all = false;
+        if (all) {
+          used[j] = true;
+          order.push_back(j);
+        }
+      }
+    };
+    int first = dpos;
+    if (first < 0)
+      for (int j = 0; j < n; j++)
+        if (!r.body[j].neg) {
+          first = j;
+          break;
+        }
+    if (first < 0) add_negs();
+    while (first >= 0) {
+      used[first] = true;
+      order.push_back(first);
+      for (auto& t : r.body[first].args)
+        if (t.kind == SK_T_VAR) bound.insert(t.a);
+      add_negs();
+      first = -1;
+      for (int j = 0; j < n && first < 0; j++) {
+        if (used[j] || r.body[j].neg) continue;
+        for (auto& t : r.body[j].args)
+          if (t.kind == SK_T_VAR && bound.count(t.a)) first = j;
+      }
+      for (int j = 0; j < n && first < 0; j++)
+        if (!used[j] && !r.body[j].neg) first = j;
+    }
+    for (int j = 0; j < n; j++)
+      if (!used[j]) order.push_back(j);
+  } else {
+    for (int j = 0; j < n; j++) order.push_back(j);
+  }
+
+  pl.unit_source = order.empty() || r.body[order[0]].neg;
+  std::set<u32> bound;
+  std::vector<int> slots;  // current table layout (all bound vars; pruned by liveness below)
+  for (size_t si = 0; si < order.size(); si++) {
+    int j = order[si];
+    const Atom& a = r.body[j];
+    StepPlan st;
+    st.lit = j;
+    st.rel = a.rel;
+    st.delta = (j == dpos);
+    st.in_slots = slots;
+    if (a.rel < 0) {
+      st.err = SK_ERR_UNKNOWN_PRED;
+      st.kind = a.neg ? 2 : (si == 0 ? 0 : 1);
+      pl.steps.push_back(st);
+      break;  // nothing after an unknown predicate can run
+    }
+    int arity = (int)P_.preds[a.rel].arity;
+    if (a.neg) {
+      st.kind = 2;
+      if (P_.preds[a.rel].stratum >= cur) st.err = SK_ERR_NEGATION_NOT_STRATIFIED;
+      for (auto& t : a.args) {
+        if (st.err) break;
+        if (t.kind == SK_T_VAR && !bound.count(t.a)) {
+          st.err = SK_ERR_UNSAFE_NEG_VAR;
+          st.err_slot = t.a;
+        } else if (t.kind == SK_T_AGG) {
+          st.err = SK_ERR_UNSUPPORTED;
+        }
+      }
+      for (auto& t : a.args) {
+        st.anti_slot.push_back(t.kind == SK_T_VAR ? (int)t.a : -1);
+        st.anti_kind.push_back(t.kind);
+        st.anti_val.push_back(t.kind == SK_T_CONST ? (i64)t.a : t.b);
+      }
+      if ((int)a.args.size() != arity) st.never = true;  // never present: the negation holds
+      st.out_slots = slots;
+      pl.steps.push_back(st);
+      if (st.err) break;
+      continue;
+    }
+    bool first_step = slots.empty() && si == 0 && !pl.unit_source;
+    if ((int)a.args.size() != arity) st.never = true;
+    for (auto& t : a.args)
+      if (t.kind == SK_T_AGG) st.never = true;
+    if (first_step) {
+      st.kind = 0;
+      std::map<u32, int> firstcol;
+      for (int c = 0; c < (int)a.args.size(); c++) {
+        const Term& t = a.args[c];
+        if (t.kind == SK_T_VAR) {
+          auto it = firstcol.find(t.a);
+          if (it == firstcol.end()) {
+            firstcol[t.a] = c;
+            st.newvars.push_back({c, (int)t.a});
+          } else {
+            st.scan_eq.push_back({it->second, c});
+          }
+        } else if (t.kind == SK_T_CONST || t.kind == SK_T_INT) {
+          st.scan_const_kind.push_back({c, t.kind});
+          st.scan_const_val.push_back(t.kind == SK_T_CONST ? (i64)t.a : t.b);
+        }
+      }
+    } else {
+      st.kind = 1;
+      std::vector<int> probe, rest;
+      for (int c = 0; c < (int)a.args.size(); c++) {
+        const Term& t = a.args[c];
+        bool isprobe = (t.kind == SK_T_VAR && bound.count(t.a)) || t.kind == SK_T_CONST || t.kind == SK_T_INT;
+        (isprobe ? probe : rest).push_back(c);
+      }
+      if (st.never) {
+        probe.clear();
+        rest.clear();
+        for (int c = 0; c < arity; c++) rest.push_back(c);
+      }
+      st.perm = probe;
+      st.perm.insert(st.perm.end(), rest.begin(), rest.end());
+      st.nprobe = (int)probe.size();
+      for (int c : probe) {
+        const Term& t = a.args[c];
+        st.probe_slot.push_back(t.kind == SK_T_VAR ? (int)t.a : -1);
+        st.probe_kind.push_back(t.kind);
+        st.probe_val.push_back(t.kind == SK_T_CONST ? (i64)t.a : t.b);
+      }
+      std::map<u32, int> firstidx;
+      for (int q = st.nprobe; q < (int)st.perm.size() && !st.never; q++) {
+        const Term& t = a.args[st.perm[q]];
+        if (t.kind != SK_T_VAR) continue;
+        auto it = firstidx.find(t.a);
+        if (it == firstidx.end()) {
+          firstidx[t.a] = q;
+          st.newvars.push_back({q, (int)t.a});
+        } else {
+          st.filt.push_back({it->second, q});
+        }
+      }
+    }
+    for (auto& nv : st.newvars)
+      if (!bound.count((u32)nv.second)) {
+        bound.insert((u32)nv.second);
+        slots.push_back(nv.second);
+      }
+    st.out_slots = slots;
+    pl.steps.push_back(st);
+  }
+
+  // head checks (build_head, crates/strata-eval/src/naive.rs:526-532)
+  std::set<u32> need;
+  if (!r.agg) {
+    for (auto& t : r.head.args) {
+      if (t.kind == SK_T_VAR) {
+        if (!bound.count(t.a)) {
+          pl.head_err = SK_ERR_UNSAFE_HEAD_VAR;
+          pl.head_err_slot = t.a;
+          break;
+        }
+        need.insert(t.a);
+      } else if (t.kind == SK_T_AGG) {
+        pl.head_err = SK_ERR_UNSUPPORTED;
+        break;
+      }
+    }
+  } else {
+    // aggregate: group keys and (except for count) the aggregand stay live
+    for (auto& t : r.head.args) {
+      if (t.kind == SK_T_VAR && bound.count(t.a)) need.insert(t.a);
+      if (t.kind == SK_T_AGG && t.a != SK_AGG_COUNT && bound.count((u32)t.b)) need.insert((u32)t.b);
+    }
+  }
+  if (pl.head_err) need.clear();
+
+  // liveness: keep only variables used later
+  std::set<u32> live = need;
+  for (int s = (int)pl.steps.size() - 1; s >= 0; s--) {
+    StepPlan& st = pl.steps[s];
+    std::vector<int> out;
+    for (int sl : st.out_slots)
+      if (live.count((u32)sl)) out.push_back(sl);
+    st.out_slots = out;
+    if (st.kind == 1)
+      for (int sl : st.probe_slot)
+        if (sl >= 0) live.insert((u32)sl);
+    if (st.kind == 2)
+      for (int sl : st.anti_slot)
+        if (sl >= 0) live.insert((u32)sl);
+  }
+  for (size_t s = 0; s < pl.steps.size(); s++) {
+    pl.steps[s].in_slots = s == 0 ? std::vector<int>() : pl.steps[s - 1].out_slots;
+    if (pl.steps[s].out_slots.size() > (size_t)kMaxCols)
+      throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, r.head.pred, "more than 8 live variables in a rule body");
+  }
+  return pl;
+}
+
+// ============================================================ plan exec
+
+static int col_index(const std::vector<int>& slots, int slot) {
+  for (size_t i = 0; i < slots.size(); i++)
+    if (slots[i] == slot) return (int)i;
+  return -1;
+}
+
+void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out) {
+  const Rule& r = P_.rules[pl.rule];
+  Exec x = ctx_->exec();
+  const int hr = r.head.rel;
+  const bool trop = pl.trop;
+  DRel& H = R_[hr];
+  const int hbits = H.bits;
+
+  Chunk ch;
+  HeadOwner ho;
+  ho.pool = x.pool;
+  ho.c = &ch;
+  ho.wide = H.wide;
+  ho.trop = trop;
+  ho.has_inf = has_inf_ && trop;
+
+  // head projection descriptor over a table layout (or the leapfrog slots)
+  auto head_desc = [&](const std::vector<int>& slots, OutDesc* o) {
+    o->ncols = (int)r.head.args.size();
+    o->bits = hbits;
+    for (int c = 0; c < o->ncols; c++) {
+      const Term& t = r.head.args[c];
+      if (t.kind == SK_T_VAR) {
+        o->kind[c] = 0;
+        o->idx[c] = col_index(slots, (int)t.a);
+      } else {
+        bool never = false;
+        o->kind[c] = 2;
+        o->cnst[c] = code_or_never(t.kind, t.kind == SK_T_CONST ? (i64)t.a : t.b, &never);
+      }
+    }
+  };
+
+  if (pl.tri) {
+    const Atom& d = r.body[pl.tri_lit[0]];
+    std::vector<int> pv = {pl.tri_col_v, pl.tri_col_w2};
+    std::vector<int> pu = {pl.tri_col_u, pl.tri_col_w3};
+    Index& iv = get_index(r.body[pl.tri_lit[1]].rel, false, pv, true, true);
+    Index& iu = get_index(r.body[pl.tri_lit[2]].rel, false, pu, true, true);
+    const RelData& dr = R_[d.rel].full;
+    TriDesc td;
+    td.bits = R_[d.rel].bits;
+    td.u_col = 0;
+    OutDesc o;
+    std::vector<int> tslots = {pl.tri_slot[0], pl.tri_slot[1], pl.tri_slot[2]};
+    head_desc(tslots, &o);
+    with_key(R_[d.rel].wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      double ll = tri_list_len<K>(x, (const K*)dr.k.ptr(), dr.n, td, iv.csr.get(), iu.csr.get());
+      triangles<K>(x, (const K*)dr.k.ptr(), dr.n, td, iv.csr.get(), iv.adj.get(), iu.csr.get(), iu.adj.get(), &o,
+                   H.wide, false, head_alloc, &ho, &ll);
+    });
+    ch.sorted_unique = pl.sorted_out;
+    out[hr].push_back(std::move(ch));
+    return;
+  }
+
+  TableOwner cur;
+  cur.pool = x.pool;
+  cur.trop = trop;
+  cur.has_inf = has_inf_ && trop;
+  if (pl.unit_source) {
+    cur.t.ncols = 0;
+    table_alloc(&cur, &cur.t, 1);
+    if (trop) {
+      memset_async(x, cur.t.w, 0, 8);  // otimes_id = Finite(0)
+      if (cur.t.inf) memset_async(x, cur.t.inf, 0, 1);
+    }
+  }
+  bool fused = false;
+  const size_t ns = pl.steps.size();
+  for (size_t s = 0; s < ns; s++) {
+    const StepPlan& st = pl.steps[s];
+    bool reached = (s == 0 && !pl.unit_source) || cur.t.n > 0;
+    if (st.err) {
+      if (!reached) {
+        cur.t.n = 0;
+        break;
+      }
+      EvalErr e = mkerr(st.err, r.body[st.lit].pred);
+      e.slot = st.err_slot;
+      if (st.err == SK_ERR_UNKNOWN_PRED) e.msg = "unknown predicate";
+      if (st.err == SK_ERR_NEGATION_NOT_STRATIFIED) e.msg = "negated predicate is not in a strictly lower stratum";
+      if (st.err == SK_ERR_UNSAFE_NEG_VAR) e.msg = "unbound variable in negated literal";
+      throw e;
+    }
+    TableOwner nxt;
+    nxt.pool = x.pool;
+    nxt.trop = trop;
+    nxt.has_inf = has_inf_ && trop;
+    nxt.t.ncols = (int)st.out_slots.size();
+    const bool last = (s + 1 == ns);
+    if (st.kind == 0) {
+      const RelData& src = source(st.rel, st.delta);
+      ScanDesc sd;
+      sd.arity = R_[st.rel].arity;
+      sd.bits = R_[st.rel].bits;
+      bool never = st.never;
+      for (size_t q = 0; q < st.scan_const_kind.size(); q++) {
+        sd.const_col[sd.nconst] = st.scan_const_kind[q].first;
+        sd.const_val[sd.nconst] = code_or_never(st.scan_const_kind[q].second, st.scan_const_val[q], &never);
+        sd.nconst++;
+      }
+      for (auto& e : st.scan_eq) {
+        sd.eq_a[sd.neq] = e.first;
+        sd.eq_b[sd.neq] = e.second;
+        sd.neq++;
+      }
+      for (int c = 0; c < nxt.t.ncols; c++) {
+        int sl = st.out_slots[c];
+        for (auto& nv : st.newvars)
+          if (nv.second == sl) sd.out_col[c] = nv.first;
+      }
+      if (never || src.n == 0) {
+        table_alloc(&nxt, &nxt.t, 0);
+      } else {
+        with_key(R_[st.rel].wide, [&](auto* tag) {
+          using K = std::remove_pointer_t<decltype(tag)>;
+          scan_rel<K>(x, (const K*)src.k.ptr(), trop ? src.w.get() : nullptr, (trop && has_inf_) ? src.inf.get() : nullptr,
+                      src.n, sd, &nxt.t, table_alloc, &nxt);
+        });
+      }
+    } else if (st.kind == 1) {
+      bool never = st.never;
+      ProbeDesc pd;
+      pd.arity = R_[st.rel].arity;
+      pd.bits = R_[st.rel].bits;
+      pd.nprobe = st.nprobe;
+      for (int j = 0; j < st.nprobe; j++) {
+        if (st.probe_slot[j] >= 0) {
+          pd.psrc[j] = col_index(st.in_slots, st.probe_slot[j]);
+        } else {
+          pd.psrc[j] = -1;
+          pd.pconst[j] = code_or_never(st.probe_kind[j], st.probe_val[j], &never);
+        }
+      }
+      for (auto& f : st.filt) {
+        pd.fa[pd.nfilt] = f.first;
+        pd.fb[pd.nfilt] = f.second;
+        pd.nfilt++;
+      }
+      const RelData& src = source(st.rel, st.delta);
+      bool static_rel = !st.delta && !in_idb_[st.rel];
+      bool final_step = last && !pl.head_err && !r.agg;
+      OutDesc o;
+      if (final_step) {
+        // fused head projection: table columns or new relation columns
+        o.ncols = (int)r.head.args.size();
+        o.bits = hbits;
+        for (int c = 0; c < o.ncols; c++) {
+          const Term& t = r.head.args[c];
+          if (t.kind == SK_T_VAR) {
+            int ci = col_index(st.in_slots, (int)t.a);
+            if (ci >= 0) {
+              o.kind[c] = 0;
+              o.idx[c] = ci;
+            } else {
+              o.kind[c] = 1;
+              for (auto& nv : st.newvars)
+                if (nv.second == (int)t.a) o.idx[c] = nv.first;
+            }
+          } else {
+            o.kind[c] = 2;
+            bool nv2 = false;
+            o.cnst[c] = code_or_never(t.kind, t.kind == SK_T_CONST ? (i64)t.a : t.b, &nv2);
+          }
+        }
+      } else {
+        o.ncols = nxt.t.ncols;
+        o.table = &nxt.t;
+        for (int c = 0; c < o.ncols; c++) {
+          int sl = st.out_slots[c];
+          int ci = col_index(st.in_slots, sl);
+          if (ci >= 0) {
+            o.kind[c] = 0;
+            o.idx[c] = ci;
+          } else {
+            o.kind[c] = 1;
+            for (auto& nv : st.newvars)
+              if (nv.second == sl) o.idx[c] = nv.first;
+          }
+        }
+      }
+      if (never || src.n == 0 || cur.t.n == 0) {
+        if (final_step) head_alloc(&ho, &o, 0);
+        else table_alloc(&nxt, &nxt.t, 0);
+      } else {
+        bool want_csr = static_rel && st.nprobe == 1 && pd.nfilt == 0;
+        Index& ix = get_index(st.rel, st.delta, st.perm, want_csr, false);
+        bool identity = true;
+        for (size_t q = 0; q < st.perm.size(); q++)
+          if (st.perm[q] != (int)q) identity = false;
+        const RelData& d = identity ? src : ix.data;
+        pd.sn = d.n;
+        Table tin = cur.t;
+        if (nparts_ && trop) {
+          // batched mode: an overflow is charged to the row's program
+          tin.part_col = col_index(st.in_slots, (int)r.var_count - 1);
+          tin.perr = part_derr_.get();
+        }
+        with_key(R_[st.rel].wide, [&](auto* tag) {
+          using K = std::remove_pointer_t<decltype(tag)>;
+          join<K>(x, tin, pd, (const K*)d.k.ptr(), trop ? d.w.get() : nullptr, (trop && has_inf_) ? d.inf.get() : nullptr,
+                  want_csr ? ix.csr.get() : nullptr, &o, final_step, H.wide, trop, table_alloc, &nxt, head_alloc, &ho);
+        });
+      }
+      if (final_step) fused = true;
+    } else {
+      // anti-join against the (lower-stratum, frozen) relation
+      const RelData& tgt = R_[st.rel].full;
+      AntiDesc ad;
+      ad.arity = R_[st.rel].arity;
+      ad.bits = R_[st.rel].bits;
+      bool never_present = st.never;
+      for (int c = 0; c < ad.arity && !st.never; c++) {
+        if (st.anti_slot[c] >= 0) {
+          ad.src[c] = col_index(st.in_slots, st.anti_slot[c]);
+        } else {
+          ad.src[c] = -1;
+          ad.cnst[c] = code_or_never(st.anti_kind[c], st.anti_val[c], &never_present);
+        }
+      }
+      ad.never_present = never_present || tgt.n == 0;
+      for (int c = 0; c < nxt.t.ncols; c++) nxt.t.srccol[c] = col_index(st.in_slots, st.out_slots[c]);
+      with_key(R_[st.rel].wide, [&](auto* tag) {
+        using K = std::remove_pointer_t<decltype(tag)>;
+        anti_join<K>(x, cur.t, ad, (const K*)tgt.k.ptr(), tgt.n, &nxt.t, table_alloc, &nxt);
+      });
+    }
+    cur = std::move(nxt);
+    if (!fused) {
+      // the table's column pointers live in `cur` now
+      for (int c = 0; c < cur.t.ncols; c++) cur.t.col[c] = cur.cols[c].get();
+      cur.t.w = cur.w.get();
+      cur.t.inf = cur.inf.get();
+    }
+  }
+
+  if (!fused) {
+    if (pl.head_err) {
+      if (cur.t.n > 0) {
+        EvalErr e = mkerr(pl.head_err, r.head.pred,
+                          pl.head_err == SK_ERR_UNSAFE_HEAD_VAR ? "unbound head variable" : "aggregate in atom position");
+        e.slot = pl.head_err_slot;
+        throw e;
+      }
+      return;
+    }
+    OutDesc o;
+    head_desc(pl.steps.empty() ? std::vector<int>() : pl.steps.back().out_slots, &o);
+    project(x, cur.t, &o, H.wide, head_alloc, &ho);
+  }
+  if (ch.n) out[hr].push_back(std::move(ch));
+}
+
+// ============================================================== finalize
+
+// ⊕-combine one round's derivations of relation `rel` into it (Jacobi
+// application, crates/strata-eval/src/seminaive.rs:192-208); the next delta
+// is the set of changed tuples with their merged values.
+u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
+  DRel& R = R_[rel];
+  Exec x = ctx_->exec();
+  const int bits = R.bits;
+  const bool trop = R.sem == SK_TROP;
+  const bool hf = trop && has_inf_;
+  u64 total = 0;
+  for (auto& c : chunks) total += c.n;
+  R.delta.release();
+  R.dversion++;
+  if (total == 0) return 0;
+  Chunk cand;
+  if (chunks.size() == 1) {
+    cand = std::move(chunks[0]);
+  } else {
+    cand.k.ensure(x.pool, total, R.wide);
+    if (trop) {
+      cand.w.reset(x.pool, total);
+      if (hf) cand.inf.reset(x.pool, total);
+    }
+    u64 off = 0;
+    size_t kb = R.wide ? 16 : 8;
+    for (auto& c : chunks) {
+      if (!c.n) continue;
+      SK_HIP(hipMemcpyAsync((char*)cand.k.ptr() + off * kb, c.k.ptr(), c.n * kb, hipMemcpyDeviceToDevice, x.stream));
+      if (trop) SK_HIP(hipMemcpyAsync(cand.w.get() + off, c.w.get(), c.n * 8, hipMemcpyDeviceToDevice, x.stream));
+      if (hf) SK_HIP(hipMemcpyAsync(cand.inf.get() + off, c.inf.get(), c.n, hipMemcpyDeviceToDevice, x.stream));
+      off += c.n;
+    }
+    cand.n = total;
+    cand.sorted_unique = false;
+  }
+  chunks.clear();
+  u64 nfresh = 0;
+  with_key(R.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    K* ck = (K*)cand.k.ptr();
+    i64* cw = trop ? cand.w.get() : nullptr;
+    u8* cf = hf ? cand.inf.get() : nullptr;
+    u64 cn = cand.n;
+    Keys uk, alt;
+    DBuf<i64> uw, w2;
+    DBuf<u8> uf, f2;
+    if (!cand.sorted_unique) {
+      alt.ensure(x.pool, cn, R.wide);
+      if (trop) w2.reset(x.pool, cn);
+      if (hf) f2.reset(x.pool, cn);
+      K* sk;
+      i64* sw;
+      u8* sf;
+      radix_sort<K>(x, ck, (K*)alt.ptr(), cw, trop ? w2.get() : nullptr, cf, hf ? f2.get() : nullptr, cn,
+                    key_bits_of(R.arity, bits), &sk, &sw, &sf);
+      uk.ensure(x.pool, cn, R.wide);
+      if (trop) uw.reset(x.pool, cn);
+      if (hf) uf.reset(x.pool, cn);
+      cn = unique_runs<K>(x, sk, trop ? sw : nullptr, hf ? sf : nullptr, cn, (K*)uk.ptr(), trop ? uw.get() : nullptr,
+                          hf ? uf.get() : nullptr);
+      ck = (K*)uk.ptr();
+      cw = trop ? uw.get() : nullptr;
+      cf = hf ? uf.get() : nullptr;
+    }
+    if (R.full.n == 0) {
+      // full is empty: every candidate is fresh and the union is the candidate set
+      RelData nf;
+      nf.n = cn;
+      nf.k.ensure(x.pool, cn, R.wide);
+      SK_HIP(hipMemcpyAsync(nf.k.ptr(), ck, cn * sizeof(K), hipMemcpyDeviceToDevice, x.stream));
+      if (trop) {
+        nf.w.reset(x.pool, cn);
+        SK_HIP(hipMemcpyAsync(nf.w.get(), cw, cn * 8, hipMemcpyDeviceToDevice, x.stream));
+        if (hf) {
+          nf.inf.reset(x.pool, cn);
+          SK_HIP(hipMemcpyAsync(nf.inf.get(), cf, cn, hipMemcpyDeviceToDevice, x.stream));
+        }
+      }
+      R.full = std::move(nf);
+      // the delta reuses the candidate buffers
+      R.delta.n = cn;
+      if (!cand.sorted_unique) {
+        R.delta.k = std::move(uk);
+        R.delta.w = std::move(uw);
+        R.delta.inf = std::move(uf);
+      } else {
+        R.delta.k = std::move(cand.k);
+        R.delta.w = std::move(cand.w);
+        R.delta.inf = std::move(cand.inf);
+      }
+      nfresh = cn;
+      return;
+    }
+    DBuf<K> u_k, f_k;
+    DBuf<i64> u_w, f_w;
+    DBuf<u8> u_f, f_f;
+    u64 nu = 0;
+    merge_diff<K>(x, ck, cw, cf, cn, (const K*)R.full.k.ptr(), trop ? R.full.w.get() : nullptr,
+                  hf ? R.full.inf.get() : nullptr, R.full.n, &u_k, &u_w, &u_f, &nu, &f_k, &f_w, &f_f, &nfresh, trop, hf);
+    if (nfresh) {
+      RelData nf;
+      nf.n = nu;
+      if constexpr (sizeof(K) == 16) {
+        nf.k.wide = true;
+        nf.k.b = std::move(u_k);
+      } else {
+        nf.k.wide = false;
+        nf.k.a = std::move(u_k);
+      }
+      nf.w = std::move(u_w);
+      nf.inf = std::move(u_f);
+      R.full = std::move(nf);
+      R.delta.n = nfresh;
+      if constexpr (sizeof(K) == 16) {
+        R.delta.k.wide = true;
+        R.delta.k.b = std::move(f_k);
+      } else {
+        R.delta.k.wide = false;
+        R.delta.k.a = std::move(f_k);
+      }
+      R.delta.w = std::move(f_w);
+      R.delta.inf = std::move(f_f);
+    }
+  });
+  if (nfresh) R.version++;
+  R.delta.n = nfresh;
+  return nfresh;
+}
+
+void Engine::check_device_error() {
+  Exec x = ctx_->exec();
+  DevErr h{};
+  SK_HIP(hipMemcpyAsync(&h, ctx_->derr, sizeof(DevErr), hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  if (h.code == SK_ERR_OVERFLOW) {
+    DevErr z{};
+    SK_HIP(hipMemcpyAsync(ctx_->derr, &z, sizeof(DevErr), hipMemcpyHostToDevice, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    EvalErr e = mkerr(SK_ERR_OVERFLOW, "", "tropical weight overflow");
+    e.lhs = h.lhs;
+    e.rhs = h.rhs;
+    throw e;
+  }
+}
+
+// ============================================================ aggregates
+
+// eval_aggregate_rule + fold_aggregate (crates/strata-eval/src/naive.rs:300-403).
+void Engine::eval_aggregate(const Rule& r) {
+  for (auto& b : r.body) {
+    if (b.rel < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, b.pred, "unknown predicate");
+    if (P_.preds[b.rel].stratum >= r.stratum)
+      throw mkerr(SK_ERR_AGGREGATE_IN_RECURSION, b.pred, "aggregate body predicate is not in a strictly lower stratum");
+  }
+  int agg_index = -1;
+  u32 op = 0, slot = 0;
+  for (size_t i = 0; i < r.head.args.size(); i++) {
+    const Term& t = r.head.args[i];
+    if (t.kind == SK_T_AGG) {
+      if (agg_index >= 0) throw mkerr(SK_ERR_MALFORMED_AGGREGATE_HEAD, "", "aggregate head must contain exactly one agg term");
+      agg_index = (int)i;
+      op = t.a;
+      slot = (u32)t.b;
+    }
+  }
+  if (agg_index < 0) throw mkerr(SK_ERR_MALFORMED_AGGREGATE_HEAD, "", "aggregate head must contain exactly one agg term");
+  if (slot >= r.var_count) throw mkerr(SK_ERR_UNSUPPORTED, "", "aggregate slot out of range");
+  int ri = (int)(&r - &P_.rules[0]);
+  RulePlan pl = plan_rule(ri, -1);
+  Exec x = ctx_->exec();
+  const int bits = D_.bits();  // group keys are packed with the dictionary width
+
+  // run the body plan to a table
+  std::map<int, std::vector<Chunk>> dummy;
+  // Execute steps (no head projection): reuse exec_plan's machinery by
+  // walking the steps here.
+  TableOwner cur;
+  cur.pool = x.pool;
+  if (pl.unit_source) {
+    cur.t.ncols = 0;
+    table_alloc(&cur, &cur.t, 1);
+  }
+  for (size_t s = 0; s < pl.steps.size(); s++) {
+    const StepPlan& st = pl.steps[s];
+    bool reached = (s == 0 && !pl.unit_source) || cur.t.n > 0;
+    if (st.err) {
+      if (!reached) {
+        cur.t.n = 0;
+        break;
+      }
+      EvalErr e = mkerr(st.err, r.body[st.lit].pred);
+      e.slot = st.err_slot;
+      throw e;
+    }
+    TableOwner nxt;
+    nxt.pool = x.pool;
+    nxt.t.ncols = (int)st.out_slots.size();
+    if (st.kind == 0) {
+      const RelData& src = R_[st.rel].full;
+      ScanDesc sd;
+      sd.arity = R_[st.rel].arity;
+      sd.bits = R_[st.rel].bits;
+      bool never = st.never;
+      for (size_t q = 0; q < st.scan_const_kind.size(); q++) {
+        sd.const_col[sd.nconst] = st.scan_const_kind[q].first;
+        sd.const_val[sd.nconst] = code_or_never(st.scan_const_kind[q].second, st.scan_const_val[q], &never);
+        sd.nconst++;
+      }
+      for (auto& e : st.scan_eq) {
+        sd.eq_a[sd.neq] = e.first;
+        sd.eq_b[sd.neq] = e.second;
+        sd.neq++;
+      }
+      for (int c = 0; c < nxt.t.ncols; c++)
+        for (auto& nv : st.newvars)
+          if (nv.second == st.out_slots[c]) sd.out_col[c] = nv.first;
+      if (never || src.n == 0) {
+        table_alloc(&nxt, &nxt.t, 0);
+      } else if (nxt.t.ncols == 0 && sd.nconst == 0 && sd.neq == 0) {
+        // no live columns and no filter: the table is just a row count
+        table_alloc(&nxt, &nxt.t, 0);
+        nxt.t.n = src.n;
+      } else {
+        with_key(R_[st.rel].wide, [&](auto* tag) {
+          using K = std::remove_pointer_t<decltype(tag)>;
+          scan_rel<K>(x, (const K*)src.k.ptr(), nullptr, nullptr, src.n, sd, &nxt.t, table_alloc, &nxt);
+        });
+      }
+    } else if (st.kind == 1) {
+      bool never = st.never;
+      ProbeDesc pd;
+      pd.arity = R_[st.rel].arity;
+      pd.bits = R_[st.rel].bits;
+      pd.nprobe = st.nprobe;
+      for (int j = 0; j < st.nprobe; j++) {
+        if (st.probe_slot[j] >= 0) pd.psrc[j] = col_index(st.in_slots, st.probe_slot[j]);
+        else {
+          pd.psrc[j] = -1;
+          pd.pconst[j] = code_or_never(st.probe_kind[j], st.probe_val[j], &never);
+        }
+      }
+      for (auto& f : st.filt) {
+        pd.fa[pd.nfilt] = f.first;
+        pd.fb[pd.nfilt] = f.second;
+        pd.nfilt++;
+      }
+      OutDesc o;
+      o.ncols = nxt.t.ncols;
+      o.table = &nxt.t;
+      for (int c = 0; c < o.ncols; c++) {
+        int sl = st.out_slots[c];
+        int ci = col_index(st.in_slots, sl);
+        if (ci >= 0) {
+          o.kind[c] = 0;
+          o.idx[c] = ci;
+        } else {
+          o.kind[c] = 1;
+          for (auto& nv : st.newvars)
+            if (nv.second == sl) o.idx[c] = nv.first;
+        }
+      }
+      const RelData& src = R_[st.rel].full;
+      if (never || src.n == 0 || cur.t.n == 0) {
+        table_alloc(&nxt, &nxt.t, 0);
+      } else {
+        bool want_csr = st.nprobe == 1 && pd.nfilt == 0;
+        Index& ix = get_index(st.rel, false, st.perm, want_csr, false);
+        bool identity = true;
+        for (size_t q = 0; q < st.perm.size(); q++)
+          if (st.perm[q] != (int)q) identity = false;
+        const RelData& d = identity ? src : ix.data;
+        pd.sn = d.n;
+        with_key(R_[st.rel].wide, [&](auto* tag) {
+          using K = std::remove_pointer_t<decltype(tag)>;
+          join<K>(x, cur.t, pd, (const K*)d.k.ptr(), nullptr, nullptr, want_csr ? ix.csr.get() : nullptr, &o, false,
+                  false, false, table_alloc, &nxt, head_alloc, nullptr);
+        });
+      }
+    } else {
+      const RelData& tgt = R_[st.rel].full;
+      AntiDesc ad;
+      ad.arity = R_[st.rel].arity;
+      ad.bits = R_[st.rel].bits;
+      bool never_present = st.never;
+      for (int c = 0; c < ad.arity && !st.never; c++) {
+        if (st.anti_slot[c] >= 0) ad.src[c] = col_index(st.in_slots, st.anti_slot[c]);
+        else {
+          ad.src[c] = -1;
+          ad.cnst[c] = code_or_never(st.anti_kind[c], st.anti_val[c], &never_present);
+        }
+      }
+      ad.never_present = never_present || tgt.n == 0;
+      for (int c = 0; c < nxt.t.ncols; c++) nxt.t.srccol[c] = col_index(st.in_slots, st.out_slots[c]);
+      with_key(R_[st.rel].wide, [&](auto* tag) {
+        using K = std::remove_pointer_t<decltype(tag)>;
+        anti_join<K>(x, cur.t, ad, (const K*)tgt.k.ptr(), tgt.n, &nxt.t, table_alloc, &nxt);
+      });
+    }
+    u64 keep_n = nxt.t.n;
+    cur = std::move(nxt);
+    for (int c = 0; c < cur.t.ncols; c++) cur.t.col[c] = cur.cols[c].get();
+    cur.t.w = nullptr;
+    cur.t.inf = nullptr;
+    cur.t.n = keep_n;
+  }
+  u64 nsol = cur.t.n;
+  if (nsol == 0) return;
+  // unbound group key / aggregand (naive.rs:350-362)
+  std::set<int> boundset(pl.steps.empty() ? std::set<int>() : std::set<int>());
+  {
+    std::set<u32> b;
+    for (auto& a : r.body)
+      if (!a.neg)
+        for (auto& t : a.args)
+          if (t.kind == SK_T_VAR) b.insert(t.a);
+    for (size_t i = 0; i < r.head.args.size(); i++) {
+      if ((int)i == agg_index) continue;
+      const Term& t = r.head.args[i];
+      if (t.kind == SK_T_VAR && !b.count(t.a)) {
+        EvalErr e = mkerr(SK_ERR_UNSAFE_HEAD_VAR, r.head.pred, "unbound head variable");
+        e.slot = t.a;
+        throw e;
+      }
+    }
+    if (!b.count(slot)) {
+      EvalErr e = mkerr(SK_ERR_UNSAFE_HEAD_VAR, r.head.pred, "unbound head variable");
+      e.slot = slot;
+      throw e;
+    }
+  }
+  if (op == SK_AGG_PROB_OR) throw mkerr(SK_ERR_UNSUPPORTED, "", "prob-or aggregate");
+  // group keys and values
+  std::vector<std::vector<u32>> gkeys;  // per group: key codes
+  std::vector<i64> gvals;
+  std::vector<int> keycols;
+  AggDesc ad;
+  ad.bits = bits;
+  ad.op = (int)op;
+  ad.first_int = (u32)D_.sym_span;
+  ad.n_ints = (u32)D_.ints.size();
+  std::vector<int> last_slots = pl.steps.empty() ? std::vector<int>() : pl.steps.back().out_slots;
+  for (size_t i = 0; i < r.head.args.size(); i++) {
+    if ((int)i == agg_index) continue;
+    const Term& t = r.head.args[i];
+    if (t.kind == SK_T_VAR) {
+      ad.key_kind[ad.nkey] = 0;
+      ad.key_idx[ad.nkey] = col_index(last_slots, (int)t.a);
+    } else {
+      bool nv = false;
+      ad.key_kind[ad.nkey] = 2;
+      ad.key_cnst[ad.nkey] = code_or_never(t.kind, t.kind == SK_T_CONST ? (i64)t.a : t.b, &nv);
+    }
+    ad.nkey++;
+  }
+  ad.val_col = op == SK_AGG_COUNT ? -1 : col_index(last_slots, (int)slot);
+  int kbits = ad.nkey * bits;
+  bool kwide = kbits > 64;
+  if (ad.nkey == 0 && op == SK_AGG_COUNT) {
+    gkeys.push_back({});
+    gvals.push_back((i64)nsol);
+  } else {
+    bool bad = false;
+    auto run = [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      std::vector<K> ks;
+      std::vector<i64> vs;
+      aggregate<K>(x, cur.t, ad, int_vals_.get(), kbits, &ks, &vs, &bad);
+      for (size_t g = 0; g < ks.size(); g++) {
+        std::vector<u32> kc(ad.nkey);
+        K kk = ks[g];
+        for (int c = ad.nkey - 1; c >= 0; c--) {
+          kc[c] = (u32)(kk & (((K)1 << bits) - 1));
+          kk >>= bits;
+        }
+        gkeys.push_back(kc);
+        gvals.push_back(vs[g]);
+      }
+    };
+    if (kwide) run((u128*)nullptr);
+    else run((u64*)nullptr);
+    if (bad && op != SK_AGG_COUNT) throw mkerr(SK_ERR_NON_INTEGER_AGGREGAND, "", "min/max/sum require an integer aggregand");
+  }
+  // new Int values enter the dictionary (order-preserving remap)
+  Dict nd = D_;
+  std::set<i64> all(nd.ints.begin(), nd.ints.end());
+  size_t before = all.size();
+  for (i64 v : gvals) all.insert(v);
+  std::vector<u32> old_to_new;
+  if (all.size() != before) {
+    nd.ints.assign(all.begin(), all.end());
+    Dict od = D_;
+    grow_dict(nd, r.stratum);
+    for (auto& k : gkeys)
+      for (auto& c : k)
+        if (c >= od.sym_span) {
+          u32 cc = 0;
+          nd.int_code(od.ints[c - od.sym_span], &cc);
+          c = cc;
+        }
+  }
+  // insert the group tuples (Db::insert with Unit)
+  int hr = r.head.rel;
+  if (hr < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, r.head.pred, "insert into unknown relation");
+  DRel& H = R_[hr];
+  const int nb = H.bits;
+  std::vector<u32> hc(gkeys.size() * H.arity);
+  for (size_t g = 0; g < gkeys.size(); g++) {
+    int ki = 0;
+    for (int c = 0; c < H.arity; c++) {
+      if (c == agg_index) {
+        u32 cc = 0;
+        D_.int_code(gvals[g], &cc);
+        hc[g * H.arity + c] = cc;
+      } else {
+        hc[g * H.arity + c] = gkeys[g][ki++];
+      }
+    }
+  }
+  u64 ng = gkeys.size();
+  std::vector<DBuf<u32>> dc(H.arity);
+  Cols cols{};
+  for (int c = 0; c < H.arity; c++) {
+    std::vector<u32> colv(ng);
+    for (u64 g = 0; g < ng; g++) colv[g] = hc[g * H.arity + c];
+    dc[c].reset(x.pool, ng ? ng : 1);
+    SK_HIP(hipMemcpyAsync(dc[c].get(), colv.data(), ng * 4, hipMemcpyHostToDevice, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    cols.c[c] = dc[c].get();
+  }
+  Chunk chk;
+  chk.k.ensure(x.pool, ng, H.wide);
+  chk.n = ng;
+  with_key(H.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    pack_cols<K>(x, cols, H.arity, nb, ng, (K*)chk.k.ptr());
+  });
+  std::vector<Chunk> v;
+  v.push_back(std::move(chk));
+  finalize(hr, v);
+  H.delta.release();
+  H.delta.n = 0;
+}
+
+// ================================================================ strata
+
+void Engine::run_stratum(u32 k) {
+  // aggregate rules first, once (seminaive.rs:79-84)
+  in_idb_.assign(R_.size(), false);
+  for (size_t i = 0; i < P_.rules.size(); i++)
+    if (P_.rules[i].stratum == k && P_.rules[i].agg) eval_aggregate(P_.rules[i]);
+  for (auto& r : P_.rules)
+    if (r.stratum == k && !r.agg && r.head.rel >= 0) in_idb_[r.head.rel] = true;
+  bool is_trop = false;
+  for (auto& r : P_.rules)
+    if (r.stratum == k && r.head.rel >= 0 && P_.preds[r.head.rel].sem == SK_TROP) is_trop = true;
+  for (size_t i = 0; i < R_.size(); i++)
+    if (in_idb_[i]) {
+      R_[i].delta.release();
+      R_[i].dversion++;
+    }
+  const u64 cap = universe_ + 2;
+  u64 iters = 0;
+  std::vector<u8> active_prev;  // batched mode: program had a non-empty delta after the last round
+  std::map<std::pair<int, int>, RulePlan> plans;
+  auto plan_of = [&](int ri, int dpos) -> const RulePlan& {
+    auto key = std::make_pair(ri, dpos);
+    auto it = plans.find(key);
+    if (it == plans.end()) it = plans.emplace(key, plan_rule(ri, dpos)).first;
+    return it->second;
+  };
+  bool bootstrap = true;
+  for (;;) {
+    std::map<int, std::vector<Chunk>> chunks;
+    for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+      const Rule& r = P_.rules[ri];
+      if (r.stratum != k || r.agg) continue;
+      if (r.head.rel < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, r.head.pred, "unknown predicate");
+      if (bootstrap) {
+        exec_plan(plan_of((int)ri, -1), chunks);
+        continue;
+      }
+      for (size_t j = 0; j < r.body.size(); j++) {
+        const Atom& a = r.body[j];
+        if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
+        if (R_[a.rel].delta.n == 0) continue;
+        exec_plan(plan_of((int)ri, (int)j), chunks);
+      }
+    }
+    if (is_trop && !nparts_) check_device_error();
+    bool any = false;
+    for (size_t i = 0; i < R_.size(); i++) {
+      if (!in_idb_[i]) continue;
+      auto it = chunks.find((int)i);
+      if (it == chunks.end()) {
+        R_[i].delta.release();
+        R_[i].dversion++;
+      } else if (finalize((int)i, it->second) > 0) {
+        any = true;
+      }
+    }
+    if (nparts_) {
+      part_round_bookkeeping(k, is_trop, active_prev, !bootstrap);
+      any = false;
+      for (size_t i = 0; i < R_.size(); i++)
+        if (in_idb_[i] && R_[i].delta.n) any = true;
+      if (!bootstrap) iters++;
+    } else if (!bootstrap) {
+      iters++;
+      if (is_trop && iters > cap)
+        throw mkerr(SK_ERR_NEGATIVE_WEIGHT_CYCLE, "", "negative-weight cycle: tropical stratum diverges");
+    }
+    bootstrap = false;
+    if (!any) break;
+  }
+  iters_[k] = iters;
+  for (size_t i = 0; i < R_.size(); i++)
+    if (in_idb_[i]) {
+      R_[i].delta.release();
+      R_[i].dversion++;
+    }
+}
+
+void Engine::execute() {
+  if (!loaded) load();
+  iters_.assign(P_.num_strata, 0);
+  auto t0 = std::chrono::steady_clock::now();
+  for (u32 k = 0; k < P_.num_strata; k++) run_stratum(k);
+  SK_HIP(hipStreamSynchronize(ctx_->stream));
+  exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ========================================================= batched mode
+
+void Engine::set_partitions(u32 np) {
+  nparts_ = np;
+  part_universe_.assign(np, 0);
+  part_err_.assign(np, EvalErr());
+  part_iters_.assign(np, std::vector<u64>(P_.num_strata, 0));
+  part_dead_.assign(np, 0);
+  Exec x = ctx_->exec();
+  part_derr_.reset(x.pool, np ? np : 1);
+  part_dead_dev_.reset(x.pool, np ? np : 1);
+  memset_async(x, part_derr_.get(), 0, sizeof(DevErr) * (np ? np : 1));
+  memset_async(x, part_dead_dev_.get(), 0, np ? np : 1);
+}
+
+// Rows per program id of a relation state (column 0 is the program id).
+std::vector<u64> Engine::part_counts(int rel, bool delta) {
+  DRel& R = R_[rel];
+  const RelData& d = delta ? R.delta : R.full;
+  std::vector<u64> cnt(nparts_, 0);
+  if (d.n == 0) return cnt;
+  Exec x = ctx_->exec();
+  DBuf<u64> csr(x.pool, nparts_ + 1);
+  with_key(R.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    build_csr<K>(x, (const K*)d.k.ptr(), d.n, R.bits * (R.arity - 1), nparts_, csr.get());
+  });
+  std::vector<u64> h(nparts_ + 1);
+  SK_HIP(hipMemcpyAsync(h.data(), csr.get(), (nparts_ + 1) * 8, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  for (u32 p = 0; p < nparts_; p++) cnt[p] = h[p + 1] - h[p];
+  return cnt;
+}
+
+void Engine::drop_dead_from_deltas() {
+  Exec x = ctx_->exec();
+  SK_HIP(hipMemcpyAsync(part_dead_dev_.get(), part_dead_.data(), nparts_, hipMemcpyHostToDevice, x.stream));
+  for (size_t i = 0; i < R_.size(); i++) {
+    if (!in_idb_[i] || R_[i].delta.n == 0) continue;
+    DRel& R = R_[i];
+    bool trop = R.sem == SK_TROP, hf = trop && has_inf_;
+    RelData nd;
+    nd.k.ensure(x.pool, R.delta.n, R.wide);
+    if (trop) nd.w.reset(x.pool, R.delta.n);
+    if (hf) nd.inf.reset(x.pool, R.delta.n);
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      nd.n = drop_partitions<K>(x, (const K*)R.delta.k.ptr(), trop ? R.delta.w.get() : nullptr,
+                                hf ? R.delta.inf.get() : nullptr, R.delta.n, R.arity, R.bits, part_dead_dev_.get(),
+                                (K*)nd.k.ptr(), trop ? nd.w.get() : nullptr, hf ? nd.inf.get() : nullptr);
+    });
+    R.delta = std::move(nd);
+    R.dversion++;
+  }
+}
+
+// After a round in batched mode: charge overflows to their programs, count
+// each program's delta round and apply its own cap (crates/strata-eval/src/seminaive.rs:98-104).
+void Engine::part_round_bookkeeping(u32 k, bool is_trop, std::vector<u8>& active_prev, bool count_round) {
+  Exec x = ctx_->exec();
+  bool newly_dead = false;
+  if (is_trop) {
+    std::vector<DevErr> h(nparts_);
+    SK_HIP(hipMemcpyAsync(h.data(), part_derr_.get(), nparts_ * sizeof(DevErr), hipMemcpyDeviceToHost, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    bool any = false;
+    for (u32 p = 0; p < nparts_; p++)
+      if (h[p].code) {
+        any = true;
+        if (!part_dead_[p]) {
+          part_dead_[p] = 1;
+          newly_dead = true;
+          EvalErr e = mkerr(SK_ERR_OVERFLOW, "", "tropical weight overflow");
+          e.lhs = h[p].lhs;
+          e.rhs = h[p].rhs;
+          part_err_[p] = e;
+        }
+      }
+    if (any) memset_async(x, part_derr_.get(), 0, sizeof(DevErr) * nparts_);
+  }
+  if (count_round) {
+    for (u32 p = 0; p < nparts_; p++) {
+      if (part_dead_[p] || !active_prev[p]) continue;
+      part_iters_[p][k]++;
+      if (is_trop && part_iters_[p][k] > part_universe_[p] + 2) {
+        part_dead_[p] = 1;
+        newly_dead = true;
+        part_err_[p] = mkerr(SK_ERR_NEGATIVE_WEIGHT_CYCLE, "", "negative-weight cycle: tropical stratum diverges");
+      }
+    }
+  }
+  if (newly_dead) drop_dead_from_deltas();
+  active_prev.assign(nparts_, 0);
+  for (size_t i = 0; i < R_.size(); i++) {
+    if (!in_idb_[i] || R_[i].delta.n == 0) continue;
+    std::vector<u64> c = part_counts((int)i, true);
+    for (u32 p = 0; p < nparts_; p++)
+      if (c[p]) active_prev[p] = 1;
+  }
+}
+
+u64 Engine::partition_size(const std::string& pred, u32 p) {
+  int r = P_.rel_of(pred);
+  if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+  return part_counts(r, false)[p];
+}
+
+void Engine::partition_read_sym(const std::string& pred, u32 p, u32* const* cols, i64* w, u8* inf) {
+  int r = P_.rel_of(pred);
+  if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+  DRel& R = R_[r];
+  std::vector<u64> c = part_counts(r, false);
+  u64 lo = 0;
+  for (u32 q = 0; q < p; q++) lo += c[q];
+  u64 n = c[p];
+  if (!n) return;
+  Exec x = ctx_->exec();
+  std::vector<DBuf<u32>> dc(R.arity);
+  std::vector<u32*> cp(R.arity);
+  for (int j = 0; j < R.arity; j++) {
+    dc[j].reset(x.pool, n);
+    cp[j] = dc[j].get();
+  }
+  with_key(R.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    unpack_cols<K>(x, (const K*)R.full.k.ptr() + lo, n, R.arity, R.bits, cp.data());
+  });
+  for (int j = 1; j < R.arity; j++)
+    if (cols[j - 1]) SK_HIP(hipMemcpyAsync(cols[j - 1], cp[j], n * 4, hipMemcpyDeviceToHost, x.stream));
+  bool trop = R.sem == SK_TROP;
+  if (w && trop) SK_HIP(hipMemcpyAsync(w, R.full.w.get() + lo, n * 8, hipMemcpyDeviceToHost, x.stream));
+  if (inf) {
+    if (trop && has_inf_) SK_HIP(hipMemcpyAsync(inf, R.full.inf.get() + lo, n, hipMemcpyDeviceToHost, x.stream));
+    else std::memset(inf, 0, n);
+  }
+  SK_HIP(hipStreamSynchronize(x.stream));
+}
+
+void Engine::attach(const std::string& pred, u64 n, void* keys, bool wide, int bits, i64* w, u8* inf, u64* csr,
+                    u64 max_sym, u64 universe_syms) {
+  (void)universe_syms;
+  int r = P_.rel_of(pred);
+  if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+  if (attached_.empty()) attached_.assign(R_.size(), false);
+  attached_[r] = true;
+  DRel& R = R_[r];
+  R.bits = bits;
+  R.wide = wide;
+  R.full.n = n;
+  R.full.k.wide = wide;
+  // non-owning views: a DBuf without a pool never frees its pointer
+  if (wide) {
+    R.full.k.b.p = (u128*)keys;
+    R.full.k.b.n = n;
+  } else {
+    R.full.k.a.p = (u64*)keys;
+    R.full.k.a.n = n;
+  }
+  R.full.w.p = w;
+  R.full.w.n = w ? n : 0;
+  R.full.inf.p = inf;
+  R.full.inf.n = inf ? n : 0;
+  attached_max_sym_ = std::max(attached_max_sym_, max_sym);
+  if (csr) {
+    Index& ix = R.idx["f01"];
+    ix.perm = {0, 1};
+    ix.version = R.version;
+    ix.csr.p = csr;
+    ix.csr.n = max_sym + 2;
+    ix.csr_version = R.version;
+  }
+}
+
+// ================================================================ results
+
+int64_t Engine::size(const std::string& pred) {
+  int r = P_.rel_of(pred);
+  return r < 0 ? -1 : (int64_t)R_[r].full.n;
+}
+
+void Engine::read(const std::string& pred, sk_val* rows, i64* w, u8* inf, u8* is_w) {
+  int r = P_.rel_of(pred);
+  if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+  DRel& R = R_[r];
+  Exec x = ctx_->exec();
+  u64 n = R.full.n;
+  if (n == 0) return;
+  const int bits = R.bits;
+  if (rows && R.arity) {
+    std::vector<DBuf<u32>> dc(R.arity);
+    std::vector<u32*> cp(R.arity);
+    for (int j = 0; j < R.arity; j++) {
+      dc[j].reset(x.pool, n);
+      cp[j] = dc[j].get();
+    }
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      unpack_cols<K>(x, (const K*)R.full.k.ptr(), n, R.arity, bits, cp.data());
+    });
+    std::vector<u32> hcol(n);
+    for (int j = 0; j < R.arity; j++) {
+      SK_HIP(hipMemcpyAsync(hcol.data(), cp[j], n * 4, hipMemcpyDeviceToHost, x.stream));
+      SK_HIP(hipStreamSynchronize(x.stream));
+      for (u64 i = 0; i < n; i++) {
+        u32 kind;
+        i64 v;
+        D_.decode(hcol[i], &kind, &v);
+        rows[i * R.arity + j].kind = kind;
+        rows[i * R.arity + j].pad = 0;
+        rows[i * R.arity + j].v = v;
+      }
+    }
+  }
+  bool trop = R.sem == SK_TROP;
+  if (w) {
+    if (trop) SK_HIP(hipMemcpyAsync(w, R.full.w.get(), n * 8, hipMemcpyDeviceToHost, x.stream));
+    else std::memset(w, 0, n * 8);
+  }
+  if (inf) {
+    if (trop && has_inf_) SK_HIP(hipMemcpyAsync(inf, R.full.inf.get(), n, hipMemcpyDeviceToHost, x.stream));
+    else std::memset(inf, 0, n);
+  }
+  if (is_w) std::memset(is_w, trop ? 1 : 0, n);
+  SK_HIP(hipStreamSynchronize(x.stream));
+}
+
+void Engine::read_sym(const std::string& pred, u32* const* cols, i64* w, u8* inf) {
+  int r = P_.rel_of(pred);
+  if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+  DRel& R = R_[r];
+  Exec x = ctx_->exec();
+  u64 n = R.full.n;
+  if (n == 0) return;
+  const int bits = R.bits;
+  if (cols && R.arity) {
+    std::vector<DBuf<u32>> dc(R.arity);
+    std::vector<u32*> cp(R.arity);
+    for (int j = 0; j < R.arity; j++) {
+      dc[j].reset(x.pool, n);
+      cp[j] = dc[j].get();
+    }
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      unpack_cols<K>(x, (const K*)R.full.k.ptr(), n, R.arity, bits, cp.data());
+    });
+    for (int j = 0; j < R.arity; j++)
+      if (cols[j]) SK_HIP(hipMemcpyAsync(cols[j], cp[j], n * 4, hipMemcpyDeviceToHost, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    if (!D_.ints.empty())
+      for (int j = 0; j < R.arity; j++)
+        if (cols[j])
+          for (u64 i = 0; i < n; i++)
+            if (cols[j][i] >= D_.sym_span) throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "relation holds Int values");
+  }
+  bool trop = R.sem == SK_TROP;
+  if (w) {
+    if (trop) SK_HIP(hipMemcpyAsync(w, R.full.w.get(), n * 8, hipMemcpyDeviceToHost, x.stream));
+    else std::memset(w, 0, n * 8);
+  }
+  if (inf) {
+    if (trop && has_inf_) SK_HIP(hipMemcpyAsync(inf, R.full.inf.get(), n, hipMemcpyDeviceToHost, x.stream));
+    else std::memset(inf, 0, n);
+  }
+  SK_HIP(hipStreamSynchronize(x.stream));
+}
+
+}  // namespace sk

@@ -1,0 +1,294 @@
+// sk_engine.h — host-side engine: Core-IR model, value dictionary, device
+// relations, rule plans and the semi-naive driver.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sk_dev.h"
+#include "sk_join.h"
+#include "strata_k.h"
+
+namespace sk {
+
+// Evaluation error: aborts the run with an sk_status (EvalError mirror).
+struct EvalErr {
+  int code = SK_OK;
+  std::string pred;
+  u32 slot = 0;
+  i64 lhs = 0, rhs = 0;
+  std::string msg;
+};
+
+// ------------------------------------------------------------ Core-IR
+
+struct Term {
+  u32 kind = SK_T_VAR;
+  u32 a = 0;
+  i64 b = 0;
+};
+
+struct Atom {
+  std::string pred;
+  bool neg = false;
+  std::vector<Term> args;
+  int rel = -1;  // relation index, -1 = undeclared
+};
+
+struct Rule {
+  Atom head;
+  std::vector<Atom> body;
+  u32 stratum = 0, var_count = 0;
+  bool agg = false;
+};
+
+struct PredInfo {
+  std::string name;
+  u32 arity = 0, sem = SK_BOOL, stratum = 0;
+};
+
+struct Program {
+  std::vector<PredInfo> preds;  // one per relation (last declaration of a name wins)
+  std::map<std::string, int> by_name;
+  std::vector<Rule> rules;
+  u32 num_strata = 0;
+  static Program from_c(const sk_program* p);
+  int rel_of(const std::string& name) const {
+    auto it = by_name.find(name);
+    return it == by_name.end() ? -1 : it->second;
+  }
+};
+
+// --------------------------------------------------------- dictionary
+
+// Order-preserving value codes: Sym(s) -> s, Int(v) -> sym_span + rank(v).
+// GroundVal orders Sym before Int (crates/strata-ir/src/value.rs:15-24), so
+// code order equals value order and packed keys sort in BTreeMap order.
+struct Dict {
+  u64 sym_span = 1;
+  std::vector<i64> ints;
+  u64 nvals() const { return sym_span + ints.size(); }
+  int bits() const {
+    u64 n = nvals();
+    int b = 1;
+    while (b < 64 && (1ull << b) < n) b++;
+    return b;
+  }
+  bool int_code(i64 v, u32* c) const;
+  u32 encode(u32 kind, i64 v, bool* ok) const;
+  void decode(u32 code, u32* kind, i64* v) const;
+};
+
+// ------------------------------------------------------ device relations
+
+struct Keys {
+  bool wide = false;
+  DBuf<u64> a;
+  DBuf<u128> b;
+  void ensure(Pool* p, u64 n, bool w) {
+    if (w != wide) {
+      a.release();
+      b.release();
+    }
+    wide = w;
+    if (w) b.ensure(p, n ? n : 1);
+    else a.ensure(p, n ? n : 1);
+  }
+  void* ptr() const { return wide ? (void*)b.get() : (void*)a.get(); }
+  void release() {
+    a.release();
+    b.release();
+  }
+};
+
+struct RelData {
+  u64 n = 0;
+  Keys k;
+  DBuf<i64> w;
+  DBuf<u8> inf;
+  void release() {
+    n = 0;
+    k.release();
+    w.release();
+    inf.release();
+  }
+};
+
+// A permuted (and CSR-indexed) copy of a relation state.
+struct Index {
+  u64 version = ~0ull;
+  std::vector<int> perm;
+  RelData data;
+  DBuf<u64> csr;
+  u64 csr_version = ~0ull;
+  DBuf<u32> adj;  // second index column, for leapfrog lists
+  u64 adj_version = ~0ull;
+};
+
+struct DRel {
+  int arity = 0;
+  u32 sem = SK_BOOL;
+  int bits = 1;       // bits per packed column (codes < 2^bits)
+  bool wide = false;  // arity * bits > 64: 128-bit keys
+  RelData full, delta;
+  u64 version = 1, dversion = 1;
+  std::map<std::string, Index> idx;
+  // staged EDB facts (host, before sk_run_load)
+  std::vector<u32> st_kind;  // row-major kinds (empty when all Sym)
+  std::vector<i64> st_val;   // row-major values
+  std::vector<std::vector<u32>> st_sym;  // all-Sym column stage
+  std::vector<i64> st_w;
+  std::vector<u8> st_inf;
+  bool st_weighted = false;
+  u64 st_rows = 0;
+};
+
+// ------------------------------------------------------------- plans
+
+struct StepPlan {
+  int kind = 0;  // 0 scan, 1 join, 2 anti
+  int lit = -1;
+  int rel = -1;
+  bool delta = false;
+  bool never = false;    // the literal cannot match (arity / value mismatch)
+  std::vector<int> perm; // join: index column order (probe columns first)
+  int nprobe = 0;
+  std::vector<int> probe_slot;     // per probe column: var slot or -1 (constant)
+  std::vector<u32> probe_kind;     // term kind of the probe constant
+  std::vector<i64> probe_val;      // constant value
+  std::vector<std::pair<int, int>> filt;  // index-order column pairs that must be equal
+  std::vector<std::pair<int, int>> newvars;  // (index-order column, slot)
+  // scan
+  std::vector<std::pair<int, u32>> scan_const_kind;  // (column, term kind)
+  std::vector<i64> scan_const_val;
+  std::vector<std::pair<int, int>> scan_eq;
+  // anti
+  std::vector<int> anti_slot;  // per column: slot or -1
+  std::vector<u32> anti_kind;
+  std::vector<i64> anti_val;
+  // static error raised when the step is reached with a non-empty input
+  int err = 0;
+  u32 err_slot = 0;
+  std::vector<int> in_slots, out_slots;
+};
+
+struct RulePlan {
+  int rule = -1;
+  int dpos = -1;
+  std::vector<StepPlan> steps;
+  bool unit_source = false;  // body without positive literals
+  // leapfrog triangle
+  bool tri = false;
+  int tri_lit[3] = {-1, -1, -1};  // driver, lit indexed on v, lit indexed on u
+  int tri_slot[3] = {0, 0, 0};    // slots of u, v, w
+  int tri_col_v = 0, tri_col_w2 = 1, tri_col_u = 0, tri_col_w3 = 1;
+  bool sorted_out = false;
+  int head_err = 0;  // UnsafeHeadVar / Unsupported after a non-empty solve
+  u32 head_err_slot = 0;
+  bool trop = false;
+};
+
+// ------------------------------------------------------------ run state
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Pool> pool;
+  Timer timer;
+  DevErr* derr = nullptr;
+  EvalErr last;
+  Exec exec() { return Exec{pool.get(), stream, &timer, derr}; }
+};
+
+struct Chunk {
+  u64 n = 0;
+  Keys k;
+  DBuf<i64> w;
+  DBuf<u8> inf;
+  bool sorted_unique = false;
+};
+
+class Engine {
+ public:
+  Engine(Ctx* c, Program p) : ctx_(c), P_(std::move(p)) {}
+  void validate();           // static checks done at sk_prepare
+  void stage_rows(const std::string& pred, u64 n, const sk_val* rows, const i64* w, const u8* inf);
+  void stage_sym(const std::string& pred, u64 n, u32 arity, const u32* const* cols, const i64* w, const u8* inf);
+  void load();
+  void execute();
+  // results
+  int64_t size(const std::string& pred);
+  void read(const std::string& pred, sk_val* rows, i64* w, u8* inf, u8* is_w);
+  void read_sym(const std::string& pred, u32* const* cols, i64* w, u8* inf);
+  u64 iterations(u32 k) const { return k < iters_.size() ? iters_[k] : 0; }
+  u64 universe() const { return universe_; }
+  double exec_ms = 0;
+  EvalErr pending;  // staged error (reported by load / execute)
+  bool loaded = false;
+  const Program& program() const { return P_; }
+  DRel& rel(int i) { return R_[i]; }
+  const Dict& dict() const { return D_; }
+  bool has_inf() const { return has_inf_; }
+
+  // Batched mode: column 0 of every relation is a program id in [0, np).
+  // Each program keeps its own round counts, Bellman-Ford cap and errors.
+  void set_partitions(u32 np);
+  void set_partition_universe(u32 p, u64 u) { part_universe_[p] = u; }
+  EvalErr partition_error(u32 p) const { return part_err_[p]; }
+  u64 partition_iterations(u32 p, u32 k) const { return k < part_iters_[p].size() ? part_iters_[p][k] : 0; }
+  u64 partition_size(const std::string& pred, u32 p);
+  void partition_read_sym(const std::string& pred, u32 p, u32* const* cols, i64* w, u8* inf);
+  // Attach a device-resident relation owned by the caller (not freed here).
+  void attach(const std::string& pred, u64 n, void* keys, bool wide, int bits, i64* w, u8* inf, u64* csr,
+              u64 max_sym, u64 universe_syms);
+  // Use a given universe instead of scanning the relations at load.
+  void set_universe(u64 u) { universe_override_ = u; }
+
+ private:
+  u32 nparts_ = 0;
+  std::vector<u64> part_universe_;
+  std::vector<EvalErr> part_err_;
+  std::vector<std::vector<u64>> part_iters_;
+  std::vector<u8> part_dead_;
+  DBuf<DevErr> part_derr_;
+  DBuf<u8> part_dead_dev_;
+  u64 attached_max_sym_ = 0;
+  u64 universe_override_ = ~0ull;
+  std::vector<bool> attached_;
+  void part_round_bookkeeping(u32 k, bool is_trop, std::vector<u8>& active_prev, bool count_round);
+  std::vector<u64> part_counts(int rel, bool delta);
+  void drop_dead_from_deltas();
+
+  Ctx* ctx_;
+  Program P_;
+  Dict D_;
+  std::vector<DRel> R_;
+  std::vector<u64> iters_;
+  u64 universe_ = 0;
+  bool has_inf_ = false;
+  std::vector<bool> in_idb_;  // relation changes in the stratum being run
+  DBuf<i64> int_vals_;        // device copy of Dict::ints
+
+  void build_dict();
+  void upload_rel(int r);
+  void compute_universe();
+  void grow_dict(const Dict& nd, u32 stratum);
+  void run_stratum(u32 k);
+  void eval_aggregate(const Rule& r);
+  RulePlan plan_rule(int ri, int dpos);
+  void exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out);
+  u64 finalize(int rel, std::vector<Chunk>& chunks);
+  const RelData& source(int rel, bool delta) { return delta ? R_[rel].delta : R_[rel].full; }
+  Index& get_index(int rel, bool delta, const std::vector<int>& perm, bool want_csr, bool want_adj);
+  u32 code_or_never(u32 kind, i64 v, bool* never) const;
+  void check_device_error();
+};
+
+}  // namespace sk
+
+// The C ABI's context handle.
+struct sk_ctx {
+  sk::Ctx c;
+};

@@ -1,0 +1,132 @@
+// sk_join.h — descriptors and launch wrappers of the join-side kernels.
+#pragma once
+
+#include <vector>
+
+#include "sk_dev.h"
+
+namespace sk {
+
+// Binding table: one u32 column per live variable, plus the Trop
+// accumulator (w, inf) when the rule's head is Trop.
+struct Table {
+  u64 n = 0;
+  int ncols = 0;
+  u32* col[kMaxCols] = {};
+  i64* w = nullptr;
+  u8* inf = nullptr;
+  int srccol[kMaxCols] = {};  // compact_table: output column c copies input column srccol[c]
+  // batched runs: the column holding the program id and the per-program
+  // error records (an overflow is charged to that program only)
+  int part_col = -1;
+  DevErr* perr = nullptr;
+};
+
+using TableAlloc = void (*)(void* ctx, Table* t, u64 n);
+
+// Scan of a relation into a table.
+struct ScanDesc {
+  int arity = 0, bits = 0;
+  int nconst = 0;
+  int const_col[kMaxCols] = {};
+  u32 const_val[kMaxCols] = {};
+  int neq = 0;
+  int eq_a[kMaxCols] = {}, eq_b[kMaxCols] = {};
+  int out_col[kMaxCols] = {};  // table column c = relation column out_col[c]
+};
+
+// Probe of a relation index whose key order puts the bound columns first.
+struct ProbeDesc {
+  int arity = 0, bits = 0;
+  u64 sn = 0;
+  int nprobe = 0;
+  int psrc[kMaxCols] = {};  // prefix column j: table column psrc[j], or -1 = constant pconst[j]
+  u32 pconst[kMaxCols] = {};
+  int nfilt = 0;  // index columns that must be equal (repeated unbound variable)
+  int fa[kMaxCols] = {}, fb[kMaxCols] = {};
+};
+
+// Output of a join / projection: either the next table's columns or a
+// packed head key.  kind 0 = table column idx, 1 = probed relation column
+// idx (index order), 2 = constant cnst.
+struct OutDesc {
+  int ncols = 0;
+  int kind[kMaxCols] = {};
+  int idx[kMaxCols] = {};
+  u32 cnst[kMaxCols] = {};
+  u32* col[kMaxCols] = {};
+  void* keys = nullptr;
+  int bits = 0;
+  i64* w = nullptr;
+  u8* inf = nullptr;
+  Table* table = nullptr;  // host side: destination table of a non-final join
+};
+
+using HeadAlloc = void (*)(void* ctx, OutDesc* o, u64 n);
+
+struct AntiDesc {
+  int arity = 0, bits = 0;
+  int src[kMaxCols] = {};  // table column, or -1 = constant
+  u32 cnst[kMaxCols] = {};
+  int never_present = 0;   // arity mismatch: the negation always holds
+};
+
+struct AggDesc {
+  int nkey = 0;
+  int key_kind[kMaxCols] = {};  // 0 = table column, 2 = constant
+  int key_idx[kMaxCols] = {};
+  u32 key_cnst[kMaxCols] = {};
+  int bits = 0;
+  int op = 0;       // 0 count, 1 sum, 2 min, 3 max
+  int val_col = -1;
+  u32 first_int = 0, n_ints = 0;
+};
+
+struct TriDesc {
+  int bits = 0;
+  int u_col = 0;  // driver column holding u (the other holds v)
+};
+
+template <class K>
+u64 scan_rel(const Exec& x, const K* keys, const i64* w, const u8* f, u64 n, const ScanDesc& d, Table* out,
+             TableAlloc alloc, void* actx);
+
+template <class K>
+u64 join(const Exec& x, const Table& in, const ProbeDesc& p, const K* skeys, const i64* sw, const u8* sf,
+         const u64* csr, OutDesc* o, bool final_step, bool k128_out, bool trop, TableAlloc alloc, void* actx,
+         HeadAlloc halloc, void* hctx);
+
+template <class K>
+u64 anti_join(const Exec& x, const Table& in, const AntiDesc& a, const K* skeys, u64 sn, Table* out,
+              TableAlloc alloc, void* actx);
+
+void project(const Exec& x, const Table& in, OutDesc* o, bool k128_out, HeadAlloc halloc, void* hctx);
+
+template <class K>
+u64 aggregate(const Exec& x, const Table& in, const AggDesc& a, const i64* int_vals, int key_bits,
+              std::vector<K>* keys_out, std::vector<i64>* vals_out, bool* bad_aggregand);
+
+template <class K>
+u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u64* csr2, const u32* adj2,
+              const u64* csr3, const u32* adj3, OutDesc* o, bool k128_out, bool count_only, HeadAlloc halloc,
+              void* hctx, double* sum_list_len);
+
+template <class K>
+double tri_list_len(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u64* csr2, const u64* csr3);
+
+template <class K, class KO>
+void remap_keys(const Exec& x, const K* in, u64 n, int arity, int bits_in, int bits_out, const u32* map, KO* out);
+
+template <class K>
+void pack_cols(const Exec& x, const Cols& cols, int arity, int bits, u64 n, K* out);
+
+void mark_codes(const Exec& x, const u32* c, u64 n, u32 limit, u32* bitmap);
+
+// Keep rows whose column-0 program id is not flagged dead; in place
+// (stream-compaction into the given output buffers).  Returns the count.
+template <class K>
+u64 drop_partitions(const Exec& x, const K* keys, const i64* w, const u8* inf, u64 n, int arity, int bits,
+                    const u8* dead, K* ok, i64* ow, u8* of);
+u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords);
+
+}  // namespace sk

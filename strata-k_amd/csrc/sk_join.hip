@@ -1,0 +1,876 @@
+// sk_join.hip — join-side kernels of the semi-naive engine on gfx950.
+//
+// A rule body is evaluated as a chain of steps over a binding table (one u32
+// column per live variable, plus the Trop accumulator):
+//   scan    relation (full or delta) -> table, with constant / repeated-var filters
+//   join    table x relation index: per-row range count, device prefix sum,
+//           load-balanced emit (outputs split evenly over blocks; each output
+//           finds its row by binary search over the row offsets)
+//   anti    negated literal: drop rows whose ground atom is present
+//   project table -> packed head keys (+ weight)
+//   leapfrog  sorted-list intersection for triangle bodies
+// Trop ⊗ is checked i64 addition in body order with +inf absorbing; the
+// first overflow is recorded in the device error record (crates/strata-ir/src/trop.rs:56-62).
+#include "sk_join.h"
+
+namespace sk {
+
+__device__ __forceinline__ u32 lane_id_j() { return threadIdx.x & 63; }
+__device__ __forceinline__ u32 wave_id_j() { return threadIdx.x >> 6; }
+
+__device__ __forceinline__ u64 wave_incl_scan_j(u64 v) {
+  u32 l = lane_id_j();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    u64 y = __shfl_up(v, o, 64);
+    if (l >= (u32)o) v += y;
+  }
+  return v;
+}
+
+static unsigned gridj(u64 n, u64 per = kBlock) {
+  u64 g = (n + per - 1) / per;
+  if (g == 0) g = 1;
+  if (g > 0x7fffffffull) g = 0x7fffffffull;
+  return (unsigned)g;
+}
+
+// Checked Trop product a ⊗ b: +inf absorbs; overflow is recorded (first one
+// wins) and the result is meaningless (the round's result is discarded).
+__device__ __forceinline__ void trop_mul(i64 a, u8 ai, i64 b, u8 bi, i64* r, u8* ri, DevErr* err) {
+  if (ai | bi) {
+    *r = 0;
+    *ri = 1;
+    return;
+  }
+  i64 s;
+  if (__builtin_add_overflow(a, b, &s)) {
+    if (atomicCAS(&err->code, 0, 2) == 0) {
+      err->lhs = a;
+      err->rhs = b;
+    }
+    s = 0;
+  }
+  *r = s;
+  *ri = 0;
+}
+
+template <class K>
+__device__ __forceinline__ u32 col_of(K key, int arity, int bits, int j) {
+  K mask = (((K)1) << bits) - 1;
+  return (u32)((key >> (bits * (arity - 1 - j))) & mask);
+}
+
+// ==================================================================== scan
+
+// flags[i] = row i of the relation passes the atom's filters.
+template <class K>
+__global__ void k_scan_flags(const K* __restrict__ keys, u64 n, ScanDesc d, u8* __restrict__ flags) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K k = keys[i];
+  bool ok = true;
+  for (int q = 0; q < d.nconst; q++) ok &= col_of<K>(k, d.arity, d.bits, d.const_col[q]) == d.const_val[q];
+  for (int q = 0; q < d.neq; q++)
+    ok &= col_of<K>(k, d.arity, d.bits, d.eq_a[q]) == col_of<K>(k, d.arity, d.bits, d.eq_b[q]);
+  flags[i] = ok;
+}
+
+// Write table rows: pos == nullptr means every row passes (identity positions).
+template <class K>
+__global__ void k_scan_write(const K* __restrict__ keys, const i64* __restrict__ w, const u8* __restrict__ f, u64 n,
+                             ScanDesc d, const u8* __restrict__ flags, const u64* __restrict__ pos, Table out) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags && !flags[i]) return;
+  u64 o = pos ? pos[i] : i;
+  K k = keys[i];
+  for (int c = 0; c < out.ncols; c++) out.col[c][o] = col_of<K>(k, d.arity, d.bits, d.out_col[c]);
+  if (out.w) {
+    // acc = otimes_id ⊗ w = 0 + w (cannot overflow)
+    out.w[o] = w ? w[i] : 0;
+    if (out.inf) out.inf[o] = f ? f[i] : (u8)0;
+  }
+}
+
+template <class K>
+u64 scan_rel(const Exec& x, const K* keys, const i64* w, const u8* f, u64 n, const ScanDesc& d, Table* out,
+             TableAlloc alloc, void* actx) {
+  bool filtered = d.nconst > 0 || d.neq > 0;
+  if (!filtered) {
+    alloc(actx, out, n);
+    if (n) {
+      Scope sc(x.timer, "scan_write", KC_JOIN, (double)n * (sizeof(K) + 4.0 * out->ncols), n);
+      k_scan_write<K><<<gridj(n), kBlock, 0, x.stream>>>(keys, w, f, n, d, nullptr, nullptr, *out);
+      SK_HIP(hipGetLastError());
+    }
+    return n;
+  }
+  if (n == 0) {
+    alloc(actx, out, 0);
+    return 0;
+  }
+  DBuf<u8> flags(x.pool, n);
+  DBuf<u64> pos(x.pool, n);
+  DBuf<u64> tot(x.pool, 1);
+  {
+    Scope sc(x.timer, "scan_flags", KC_JOIN, (double)n * (sizeof(K) + 1.0), n);
+    k_scan_flags<K><<<gridj(n), kBlock, 0, x.stream>>>(keys, n, d, flags.get());
+  }
+  scan_u8(x, flags.get(), pos.get(), n, tot.get());
+  u64 m = read_u64(x, tot.get());
+  alloc(actx, out, m);
+  if (m) {
+    Scope sc(x.timer, "scan_write", KC_JOIN, (double)n * (sizeof(K) + 1.0) + m * 4.0 * out->ncols, n);
+    k_scan_write<K><<<gridj(n), kBlock, 0, x.stream>>>(keys, w, f, n, d, flags.get(), pos.get(), *out);
+    SK_HIP(hipGetLastError());
+  }
+  return m;
+}
+
+// ==================================================================== join
+
+// Probe prefix of index order for table row r, packed as the high bits of
+// the index key.
+template <class K>
+__device__ __forceinline__ K probe_prefix(const Table& in, const ProbeDesc& p, u64 r) {
+  K pre = 0;
+  for (int j = 0; j < p.nprobe; j++) {
+    u32 v = p.psrc[j] >= 0 ? in.col[p.psrc[j]][r] : p.pconst[j];
+    pre = (pre << p.bits) | (K)v;
+  }
+  return pre;
+}
+
+template <class K>
+__device__ __forceinline__ bool filt_ok(K key, const ProbeDesc& p) {
+  bool ok = true;
+  for (int q = 0; q < p.nfilt; q++)
+    ok &= col_of<K>(key, p.arity, p.bits, p.fa[q]) == col_of<K>(key, p.arity, p.bits, p.fb[q]);
+  return ok;
+}
+
+template <class K>
+__global__ void k_join_count(Table in, ProbeDesc p, const K* __restrict__ skeys, const u64* __restrict__ csr,
+                             u64* __restrict__ lo_out, u32* __restrict__ cnt) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n) return;
+  u64 lo, hi;
+  if (p.nprobe == 0) {
+    lo = 0;
+    hi = p.sn;
+  } else if (csr) {
+    u32 v = p.psrc[0] >= 0 ? in.col[p.psrc[0]][r] : p.pconst[0];
+    lo = csr[v];
+    hi = csr[v + 1];
+  } else {
+    K pre = probe_prefix<K>(in, p, r);
+    int rem = p.bits * (p.arity - p.nprobe);
+    K klo = pre << rem;
+    K khi = rem ? (klo | ((((K)1) << rem) - 1)) : klo;
+    u64 a = 0, b = p.sn;
+    while (a < b) {
+      u64 m = (a + b) >> 1;
+      if (skeys[m] < klo) a = m + 1;
+      else b = m;
+    }
+    lo = a;
+    b = p.sn;
+    while (a < b) {
+      u64 m = (a + b) >> 1;
+      if (skeys[m] <= khi) a = m + 1;
+      else b = m;
+    }
+    hi = a;
+  }
+  u64 c = hi - lo;
+  if (p.nfilt) {
+    c = 0;
+    for (u64 s = lo; s < hi; s++) c += filt_ok<K>(skeys[s], p);
+  }
+  lo_out[r] = lo;
+  cnt[r] = (u32)c;
+}
+
+// Output value for output column c of emitted row (table row r, relation row key).
+template <class K>
+__device__ __forceinline__ u32 out_val(const OutDesc& o, int c, const Table& in, u64 r, K skey, const ProbeDesc& p) {
+  int kind = o.kind[c];
+  if (kind == 0) return in.col[o.idx[c]][r];
+  if (kind == 1) return col_of<K>(skey, p.arity, p.bits, o.idx[c]);
+  return o.cnst[c];
+}
+
+template <class K, class KO, bool FINAL, bool TROP>
+__device__ __forceinline__ void emit_one(const Table& in, const ProbeDesc& p, const OutDesc& o, u64 r, u64 s,
+                                         const K* __restrict__ skeys, const i64* __restrict__ sw,
+                                         const u8* __restrict__ sf, u64 oi, DevErr* err) {
+  K sk = skeys[s];
+  if constexpr (FINAL) {
+    KO key = 0;
+    for (int c = 0; c < o.ncols; c++) key = (key << o.bits) | (KO)out_val<K>(o, c, in, r, sk, p);
+    ((KO*)o.keys)[oi] = key;
+  } else {
+    for (int c = 0; c < o.ncols; c++) o.col[c][oi] = out_val<K>(o, c, in, r, sk, p);
+  }
+  if constexpr (TROP) {
+    i64 res;
+    u8 rf;
+    u8 af = in.inf ? in.inf[r] : (u8)0;
+    u8 bf = sf ? sf[s] : (u8)0;
+    DevErr* e = in.part_col >= 0 ? &in.perr[in.col[in.part_col][r]] : err;
+    trop_mul(in.w[r], af, sw[s], bf, &res, &rf, e);
+    o.w[oi] = res;
+    if (o.inf) o.inf[oi] = rf;
+  }
+}
+
+constexpr int kEmitItems = 8;
+constexpr u64 kEmitTile = (u64)kBlock * kEmitItems;
+constexpr int kEmitLds = 2048;
+
+// Load-balanced emit: block b owns outputs [b*tile, (b+1)*tile).  The rows
+// spanning the tile are found by binary search over the exclusive offsets;
+// when they fit, their offsets are staged in LDS and each output searches
+// there.  Output writes and relation reads are coalesced along the output.
+template <class K, class KO, bool FINAL, bool TROP>
+__global__ __launch_bounds__(kBlock) void k_join_emit_lb(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
+                                                         const i64* __restrict__ sw, const u8* __restrict__ sf,
+                                                         const u64* __restrict__ lo, const u64* __restrict__ off,
+                                                         u64 total, DevErr* err) {
+  __shared__ u64 s_off[kEmitLds];
+  __shared__ u64 s_r[2];
+  const u64 o0 = (u64)blockIdx.x * kEmitTile;
+  const u64 o1 = min(o0 + kEmitTile, total);
+  const u64 nrows = in.n;
+  if (threadIdx.x < 2) {
+    // row containing output q: last r with off[r] <= q
+    u64 q = threadIdx.x == 0 ? o0 : o1 - 1;
+    u64 a = 0, b = nrows;
+    while (a < b) {
+      u64 m = (a + b) >> 1;
+      if (off[m] <= q) a = m + 1;
+      else b = m;
+    }
+    s_r[threadIdx.x] = a - 1;
+  }
+  __syncthreads();
+  const u64 r0 = s_r[0], r1 = s_r[1];
+  const u64 span = r1 - r0 + 1;
+  const bool staged = span <= (u64)kEmitLds;
+  if (staged)
+    for (u64 q = threadIdx.x; q < span; q += kBlock) s_off[q] = off[r0 + q];
+  __syncthreads();
+  for (int k = 0; k < kEmitItems; k++) {
+    u64 oi = o0 + (u64)k * kBlock + threadIdx.x;
+    if (oi >= o1) break;
+    u64 a, b;
+    u64 r;
+    if (staged) {
+      a = 0;
+      b = span;
+      while (a < b) {
+        u64 m = (a + b) >> 1;
+        if (s_off[m] <= oi) a = m + 1;
+        else b = m;
+      }
+      r = r0 + a - 1;
+    } else {
+      a = r0;
+      b = r1 + 1;
+      while (a < b) {
+        u64 m = (a + b) >> 1;
+        if (off[m] <= oi) a = m + 1;
+        else b = m;
+      }
+      r = a - 1;
+    }
+    u64 s = lo[r] + (oi - (staged ? s_off[r - r0] : off[r]));
+    emit_one<K, KO, FINAL, TROP>(in, p, o, r, s, skeys, sw, sf, oi, err);
+  }
+}
+
+// Filtered emit (repeated unbound variables inside the probed atom): one
+// thread per table row walks its range.
+template <class K, class KO, bool FINAL, bool TROP>
+__global__ void k_join_emit_rows(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
+                                 const i64* __restrict__ sw, const u8* __restrict__ sf, const u64* __restrict__ lo,
+                                 const u32* __restrict__ cnt, const u64* __restrict__ off, DevErr* err) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n || cnt[r] == 0) return;
+  u64 oi = off[r];
+  u64 s = lo[r];
+  for (u32 k = 0; k < cnt[r]; s++) {
+    if (!filt_ok<K>(skeys[s], p)) continue;
+    emit_one<K, KO, FINAL, TROP>(in, p, o, r, s, skeys, sw, sf, oi, err);
+    oi++;
+    k++;
+  }
+}
+
+template <class K, class KO, bool FINAL, bool TROP>
+static void emit_launch(const Exec& x, const Table& in, const ProbeDesc& p, const OutDesc& o, const K* skeys,
+                        const i64* sw, const u8* sf, const u64* lo, const u32* cnt, const u64* off, u64 total) {
+  if (p.nfilt == 0) {
+    u64 nb = (total + kEmitTile - 1) / kEmitTile;
+    k_join_emit_lb<K, KO, FINAL, TROP><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total,
+                                                                               x.err);
+  } else {
+    k_join_emit_rows<K, KO, FINAL, TROP><<<gridj(in.n), kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, cnt, off,
+                                                                                x.err);
+  }
+  SK_HIP(hipGetLastError());
+}
+
+template <class K>
+u64 join(const Exec& x, const Table& in, const ProbeDesc& p, const K* skeys, const i64* sw, const u8* sf,
+         const u64* csr, OutDesc* o, bool final_step, bool k128_out, bool trop, TableAlloc alloc, void* actx,
+         HeadAlloc halloc, void* hctx) {
+  if (in.n == 0) {
+    if (final_step) halloc(hctx, o, 0);
+    else alloc(actx, o->table, 0);
+    return 0;
+  }
+  DBuf<u64> lo(x.pool, in.n);
+  DBuf<u32> cnt(x.pool, in.n);
+  DBuf<u64> off(x.pool, in.n);
+  DBuf<u64> tot(x.pool, 1);
+  {
+    Scope sc(x.timer, "join_count", KC_JOIN, (double)in.n * (4.0 * p.nprobe + 12.0), in.n);
+    k_join_count<K><<<gridj(in.n), kBlock, 0, x.stream>>>(in, p, skeys, csr, lo.get(), cnt.get());
+    SK_HIP(hipGetLastError());
+  }
+  scan_u32(x, cnt.get(), off.get(), in.n, tot.get());
+  u64 total = read_u64(x, tot.get());
+  if (final_step) halloc(hctx, o, total);
+  else {
+    alloc(actx, o->table, total);
+    for (int c = 0; c < o->ncols; c++) o->col[c] = o->table->col[c];
+    o->w = o->table->w;
+    o->inf = o->table->inf;
+  }
+  if (total == 0) return 0;
+  // algorithmic bytes: every emitted row reads one relation record and
+  // writes one output record
+  double rec_in = sizeof(K) + (trop ? 8.0 + (sf ? 1.0 : 0) : 0);
+  double rec_out = (final_step ? (k128_out ? 16.0 : 8.0) : 4.0 * o->ncols) + (trop ? 8.0 + (o->inf ? 1.0 : 0) : 0);
+  Scope sc(x.timer, "join_emit", KC_JOIN, (double)total * (rec_in + rec_out), total);
+  if (final_step) {
+    if (k128_out) {
+      if (trop) emit_launch<K, u128, true, true>(x, in, p, *o, skeys, sw, sf, lo.get(), cnt.get(), off.get(), total);
+      else emit_launch<K, u128, true, false>(x, in, p, *o, skeys, sw, sf, lo.get(), cnt.get(), off.get(), total);
+    } else {
+      if (trop) emit_launch<K, u64, true, true>(x, in, p, *o, skeys, sw, sf, lo.get(), cnt.get(), off.get(), total);
+      else emit_launch<K, u64, true, false>(x, in, p, *o, skeys, sw, sf, lo.get(), cnt.get(), off.get(), total);
+    }
+  } else {
+    if (trop) emit_launch<K, u64, false, true>(x, in, p, *o, skeys, sw, sf, lo.get(), cnt.get(), off.get(), total);
+    else emit_launch<K, u64, false, false>(x, in, p, *o, skeys, sw, sf, lo.get(), cnt.get(), off.get(), total);
+  }
+  return total;
+}
+
+// ============================================================== anti-join
+
+template <class K>
+__global__ void k_anti_flags(Table in, AntiDesc a, const K* __restrict__ skeys, u64 sn, u8* __restrict__ keep) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n) return;
+  K key = 0;
+  for (int j = 0; j < a.arity; j++) {
+    u32 v = a.src[j] >= 0 ? in.col[a.src[j]][r] : a.cnst[j];
+    key = (key << a.bits) | (K)v;
+  }
+  u64 lo = 0, hi = sn;
+  while (lo < hi) {
+    u64 m = (lo + hi) >> 1;
+    if (skeys[m] < key) lo = m + 1;
+    else hi = m;
+  }
+  bool present = lo < sn && skeys[lo] == key;
+  keep[r] = present ? 0 : 1;
+}
+
+__global__ void k_compact_table(Table in, const u8* __restrict__ keep, const u64* __restrict__ pos, Table out) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n || !keep[r]) return;
+  u64 o = pos[r];
+  for (int c = 0; c < out.ncols; c++) out.col[c][o] = in.col[out.srccol[c]][r];
+  if (out.w) {
+    out.w[o] = in.w[r];
+    if (out.inf) out.inf[o] = in.inf ? in.inf[r] : (u8)0;
+  }
+}
+
+template <class K>
+u64 anti_join(const Exec& x, const Table& in, const AntiDesc& a, const K* skeys, u64 sn, Table* out,
+              TableAlloc alloc, void* actx) {
+  if (in.n == 0) {
+    alloc(actx, out, 0);
+    return 0;
+  }
+  DBuf<u8> keep(x.pool, in.n);
+  DBuf<u64> pos(x.pool, in.n);
+  DBuf<u64> tot(x.pool, 1);
+  if (a.never_present) {
+    fill_u8(x, keep.get(), 1, in.n);
+  } else {
+    Scope sc(x.timer, "anti_flags", KC_JOIN, (double)in.n * (4.0 * a.arity + 1.0), in.n);
+    k_anti_flags<K><<<gridj(in.n), kBlock, 0, x.stream>>>(in, a, skeys, sn, keep.get());
+    SK_HIP(hipGetLastError());
+  }
+  scan_u8(x, keep.get(), pos.get(), in.n, tot.get());
+  u64 m = read_u64(x, tot.get());
+  alloc(actx, out, m);
+  if (m) {
+    Scope sc(x.timer, "compact_table", KC_JOIN, (double)in.n * 1.0 + m * 4.0 * out->ncols, in.n);
+    k_compact_table<<<gridj(in.n), kBlock, 0, x.stream>>>(in, keep.get(), pos.get(), *out);
+    SK_HIP(hipGetLastError());
+  }
+  return m;
+}
+
+// ================================================================ project
+
+template <class KO>
+__global__ void k_project(Table in, OutDesc o) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n) return;
+  KO key = 0;
+  for (int c = 0; c < o.ncols; c++) {
+    u32 v = o.kind[c] == 0 ? in.col[o.idx[c]][r] : o.cnst[c];
+    key = (key << o.bits) | (KO)v;
+  }
+  ((KO*)o.keys)[r] = key;
+  if (o.w) {
+    o.w[r] = in.w ? in.w[r] : 0;
+    if (o.inf) o.inf[r] = in.inf ? in.inf[r] : (u8)0;
+  }
+}
+
+void project(const Exec& x, const Table& in, OutDesc* o, bool k128_out, HeadAlloc halloc, void* hctx) {
+  halloc(hctx, o, in.n);
+  if (!in.n) return;
+  Scope sc(x.timer, "project", KC_JOIN, (double)in.n * (4.0 * o->ncols + (k128_out ? 16 : 8)), in.n);
+  if (k128_out) k_project<u128><<<gridj(in.n), kBlock, 0, x.stream>>>(in, *o);
+  else k_project<u64><<<gridj(in.n), kBlock, 0, x.stream>>>(in, *o);
+  SK_HIP(hipGetLastError());
+}
+
+// ========================================================== aggregation
+
+// Group key (packed) and aggregand of each table row.  The aggregand must be
+// an Int code (>= first_int); its value comes from the Int table.
+template <class K>
+__global__ void k_agg_prep(Table in, AggDesc a, const i64* __restrict__ int_vals, K* __restrict__ gk,
+                           i64* __restrict__ val, int* __restrict__ bad) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n) return;
+  K key = 0;
+  for (int c = 0; c < a.nkey; c++) {
+    u32 v = a.key_kind[c] == 0 ? in.col[a.key_idx[c]][r] : a.key_cnst[c];
+    key = (key << a.bits) | (K)v;
+  }
+  gk[r] = key;
+  i64 v = 0;
+  if (a.op != 0) {
+    u32 code = a.val_col >= 0 ? in.col[a.val_col][r] : 0;
+    if (code < a.first_int || code >= a.first_int + a.n_ints) {
+      atomicOr(bad, 1);
+    } else {
+      v = int_vals[code - a.first_int];
+    }
+  }
+  val[r] = v;
+}
+
+// Segmented reduction over sorted group keys: one output per run.
+template <class K>
+__global__ void k_agg_reduce(const K* __restrict__ gk, const i64* __restrict__ val, u64 n, int op,
+                             const u64* __restrict__ pos, K* __restrict__ ok, i64* __restrict__ ov) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K k = gk[i];
+  if (i > 0 && gk[i - 1] == k) return;
+  i64 acc = op == 0 ? 0 : val[i];
+  u64 j = i;
+  u64 cnt = 0;
+  for (; j < n && gk[j] == k; j++) {
+    cnt++;
+    i64 v = val[j];
+    if (j == i) continue;
+    if (op == 1) acc = (i64)((u64)acc + (u64)v);
+    else if (op == 2) acc = v < acc ? v : acc;
+    else if (op == 3) acc = v > acc ? v : acc;
+  }
+  if (op == 0) acc = (i64)cnt;
+  u64 o = pos[i];
+  ok[o] = k;
+  ov[o] = acc;
+}
+
+__global__ void k_run_flags64(const u64* __restrict__ k, u64 n, u8* __restrict__ f) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = (i == 0 || k[i] != k[i - 1]);
+}
+__global__ void k_run_flags128(const u128* __restrict__ k, u64 n, u8* __restrict__ f) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = (i == 0 || k[i] != k[i - 1]);
+}
+
+template <class K>
+u64 aggregate(const Exec& x, const Table& in, const AggDesc& a, const i64* int_vals, int key_bits,
+              std::vector<K>* keys_out, std::vector<i64>* vals_out, bool* bad_aggregand) {
+  *bad_aggregand = false;
+  keys_out->clear();
+  vals_out->clear();
+  u64 n = in.n;
+  if (n == 0) return 0;
+  DBuf<K> gk(x.pool, n), gk2(x.pool, n);
+  DBuf<i64> v(x.pool, n), v2(x.pool, n);
+  DBuf<int> bad(x.pool, 1);
+  memset_async(x, bad.get(), 0, sizeof(int));
+  k_agg_prep<K><<<gridj(n), kBlock, 0, x.stream>>>(in, a, int_vals, gk.get(), v.get(), bad.get());
+  SK_HIP(hipGetLastError());
+  K* sk;
+  i64* sv;
+  radix_sort<K>(x, gk.get(), gk2.get(), v.get(), v2.get(), nullptr, nullptr, n, key_bits, &sk, &sv, nullptr);
+  DBuf<u8> fl(x.pool, n);
+  DBuf<u64> pos(x.pool, n);
+  DBuf<u64> tot(x.pool, 1);
+  if constexpr (sizeof(K) == 8) k_run_flags64<<<gridj(n), kBlock, 0, x.stream>>>((const u64*)sk, n, fl.get());
+  else k_run_flags128<<<gridj(n), kBlock, 0, x.stream>>>((const u128*)sk, n, fl.get());
+  scan_u8(x, fl.get(), pos.get(), n, tot.get());
+  u64 m = read_u64(x, tot.get());
+  DBuf<K> ok(x.pool, m);
+  DBuf<i64> ov(x.pool, m);
+  k_agg_reduce<K><<<gridj(n), kBlock, 0, x.stream>>>(sk, sv, n, a.op, pos.get(), ok.get(), ov.get());
+  SK_HIP(hipGetLastError());
+  int hb = 0;
+  SK_HIP(hipMemcpyAsync(&hb, bad.get(), sizeof(int), hipMemcpyDeviceToHost, x.stream));
+  keys_out->resize(m);
+  vals_out->resize(m);
+  SK_HIP(hipMemcpyAsync(keys_out->data(), ok.get(), m * sizeof(K), hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipMemcpyAsync(vals_out->data(), ov.get(), m * sizeof(i64), hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  *bad_aggregand = hb != 0;
+  return m;
+}
+
+// ============================================================== leapfrog
+//
+// Triangle body R1(u,v), R2(v,w), R3(u,w) (any column order): for every
+// driver row (u, v) of R1 the candidates w are the intersection of two
+// sorted adjacency lists, adj2(v) from R2 indexed on its v column and
+// adj3(u) from R3 indexed on its u column.  Count pass, device prefix sum,
+// emit pass; with the driver scanned in key order and w ascending, the
+// bindings (u, v, w) come out sorted.
+
+template <class K>
+__device__ __forceinline__ u64 intersect_count(const u32* __restrict__ A, u64 na, const u32* __restrict__ B, u64 nb) {
+  u64 i = 0, j = 0, c = 0;
+  while (i < na && j < nb) {
+    u32 x = A[i], y = B[j];
+    c += (x == y);
+    i += (x <= y);
+    j += (y <= x);
+  }
+  return c;
+}
+
+template <class K>
+__global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
+                            const u32* __restrict__ adj2, const u64* __restrict__ csr3, const u32* __restrict__ adj3,
+                            u32* __restrict__ cnt) {
+  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nd) return;
+  K k = d_keys[e];
+  u32 u = col_of<K>(k, 2, t.bits, t.u_col);
+  u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
+  u64 a0 = csr2[v], a1 = csr2[v + 1];
+  u64 b0 = csr3[u], b1 = csr3[u + 1];
+  cnt[e] = (u32)intersect_count<K>(adj2 + a0, a1 - a0, adj3 + b0, b1 - b0);
+}
+
+template <class K, class KO>
+__global__ void k_tri_emit(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
+                           const u32* __restrict__ adj2, const u64* __restrict__ csr3, const u32* __restrict__ adj3,
+                           const u64* __restrict__ off, OutDesc o) {
+  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nd) return;
+  K k = d_keys[e];
+  u32 u = col_of<K>(k, 2, t.bits, t.u_col);
+  u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
+  u64 a = csr2[v], a1 = csr2[v + 1];
+  u64 b = csr3[u], b1 = csr3[u + 1];
+  u64 oi = off[e];
+  while (a < a1 && b < b1) {
+    u32 x = adj2[a], y = adj3[b];
+    if (x == y) {
+      u32 vals[3] = {u, v, x};  // binding slots: 0 = u, 1 = v, 2 = w
+      KO key = 0;
+      for (int c = 0; c < o.ncols; c++) {
+        u32 val = o.kind[c] == 0 ? vals[o.idx[c]] : o.cnst[c];
+        key = (key << o.bits) | (KO)val;
+      }
+      ((KO*)o.keys)[oi++] = key;
+    }
+    a += (x <= y);
+    b += (y <= x);
+  }
+}
+
+template <class K>
+u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u64* csr2, const u32* adj2,
+              const u64* csr3, const u32* adj3, OutDesc* o, bool k128_out, bool count_only, HeadAlloc halloc,
+              void* hctx, double* sum_list_len) {
+  DBuf<u32> cnt(x.pool, nd ? nd : 1);
+  DBuf<u64> off(x.pool, nd ? nd : 1);
+  DBuf<u64> tot(x.pool, 1);
+  if (nd == 0) {
+    if (!count_only) halloc(hctx, o, 0);
+    return 0;
+  }
+  double lb = sum_list_len ? *sum_list_len : 0;
+  {
+    Scope sc(x.timer, "leapfrog_count", KC_LEAPFROG, (double)nd * sizeof(K) + lb * 4.0, nd);
+    k_tri_count<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, cnt.get());
+    SK_HIP(hipGetLastError());
+  }
+  scan_u32(x, cnt.get(), off.get(), nd, tot.get());
+  u64 total = read_u64(x, tot.get());
+  if (count_only) return total;
+  halloc(hctx, o, total);
+  if (total == 0) return 0;
+  Scope sc(x.timer, "leapfrog_emit", KC_LEAPFROG,
+           (double)nd * sizeof(K) + lb * 4.0 + (double)total * (k128_out ? 16.0 : 8.0), nd);
+  if (k128_out)
+    k_tri_emit<K, u128><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, off.get(), *o);
+  else
+    k_tri_emit<K, u64><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, off.get(), *o);
+  SK_HIP(hipGetLastError());
+  return total;
+}
+
+// Σ over driver rows of |adj2(v)| + |adj3(u)| (the leapfrog's list reads).
+template <class K>
+__global__ void k_tri_listlen(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
+                              const u64* __restrict__ csr3, unsigned long long* __restrict__ acc) {
+  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 s = 0;
+  if (e < nd) {
+    K k = d_keys[e];
+    u32 u = col_of<K>(k, 2, t.bits, t.u_col);
+    u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
+    s = (csr2[v + 1] - csr2[v]) + (csr3[u + 1] - csr3[u]);
+  }
+  s = wave_incl_scan_j(s);
+  if (lane_id_j() == 63 && s) atomicAdd(acc, (unsigned long long)s);
+}
+
+template <class K>
+double tri_list_len(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u64* csr2, const u64* csr3) {
+  if (!nd) return 0;
+  DBuf<u64> acc(x.pool, 1);
+  memset_async(x, acc.get(), 0, 8);
+  k_tri_listlen<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, csr3, (unsigned long long*)acc.get());
+  SK_HIP(hipGetLastError());
+  return (double)read_u64(x, acc.get());
+}
+
+// ================================================================ remap
+
+template <class K, class KO>
+__global__ void k_remap(const K* __restrict__ in, u64 n, int arity, int bits_in, int bits_out,
+                        const u32* __restrict__ map, KO* __restrict__ out) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K k = in[i];
+  KO o = 0;
+  for (int j = 0; j < arity; j++) o = (o << bits_out) | (KO)map[col_of<K>(k, arity, bits_in, j)];
+  out[i] = o;
+}
+
+template <class K, class KO>
+void remap_keys(const Exec& x, const K* in, u64 n, int arity, int bits_in, int bits_out, const u32* map, KO* out) {
+  if (!n) return;
+  k_remap<K, KO><<<gridj(n), kBlock, 0, x.stream>>>(in, n, arity, bits_in, bits_out, map, out);
+  SK_HIP(hipGetLastError());
+}
+
+// ================================================================ encode
+
+// Pack staged u32 code columns into keys.
+template <class K>
+__global__ void k_pack(Cols cols, int arity, int bits, u64 n, K* __restrict__ out) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K k = 0;
+  for (int j = 0; j < arity; j++) k = (k << bits) | (K)cols.c[j][i];
+  out[i] = k;
+}
+
+template <class K>
+void pack_cols(const Exec& x, const Cols& cols, int arity, int bits, u64 n, K* out) {
+  if (!n) return;
+  k_pack<K><<<gridj(n), kBlock, 0, x.stream>>>(cols, arity, bits, n, out);
+  SK_HIP(hipGetLastError());
+}
+
+// Mark codes present (universe bitmap).
+__global__ void k_mark(const u32* __restrict__ c, u64 n, u32 limit, u32* __restrict__ bitmap) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 v = c[i];
+  if (v < limit) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+}
+
+__global__ void k_popcount(const u32* __restrict__ b, u64 n, unsigned long long* acc) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 s = i < n ? (u64)__popc(b[i]) : 0;
+  s = wave_incl_scan_j(s);
+  if (lane_id_j() == 63 && s) atomicAdd(acc, (unsigned long long)s);
+}
+
+void mark_codes(const Exec& x, const u32* c, u64 n, u32 limit, u32* bitmap) {
+  if (!n) return;
+  k_mark<<<gridj(n), kBlock, 0, x.stream>>>(c, n, limit, bitmap);
+  SK_HIP(hipGetLastError());
+}
+
+u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords) {
+  DBuf<u64> acc(x.pool, 1);
+  memset_async(x, acc.get(), 0, 8);
+  if (nwords) k_popcount<<<gridj(nwords), kBlock, 0, x.stream>>>(bitmap, nwords, (unsigned long long*)acc.get());
+  SK_HIP(hipGetLastError());
+  return read_u64(x, acc.get());
+}
+
+// ============================================================ instances
+
+#define SK_JOIN_INST(K)                                                                                          \
+  template u64 scan_rel<K>(const Exec&, const K*, const i64*, const u8*, u64, const ScanDesc&, Table*, TableAlloc, \
+                           void*);                                                                               \
+  template u64 join<K>(const Exec&, const Table&, const ProbeDesc&, const K*, const i64*, const u8*, const u64*, \
+                       OutDesc*, bool, bool, bool, TableAlloc, void*, HeadAlloc, void*);                         \
+  template u64 anti_join<K>(const Exec&, const Table&, const AntiDesc&, const K*, u64, Table*, TableAlloc, void*); \
+  template u64 aggregate<K>(const Exec&, const Table&, const AggDesc&, const i64*, int, std::vector<K>*,         \
+                            std::vector<i64>*, bool*);                                                          \
+  template u64 triangles<K>(const Exec&, const K*, u64, const TriDesc&, const u64*, const u32*, const u64*,       \
+                            const u32*, OutDesc*, bool, bool, HeadAlloc, void*, double*);                        \
+  template double tri_list_len<K>(const Exec&, const K*, u64, const TriDesc&, const u64*, const u64*);           \
+  template void pack_cols<K>(const Exec&, const Cols&, int, int, u64, K*);
+
+SK_JOIN_INST(u64)
+SK_JOIN_INST(u128)
+template void remap_keys<u64, u64>(const Exec&, const u64*, u64, int, int, int, const u32*, u64*);
+template void remap_keys<u64, u128>(const Exec&, const u64*, u64, int, int, int, const u32*, u128*);
+template void remap_keys<u128, u64>(const Exec&, const u128*, u64, int, int, int, const u32*, u64*);
+template void remap_keys<u128, u128>(const Exec&, const u128*, u64, int, int, int, const u32*, u128*);
+
+// ============================================================ 4-cliques
+//
+// For each id-oriented edge (u, v): every w1 in adj(u) ∩ adj(v) closes a
+// triangle; the 4-cliques through it are the w2 in adj(u) ∩ adj(v) ∩ adj(w1)
+// beyond w1 (crates/strata-gpu/src/wcoj.rs:49-77 counts the same set).
+__global__ void k_k4(const u64* __restrict__ keys, u64 ne, const u64* __restrict__ csr, const u32* __restrict__ adj,
+                     int bits, unsigned long long* __restrict__ total) {
+  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 c = 0;
+  if (e < ne) {
+    u64 k = keys[e];
+    u32 u = (u32)(k >> bits), v = (u32)(k & ((1ull << bits) - 1));
+    u64 a = csr[u], ae = csr[u + 1], b = csr[v], be = csr[v + 1];
+    while (a < ae && b < be) {
+      u32 x = adj[a], y = adj[b];
+      if (x < y) { a++; continue; }
+      if (x > y) { b++; continue; }
+      u64 p = a + 1, q = b + 1, r = csr[x], re = csr[x + 1];
+      while (p < ae && q < be && r < re) {
+        u32 xp = adj[p], xq = adj[q], xr = adj[r];
+        u32 mx = max(xp, max(xq, xr));
+        if (xp < mx) p++;
+        else if (xq < mx) q++;
+        else if (xr < mx) r++;
+        else { c++; p++; q++; r++; }
+      }
+      a++;
+      b++;
+    }
+  }
+  c = wave_incl_scan_j(c);
+  if (lane_id_j() == 63 && c) atomicAdd(total, (unsigned long long)c);
+}
+
+u64 k4_count(const Exec& x, const u64* keys, u64 ne, const u64* csr, const u32* adj, int bits) {
+  DBuf<u64> acc(x.pool, 1);
+  memset_async(x, acc.get(), 0, 8);
+  if (ne) {
+    Scope sc(x.timer, "k4_count", KC_LEAPFROG, (double)ne * 8.0, ne);
+    k_k4<<<gridj(ne), kBlock, 0, x.stream>>>(keys, ne, csr, adj, bits, (unsigned long long*)acc.get());
+    SK_HIP(hipGetLastError());
+  }
+  return read_u64(x, acc.get());
+}
+
+// Interned id = position of each key in the sorted distinct table.
+__global__ void k_intern(const u64* __restrict__ t, u64 nt, const u64* __restrict__ q, u64 nq, u32* __restrict__ ids) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  u64 k = q[i], lo = 0, hi = nt;
+  while (lo < hi) {
+    u64 m = (lo + hi) >> 1;
+    if (t[m] < k) lo = m + 1;
+    else hi = m;
+  }
+  ids[i] = (u32)lo;
+}
+
+void intern_lookup(const Exec& x, const u64* table, u64 nt, const u64* q, u64 nq, u32* ids) {
+  if (!nq) return;
+  k_intern<<<gridj(nq), kBlock, 0, x.stream>>>(table, nt, q, nq, ids);
+  SK_HIP(hipGetLastError());
+}
+
+// ====================================================== partition filter
+
+template <class K>
+__global__ void k_part_flags(const K* __restrict__ keys, u64 n, int arity, int bits, const u8* __restrict__ dead,
+                             u8* __restrict__ keep) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keep[i] = dead[col_of<K>(keys[i], arity, bits, 0)] ? 0 : 1;
+}
+
+template <class K>
+__global__ void k_part_write(const K* __restrict__ keys, const i64* __restrict__ w, const u8* __restrict__ f, u64 n,
+                             const u8* __restrict__ keep, const u64* __restrict__ pos, K* __restrict__ ok,
+                             i64* __restrict__ ow, u8* __restrict__ of) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !keep[i]) return;
+  u64 o = pos[i];
+  ok[o] = keys[i];
+  if (w) ow[o] = w[i];
+  if (f) of[o] = f[i];
+}
+
+template <class K>
+u64 drop_partitions(const Exec& x, const K* keys, const i64* w, const u8* inf, u64 n, int arity, int bits,
+                    const u8* dead, K* ok, i64* ow, u8* of) {
+  if (!n) return 0;
+  DBuf<u8> keep(x.pool, n);
+  DBuf<u64> pos(x.pool, n);
+  DBuf<u64> tot(x.pool, 1);
+  k_part_flags<K><<<gridj(n), kBlock, 0, x.stream>>>(keys, n, arity, bits, dead, keep.get());
+  scan_u8(x, keep.get(), pos.get(), n, tot.get());
+  k_part_write<K><<<gridj(n), kBlock, 0, x.stream>>>(keys, w, inf, n, keep.get(), pos.get(), ok, ow, of);
+  SK_HIP(hipGetLastError());
+  return read_u64(x, tot.get());
+}
+
+template u64 drop_partitions<u64>(const Exec&, const u64*, const i64*, const u8*, u64, int, int, const u8*, u64*,
+                                  i64*, u8*);
+template u64 drop_partitions<u128>(const Exec&, const u128*, const i64*, const u8*, u64, int, int, const u8*, u128*,
+                                   i64*, u8*);
+
+}  // namespace sk

@@ -1,0 +1,965 @@
+// sk_prims.hip — data-parallel primitives for sorted packed-key relations on
+// gfx950: device scan, LSD radix sort, run collapse (dedup / Trop min),
+// merge-path union and difference, CSR build, pack/unpack, membership.
+//
+// Every kernel is written for 64-wide wavefronts and 256-thread blocks.
+// Global reads and writes are coalesced along the element index; ranking and
+// merging happen in LDS.
+#include "sk_dev.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace sk {
+
+// ============================================================ device helpers
+
+__device__ __forceinline__ u32 lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ u32 wave_id() { return threadIdx.x >> 6; }
+__device__ __forceinline__ u64 lanemask_lt() {
+  u32 l = lane_id();
+  return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
+// Wave-level inclusive scan of u64.
+__device__ __forceinline__ u64 wave_incl_scan(u64 v) {
+  u32 l = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    u64 y = __shfl_up(v, o, 64);
+    if (l >= (u32)o) v += y;
+  }
+  return v;
+}
+
+// Block-level exclusive scan; `tmp` is __shared__ u64[kWaves]; returns the
+// exclusive prefix and the block total.  Must be reached by every thread.
+__device__ __forceinline__ u64 block_excl_scan(u64 v, u64* tmp, u64* total) {
+  u64 inc = wave_incl_scan(v);
+  if (lane_id() == 63) tmp[wave_id()] = inc;
+  __syncthreads();
+  u64 off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    u64 s = tmp[w];
+    if ((u32)w < wave_id()) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+// Weight order with +inf above every finite value.
+__device__ __forceinline__ bool w_less(i64 a, u8 ai, i64 b, u8 bi) {
+  if (ai) return false;
+  if (bi) return true;
+  return a < b;
+}
+
+// ===================================================================== fill
+
+__global__ void k_fill_u8(u8* p, u8 v, u64 n) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  u64 stride = (u64)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+static unsigned grid_for(u64 n, u64 per = kBlock, unsigned cap = 65535u * 4) {
+  u64 g = (n + per - 1) / per;
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+void fill_u8(const Exec& x, u8* p, u8 v, u64 n) {
+  if (!n) return;
+  k_fill_u8<<<grid_for(n, kBlock * 8), kBlock, 0, x.stream>>>(p, v, n);
+  SK_HIP(hipGetLastError());
+}
+
+void memset_async(const Exec& x, void* p, int v, u64 bytes) { fill_u8(x, (u8*)p, (u8)v, bytes); }
+
+u64 read_u64(const Exec& x, const u64* d) {
+  static thread_local u64* host = nullptr;
+  if (!host) SK_HIP(hipHostMalloc((void**)&host, 64, hipHostMallocDefault));
+  SK_HIP(hipMemcpyAsync(host, d, sizeof(u64), hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  return *host;
+}
+
+// ===================================================================== scan
+
+constexpr int kScanItems = 16;
+constexpr u64 kScanTile = (u64)kBlock * kScanItems;  // 4096
+constexpr unsigned kScanMaxBlocks = 2048;
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const T* __restrict__ in, u64 n, u64 per,
+                                                        u64* __restrict__ sums) {
+  __shared__ u64 tmp[kWaves];
+  u64 lo = (u64)blockIdx.x * per, hi = min(n, lo + per);
+  u64 acc = 0;
+  for (u64 i = lo + threadIdx.x; i < hi; i += kBlock) acc += (u64)in[i];
+  u64 tot;
+  block_excl_scan(acc, tmp, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Single-block exclusive scan of block sums in place; writes the grand total.
+__global__ __launch_bounds__(kBlock) void k_scan_sums(u64* sums, u32 nb, u64* total) {
+  __shared__ u64 tmp[kWaves];
+  u64 carry = 0;
+  for (u32 base = 0; base < nb; base += kBlock) {
+    u32 i = base + threadIdx.x;
+    u64 v = i < nb ? sums[i] : 0;
+    u64 tot;
+    u64 ex = block_excl_scan(v, tmp, &tot);
+    if (i < nb) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, u64* __restrict__ out,
+                                                      u64 n, u64 per, const u64* __restrict__ offs) {
+  __shared__ u64 tmp[kWaves];
+  u64 lo = (u64)blockIdx.x * per, hi = min(n, lo + per);
+  u64 carry = offs[blockIdx.x];
+  for (u64 base = lo; base < hi; base += kBlock) {
+    u64 i = base + threadIdx.x;
+    u64 v = i < hi ? (u64)in[i] : 0;
+    u64 tot;
+    u64 ex = block_excl_scan(v, tmp, &tot);
+    if (i < hi) out[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+template <class T>
+static void scan_impl(const Exec& x, const T* in, u64* out, u64 n, u64* d_total) {
+  if (n == 0) {
+    if (d_total) memset_async(x, d_total, 0, sizeof(u64));
+    return;
+  }
+  u64 nb = (n + kScanTile - 1) / kScanTile;
+  if (nb > kScanMaxBlocks) nb = kScanMaxBlocks;
+  u64 per = (n + nb - 1) / nb;
+  per = (per + kBlock - 1) / kBlock * kBlock;
+  nb = (n + per - 1) / per;
+  DBuf<u64> sums(x.pool, nb);
+  {
+    Scope sc(x.timer, "scan_reduce", KC_MISC, (double)n * sizeof(T), n);
+    k_scan_reduce<T><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, n, per, sums.get());
+  }
+  k_scan_sums<<<1, kBlock, 0, x.stream>>>(sums.get(), (u32)nb, d_total);
+  {
+    Scope sc(x.timer, "scan_down", KC_MISC, (double)n * (sizeof(T) + 8), n);
+    k_scan_down<T><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, out, n, per, sums.get());
+  }
+  SK_HIP(hipGetLastError());
+}
+
+void scan_u32(const Exec& x, const u32* in, u64* out, u64 n, u64* t) { scan_impl<u32>(x, in, out, n, t); }
+void scan_u64(const Exec& x, const u64* in, u64* out, u64 n, u64* t) { scan_impl<u64>(x, in, out, n, t); }
+void scan_u8(const Exec& x, const u8* in, u64* out, u64 n, u64* t) { scan_impl<u8>(x, in, out, n, t); }
+
+// =============================================================== radix sort
+//
+// One LSD pass over 8-bit digit `shift`:
+//   k_radix_hist     per-block digit histogram (wave ballot match + LDS)
+//   scan             exclusive scan of counts in digit-major order
+//   k_radix_scatter  per tile: stable in-wave ranking by ballot match,
+//                    tile-local counting sort in LDS, then coalesced runs
+//                    written to the digit's global position.
+// Each block owns a contiguous chunk, so the pass is stable.
+
+template <class K>
+struct RadixCfg;
+template <>
+struct RadixCfg<u64> {
+  static constexpr int kItems = 16;
+};
+template <>
+struct RadixCfg<u128> {
+  static constexpr int kItems = 8;
+};
+
+// Lanes of the wave (among `valid` lanes) holding the same 8-bit digit.
+__device__ __forceinline__ u64 match_digit(u32 d, bool valid) {
+  u64 m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    bool bit = (d >> b) & 1u;
+    u64 bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+template <class K>
+__device__ __forceinline__ u32 digit_of(K k, int shift) {
+  return (u32)(k >> shift) & 0xffu;
+}
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, u64 n, u64 chunk,
+                                                       int shift, u32* __restrict__ counts, u32 G) {
+  __shared__ u32 h[kWaves][256];
+  for (u32 i = threadIdx.x; i < kWaves * 256; i += kBlock) (&h[0][0])[i] = 0;
+  __syncthreads();
+  u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  u32 w = wave_id();
+  for (u64 base = lo; base < hi; base += kBlock) {
+    u64 i = base + threadIdx.x;
+    bool valid = i < hi;
+    u32 d = valid ? digit_of<K>(keys[i], shift) : 0;
+    u64 m = match_digit(d, valid);
+    if (valid && (int)lane_id() == (m ? (int)__builtin_ctzll(m) : -1)) atomicAdd(&h[w][d], (u32)__popcll(m));
+  }
+  __syncthreads();
+  for (u32 d = threadIdx.x; d < 256; d += kBlock) {
+    u32 s = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; q++) s += h[q][d];
+    counts[(u64)d * G + blockIdx.x] = s;
+  }
+}
+
+template <class K, bool HASW>
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    const K* __restrict__ kin, K* __restrict__ kout, const i64* __restrict__ win, i64* __restrict__ wout,
+    const u8* __restrict__ fin, u8* __restrict__ fout, u64 n, u64 chunk, int shift,
+    const u64* __restrict__ offs, u32 G) {
+  constexpr int IT = RadixCfg<K>::kItems;
+  constexpr int TILE = kBlock * IT;
+  __shared__ K s_key[TILE];
+  __shared__ i64 s_w[HASW ? TILE : 1];
+  __shared__ u8 s_f[HASW ? TILE : 1];
+  __shared__ u32 s_wcnt[kWaves][256];
+  __shared__ u32 s_start[256];
+  __shared__ u32 s_tot[256];
+  __shared__ u64 s_gbase[256];
+  __shared__ u64 tmp[kWaves];
+
+  const u32 tid = threadIdx.x, lane = lane_id(), wv = wave_id();
+  const u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  for (u32 d = tid; d < 256; d += kBlock) s_gbase[d] = offs[(u64)d * G + blockIdx.x];
+  const bool has_f = HASW && fin != nullptr;
+
+  for (u64 tb = lo; tb < hi; tb += TILE) {
+    for (u32 i = tid; i < kWaves * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+    __syncthreads();
+    K key[IT];
+    i64 wt[HASW ? IT : 1];
+    u8 fl[HASW ? IT : 1];
+    u32 rk[IT];
+    const u64 wb = tb + (u64)wv * 64 * IT;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      u64 idx = wb + (u64)it * 64 + lane;
+      bool valid = idx < hi;
+      key[it] = valid ? kin[idx] : (K)0;
+      if constexpr (HASW) {
+        wt[it] = valid ? win[idx] : 0;
+        fl[it] = (valid && has_f) ? fin[idx] : (u8)0;
+      }
+      u32 d = digit_of<K>(key[it], shift);
+      u64 m = match_digit(d, valid);
+      int leader = (m ? (int)__builtin_ctzll(m) : -1);
+      u32 old = 0;
+      if (valid && (int)lane == leader) old = atomicAdd(&s_wcnt[wv][d], (u32)__popcll(m));
+      old = __shfl(old, leader < 0 ? 0 : leader, 64);
+      rk[it] = old + (u32)__popcll(m & lanemask_lt());
+    }
+    __syncthreads();
+    // digit totals of the tile and per-wave offsets
+    u32 dsum = 0;
+    {
+      u32 d = tid;  // kBlock == 256 digits
+      u32 run = 0;
+#pragma unroll
+      for (int q = 0; q < kWaves; q++) {
+        u32 c = s_wcnt[q][d];
+        s_wcnt[q][d] = run;
+        run += c;
+      }
+      dsum = run;
+      s_tot[d] = run;
+    }
+    u64 ttot;
+    u64 st = block_excl_scan((u64)dsum, tmp, &ttot);
+    s_start[tid] = (u32)st;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      u64 idx = wb + (u64)it * 64 + lane;
+      if (idx < hi) {
+        u32 d = digit_of<K>(key[it], shift);
+        u32 pos = s_start[d] + s_wcnt[wv][d] + rk[it];
+        s_key[pos] = key[it];
+        if constexpr (HASW) {
+          s_w[pos] = wt[it];
+          s_f[pos] = fl[it];
+        }
+      }
+    }
+    __syncthreads();
+    const u32 tn = (u32)min((u64)TILE, hi - tb);
+    for (u32 p = tid; p < tn; p += kBlock) {
+      K k = s_key[p];
+      u32 d = digit_of<K>(k, shift);
+      u64 o = s_gbase[d] + (p - s_start[d]);
+      kout[o] = k;
+      if constexpr (HASW) {
+        wout[o] = s_w[p];
+        if (has_f) fout[o] = s_f[p];
+      }
+    }
+    __syncthreads();
+    s_gbase[tid] += s_tot[tid];
+    __syncthreads();
+  }
+}
+
+template <class K>
+void radix_sort(const Exec& x, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt, u64 n,
+                int key_bits, K** out_keys, i64** out_w, u8** out_inf) {
+  *out_keys = keys;
+  if (out_w) *out_w = w;
+  if (out_inf) *out_inf = inf;
+  if (n <= 1 || key_bits <= 0) return;
+  constexpr u64 TILE = (u64)kBlock * RadixCfg<K>::kItems;
+  const bool hasw = (w != nullptr);
+  int passes = (key_bits + 7) / 8;
+  const double rec = sizeof(K) + (hasw ? 8.0 : 0) + (inf ? 1.0 : 0);
+  u64 G = (n + TILE - 1) / TILE;
+  u64 maxG = 2048;
+  if (G > maxG) G = maxG;
+  u64 chunk = (n + G - 1) / G;
+  chunk = (chunk + TILE - 1) / TILE * TILE;
+  G = (n + chunk - 1) / chunk;
+  DBuf<u32> counts(x.pool, 256 * G);
+  DBuf<u64> offs(x.pool, 256 * G);
+  K* a = keys;
+  K* b = keys_alt;
+  i64* wa = w;
+  i64* wb = w_alt;
+  u8* fa = inf;
+  u8* fb = inf_alt;
+  for (int p = 0; p < passes; p++) {
+    int shift = 8 * p;
+    {
+      Scope sc(x.timer, "radix_hist", KC_SORT, (double)n * sizeof(K), n);
+      k_radix_hist<K><<<(unsigned)G, kBlock, 0, x.stream>>>(a, n, chunk, shift, counts.get(), (u32)G);
+    }
+    scan_u32(x, counts.get(), offs.get(), 256 * G, nullptr);
+    // algorithmic bytes of one scatter pass: read + write every record
+    Scope sc(x.timer, "radix_scatter", KC_SORT, 2.0 * rec * (double)n, n);
+    if (hasw)
+      k_radix_scatter<K, true><<<(unsigned)G, kBlock, 0, x.stream>>>(a, b, wa, wb, fa, fb, n, chunk, shift,
+                                                                     offs.get(), (u32)G);
+    else
+      k_radix_scatter<K, false><<<(unsigned)G, kBlock, 0, x.stream>>>(a, b, nullptr, nullptr, nullptr, nullptr,
+                                                                      n, chunk, shift, offs.get(), (u32)G);
+    SK_HIP(hipGetLastError());
+    std::swap(a, b);
+    std::swap(wa, wb);
+    std::swap(fa, fb);
+  }
+  *out_keys = a;
+  if (out_w) *out_w = wa;
+  if (out_inf) *out_inf = fa;
+}
+
+// ============================================================ run collapse
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_runs_count(const K* __restrict__ keys, u64 n, u64 per,
+                                                       u64* __restrict__ cnt) {
+  __shared__ u64 tmp[kWaves];
+  u64 lo = (u64)blockIdx.x * per, hi = min(n, lo + per);
+  u64 c = 0;
+  for (u64 i = lo + threadIdx.x; i < hi; i += kBlock) c += (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+  u64 tot;
+  block_excl_scan(c, tmp, &tot);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+template <class K, bool HASW>
+__global__ __launch_bounds__(kBlock) void k_runs_write(const K* __restrict__ keys, const i64* __restrict__ w,
+                                                       const u8* __restrict__ f, u64 n, u64 per,
+                                                       const u64* __restrict__ offs, K* __restrict__ ok,
+                                                       i64* __restrict__ ow, u8* __restrict__ of) {
+  __shared__ u64 tmp[kWaves];
+  u64 lo = (u64)blockIdx.x * per, hi = min(n, lo + per);
+  u64 carry = offs[blockIdx.x];
+  for (u64 base = lo; base < hi; base += kBlock) {
+    u64 i = base + threadIdx.x;
+    bool first = false;
+    K k = 0;
+    if (i < hi) {
+      k = keys[i];
+      first = (i == 0 || keys[i - 1] != k);
+    }
+    u64 tot;
+    u64 ex = block_excl_scan(first ? 1 : 0, tmp, &tot);
+    if (first) {
+      u64 o = carry + ex;
+      ok[o] = k;
+      if constexpr (HASW) {
+        i64 bw = w[i];
+        u8 bf = f ? f[i] : (u8)0;
+        for (u64 j = i + 1; j < n && keys[j] == k; j++) {
+          i64 cw = w[j];
+          u8 cf = f ? f[j] : (u8)0;
+          if (w_less(cw, cf, bw, bf)) {
+            bw = cw;
+            bf = cf;
+          }
+        }
+        ow[o] = bw;
+        if (of) of[o] = bf;
+      }
+    }
+    carry += tot;
+  }
+}
+
+template <class K>
+u64 unique_runs(const Exec& x, const K* keys, const i64* w, const u8* inf, u64 n, K* ok, i64* ow, u8* of) {
+  if (n == 0) return 0;
+  u64 nb = (n + kScanTile - 1) / kScanTile;
+  if (nb > kScanMaxBlocks) nb = kScanMaxBlocks;
+  u64 per = (n + nb - 1) / nb;
+  per = (per + kBlock - 1) / kBlock * kBlock;
+  nb = (n + per - 1) / per;
+  DBuf<u64> cnt(x.pool, nb + 1);
+  {
+    Scope sc(x.timer, "runs_count", KC_DEDUP, (double)n * sizeof(K), n);
+    k_runs_count<K><<<(unsigned)nb, kBlock, 0, x.stream>>>(keys, n, per, cnt.get());
+  }
+  k_scan_sums<<<1, kBlock, 0, x.stream>>>(cnt.get(), (u32)nb, cnt.get() + nb);
+  {
+  const double rec = sizeof(K) + (w ? 8.0 : 0) + (inf ? 1.0 : 0);
+  Scope sc(x.timer, "runs_write", KC_DEDUP, (double)n * rec, n);
+  if (w)
+    k_runs_write<K, true><<<(unsigned)nb, kBlock, 0, x.stream>>>(keys, w, inf, n, per, cnt.get(), ok, ow, of);
+  else
+    k_runs_write<K, false><<<(unsigned)nb, kBlock, 0, x.stream>>>(keys, nullptr, nullptr, n, per, cnt.get(), ok,
+                                                                  nullptr, nullptr);
+  }
+  SK_HIP(hipGetLastError());
+  return read_u64(x, cnt.get() + nb);
+}
+
+// ============================================================== merge path
+//
+// The merged order takes full (B) before cand (A) on equal keys, so a match
+// is an adjacent (B, A) pair.  Each tile of kMergeTile merged positions
+// loads its A and B ranges into LDS, every thread merges kMergeItems
+// consecutive positions, and neighbours across threads / tiles are exchanged
+// through LDS and one global look-behind / look-ahead.
+
+constexpr int kMergeItems = 8;
+constexpr u64 kMergeTile = (u64)kBlock * kMergeItems;  // 2048
+
+template <class K>
+__device__ __forceinline__ u64 merge_path(const K* a, u64 m, const K* b, u64 n, u64 d) {
+  u64 lo = d > n ? d - n : 0, hi = d < m ? d : m;
+  while (lo < hi) {
+    u64 mid = (lo + hi) >> 1;
+    if (a[mid] < b[d - 1 - mid]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <class K>
+__global__ void k_merge_partition(const K* __restrict__ a, u64 m, const K* __restrict__ b, u64 n, u64 ntiles,
+                                  u64* __restrict__ part) {
+  u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  u64 d = min(t * kMergeTile, m + n);
+  part[t] = merge_path(a, m, b, n, d);
+}
+
+template <class K>
+struct MItem {
+  K key;
+  i64 w;
+  u8 f;
+  u8 src;  // 0 = cand (A), 1 = full (B)
+};
+
+// PASS 0: count union / fresh rows per tile.  PASS 1: write them.
+template <class K, bool HASW, int PASS>
+__global__ __launch_bounds__(kBlock) void k_merge(
+    const K* __restrict__ a, const i64* __restrict__ aw, const u8* __restrict__ af, u64 m,
+    const K* __restrict__ b, const i64* __restrict__ bw, const u8* __restrict__ bf, u64 n,
+    const u64* __restrict__ part, u64* __restrict__ ucnt, u64* __restrict__ fcnt,
+    K* __restrict__ uk, i64* __restrict__ uw, u8* __restrict__ uf,
+    K* __restrict__ fk, i64* __restrict__ fw, u8* __restrict__ ff) {
+  __shared__ K sa[kMergeTile];
+  __shared__ K sb[kMergeTile];
+  __shared__ i64 saw[HASW ? kMergeTile : 1];
+  __shared__ i64 sbw[HASW ? kMergeTile : 1];
+  __shared__ u8 saf[HASW ? kMergeTile : 1];
+  __shared__ u8 sbf[HASW ? kMergeTile : 1];
+  __shared__ MItem<K> edge_first[kBlock];
+  __shared__ MItem<K> edge_last[kBlock];
+  __shared__ u64 tmp[kWaves];
+
+  const u64 t = blockIdx.x;
+  const u64 d0 = t * kMergeTile, d1 = min(d0 + kMergeTile, m + n);
+  const u64 i0 = part[t], i1 = part[t + 1];
+  const u64 j0 = d0 - i0, j1 = d1 - i1;
+  const u32 na = (u32)(i1 - i0), nb = (u32)(j1 - j0);
+  const bool has_f = HASW && (af != nullptr);
+  for (u32 q = threadIdx.x; q < na; q += kBlock) {
+    sa[q] = a[i0 + q];
+    if constexpr (HASW) {
+      saw[q] = aw[i0 + q];
+      saf[q] = has_f ? af[i0 + q] : (u8)0;
+    }
+  }
+  for (u32 q = threadIdx.x; q < nb; q += kBlock) {
+    sb[q] = b[j0 + q];
+    if constexpr (HASW) {
+      sbw[q] = bw[j0 + q];
+      sbf[q] = has_f ? bf[j0 + q] : (u8)0;
+    }
+  }
+  __syncthreads();
+
+  // thread-local merge of kMergeItems positions
+  const u32 tn = (u32)(d1 - d0);
+  u32 ld = min((u32)threadIdx.x * kMergeItems, tn);
+  u32 lhi = min(ld + kMergeItems, tn);
+  u32 cnt = lhi - ld;
+  u32 li = (u32)merge_path<K>(sa, na, sb, nb, ld);
+  u32 lj = ld - li;
+  MItem<K> it[kMergeItems];
+#pragma unroll
+  for (int k = 0; k < kMergeItems; k++) {
+    if ((u32)k < cnt) {
+      bool takeB = (lj < nb) && (li >= na || !(sa[li] < sb[lj]));
+      if (takeB) {
+        it[k].key = sb[lj];
+        it[k].src = 1;
+        if constexpr (HASW) { it[k].w = sbw[lj]; it[k].f = sbf[lj]; }
+        lj++;
+      } else {
+        it[k].key = sa[li];
+        it[k].src = 0;
+        if constexpr (HASW) { it[k].w = saw[li]; it[k].f = saf[li]; }
+        li++;
+      }
+    }
+  }
+  if (cnt) {
+    edge_first[threadIdx.x] = it[0];
+    edge_last[threadIdx.x] = it[cnt - 1];
+  }
+  __syncthreads();
+
+  // neighbours of this thread's first and last item
+  bool has_prev = false, has_next = false;
+  MItem<K> prev, next;
+  if (cnt) {
+    if (threadIdx.x > 0) {
+      prev = edge_last[threadIdx.x - 1];
+      has_prev = true;
+    } else if (d0 > 0) {
+      // merged element d0-1: the later (in merge order) of a[i0-1], b[j0-1]
+      bool ha = i0 > 0, hb = j0 > 0;
+      bool pickA = ha && (!hb || !(a[i0 - 1] < b[j0 - 1]));
+      if (pickA) {
+        prev.key = a[i0 - 1]; prev.src = 0;
+        if constexpr (HASW) { prev.w = aw[i0 - 1]; prev.f = has_f ? af[i0 - 1] : (u8)0; }
+      } else {
+        prev.key = b[j0 - 1]; prev.src = 1;
+        if constexpr (HASW) { prev.w = bw[j0 - 1]; prev.f = has_f ? bf[j0 - 1] : (u8)0; }
+      }
+      has_prev = true;
+    }
+    u32 nthr = (tn + kMergeItems - 1) / kMergeItems;
+    if (threadIdx.x + 1 < nthr) {
+      next = edge_first[threadIdx.x + 1];
+      has_next = true;
+    } else if (d1 < m + n) {
+      // merged element d1: the earlier of a[i1], b[j1] (B first on ties)
+      bool ha = i1 < m, hb = j1 < n;
+      bool pickB = hb && (!ha || !(a[i1] < b[j1]));
+      if (pickB) { next.key = b[j1]; next.src = 1; }
+      else { next.key = a[i1]; next.src = 0; }
+      has_next = true;
+    }
+  }
+
+  u32 ucount = 0, fcount = 0;
+  bool uemit[kMergeItems], femit[kMergeItems];
+  i64 uwv[HASW ? kMergeItems : 1];
+  u8 ufv[HASW ? kMergeItems : 1];
+#pragma unroll
+  for (int k = 0; k < kMergeItems; k++) {
+    uemit[k] = false;
+    femit[k] = false;
+    if ((u32)k >= cnt) continue;
+    bool hp = k > 0 || has_prev;
+    const MItem<K>& pv = k > 0 ? it[k > 0 ? k - 1 : 0] : prev;
+    bool hn = (u32)(k + 1) < cnt || has_next;
+    K nk = (u32)(k + 1) < cnt ? it[(k + 1) < kMergeItems ? k + 1 : k].key : next.key;
+    bool dup_prev = hp && pv.key == it[k].key;
+    bool dup_next = hn && nk == it[k].key;
+    if (!dup_next) {
+      uemit[k] = true;
+      if constexpr (HASW) {
+        i64 ww = it[k].w;
+        u8 ff2 = it[k].f;
+        if (dup_prev && w_less(pv.w, pv.f, ww, ff2)) { ww = pv.w; ff2 = pv.f; }
+        uwv[k] = ww;
+        ufv[k] = ff2;
+      }
+    }
+    if (it[k].src == 0) {
+      bool fresh = !dup_prev;
+      if constexpr (HASW) {
+        if (dup_prev && w_less(it[k].w, it[k].f, pv.w, pv.f)) fresh = true;
+      }
+      femit[k] = fresh;
+    }
+    ucount += uemit[k];
+    fcount += femit[k];
+  }
+
+  if (PASS == 0) {
+    u64 ut, ft;
+    block_excl_scan(ucount, tmp, &ut);
+    block_excl_scan(fcount, tmp, &ft);
+    if (threadIdx.x == 0) {
+      ucnt[t] = ut;
+      fcnt[t] = ft;
+    }
+  } else {
+    u64 ut, ft;
+    u64 uo = block_excl_scan(ucount, tmp, &ut) + ucnt[t];
+    u64 fo = block_excl_scan(fcount, tmp, &ft) + fcnt[t];
+#pragma unroll
+    for (int k = 0; k < kMergeItems; k++) {
+      if (uemit[k] && uk) {
+        uk[uo] = it[k].key;
+        if constexpr (HASW) {
+          uw[uo] = uwv[k];
+          if (uf) uf[uo] = ufv[k];
+        }
+        uo++;
+      }
+      if (femit[k]) {
+        fk[fo] = it[k].key;
+        if constexpr (HASW) {
+          fw[fo] = it[k].w;
+          if (ff) ff[fo] = it[k].f;
+        }
+        fo++;
+      }
+    }
+  }
+}
+
+template <class K>
+void merge_diff(const Exec& x, const K* a, const i64* aw, const u8* ainf, u64 m, const K* b, const i64* bw,
+                const u8* binf, u64 n, DBuf<K>* u_keys, DBuf<i64>* u_w, DBuf<u8>* u_inf, u64* n_union,
+                DBuf<K>* f_keys, DBuf<i64>* f_w, DBuf<u8>* f_inf, u64* n_fresh, bool weighted, bool has_inf) {
+  *n_union = n;
+  *n_fresh = 0;
+  if (m == 0) return;
+  double rb = sizeof(K) + (weighted ? 8.0 + (has_inf ? 1.0 : 0) : 0);
+  u64 tot = m + n;
+  u64 nt = (tot + kMergeTile - 1) / kMergeTile;
+  DBuf<u64> part(x.pool, nt + 1);
+  DBuf<u64> uc(x.pool, nt + 1), fc(x.pool, nt + 1);
+  k_merge_partition<K><<<grid_for(nt + 1), kBlock, 0, x.stream>>>(a, m, b, n, nt, part.get());
+  const u8* af = has_inf ? ainf : nullptr;
+  const u8* bf = has_inf ? binf : nullptr;
+  {
+  Scope sc(x.timer, "merge_count", KC_MERGE, (double)(m + n) * rb, m + n);
+  if (weighted)
+    k_merge<K, true, 0><<<(unsigned)nt, kBlock, 0, x.stream>>>(a, aw, af, m, b, bw, bf, n, part.get(), uc.get(),
+                                                               fc.get(), nullptr, nullptr, nullptr, nullptr,
+                                                               nullptr, nullptr);
+  else
+    k_merge<K, false, 0><<<(unsigned)nt, kBlock, 0, x.stream>>>(a, nullptr, nullptr, m, b, nullptr, nullptr, n,
+                                                                part.get(), uc.get(), fc.get(), nullptr, nullptr,
+                                                                nullptr, nullptr, nullptr, nullptr);
+  }
+  k_scan_sums<<<1, kBlock, 0, x.stream>>>(uc.get(), (u32)nt, uc.get() + nt);
+  k_scan_sums<<<1, kBlock, 0, x.stream>>>(fc.get(), (u32)nt, fc.get() + nt);
+  SK_HIP(hipGetLastError());
+  u64 nu = read_u64(x, uc.get() + nt);
+  u64 nf = read_u64(x, fc.get() + nt);
+  *n_fresh = nf;
+  if (nf == 0) return;
+  *n_union = nu;
+  u_keys->ensure(x.pool, nu);
+  f_keys->ensure(x.pool, nf);
+  // algorithmic bytes: read both inputs once, write union and fresh once
+  Scope sc(x.timer, "merge_write", KC_MERGE, (double)(m + n + nu + nf) * rb, m + n);
+  if (weighted) {
+    u_w->ensure(x.pool, nu);
+    f_w->ensure(x.pool, nf);
+    if (has_inf) {
+      u_inf->ensure(x.pool, nu);
+      f_inf->ensure(x.pool, nf);
+    }
+    k_merge<K, true, 1><<<(unsigned)nt, kBlock, 0, x.stream>>>(
+        a, aw, af, m, b, bw, bf, n, part.get(), uc.get(), fc.get(), u_keys->get(), u_w->get(),
+        has_inf ? u_inf->get() : nullptr, f_keys->get(), f_w->get(), has_inf ? f_inf->get() : nullptr);
+  } else {
+    k_merge<K, false, 1><<<(unsigned)nt, kBlock, 0, x.stream>>>(a, nullptr, nullptr, m, b, nullptr, nullptr, n,
+                                                                part.get(), uc.get(), fc.get(), u_keys->get(),
+                                                                nullptr, nullptr, f_keys->get(), nullptr, nullptr);
+  }
+  SK_HIP(hipGetLastError());
+}
+
+// ===================================================================== CSR
+
+template <class K>
+__global__ void k_csr(const K* __restrict__ keys, u64 n, int shift, u64 nvals, u64* __restrict__ csr) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  // rows [i-1, i) boundary: values (prev, cur] start at row i
+  long long cur = i < n ? (long long)(u64)(keys[i] >> shift) : (long long)nvals;
+  long long prev = i > 0 ? (long long)(u64)(keys[i - 1] >> shift) : -1;
+  if (cur > (long long)nvals) cur = (long long)nvals;
+  for (long long v = prev + 1; v <= cur; v++) csr[v] = i;
+}
+
+template <class K>
+void build_csr(const Exec& x, const K* keys, u64 n, int shift, u64 nvals, u64* csr) {
+  Scope sc(x.timer, "csr", KC_MISC, (double)n * sizeof(K) + (nvals + 1) * 8.0, n);
+  k_csr<K><<<grid_for(n + 1), kBlock, 0, x.stream>>>(keys, n, shift, nvals, csr);
+  SK_HIP(hipGetLastError());
+}
+
+// ========================================================== pack / unpack
+
+template <class K>
+__global__ void k_unpack(const K* __restrict__ keys, u64 n, int arity, int bits, Cols cols) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K k = keys[i];
+  K mask = (((K)1) << bits) - 1;
+  for (int j = 0; j < arity; j++) {
+    int sh = bits * (arity - 1 - j);
+    cols.c[j][i] = (u32)((k >> sh) & mask);
+  }
+}
+
+template <class K>
+void unpack_cols(const Exec& x, const K* keys, u64 n, int arity, int bits, u32* const* cols) {
+  if (!n || !arity) return;
+  Scope sc(x.timer, "unpack", KC_MISC, (double)n * (sizeof(K) + 4.0 * arity), n);
+  Cols c{};
+  for (int j = 0; j < arity && j < kMaxCols; j++) c.c[j] = cols[j];
+  k_unpack<K><<<grid_for(n), kBlock, 0, x.stream>>>(keys, n, arity, bits, c);
+  SK_HIP(hipGetLastError());
+}
+
+struct Perm8 {
+  int p[8];
+};
+
+template <class K, class KO>
+__global__ void k_repack(const K* __restrict__ keys, u64 n, int arity, int bits, Perm8 perm, KO* __restrict__ out) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K k = keys[i];
+  K mask = (((K)1) << bits) - 1;
+  KO o = 0;
+  for (int j = 0; j < arity; j++) {
+    int src = perm.p[j];
+    K c = (k >> (bits * (arity - 1 - src))) & mask;
+    o = (o << bits) | (KO)c;
+  }
+  out[i] = o;
+}
+
+template <class K, class KO>
+void repack(const Exec& x, const K* keys, u64 n, int arity, int bits, const int* perm, KO* out) {
+  if (!n) return;
+  Scope sc(x.timer, "repack", KC_MISC, (double)n * (sizeof(K) + sizeof(KO)), n);
+  Perm8 p8;
+  for (int j = 0; j < 8; j++) p8.p[j] = j < arity ? perm[j] : 0;
+  k_repack<K, KO><<<grid_for(n), kBlock, 0, x.stream>>>(keys, n, arity, bits, p8, out);
+  SK_HIP(hipGetLastError());
+}
+
+// ============================================================ membership
+
+template <class K>
+__global__ void k_member(const K* __restrict__ s, u64 ns, const K* __restrict__ q, u64 nq, u8* __restrict__ flags) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  K k = q[i];
+  u64 lo = 0, hi = ns;
+  while (lo < hi) {
+    u64 mid = (lo + hi) >> 1;
+    if (s[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  flags[i] = (lo < ns && s[lo] == k) ? 1 : 0;
+}
+
+template <class K>
+void member(const Exec& x, const K* s, u64 ns, const K* q, u64 nq, u8* flags) {
+  if (!nq) return;
+  Scope sc(x.timer, "member", KC_MISC, (double)nq * (sizeof(K) + 1.0), nq);
+  k_member<K><<<grid_for(nq), kBlock, 0, x.stream>>>(s, ns, q, nq, flags);
+  SK_HIP(hipGetLastError());
+}
+
+// ======================================================= instantiations
+
+#define SK_INST_K(K)                                                                                     \
+  template void radix_sort<K>(const Exec&, K*, K*, i64*, i64*, u8*, u8*, u64, int, K**, i64**, u8**);  \
+  template u64 unique_runs<K>(const Exec&, const K*, const i64*, const u8*, u64, K*, i64*, u8*);        \
+  template void merge_diff<K>(const Exec&, const K*, const i64*, const u8*, u64, const K*, const i64*, \
+                              const u8*, u64, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, u64*, DBuf<K>*,          \
+                              DBuf<i64>*, DBuf<u8>*, u64*, bool, bool);                                 \
+  template void build_csr<K>(const Exec&, const K*, u64, int, u64, u64*);                               \
+  template void unpack_cols<K>(const Exec&, const K*, u64, int, int, u32* const*);                      \
+  template void member<K>(const Exec&, const K*, u64, const K*, u64, u8*);
+
+SK_INST_K(u64)
+SK_INST_K(u128)
+template void repack<u64, u64>(const Exec&, const u64*, u64, int, int, const int*, u64*);
+template void repack<u64, u128>(const Exec&, const u64*, u64, int, int, const int*, u128*);
+template void repack<u128, u64>(const Exec&, const u128*, u64, int, int, const int*, u64*);
+template void repack<u128, u128>(const Exec&, const u128*, u64, int, int, const int*, u128*);
+
+// ======================================================= pool and timer
+
+u64 Pool::size_class(u64 b) {
+  if (b <= 256) return 256;
+  // classes at 1/4 steps between powers of two
+  int lg = 63 - __builtin_clzll(b - 1);
+  u64 base = 1ull << lg;
+  u64 step = base / 4;
+  return (b + step - 1) / step * step;
+}
+
+void* Pool::alloc(u64 bytes) {
+  u64 c = size_class(bytes);
+  auto it = free_.lower_bound(c);
+  if (it != free_.end() && it->first <= c + c / 4) {
+    void* p = it->second;
+    u64 sz = it->first;
+    free_.erase(it);
+    live_[p] = sz;
+    in_use_ += sz;
+    return p;
+  }
+  if (held_ + c > budget_) trim();
+  if (held_ + c > budget_) throw BudgetError{bytes};
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim();
+    e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw BudgetError{bytes};
+    }
+  }
+  live_[p] = c;
+  held_ += c;
+  in_use_ += c;
+  if (held_ > peak_) peak_ = held_;
+  return p;
+}
+
+void Pool::free(void* p) {
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  in_use_ -= it->second;
+  free_.emplace(it->second, p);
+  live_.erase(it);
+}
+
+void Pool::trim() {
+  if (free_.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_) {
+    (void)hipFree(kv.second);
+    held_ -= kv.first;
+  }
+  free_.clear();
+}
+
+Pool::~Pool() {
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  for (auto& kv : live_) (void)hipFree(kv.first);
+}
+
+hipEvent_t Timer::get_event() {
+  if (!spare_.empty()) {
+    hipEvent_t e = spare_.back();
+    spare_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  SK_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Timer::begin(const char* name, int cls, double bytes, u64 items) {
+  if (open_) end();
+  cur_ = Pending{get_event(), nullptr, name, cls, bytes, items};
+  SK_HIP(hipEventRecord(cur_.a, stream));
+  open_ = true;
+}
+
+void Timer::end() {
+  if (!open_) return;
+  cur_.b = get_event();
+  SK_HIP(hipEventRecord(cur_.b, stream));
+  pending_.push_back(cur_);
+  open_ = false;
+  if (pending_.size() > 4096) collect();
+}
+
+void Timer::collect() {
+  if (pending_.empty()) return;
+  SK_HIP(hipStreamSynchronize(stream));
+  for (auto& p : pending_) {
+    float ms = 0;
+    SK_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    stats[p.cls].launches++;
+    stats[p.cls].ms += ms;
+    stats[p.cls].bytes += p.bytes;
+    stats[p.cls].items += p.items;
+    KRecord& r = kernels[p.name];
+    r.name = p.name;
+    r.cls = p.cls;
+    r.launches++;
+    r.ms += ms;
+    r.bytes += p.bytes;
+    r.items += p.items;
+    spare_.push_back(p.a);
+    spare_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+void Timer::reset() {
+  collect();
+  for (auto& st : stats) st = KStat();
+  kernels.clear();
+}
+
+}  // namespace sk
