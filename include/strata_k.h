@@ -183,6 +183,9 @@ sk_status sk_run_add_facts_sym(sk_run* run, const char* pred, uint64_t n, uint32
 sk_status sk_run_load(sk_run* run);
 /* The device fixpoint: every stratum in order (seminaive.rs:52-65). */
 sk_status sk_run_execute(sk_run* run);
+/* Restore the relations to their state right after sk_run_load, so the
+ * fixpoint can be executed again over the same resident EDB. */
+sk_status sk_run_reset(sk_run* run);
 void sk_run_error(const sk_run* run, sk_error_info* out);
 
 /* Results: rows in BTreeMap order (sorted by GroundVal tuple). */

@@ -118,6 +118,8 @@ def lib():
         L.orc_rel_arity.argtypes = [C.c_void_p, C.c_char_p]
         L.orc_rel_copy.restype = C.c_int32
         L.orc_rel_copy.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p]
+        L.orc_tri_count_oriented.restype = C.c_uint64
+        L.orc_tri_count_oriented.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64]
         _lib = L
     return _lib
 
@@ -307,3 +309,10 @@ def run(prog, edb=(), bulk: Optional[dict] = None, semi: bool = True, indexed: b
         return res
     finally:
         L.orc_db_free(db)
+
+
+def tri_count_oriented(a, b, nnodes: int) -> int:
+    """Triangles of an oriented simple graph given as sorted unique pairs."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    return int(lib().orc_tri_count_oriented(a.ctypes.data, b.ctypes.data, len(a), nnodes))

@@ -1247,3 +1247,32 @@ int32_t orc_rel_copy(orc_db* db, const char* pred, orc_val* rows, orc_ann* anns)
     }
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------- triangles */
+
+/* crates/strata-gpu/src/wcoj.rs:27-43 (tri_count) over a given orientation:
+ * the oriented simple graph (a[i] -> b[i], sorted by (a, b), unique) as a CSR
+ * trie, and for every oriented edge (u, v) the size of out(u) ∩ out(v) by a
+ * two-pointer merge.  With any acyclic orientation of a simple graph each
+ * triangle is counted exactly once. */
+uint64_t orc_tri_count_oriented(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t nnodes) {
+    uint64_t* off = (uint64_t*)calloc(nnodes + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; i++) off[a[i] + 1]++;
+    for (uint64_t v = 0; v < nnodes; v++) off[v + 1] += off[v];
+    uint64_t total = 0;
+#pragma omp parallel for reduction(+ : total) schedule(dynamic, 4096)
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t u = a[i], v = b[i];
+        uint64_t p = off[u], pe = off[u + 1], q = off[v], qe = off[v + 1];
+        uint64_t c = 0;
+        while (p < pe && q < qe) {
+            uint32_t x = b[p], y = b[q];
+            if (x < y) p++;
+            else if (x > y) q++;
+            else { c++; p++; q++; }
+        }
+        total += c;
+    }
+    free(off);
+    return total;
+}

@@ -166,6 +166,10 @@ int64_t orc_rel_size(orc_db* db, const char* pred);
 int64_t orc_rel_arity(orc_db* db, const char* pred);
 int32_t orc_rel_copy(orc_db* db, const char* pred, orc_val* rows, orc_ann* anns);
 
+/* Triangle count of an acyclically oriented simple graph (sorted unique
+ * pairs a[i] -> b[i], ids < nnodes): crates/strata-gpu/src/wcoj.rs:27-43. */
+uint64_t orc_tri_count_oriented(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t nnodes);
+
 #ifdef __cplusplus
 }
 #endif

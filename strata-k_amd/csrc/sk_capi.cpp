@@ -199,6 +199,12 @@ sk_status sk_run_execute(sk_run* run) {
   return s;
 }
 
+sk_status sk_run_reset(sk_run* run) {
+  if (!run) return SK_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(run->ctx->c.device);
+  return guarded(&run->err, [&] { run->eng->reset(); });
+}
+
 void sk_run_error(const sk_run* run, sk_error_info* out) {
   if (run) set_info(run->err, out);
 }

@@ -241,8 +241,8 @@ void Engine::stage_rows(const std::string& pred, u64 n, const sk_val* rows, cons
   // appended after any all-Sym stage of the same relation at load time
   if (weighted)
     for (u64 i = 0; i < n; i++) {
-      R.st_w.push_back(w[i]);
       u8 f = inf ? inf[i] : 0;
+      R.st_w.push_back(f ? 0 : w[i]);  // +inf rows carry weight 0, like every derived +inf
       R.st_inf.push_back(f);
       if (f) has_inf_ = true;
     }
@@ -281,8 +281,8 @@ void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* con
   }
   if (weighted)
     for (u64 i = 0; i < n; i++) {
-      R.st_w.push_back(w[i]);
       u8 f = inf ? inf[i] : 0;
+      R.st_w.push_back(f ? 0 : w[i]);  // +inf rows carry weight 0, like every derived +inf
       R.st_inf.push_back(f);
       if (f) has_inf_ = true;
     }
@@ -443,8 +443,86 @@ void Engine::load() {
     int_vals_.reset(x.pool, D_.ints.size());
     SK_HIP(hipMemcpyAsync(int_vals_.get(), D_.ints.data(), D_.ints.size() * 8, hipMemcpyHostToDevice, x.stream));
   }
+  // snapshot relations that rules write (heads), so reset() can restore them
+  snap_.clear();
+  snap_.resize(R_.size());
+  std::vector<bool> written = snapshot_set();
+  for (size_t i = 0; i < R_.size(); i++) {
+    if (!written[i] || R_[i].full.n == 0) continue;
+    const RelData& s = R_[i].full;
+    RelData& d = snap_[i];
+    d.n = s.n;
+    d.k.ensure(x.pool, s.n, R_[i].wide);
+    SK_HIP(hipMemcpyAsync(d.k.ptr(), s.k.ptr(), s.n * (R_[i].wide ? 16 : 8), hipMemcpyDeviceToDevice, x.stream));
+    if (s.w.get()) {
+      d.w.reset(x.pool, s.n);
+      SK_HIP(hipMemcpyAsync(d.w.get(), s.w.get(), s.n * 8, hipMemcpyDeviceToDevice, x.stream));
+    }
+    if (s.inf.get()) {
+      d.inf.reset(x.pool, s.n);
+      SK_HIP(hipMemcpyAsync(d.inf.get(), s.inf.get(), s.n, hipMemcpyDeviceToDevice, x.stream));
+    }
+  }
+  snap_dict_ = D_;
+  snap_bits_.clear();
+  for (auto& R : R_) snap_bits_.push_back(R.bits);
   SK_HIP(hipStreamSynchronize(x.stream));
   loaded = true;
+}
+
+// Relations reset() must restore: those written by rules; with aggregates
+// (which may grow the dictionary and re-pack relations) every relation.
+std::vector<bool> Engine::snapshot_set() const {
+  bool any_agg = false;
+  for (auto& r : P_.rules) any_agg |= r.agg;
+  std::vector<bool> s(R_.size(), any_agg);
+  for (auto& r : P_.rules)
+    if (r.head.rel >= 0) s[r.head.rel] = true;
+  if (!attached_.empty())
+    for (size_t i = 0; i < R_.size(); i++)
+      if (attached_[i]) s[i] = false;
+  return s;
+}
+
+void Engine::reset() {
+  if (!loaded) return;
+  Exec x = ctx_->exec();
+  std::vector<bool> written = snapshot_set();
+  if (snap_dict_.nvals() != D_.nvals()) {
+    D_ = snap_dict_;
+    int_vals_.release();
+    if (!D_.ints.empty()) {
+      int_vals_.reset(x.pool, D_.ints.size());
+      SK_HIP(hipMemcpyAsync(int_vals_.get(), D_.ints.data(), D_.ints.size() * 8, hipMemcpyHostToDevice, x.stream));
+    }
+  }
+  for (size_t i = 0; i < R_.size(); i++) {
+    DRel& R = R_[i];
+    R.delta.release();
+    R.dversion++;
+    if (!written[i]) continue;
+    R.bits = snap_bits_[i];
+    R.wide = key_bits_of(R.arity, R.bits) > 64;
+    const RelData& s = snap_[i];
+    RelData d;
+    d.n = s.n;
+    if (s.n) {
+      d.k.ensure(x.pool, s.n, R.wide);
+      SK_HIP(hipMemcpyAsync(d.k.ptr(), s.k.ptr(), s.n * (R.wide ? 16 : 8), hipMemcpyDeviceToDevice, x.stream));
+      if (s.w.get()) {
+        d.w.reset(x.pool, s.n);
+        SK_HIP(hipMemcpyAsync(d.w.get(), s.w.get(), s.n * 8, hipMemcpyDeviceToDevice, x.stream));
+      }
+      if (s.inf.get()) {
+        d.inf.reset(x.pool, s.n);
+        SK_HIP(hipMemcpyAsync(d.inf.get(), s.inf.get(), s.n, hipMemcpyDeviceToDevice, x.stream));
+      }
+    }
+    R.full = std::move(d);
+    R.version++;
+    R.idx.clear();
+  }
+  SK_HIP(hipStreamSynchronize(x.stream));
 }
 
 // Grow the dictionary with new Int values (aggregate results).  New codes

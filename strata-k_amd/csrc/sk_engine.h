@@ -218,6 +218,9 @@ class Engine {
   void stage_sym(const std::string& pred, u64 n, u32 arity, const u32* const* cols, const i64* w, const u8* inf);
   void load();
   void execute();
+  // Restore every relation to its state right after load (re-run support).
+  void reset();
+  std::vector<bool> snapshot_set() const;
   // results
   int64_t size(const std::string& pred);
   void read(const std::string& pred, sk_val* rows, i64* w, u8* inf, u8* is_w);
@@ -247,6 +250,9 @@ class Engine {
   void set_universe(u64 u) { universe_override_ = u; }
 
  private:
+  std::vector<RelData> snap_;  // post-load state of relations written by rules
+  Dict snap_dict_;
+  std::vector<int> snap_bits_;
   u32 nparts_ = 0;
   std::vector<u64> part_universe_;
   std::vector<EvalErr> part_err_;

@@ -427,9 +427,218 @@ __global__ __launch_bounds__(kBlock) void k_runs_write(const K* __restrict__ key
   }
 }
 
+// ---- weighted runs: segmented minimum without serial run walks
+//
+// Pass 1 (k_runs_sum_w): per chunk, the number of run heads, the minimum of
+// the elements before the chunk's first head (the tail of a run that started
+// earlier) and the minimum of the run still open at the chunk's end.
+// Carry (k_runs_carry): one thread walks the chunks in order and produces,
+// per chunk, the minimum carried into it by a run that started before it,
+// and the number of heads before it.
+// Pass 2 (k_runs_write_w): each 256-element tile does a segmented inclusive
+// min-scan (wave shuffles + LDS across waves) seeded with the carry; the last
+// element of every run writes (key, run minimum) at the run's rank.
+
+struct WMin {
+  i64 w;
+  u8 f;  // +inf
+};
+
+__device__ __forceinline__ WMin wmin(WMin a, WMin b) { return w_less(b.w, b.f, a.w, a.f) ? b : a; }
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_runs_sum_w(const K* __restrict__ keys, const i64* __restrict__ w,
+                                                       const u8* __restrict__ f, u64 n, u64 per,
+                                                       u64* __restrict__ heads, i64* __restrict__ pre_w,
+                                                       u8* __restrict__ pre_f, i64* __restrict__ tail_w,
+                                                       u8* __restrict__ tail_f, u64* __restrict__ first_head) {
+  __shared__ u64 tmp[kWaves];
+  __shared__ u64 s_first[kWaves], s_last[kWaves];
+  __shared__ i64 s_pw[kWaves], s_tw[kWaves];
+  __shared__ u8 s_pf[kWaves], s_tf[kWaves];
+  const u64 lo = (u64)blockIdx.x * per, hi = min(n, lo + per);
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  u64 nh = 0;
+  u64 firsth = hi;              // position of the chunk's first head
+  WMin pre{0, 1}, tail{0, 1};   // +inf identities
+  for (u64 base = lo; base < hi; base += kBlock) {
+    u64 i = base + threadIdx.x;
+    bool valid = i < hi;
+    bool head = valid && (i == 0 || keys[i] != keys[i - 1]);
+    WMin v{valid ? w[i] : 0, (u8)(valid ? (f ? f[i] : 0) : 1)};
+    u64 hb = __ballot(head);
+    if (lane == 0) {
+      s_first[wv] = hb ? base + wv * 64 + __builtin_ctzll(hb) : ~0ull;
+      s_last[wv] = hb ? base + wv * 64 + 63 - __builtin_clzll(hb) : ~0ull;
+    }
+    u64 t;
+    block_excl_scan(head ? 1 : 0, tmp, &t);  // its barriers publish s_first / s_last
+    u64 tf = ~0ull, tl = ~0ull;
+    for (int q = 0; q < kWaves; q++) {
+      if (s_first[q] != ~0ull && tf == ~0ull) tf = s_first[q];
+      if (s_last[q] != ~0ull) tl = s_last[q];
+    }
+    // min of the elements before the tile's first head, and from its last head on
+    WMin pv = (valid && i < tf) ? v : WMin{0, 1};
+    WMin tv = (valid && tl != ~0ull && i >= tl) ? v : WMin{0, 1};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      WMin a{__shfl_xor(pv.w, o, 64), (u8)__shfl_xor((int)pv.f, o, 64)};
+      WMin b{__shfl_xor(tv.w, o, 64), (u8)__shfl_xor((int)tv.f, o, 64)};
+      pv = wmin(pv, a);
+      tv = wmin(tv, b);
+    }
+    if (lane == 0) {
+      s_pw[wv] = pv.w;
+      s_pf[wv] = pv.f;
+      s_tw[wv] = tv.w;
+      s_tf[wv] = tv.f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      WMin pt{0, 1}, tt{0, 1};
+      for (int q = 0; q < kWaves; q++) {
+        pt = wmin(pt, WMin{s_pw[q], s_pf[q]});
+        tt = wmin(tt, WMin{s_tw[q], s_tf[q]});
+      }
+      // elements before the tile's first head extend the run open so far
+      if (nh == 0) pre = wmin(pre, pt);
+      else tail = wmin(tail, pt);
+      if (tf != ~0ull) {
+        if (nh == 0) firsth = tf;
+        tail = tt;
+      }
+    }
+    nh += t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    heads[blockIdx.x] = nh;
+    pre_w[blockIdx.x] = pre.w;
+    pre_f[blockIdx.x] = pre.f;
+    tail_w[blockIdx.x] = nh ? tail.w : pre.w;
+    tail_f[blockIdx.x] = nh ? tail.f : pre.f;
+    first_head[blockIdx.x] = firsth;
+  }
+}
+
+// Sequential over chunks: carry-in minimum and head offsets.  The grand
+// total of heads lands in heads[nb].
+__global__ void k_runs_carry(u64* heads, const i64* pre_w, const u8* pre_f, const i64* tail_w, const u8* tail_f,
+                             i64* cin_w, u8* cin_f, u32 nb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  WMin carry{0, 1};
+  u64 acc = 0;
+  for (u32 b = 0; b < nb; b++) {
+    cin_w[b] = carry.w;
+    cin_f[b] = carry.f;
+    u64 h = heads[b];
+    heads[b] = acc;
+    acc += h;
+    if (h) carry = WMin{tail_w[b], tail_f[b]};
+    else carry = wmin(carry, WMin{pre_w[b], pre_f[b]});
+  }
+  heads[nb] = acc;
+}
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_runs_write_w(const K* __restrict__ keys, const i64* __restrict__ w,
+                                                         const u8* __restrict__ f, u64 n, u64 per,
+                                                         const u64* __restrict__ offs, const i64* __restrict__ cin_w,
+                                                         const u8* __restrict__ cin_f, K* __restrict__ ok,
+                                                         i64* __restrict__ ow, u8* __restrict__ of) {
+  __shared__ u64 tmp[kWaves];
+  __shared__ i64 s_ww[kWaves];
+  __shared__ u8 s_wf[kWaves], s_wh[kWaves];
+  const u64 lo = (u64)blockIdx.x * per, hi = min(n, lo + per);
+  u64 carry_h = offs[blockIdx.x];  // heads before this tile
+  WMin carry{cin_w[blockIdx.x], cin_f[blockIdx.x]};
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (u64 base = lo; base < hi; base += kBlock) {
+    u64 i = base + threadIdx.x;
+    bool valid = i < hi;
+    K k = valid ? keys[i] : (K)0;
+    bool head = valid && (i == 0 || keys[i - 1] != k);
+    bool last = valid && (i + 1 == n || keys[i + 1] != k);
+    i64 vw = valid ? w[i] : 0;
+    u8 vf = valid ? (f ? f[i] : (u8)0) : (u8)1;
+    u8 fl = head ? 1 : 0;
+    // wave segmented inclusive min-scan
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      i64 pw = __shfl_up(vw, o, 64);
+      int pf = __shfl_up((int)vf, o, 64);
+      int ph = __shfl_up((int)fl, o, 64);
+      if (lane >= (u32)o) {
+        if (!fl && w_less(pw, (u8)pf, vw, vf)) {
+          vw = pw;
+          vf = (u8)pf;
+        }
+        fl |= (u8)ph;
+      }
+    }
+    if (lane == 63) {
+      s_ww[wv] = vw;
+      s_wf[wv] = vf;
+      s_wh[wv] = fl;
+    }
+    u64 tot;
+    u64 ex = block_excl_scan(head ? 1 : 0, tmp, &tot);  // barriers publish s_ww / s_wf / s_wh
+    // carry into this wave: the tile carry combined with earlier waves
+    WMin c = carry;
+    for (u32 q = 0; q < wv; q++) {
+      if (s_wh[q]) c = WMin{s_ww[q], s_wf[q]};
+      else c = wmin(c, WMin{s_ww[q], s_wf[q]});
+    }
+    WMin mine{vw, vf};
+    if (!fl) mine = wmin(c, mine);
+    if (last) {
+      // run rank = heads before this element, inclusive of its own head
+      u64 o = carry_h + ex + (head ? 1 : 0) - 1;
+      ok[o] = k;
+      ow[o] = mine.w;
+      if (of) of[o] = mine.f;
+    }
+    // tile carry: the state after the tile's last element
+    WMin nc = carry;
+    for (u32 q = 0; q < (u32)kWaves; q++) {
+      if (s_wh[q]) nc = WMin{s_ww[q], s_wf[q]};
+      else nc = wmin(nc, WMin{s_ww[q], s_wf[q]});
+    }
+    __syncthreads();
+    carry = nc;
+    carry_h += tot;
+  }
+}
+
 template <class K>
 u64 unique_runs(const Exec& x, const K* keys, const i64* w, const u8* inf, u64 n, K* ok, i64* ow, u8* of) {
   if (n == 0) return 0;
+  if (w) {
+    u64 nb = (n + kScanTile - 1) / kScanTile;
+    if (nb > kScanMaxBlocks) nb = kScanMaxBlocks;
+    u64 per = (n + nb - 1) / nb;
+    per = (per + kBlock - 1) / kBlock * kBlock;
+    nb = (n + per - 1) / per;
+    DBuf<u64> heads(x.pool, nb + 1), firsth(x.pool, nb);
+    DBuf<i64> pw(x.pool, nb), tw(x.pool, nb), cw(x.pool, nb);
+    DBuf<u8> pf(x.pool, nb), tf(x.pool, nb), cf(x.pool, nb);
+    const double rec = sizeof(K) + 8.0 + (inf ? 1.0 : 0);
+    {
+      Scope sc(x.timer, "runs_count", KC_DEDUP, (double)n * rec, n);
+      k_runs_sum_w<K><<<(unsigned)nb, kBlock, 0, x.stream>>>(keys, w, inf, n, per, heads.get(), pw.get(), pf.get(),
+                                                             tw.get(), tf.get(), firsth.get());
+    }
+    k_runs_carry<<<1, 64, 0, x.stream>>>(heads.get(), pw.get(), pf.get(), tw.get(), tf.get(), cw.get(), cf.get(),
+                                         (u32)nb);
+    {
+      Scope sc(x.timer, "runs_write", KC_DEDUP, (double)n * rec, n);
+      k_runs_write_w<K><<<(unsigned)nb, kBlock, 0, x.stream>>>(keys, w, inf, n, per, heads.get(), cw.get(), cf.get(),
+                                                               ok, ow, of);
+    }
+    SK_HIP(hipGetLastError());
+    return read_u64(x, heads.get() + nb);
+  }
   u64 nb = (n + kScanTile - 1) / kScanTile;
   if (nb > kScanMaxBlocks) nb = kScanMaxBlocks;
   u64 per = (n + nb - 1) / nb;

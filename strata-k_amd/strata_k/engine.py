@@ -140,6 +140,7 @@ SIGNATURES = [
     ("sk_run_add_facts_sym", C.c_int, [_VP, C.c_char_p, _U64, _U32, _VP, _VP, _VP]),
     ("sk_run_load", C.c_int, [_VP]),
     ("sk_run_execute", C.c_int, [_VP]),
+    ("sk_run_reset", C.c_int, [_VP]),
     ("sk_run_error", None, [_VP, C.POINTER(SkErrorInfo)]),
     ("sk_result_size", _I64, [_VP, C.c_char_p]),
     ("sk_result_arity", _I64, [_VP, C.c_char_p]),
@@ -334,9 +335,17 @@ class Context:
         if code:
             _raise(code, {"message": "sk_ctx_create failed"})
         self.h = h
+        import weakref
+        self._children = weakref.WeakSet()  # runs, batches and servers on this context
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def close(self):
         if self.h:
+            # every object living on the context is released before it
+            for ch in list(self._children):
+                ch.free()
             self.L.sk_ctx_destroy(self.h)
             self.h = None
 
@@ -400,12 +409,13 @@ class Run:
             self.L.sk_ctx_error(ctx.h, C.byref(info))
             _raise(code, _err_info(info))
         self.h = h
+        ctx._adopt(self)
         self._keep = []
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.L.sk_run_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -476,6 +486,10 @@ class Run:
 
     def execute(self):
         self._check(self.L.sk_run_execute(self.h))
+
+    def reset(self):
+        """Restore the post-load relation state (re-run over the resident EDB)."""
+        self._check(self.L.sk_run_reset(self.h))
 
     @property
     def execute_ms(self) -> float:
@@ -664,6 +678,7 @@ class Batch:
         if code:
             _raise(code)
         self.h = h
+        ctx._adopt(self)
         self.progs = []
         self._keep = []
 
@@ -736,9 +751,9 @@ class Batch:
         return [c[:n] for c in cols], (w[:n] if trop else None)
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.L.sk_batch_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -764,6 +779,7 @@ class Server:
         if code:
             _raise(code)
         self.h = h
+        ctx._adopt(self)
 
     def submit(self, sources, targets=()) -> int:
         s = np.ascontiguousarray(np.asarray(sources, dtype=np.uint32))
@@ -805,9 +821,9 @@ class Server:
         self.L.sk_server_release(self.h, ticket)
 
     def free(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.L.sk_server_free(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -1,0 +1,171 @@
+"""GPU: batched evaluation and query serving, each answer bit-exact against
+the plain single-program run and the oracle."""
+import numpy as np
+import pytest
+
+import strata_k as sk
+from oracle import oracle as O
+from strata_k import workloads as wl
+from strata_k.core_ir import INF, Sym, cost_program, tc_program, TROP
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_cols(prog, bulk, out):
+    b = {k: (v[0], v[1], None) for k, v in bulk.items()}
+    r = O.run(prog, bulk=b, want=[out])
+    assert r.ok, r.status
+    rel = r.relations[out]
+    return [rel.vals[:, j] for j in range(rel.arity)], (rel.w if prog.predicates[-1].semiring == TROP else None), \
+        r.iterations
+
+
+def test_batch_matches_plain_and_oracle(ctx):
+    progs = wl.batch_programs(seed=44, count=24, n_lo=30, n_hi=600, m_lo=60, m_hi=2500, cost_sources=2)
+    b = sk.Batch(ctx)
+    for p in progs:
+        b.add(p.program, p.bulk)
+    b.load()
+    b.execute()
+    for i, p in enumerate(progs):
+        code, info = b.status(i)
+        assert code == 0, (i, info)
+        cols, w = b.relation_sym(i, p.output)
+        ocols, ow, oit = oracle_cols(p.program, p.bulk, p.output)
+        assert len(cols[0]) == len(ocols[0]), i
+        for c, oc in zip(cols, ocols):
+            assert (c == oc).all(), i
+        if w is not None:
+            assert (w == ow).all(), i
+        assert b.iterations(i) == oit, (i, b.iterations(i), oit)
+        # the same program alone through the plain engine
+        db = ctx.run_semi_naive(p.program, bulk={k: (v[0], v[1], None) for k, v in p.bulk.items()}, want=[p.output])
+        assert db.iterations == oit
+    b.free()
+
+
+def test_batch_isolates_errors(ctx):
+    """A program that fails (negative cycle, overflow) stops alone; the others
+    are unaffected (each keeps its own cap universe+2, seminaive.rs:94-103)."""
+    good = wl.batch_programs(seed=45, count=6, n_lo=20, n_hi=200, m_lo=40, m_hi=600)
+    b = sk.Batch(ctx)
+    idx = {}
+    for p in good:
+        b.add(p.program, p.bulk)
+    # negative cycle: a->b 1, b->a -2 with a source
+    neg = {"edge": ([np.array([0, 1], np.uint32), np.array([1, 0], np.uint32)], np.array([1, -2], np.int64)),
+           "src": ([np.array([0], np.uint32)], np.array([0], np.int64))}
+    idx["neg"] = b.add(cost_program(), neg)
+    ovf = {"edge": ([np.array([0, 1], np.uint32), np.array([1, 2], np.uint32)], np.array([2**63 - 1, 1], np.int64)),
+           "src": ([np.array([0], np.uint32)], np.array([0], np.int64))}
+    idx["ovf"] = b.add(cost_program(), ovf)
+    b.load()
+    b.execute()
+    assert b.status(idx["neg"])[0] == 6  # NegativeWeightCycle
+    code, info = b.status(idx["ovf"])
+    assert code == 2 and (info["lhs"], info["rhs"]) == (2**63 - 1, 1)
+    for i, p in enumerate(good):
+        assert b.status(i)[0] == 0
+        cols, w = b.relation_sym(i, p.output)
+        ocols, ow, oit = oracle_cols(p.program, p.bulk, p.output)
+        assert all((c == oc).all() for c, oc in zip(cols, ocols))
+        assert b.iterations(i) == oit
+    b.free()
+
+
+def _server_graph():
+    s, d = wl.rmat_graph(51, 12, 60_000)
+    w = wl.weights(51, len(s))
+    return s, d, w, 1 << 12
+
+
+def _oracle_answer(s, d, w, q):
+    prog = wl.query_program(len(q["targets"]) > 0)
+    bulk = {"edge": ([s, d], w, None), "src": ([q["sources"]], np.zeros(len(q["sources"]), np.int64), None)}
+    out = "cost"
+    if len(q["targets"]):
+        bulk["tgt"] = ([q["targets"]], np.zeros(len(q["targets"]), np.int64), None)
+        out = "ans"
+    r = O.run(prog, bulk=bulk, want=[out, "cost"])
+    rel = r.relations[out]
+    return rel.vals[:, 0], rel.vals[:, 1], rel.w, r.iterations[0]
+
+
+def test_wide_trop_keys(ctx):
+    """Trop relations whose packed tuples exceed 64 bits (arity 3, ids >= 2^22):
+    the fused cost(Q,S,Z) shape the server uses, through the general engine."""
+    from strata_k.core_ir import CoreProgram, atom, pos, pred, rule, v
+    prog = CoreProgram(
+        [pred("e", 2, TROP), pred("s", 2, TROP), pred("c", 3, TROP)],
+        [rule(atom("c", v(0), v(1), v(2)), [pos("s", v(0), v(1)), pos("e", v(1), v(2))], 3),
+         rule(atom("c", v(0), v(1), v(3)), [pos("c", v(0), v(1), v(2)), pos("e", v(2), v(3))], 4)], 1)
+    s, d = wl.rmat_graph(61, 10, 8000)
+    scale = 4099  # spread ids up to ~2^22
+    w = wl.weights(61, len(s))
+    rng = np.random.default_rng(5)
+    q = rng.integers(0, 5, size=12).astype(np.uint32)
+    src = (rng.integers(0, 1024, size=12) * scale).astype(np.uint32)
+    bulk = {"e": ([s * scale, d * scale], w, None), "s": ([q, src], np.zeros(12, np.int64), None)}
+    ref = O.run(prog, bulk=bulk, want=["c"])
+    db = ctx.run_semi_naive(prog, bulk=bulk, want=["c"])
+    g, r = db.relation("c"), ref.relations["c"]
+    assert len(g) == len(r)
+    assert (g.vals == r.vals).all()
+    assert (g.w == r.w).all()
+    assert db.iterations == ref.iterations
+
+
+def test_server_single_queries(ctx):
+    s, d, w, n = _server_graph()
+    qs = wl.config5_queries(seed=53, n_queries=6, n_nodes=n, max_sources=6, n_targets=16, filter_frac=0.3)
+    for q in qs:
+        srv = sk.Server(ctx, s, d, w, max_batch=4)
+        t = srv.submit(q["sources"], q["targets"])
+        while srv.step()[0]:
+            pass
+        res = srv.poll(t)
+        a, b, ww = srv.take(t, res["rows"])
+        oa, ob, ow, oit = _oracle_answer(s, d, w, q)
+        assert len(a) == len(oa)
+        assert (a == oa).all() and (b == ob).all()
+        assert (ww == ow).all(), np.nonzero(ww != ow)[0][:10]
+        assert res["iterations"] == oit
+        srv.free()
+
+
+def test_server_answers_exact_alone_and_in_mix(ctx):
+    s, d, w, n = _server_graph()
+    qs = wl.config5_queries(seed=52, n_queries=16, n_nodes=n, max_sources=12, n_targets=16, filter_frac=0.25)
+    srv = sk.Server(ctx, s, d, w, max_batch=8)
+    tickets = []
+    # half arrive up front, the rest while the first ones are running
+    for q in qs[:8]:
+        tickets.append(srv.submit(q["sources"], q["targets"]))
+    srv.step()
+    srv.step()
+    for q in qs[8:]:
+        tickets.append(srv.submit(q["sources"], q["targets"]))
+    for _ in range(10_000):
+        a, wt = srv.step()
+        if a == 0 and wt == 0:
+            break
+    mix = []
+    for q, t in zip(qs, tickets):
+        res = srv.poll(t)
+        assert res is not None
+        a, b, ww = srv.take(t, res["rows"])
+        mix.append((a, b, ww, res["iterations"]))
+        oa, ob, ow, oit = _oracle_answer(s, d, w, q)
+        assert (a == oa).all() and (b == ob).all() and (ww == ow).all()
+        assert res["iterations"] == oit
+    srv.free()
+    # each query alone on a fresh server gives the same answer and round count
+    for q, m in zip(qs[:6], mix):
+        srv = sk.Server(ctx, s, d, w, max_batch=8)
+        t = srv.submit(q["sources"], q["targets"])
+        while srv.step()[0]:
+            pass
+        res = srv.poll(t)
+        a, b, ww = srv.take(t, res["rows"])
+        assert (a == m[0]).all() and (b == m[1]).all() and (ww == m[2]).all() and res["iterations"] == m[3]
+        srv.free()
