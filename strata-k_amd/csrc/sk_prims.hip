@@ -667,12 +667,17 @@ u64 unique_runs(const Exec& x, const K* keys, const i64* w, const u8* inf, u64 n
 //
 // The merged order takes full (B) before cand (A) on equal keys, so a match
 // is an adjacent (B, A) pair.  Each tile of kMergeTile merged positions
-// loads its A and B ranges into LDS, every thread merges kMergeItems
-// consecutive positions, and neighbours across threads / tiles are exchanged
-// through LDS and one global look-behind / look-ahead.
+// loads its A range and its B range into one LDS buffer, each range with a
+// one-element halo on both sides (the global neighbours of the tile).  Every
+// thread finds its split of the tile by merge path, reads the element just
+// before its first position from the split itself, and then streams its
+// positions in order: an element's union row is decided when the element
+// after it is known, its fresh row as soon as it arrives.  Nothing crosses
+// threads but the block scan of the row counts.
 
 constexpr int kMergeItems = 8;
 constexpr u64 kMergeTile = (u64)kBlock * kMergeItems;  // 2048
+constexpr u32 kMergeSlots = (u32)kMergeTile + 4;       // A and B ranges plus four halo slots
 
 template <class K>
 __device__ __forceinline__ u64 merge_path(const K* a, u64 m, const K* b, u64 n, u64 d) {
@@ -694,13 +699,72 @@ __global__ void k_merge_partition(const K* __restrict__ a, u64 m, const K* __res
   part[t] = merge_path(a, m, b, n, d);
 }
 
-template <class K>
-struct MItem {
-  K key;
-  i64 w;
-  u8 f;
-  u8 src;  // 0 = cand (A), 1 = full (B)
-};
+// One thread's walk over `cnt` merged positions starting at tile-local split
+// (li, lj).  Slot layout of the LDS buffer: A[x] is slot 1 + x and B[x] is
+// slot boff + 1 + x, for x in [-1, na] resp. [-1, nb]; a halo slot is valid
+// only when its global index lies inside the input.
+template <class K, bool HASW, class FU, class FF>
+__device__ __forceinline__ void merge_walk(const K* sk, const i64* sw, const u8* sf, u32 boff, u32 na, u32 nb,
+                                           u64 i0, u64 j0, u64 m, u64 n, u32 li, u32 lj, u32 cnt, FU&& on_union,
+                                           FF&& on_fresh) {
+  if (cnt == 0) return;
+  // the merged element just before the first position: the later of
+  // A[li-1] and B[lj-1] (A is later on equal keys)
+  bool haslast = false;
+  K lk{};
+  i64 lw = 0;
+  u8 lf = 0;
+  {
+    bool va = i0 + li > 0, vb = j0 + lj > 0;
+    if (va || vb) {
+      bool pick_a = va && (!vb || !(sk[li] < sk[boff + lj]));
+      u32 s = pick_a ? li : boff + lj;
+      lk = sk[s];
+      if constexpr (HASW) { lw = sw[s]; lf = sf[s]; }
+      haslast = true;
+    }
+  }
+  bool held = false;
+  K hk{};
+  i64 hw = 0;
+  u8 hf = 0;
+#pragma unroll
+  for (int k = 0; k < kMergeItems; k++) {
+    if ((u32)k < cnt) {
+      bool take_b = (lj < nb) && (li >= na || !(sk[1 + li] < sk[boff + 1 + lj]));
+      u32 s = take_b ? boff + 1 + lj : 1 + li;
+      K ck = sk[s];
+      i64 cw = 0;
+      u8 cf = 0;
+      if constexpr (HASW) { cw = sw[s]; cf = sf[s]; }
+      if (take_b) lj++;
+      else li++;
+      bool dup = haslast && lk == ck;
+      if (held && !dup) on_union(hk, hw, hf);
+      i64 uw = cw;
+      u8 uf = cf;
+      bool fresh = !dup;
+      if constexpr (HASW) {
+        if (dup && w_less(lw, lf, cw, cf)) { uw = lw; uf = lf; }
+        if (dup && w_less(cw, cf, lw, lf)) fresh = true;
+      }
+      if (!take_b && fresh) on_fresh(ck, cw, cf);
+      held = true;
+      hk = ck; hw = uw; hf = uf;
+      lk = ck; lw = cw; lf = cf;
+      haslast = true;
+    }
+  }
+  // the merged element just after the last position: the earlier of A[li]
+  // and B[lj] (B is earlier on equal keys)
+  bool va = i0 + li < m, vb = j0 + lj < n;
+  bool dup_next = false;
+  if (va || vb) {
+    bool pick_b = vb && (!va || !(sk[1 + li] < sk[boff + 1 + lj]));
+    dup_next = (pick_b ? sk[boff + 1 + lj] : sk[1 + li]) == hk;
+  }
+  if (!dup_next) on_union(hk, hw, hf);
+}
 
 // PASS 0: count union / fresh rows per tile.  PASS 1: write them.
 template <class K, bool HASW, int PASS>
@@ -710,14 +774,9 @@ __global__ __launch_bounds__(kBlock) void k_merge(
     const u64* __restrict__ part, u64* __restrict__ ucnt, u64* __restrict__ fcnt,
     K* __restrict__ uk, i64* __restrict__ uw, u8* __restrict__ uf,
     K* __restrict__ fk, i64* __restrict__ fw, u8* __restrict__ ff) {
-  __shared__ K sa[kMergeTile];
-  __shared__ K sb[kMergeTile];
-  __shared__ i64 saw[HASW ? kMergeTile : 1];
-  __shared__ i64 sbw[HASW ? kMergeTile : 1];
-  __shared__ u8 saf[HASW ? kMergeTile : 1];
-  __shared__ u8 sbf[HASW ? kMergeTile : 1];
-  __shared__ MItem<K> edge_first[kBlock];
-  __shared__ MItem<K> edge_last[kBlock];
+  __shared__ K sk[kMergeSlots];
+  __shared__ i64 sw[HASW ? kMergeSlots : 1];
+  __shared__ u8 sf[HASW ? kMergeSlots : 1];
   __shared__ u64 tmp[kWaves];
 
   const u64 t = blockIdx.x;
@@ -725,155 +784,69 @@ __global__ __launch_bounds__(kBlock) void k_merge(
   const u64 i0 = part[t], i1 = part[t + 1];
   const u64 j0 = d0 - i0, j1 = d1 - i1;
   const u32 na = (u32)(i1 - i0), nb = (u32)(j1 - j0);
+  const u32 boff = na + 2;
   const bool has_f = HASW && (af != nullptr);
-  for (u32 q = threadIdx.x; q < na; q += kBlock) {
-    sa[q] = a[i0 + q];
-    if constexpr (HASW) {
-      saw[q] = aw[i0 + q];
-      saf[q] = has_f ? af[i0 + q] : (u8)0;
+  // slot s of the A range holds a[i0 - 1 + s]; of the B range b[j0 - 1 + s]
+  for (u32 s = threadIdx.x; s < na + 2; s += kBlock) {
+    u64 g = i0 + s - 1;  // wraps past m for s == 0 when i0 == 0
+    if (g < m) {
+      sk[s] = a[g];
+      if constexpr (HASW) {
+        sw[s] = aw[g];
+        sf[s] = has_f ? af[g] : (u8)0;
+      }
     }
   }
-  for (u32 q = threadIdx.x; q < nb; q += kBlock) {
-    sb[q] = b[j0 + q];
-    if constexpr (HASW) {
-      sbw[q] = bw[j0 + q];
-      sbf[q] = has_f ? bf[j0 + q] : (u8)0;
+  for (u32 s = threadIdx.x; s < nb + 2; s += kBlock) {
+    u64 g = j0 + s - 1;
+    if (g < n) {
+      sk[boff + s] = b[g];
+      if constexpr (HASW) {
+        sw[boff + s] = bw[g];
+        sf[boff + s] = has_f ? bf[g] : (u8)0;
+      }
     }
   }
   __syncthreads();
 
-  // thread-local merge of kMergeItems positions
   const u32 tn = (u32)(d1 - d0);
-  u32 ld = min((u32)threadIdx.x * kMergeItems, tn);
-  u32 lhi = min(ld + kMergeItems, tn);
-  u32 cnt = lhi - ld;
-  u32 li = (u32)merge_path<K>(sa, na, sb, nb, ld);
-  u32 lj = ld - li;
-  MItem<K> it[kMergeItems];
-#pragma unroll
-  for (int k = 0; k < kMergeItems; k++) {
-    if ((u32)k < cnt) {
-      bool takeB = (lj < nb) && (li >= na || !(sa[li] < sb[lj]));
-      if (takeB) {
-        it[k].key = sb[lj];
-        it[k].src = 1;
-        if constexpr (HASW) { it[k].w = sbw[lj]; it[k].f = sbf[lj]; }
-        lj++;
-      } else {
-        it[k].key = sa[li];
-        it[k].src = 0;
-        if constexpr (HASW) { it[k].w = saw[li]; it[k].f = saf[li]; }
-        li++;
-      }
-    }
-  }
-  if (cnt) {
-    edge_first[threadIdx.x] = it[0];
-    edge_last[threadIdx.x] = it[cnt - 1];
-  }
-  __syncthreads();
-
-  // neighbours of this thread's first and last item
-  bool has_prev = false, has_next = false;
-  MItem<K> prev, next;
-  if (cnt) {
-    if (threadIdx.x > 0) {
-      prev = edge_last[threadIdx.x - 1];
-      has_prev = true;
-    } else if (d0 > 0) {
-      // merged element d0-1: the later (in merge order) of a[i0-1], b[j0-1]
-      bool ha = i0 > 0, hb = j0 > 0;
-      bool pickA = ha && (!hb || !(a[i0 - 1] < b[j0 - 1]));
-      if (pickA) {
-        prev.key = a[i0 - 1]; prev.src = 0;
-        if constexpr (HASW) { prev.w = aw[i0 - 1]; prev.f = has_f ? af[i0 - 1] : (u8)0; }
-      } else {
-        prev.key = b[j0 - 1]; prev.src = 1;
-        if constexpr (HASW) { prev.w = bw[j0 - 1]; prev.f = has_f ? bf[j0 - 1] : (u8)0; }
-      }
-      has_prev = true;
-    }
-    u32 nthr = (tn + kMergeItems - 1) / kMergeItems;
-    if (threadIdx.x + 1 < nthr) {
-      next = edge_first[threadIdx.x + 1];
-      has_next = true;
-    } else if (d1 < m + n) {
-      // merged element d1: the earlier of a[i1], b[j1] (B first on ties)
-      bool ha = i1 < m, hb = j1 < n;
-      bool pickB = hb && (!ha || !(a[i1] < b[j1]));
-      if (pickB) { next.key = b[j1]; next.src = 1; }
-      else { next.key = a[i1]; next.src = 0; }
-      has_next = true;
-    }
-  }
+  const u32 ld = min((u32)threadIdx.x * kMergeItems, tn);
+  const u32 cnt = min(ld + (u32)kMergeItems, tn) - ld;
+  const u32 li = (u32)merge_path<K>(sk + 1, na, sk + boff + 1, nb, ld);
+  const u32 lj = ld - li;
 
   u32 ucount = 0, fcount = 0;
-  bool uemit[kMergeItems], femit[kMergeItems];
-  i64 uwv[HASW ? kMergeItems : 1];
-  u8 ufv[HASW ? kMergeItems : 1];
-#pragma unroll
-  for (int k = 0; k < kMergeItems; k++) {
-    uemit[k] = false;
-    femit[k] = false;
-    if ((u32)k >= cnt) continue;
-    bool hp = k > 0 || has_prev;
-    const MItem<K>& pv = k > 0 ? it[k > 0 ? k - 1 : 0] : prev;
-    bool hn = (u32)(k + 1) < cnt || has_next;
-    K nk = (u32)(k + 1) < cnt ? it[(k + 1) < kMergeItems ? k + 1 : k].key : next.key;
-    bool dup_prev = hp && pv.key == it[k].key;
-    bool dup_next = hn && nk == it[k].key;
-    if (!dup_next) {
-      uemit[k] = true;
-      if constexpr (HASW) {
-        i64 ww = it[k].w;
-        u8 ff2 = it[k].f;
-        if (dup_prev && w_less(pv.w, pv.f, ww, ff2)) { ww = pv.w; ff2 = pv.f; }
-        uwv[k] = ww;
-        ufv[k] = ff2;
-      }
-    }
-    if (it[k].src == 0) {
-      bool fresh = !dup_prev;
-      if constexpr (HASW) {
-        if (dup_prev && w_less(it[k].w, it[k].f, pv.w, pv.f)) fresh = true;
-      }
-      femit[k] = fresh;
-    }
-    ucount += uemit[k];
-    fcount += femit[k];
-  }
-
-  if (PASS == 0) {
-    u64 ut, ft;
-    block_excl_scan(ucount, tmp, &ut);
-    block_excl_scan(fcount, tmp, &ft);
+  merge_walk<K, HASW>(sk, sw, sf, boff, na, nb, i0, j0, m, n, li, lj, cnt,
+                      [&](K, i64, u8) { ucount++; }, [&](K, i64, u8) { fcount++; });
+  u64 ut, ft;
+  u64 uo = block_excl_scan(ucount, tmp, &ut);
+  u64 fo = block_excl_scan(fcount, tmp, &ft);
+  if constexpr (PASS == 0) {
     if (threadIdx.x == 0) {
       ucnt[t] = ut;
       fcnt[t] = ft;
     }
   } else {
-    u64 ut, ft;
-    u64 uo = block_excl_scan(ucount, tmp, &ut) + ucnt[t];
-    u64 fo = block_excl_scan(fcount, tmp, &ft) + fcnt[t];
-#pragma unroll
-    for (int k = 0; k < kMergeItems; k++) {
-      if (uemit[k] && uk) {
-        uk[uo] = it[k].key;
-        if constexpr (HASW) {
-          uw[uo] = uwv[k];
-          if (uf) uf[uo] = ufv[k];
-        }
-        uo++;
-      }
-      if (femit[k]) {
-        fk[fo] = it[k].key;
-        if constexpr (HASW) {
-          fw[fo] = it[k].w;
-          if (ff) ff[fo] = it[k].f;
-        }
-        fo++;
-      }
-    }
+    uo += ucnt[t];
+    fo += fcnt[t];
+    merge_walk<K, HASW>(
+        sk, sw, sf, boff, na, nb, i0, j0, m, n, li, lj, cnt,
+        [&](K key, i64 w, u8 f) {
+          uk[uo] = key;
+          if constexpr (HASW) {
+            uw[uo] = w;
+            if (uf) uf[uo] = f;
+          }
+          uo++;
+        },
+        [&](K key, i64 w, u8 f) {
+          fk[fo] = key;
+          if constexpr (HASW) {
+            fw[fo] = w;
+            if (ff) ff[fo] = f;
+          }
+          fo++;
+        });
   }
 }
 
