@@ -31,14 +31,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def dominant_kernel(run):
-    recs = run.kernel_records()
-    if not recs:
-        return None, recs
-    top = max(recs, key=lambda r: r["ms"])
-    return top, recs
-
-
 def roofline_of(top, traffic=None):
     if not top or top["ms"] <= 0:
         return None
@@ -53,7 +45,7 @@ def roofline_of(top, traffic=None):
 
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprof PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
     try:
         with open(path) as f:
             d = json.load(f)
@@ -78,16 +70,40 @@ def cpu_baseline_cfg2(wl_full, k_sources=1):
 
 
 def run_steps(ctx, run, steps, warmup):
+    """Times `steps` fixpoints.  Every launch in the timed steps is bracketed by
+    HIP events on the engine's stream (no host synchronisation between
+    launches); the per-kernel records are summed over the timed steps."""
     for _ in range(warmup):
         run.reset()
         run.execute()
     ctx.synchronize()
+    ctx.set_profiling(True)
+    acc = {}
     t0 = time.perf_counter()
     for _ in range(steps):
         run.reset()
         run.execute()
+        for r in run.kernel_records():
+            a = acc.setdefault(r["name"], dict(r, launches=0, ms=0.0, bytes=0.0, items=0))
+            for k in ("launches", "ms", "bytes", "items"):
+                a[k] += r[k]
     ctx.synchronize()
-    return (time.perf_counter() - t0) * 1e3 / max(1, steps)
+    ms = (time.perf_counter() - t0) * 1e3 / max(1, steps)
+    ctx.set_profiling(False)
+    return ms, list(acc.values())
+
+
+def copy_probe(mib):
+    """One device-to-device pass of `mib` MiB through an elementwise kernel
+    (int64 negation: reads and writes exactly mib MiB each), the known-size
+    launch that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.int64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy probe: {mib} MiB read, {mib} MiB written")
 
 
 def bench_cfg2(args):
@@ -103,18 +119,18 @@ def bench_cfg2(args):
     t = time.time()
     run.load()
     log(f"EDB resident in HBM ({time.time() - t:.2f}s incl. H2D + device sort)")
-    ms = run_steps(ctx, run, args.steps, args.warmup)
+    ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n_cost = run.size("cost")
     iters = run.iterations()
-    # profiled step (events around each launch) for the roofline
-    ctx.set_profiling(True)
-    run.reset()
-    run.execute()
-    ctx.set_profiling(False)
-    top, recs = dominant_kernel(run)
-    stats = run.stats()
+    top = max(recs, key=lambda r: r["ms"]) if recs else None
     roof = roofline_of(top, measured_traffic(top["name"]) if top else None)
-    all_kernel_ms = stats["all"]["ms"]
+    stats = {}
+    for r in recs:
+        c = stats.setdefault(r["cls"], {"ms": 0.0, "bytes": 0.0})
+        c["ms"] += r["ms"] / args.steps
+        c["bytes"] += r["bytes"] / args.steps
+    all_kernel_ms = sum(c["ms"] for c in stats.values())
+    all_kernel_bytes = sum(c["bytes"] for c in stats.values())
     line = {
         "metric": "derived tuples/s", "value": round(n_cost / (ms / 1e3), 1), "unit": "tuples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -124,9 +140,9 @@ def bench_cfg2(args):
                                f"{w.params['n_edges']} edges), 64 sources",
                    "derived_tuples": n_cost, "semi_naive_rounds": iters[0], "seed": args.seed},
         "roofline": roof,
-        "kernel_ms_share": {k: round(v["ms"], 3) for k, v in stats.items() if k != "all"},
+        "kernel_ms_per_step": {k: round(v["ms"], 3) for k, v in stats.items()},
         "kernel_ms_total": round(all_kernel_ms, 3),
-        "hbm_gbps_all_kernels": round(stats["all"]["bytes"] / (all_kernel_ms * 1e-3) / 1e9, 1) if all_kernel_ms else None,
+        "hbm_gbps_all_kernels": round(all_kernel_bytes / (all_kernel_ms * 1e-3) / 1e9, 1) if all_kernel_ms else None,
     }
     if args.kernels:
         log(json.dumps(sorted(recs, key=lambda r: -r["ms"]), indent=1))
@@ -148,17 +164,13 @@ def bench_cfg1(args):
     run = ctx.prepare(w.program)
     run.add_sym("edge", *w.bulk["edge"][:1], None)
     run.load()
-    ms = run_steps(ctx, run, args.steps, args.warmup)
+    ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n = run.size("path")
-    ctx.set_profiling(True)
-    run.reset()
-    run.execute()
-    ctx.set_profiling(False)
-    top, recs = dominant_kernel(run)
-    st = run.stats()
+    top = max(recs, key=lambda r: r["ms"]) if recs else None
     print(json.dumps({"config": "bool_tc_uniform_2000_10000", "derived_tuples": n, "rounds": run.iterations()[0],
                       "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
-                      "kernel_ms_total": round(st["all"]["ms"], 3), "launches": st["all"]["launches"],
+                      "kernel_ms_per_step": round(sum(r["ms"] for r in recs) / args.steps, 3),
+                      "launches_per_step": sum(r["launches"] for r in recs) // args.steps,
                       "roofline": roofline_of(top)}), flush=True)
 
 
@@ -172,14 +184,10 @@ def bench_cfg3(args):
     run = ctx.prepare(w.program)
     run.add_sym("e", *w.bulk["e"][:1], None)
     run.load()
-    ms = run_steps(ctx, run, args.steps, args.warmup)
+    ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n = run.size("tri")
     ntri = run.relation("ntri").rows()
-    ctx.set_profiling(True)
-    run.reset()
-    run.execute()
-    ctx.set_profiling(False)
-    top, recs = dominant_kernel(run)
+    top = max(recs, key=lambda r: r["ms"]) if recs else None
     if args.kernels:
         log(json.dumps(sorted(recs, key=lambda r: -r["ms"]), indent=1))
     print(json.dumps({"config": "wcoj_triangles_rmat", "params": w.params, "tri": n, "ntri": str(ntri),
@@ -197,11 +205,15 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one known-size copy kernel (PMC counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (per-GPU shards would be "
-              "independent source sets, see DESIGN.md); run with --gpus 1.", file=sys.stderr)
+              "independent source sets); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3}[args.config](args)
 
 
