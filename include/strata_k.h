@@ -186,6 +186,13 @@ sk_status sk_run_execute(sk_run* run);
 /* Restore the relations to their state right after sk_run_load, so the
  * fixpoint can be executed again over the same resident EDB. */
 sk_status sk_run_reset(sk_run* run);
+/* Evaluation path switch (no counterpart in the reference; a tuning and
+ * testing knob).  enabled != 0 (the default): a relation of a recursive
+ * stratum whose column domains multiply out to a bounded number of cells is
+ * ⊕-applied in place in a dense cell array while its stratum runs;
+ * enabled == 0: every relation uses the sorted-list path (sort, ⊕-unique,
+ * merge).  Results, round counts and errors are identical either way. */
+sk_status sk_run_set_dense(sk_run* run, int enabled);
 void sk_run_error(const sk_run* run, sk_error_info* out);
 
 /* Results: rows in BTreeMap order (sorted by GroundVal tuple). */

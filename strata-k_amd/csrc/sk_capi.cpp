@@ -205,6 +205,12 @@ sk_status sk_run_reset(sk_run* run) {
   return guarded(&run->err, [&] { run->eng->reset(); });
 }
 
+sk_status sk_run_set_dense(sk_run* run, int enabled) {
+  if (!run) return SK_ERR_INVALID_ARGUMENT;
+  run->eng->set_dense(enabled != 0);
+  return SK_OK;
+}
+
 void sk_run_error(const sk_run* run, sk_error_info* out) {
   if (run) set_info(run->err, out);
 }

@@ -1,0 +1,138 @@
+// sk_cells.h — dense cell state of a relation while its stratum runs.
+//
+// A relation whose column domains multiply out to a modest number of cells
+// is kept as one cell per possible tuple instead of a sorted key list:
+//   Bool  one presence bit per cell;
+//   Trop  one u64 per cell holding the order-preserving code of the least
+//         weight (finite w -> w ^ 2^63; kCellTop = absent), plus two rare-case
+//         bitmaps: `maxf` (present with the finite weight INT64_MAX, whose
+//         code collides with kCellTop) and `infb` (present with +inf).
+// Derivations are ⊕-applied in place (atomic min / or); a derivation that
+// strictly improves its cell sets the cell's bit in `touched`.  Every round
+// is one Jacobi application: the rules read last round's delta list, never
+// the cells, so updating in place during the round changes nothing they
+// see; the next delta is the touched set with the cells' final values, which
+// compacts in cell order, i.e. sorted by packed key.
+//
+// Column c of a cell index holds the column value's rank in its domain
+// (identity when dom_bits[c] is null, else the number of domain codes below
+// it: dom_pre[word] + popcount of the lower bits in the word); ranks are
+// monotone in the code, so cell order equals key order.
+#pragma once
+
+#include "sk_dev.h"
+
+namespace sk {
+
+constexpr u64 kCellTop = ~0ull;
+
+struct CellDesc {
+  int on = 0;
+  int trop = 0;
+  int ncols = 0;
+  u64 ncodes = 0;            // dictionary size: every valid code is below it
+  int shift[kMaxCols] = {};  // cell = sum of rank_c << shift[c]
+  int ebits[kMaxCols] = {};  // rank_c < 2^ebits[c]
+  const u32* dom_bits[kMaxCols] = {};  // domain bitmap over codes, or null = identity
+  const u64* dom_pre[kMaxCols] = {};   // per-word count of domain codes in earlier words
+  const u32* dom_list[kMaxCols] = {};  // rank -> code (null = identity)
+  u64* cell = nullptr;     // Trop weight codes
+  u32* present = nullptr;  // Bool presence bits
+  u32* maxf = nullptr;     // Trop: present with finite INT64_MAX
+  u32* infb = nullptr;     // Trop: present with +inf
+  u32* touched = nullptr;  // strictly improved in the current round
+  u32* bad = nullptr;      // set when a value falls outside its column domain
+};
+
+__device__ __forceinline__ u64 cell_enc(i64 w) { return (u64)w ^ (1ull << 63); }
+__device__ __forceinline__ i64 cell_dec(u64 e) { return (i64)(e ^ (1ull << 63)); }
+
+__device__ __forceinline__ bool cell_rank(const CellDesc& cd, int c, u32 code, u64* r) {
+  if (code >= cd.ncodes) return false;
+  const u32* b = cd.dom_bits[c];
+  if (!b) {
+    *r = code;
+    return true;
+  }
+  u32 w = b[code >> 5], m = 1u << (code & 31);
+  if (!(w & m)) return false;
+  *r = cd.dom_pre[c][code >> 5] + (u64)__popc(w & (m - 1));
+  return true;
+}
+
+__device__ __forceinline__ void cell_touch(const CellDesc& cd, u64 c) {
+  atomicOr(&cd.touched[c >> 5], 1u << (c & 31));
+}
+
+// ⊕-apply one derivation to cell c.  Plain loads may be stale within the
+// kernel, but cell states only decrease, so a stale value can only send a
+// derivation to the atomic, which decides.
+__device__ __forceinline__ void cell_insert(const CellDesc& cd, u64 c, i64 w, u8 inf) {
+  const u32 m = 1u << (c & 31);
+  if (!cd.trop) {
+    u32* p = &cd.present[c >> 5];
+    if (*p & m) return;
+    if (!(atomicOr(p, m) & m)) cell_touch(cd, c);
+    return;
+  }
+  unsigned long long* p = (unsigned long long*)&cd.cell[c];
+  if (!inf) {
+    const u64 e = cell_enc(w);
+    if (e != kCellTop) {
+      if (e >= (u64)*p) return;
+      if (e < (u64)atomicMin(p, (unsigned long long)e)) cell_touch(cd, c);
+      return;
+    }
+    // finite INT64_MAX: better only than absent or +inf
+    if ((u64)*p != kCellTop) return;
+    if (!(atomicOr(&cd.maxf[c >> 5], m) & m)) cell_touch(cd, c);
+    return;
+  }
+  // +inf: better only than absent
+  if ((u64)*p != kCellTop) return;
+  if (cd.maxf[c >> 5] & m) return;
+  if (!(atomicOr(&cd.infb[c >> 5], m) & m)) cell_touch(cd, c);
+}
+
+// Cell index of a tuple given its column codes; false (and `bad` set) when a
+// code lies outside its column's domain.
+template <class GetCode>
+__device__ __forceinline__ bool cell_of(const CellDesc& cd, GetCode code_of, u64* cell) {
+  u64 c = 0;
+  for (int j = 0; j < cd.ncols; j++) {
+    u64 r;
+    if (!cell_rank(cd, j, code_of(j), &r) || (r >> cd.ebits[j])) {
+      atomicOr(cd.bad, 1u);
+      return false;
+    }
+    c |= r << cd.shift[j];
+  }
+  *cell = c;
+  return true;
+}
+
+// ----------------------------------------------------------- host wrappers
+
+// Bitmap of the codes appearing in column j of a packed relation.
+template <class K>
+void cells_mark_col(const Exec& x, const K* keys, u64 n, int arity, int bits, int j, u32* bitmap);
+void cells_mark_code(const Exec& x, u32* bitmap, u32 code);
+void cells_or(const Exec& x, u32* dst, const u32* src, u64 nwords);
+// Domain tables of a bitmap: per-word prefix counts and the rank -> code list.
+// Returns the number of codes in the domain.
+u64 cells_domain(const Exec& x, const u32* bitmap, u64 nwords, u64* pre, DBuf<u32>* list);
+// Load a relation's tuples into empty cells (no touched bits).
+template <class K>
+void cells_load(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, const u8* inf, u64 n, int arity,
+                int bits);
+// ⊕-apply packed derivations (marks touched cells).
+template <class K>
+void cells_insert(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, const u8* inf, u64 n, int arity,
+                  int bits);
+// Compact the touched cells (clearing them) or all present cells into a
+// sorted packed relation; returns its size.  The outputs are (re)allocated.
+template <class K>
+u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
+                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf);
+
+}  // namespace sk

@@ -1,0 +1,229 @@
+// sk_cells.hip — dense cell state kernels: column domains, loading a
+// relation into cells, ⊕-applying derivations, and compacting touched or
+// present cells back into sorted packed relations.
+#include <climits>
+
+#include "sk_cells.h"
+
+namespace sk {
+
+static unsigned gridc(u64 n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <class K>
+__device__ __forceinline__ u32 code_at(K key, int arity, int bits, int j) {
+  K mask = (((K)1) << bits) - 1;
+  return (u32)((key >> (bits * (arity - 1 - j))) & mask);
+}
+
+// ----------------------------------------------------------------- domains
+
+template <class K>
+__global__ void k_cells_mark_col(const K* __restrict__ keys, u64 n, int arity, int bits, int j, u32* bitmap) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u32 v = code_at<K>(keys[i], arity, bits, j);
+  u32 m = 1u << (v & 31);
+  // hub values repeat: skip the atomic when the bit is already visible
+  if (!(bitmap[v >> 5] & m)) atomicOr(&bitmap[v >> 5], m);
+}
+
+__global__ void k_cells_mark_code(u32* bitmap, u32 code) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(&bitmap[code >> 5], 1u << (code & 31));
+}
+
+__global__ void k_cells_or(u32* __restrict__ dst, const u32* __restrict__ src, u64 nwords) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) dst[i] |= src[i];
+}
+
+__global__ void k_cells_popc(const u32* __restrict__ b, u64 nwords, u32* __restrict__ cnt) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) cnt[i] = (u32)__popc(b[i]);
+}
+
+__global__ void k_cells_list(const u32* __restrict__ b, const u64* __restrict__ pre, u64 nwords,
+                             u32* __restrict__ list) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwords) return;
+  u32 w = b[i];
+  u64 o = pre[i];
+  while (w) {
+    int bit = __ffs(w) - 1;
+    list[o++] = (u32)(i * 32 + bit);
+    w &= w - 1;
+  }
+}
+
+template <class K>
+void cells_mark_col(const Exec& x, const K* keys, u64 n, int arity, int bits, int j, u32* bitmap) {
+  if (!n) return;
+  Scope sc(x.timer, "cells_domain", KC_MISC, (double)n * sizeof(K), n);
+  k_cells_mark_col<K><<<gridc(n), kBlock, 0, x.stream>>>(keys, n, arity, bits, j, bitmap);
+  SK_HIP(hipGetLastError());
+}
+
+void cells_mark_code(const Exec& x, u32* bitmap, u32 code) {
+  k_cells_mark_code<<<1, 64, 0, x.stream>>>(bitmap, code);
+  SK_HIP(hipGetLastError());
+}
+
+void cells_or(const Exec& x, u32* dst, const u32* src, u64 nwords) {
+  if (!nwords) return;
+  k_cells_or<<<gridc(nwords), kBlock, 0, x.stream>>>(dst, src, nwords);
+  SK_HIP(hipGetLastError());
+}
+
+u64 cells_domain(const Exec& x, const u32* bitmap, u64 nwords, u64* pre, DBuf<u32>* list) {
+  DBuf<u32> cnt(x.pool, nwords);
+  DBuf<u64> tot(x.pool, 1);
+  k_cells_popc<<<gridc(nwords), kBlock, 0, x.stream>>>(bitmap, nwords, cnt.get());
+  SK_HIP(hipGetLastError());
+  scan_u32(x, cnt.get(), pre, nwords, tot.get());
+  u64 n = read_u64(x, tot.get());
+  list->reset(x.pool, n ? n : 1);
+  k_cells_list<<<gridc(nwords), kBlock, 0, x.stream>>>(bitmap, pre, nwords, list->get());
+  SK_HIP(hipGetLastError());
+  return n;
+}
+
+// ------------------------------------------------------ load / ⊕-apply
+
+template <class K>
+__global__ void k_cells_load(CellDesc cd, const K* __restrict__ keys, const i64* __restrict__ w,
+                             const u8* __restrict__ inf, u64 n, int arity, int bits) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K key = keys[i];
+  u64 c;
+  if (!cell_of(cd, [&](int j) { return code_at<K>(key, arity, bits, j); }, &c)) return;
+  u32 m = 1u << (c & 31);
+  if (!cd.trop) {
+    atomicOr(&cd.present[c >> 5], m);
+  } else if (inf && inf[i]) {
+    atomicOr(&cd.infb[c >> 5], m);
+  } else if (w[i] == LLONG_MAX) {
+    atomicOr(&cd.maxf[c >> 5], m);
+  } else {
+    cd.cell[c] = cell_enc(w[i]);  // keys are unique: one writer per cell
+  }
+}
+
+template <class K>
+__global__ void k_cells_insert(CellDesc cd, const K* __restrict__ keys, const i64* __restrict__ w,
+                               const u8* __restrict__ inf, u64 n, int arity, int bits) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  K key = keys[i];
+  u64 c;
+  if (!cell_of(cd, [&](int j) { return code_at<K>(key, arity, bits, j); }, &c)) return;
+  cell_insert(cd, c, cd.trop ? w[i] : 0, (cd.trop && inf) ? inf[i] : (u8)0);
+}
+
+template <class K>
+void cells_load(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, const u8* inf, u64 n, int arity,
+                int bits) {
+  if (!n) return;
+  Scope sc(x.timer, "cells_load", KC_MERGE, (double)n * (sizeof(K) + (cd.trop ? 16.0 : 4.0)), n);
+  k_cells_load<K><<<gridc(n), kBlock, 0, x.stream>>>(cd, keys, w, inf, n, arity, bits);
+  SK_HIP(hipGetLastError());
+}
+
+template <class K>
+void cells_insert(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, const u8* inf, u64 n, int arity,
+                  int bits) {
+  if (!n) return;
+  Scope sc(x.timer, "cells_insert", KC_MERGE, (double)n * (sizeof(K) + (cd.trop ? 16.0 : 4.0)), n);
+  k_cells_insert<K><<<gridc(n), kBlock, 0, x.stream>>>(cd, keys, w, inf, n, arity, bits);
+  SK_HIP(hipGetLastError());
+}
+
+// ----------------------------------------------------------------- compact
+
+__device__ __forceinline__ u32 cells_word(const CellDesc& cd, u64 wi, bool touched_only) {
+  if (touched_only) return cd.touched[wi];
+  if (!cd.trop) return cd.present[wi];
+  u32 b = cd.maxf[wi] | (cd.infb ? cd.infb[wi] : 0u);
+  const u64* p = cd.cell + wi * 32;
+#pragma unroll 8
+  for (int k = 0; k < 32; k++)
+    if (p[k] != kCellTop) b |= 1u << k;
+  return b;
+}
+
+__global__ void k_cells_count(CellDesc cd, u64 nwords, int touched_only, u32* __restrict__ cnt) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nwords) cnt[i] = (u32)__popc(cells_word(cd, i, touched_only != 0));
+}
+
+template <class K>
+__global__ void k_cells_write(CellDesc cd, u64 nwords, int touched_only, const u64* __restrict__ off, int bits,
+                              K* __restrict__ keys, i64* __restrict__ w, u8* __restrict__ inf) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwords) return;
+  u32 b = cells_word(cd, i, touched_only != 0);
+  if (touched_only && b) cd.touched[i] = 0;
+  u64 o = off[i];
+  while (b) {
+    int bit = __ffs(b) - 1;
+    b &= b - 1;
+    u64 c = i * 32 + (u64)bit;
+    K key = 0;
+    for (int j = 0; j < cd.ncols; j++) {
+      u64 r = (c >> cd.shift[j]) & ((1ull << cd.ebits[j]) - 1);
+      u32 code = cd.dom_list[j] ? cd.dom_list[j][r] : (u32)r;
+      key = (key << bits) | (K)code;
+    }
+    keys[o] = key;
+    if (cd.trop) {
+      u64 e = cd.cell[c];
+      i64 wv = 0;
+      u8 f = 0;
+      if (e != kCellTop) wv = cell_dec(e);
+      else if (cd.maxf[i] & (1u << bit)) wv = LLONG_MAX;
+      else f = 1;
+      w[o] = wv;
+      if (inf) inf[o] = f;
+    }
+    o++;
+  }
+}
+
+template <class K>
+u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
+                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf) {
+  const u64 nwords = (ncells + 31) / 32;
+  DBuf<u32> cnt(x.pool, nwords);
+  DBuf<u64> off(x.pool, nwords);
+  DBuf<u64> tot(x.pool, 1);
+  const double scan_bytes = touched_only ? 4.0 * nwords : (cd.trop ? 8.0 * ncells + 8.0 * nwords : 4.0 * nwords);
+  {
+    Scope sc(x.timer, "cells_count", KC_DEDUP, scan_bytes, ncells);
+    k_cells_count<<<gridc(nwords), kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, cnt.get());
+    SK_HIP(hipGetLastError());
+  }
+  scan_u32(x, cnt.get(), off.get(), nwords, tot.get());
+  u64 n = read_u64(x, tot.get());
+  if (n == 0) return 0;
+  keys->reset(x.pool, n);
+  if (cd.trop) {
+    w->reset(x.pool, n);
+    if (want_inf) inf->reset(x.pool, n);
+  }
+  const double rec = sizeof(K) + (cd.trop ? 16.0 + (want_inf ? 1.0 : 0) : 0);
+  Scope sc(x.timer, "cells_write", KC_DEDUP, scan_bytes + rec * n, ncells);
+  k_cells_write<K><<<gridc(nwords), kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, off.get(), bits,
+                                                           keys->get(), cd.trop ? w->get() : nullptr,
+                                                           (cd.trop && want_inf) ? inf->get() : nullptr);
+  SK_HIP(hipGetLastError());
+  return n;
+}
+
+#define SK_CELLS_INST(K)                                                                                      \
+  template void cells_mark_col<K>(const Exec&, const K*, u64, int, int, int, u32*);                          \
+  template void cells_load<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int);  \
+  template void cells_insert<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int); \
+  template u64 cells_compact<K>(const Exec&, const CellDesc&, u64, bool, int, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, bool);
+SK_CELLS_INST(u64)
+SK_CELLS_INST(u128)
+
+}  // namespace sk

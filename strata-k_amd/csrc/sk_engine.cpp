@@ -158,6 +158,14 @@ struct HeadOwner {
 
 static void head_alloc(void* ctx, OutDesc* o, u64 n) {
   HeadOwner* h = (HeadOwner*)ctx;
+  if (o->cd.on) {
+    // rows are ⊕-applied to the head's cells by the emitting kernel
+    o->keys = nullptr;
+    o->w = nullptr;
+    o->inf = nullptr;
+    h->c->n = 0;
+    return;
+  }
   h->c->k.ensure(h->pool, n, h->wide);
   o->keys = h->c->k.ptr();
   if (h->trop) {
@@ -487,6 +495,7 @@ std::vector<bool> Engine::snapshot_set() const {
 void Engine::reset() {
   if (!loaded) return;
   Exec x = ctx_->exec();
+  dense_.clear();
   std::vector<bool> written = snapshot_set();
   if (snap_dict_.nvals() != D_.nvals()) {
     D_ = snap_dict_;
@@ -1145,6 +1154,7 @@ void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& ou
             o.cnst[c] = code_or_never(t.kind, t.kind == SK_T_CONST ? (i64)t.a : t.b, &nv2);
           }
         }
+        if (DenseRel* dh = dense_of(hr)) o.cd = dh->cd;
       } else {
         o.ncols = nxt.t.ncols;
         o.table = &nxt.t;
@@ -1228,6 +1238,7 @@ void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& ou
     }
     OutDesc o;
     head_desc(pl.steps.empty() ? std::vector<int>() : pl.steps.back().out_slots, &o);
+    if (DenseRel* dh = dense_of(hr)) o.cd = dh->cd;
     project(x, cur.t, &o, H.wide, head_alloc, &ho);
   }
   if (ch.n) out[hr].push_back(std::move(ch));
@@ -1248,6 +1259,50 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
   for (auto& c : chunks) total += c.n;
   R.delta.release();
   R.dversion++;
+  if (DenseRel* dn = dense_of(rel)) {
+    // rows not applied by their emitting kernel (leapfrog output) go in now;
+    // the delta is the set of cells this round improved, in key order
+    u64 nd = 0;
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      for (auto& c : chunks)
+        if (c.n) cells_insert<K>(x, dn->cd, (const K*)c.k.ptr(), trop ? c.w.get() : nullptr, hf ? c.inf.get() : nullptr,
+                                 c.n, R.arity, bits);
+      chunks.clear();
+      DBuf<K> dk;
+      DBuf<i64> dw;
+      DBuf<u8> df;
+      nd = cells_compact<K>(x, dn->cd, dn->ncells, true, bits, &dk, &dw, &df, hf);
+      if (!nd) return;
+      R.delta.n = nd;
+      R.delta.k.wide = sizeof(K) == 16;
+      if constexpr (sizeof(K) == 16) R.delta.k.b = std::move(dk);
+      else R.delta.k.a = std::move(dk);
+      R.delta.w = std::move(dw);
+      R.delta.inf = std::move(df);
+      if (dn->keep_full) {
+        DBuf<K> u_k, f_k;
+        DBuf<i64> u_w, f_w;
+        DBuf<u8> u_f, f_f;
+        u64 nu = 0, nf = 0;
+        merge_diff<K>(x, (const K*)R.delta.k.ptr(), trop ? R.delta.w.get() : nullptr, hf ? R.delta.inf.get() : nullptr,
+                      nd, (const K*)R.full.k.ptr(), trop ? R.full.w.get() : nullptr, hf ? R.full.inf.get() : nullptr,
+                      R.full.n, &u_k, &u_w, &u_f, &nu, &f_k, &f_w, &f_f, &nf, trop, hf);
+        // every delta row strictly improves its cell, so all of them are
+        // fresh and the union is always written
+        RelData nfull;
+        nfull.n = nu;
+        nfull.k.wide = sizeof(K) == 16;
+        if constexpr (sizeof(K) == 16) nfull.k.b = std::move(u_k);
+        else nfull.k.a = std::move(u_k);
+        nfull.w = std::move(u_w);
+        nfull.inf = std::move(u_f);
+        R.full = std::move(nfull);
+        R.version++;
+      }
+    });
+    return nd;
+  }
   if (total == 0) return 0;
   Chunk cand;
   if (chunks.size() == 1) {
@@ -1680,6 +1735,256 @@ void Engine::eval_aggregate(const Rule& r) {
 
 // ================================================================ strata
 
+// ============================================================ dense cells
+
+static int ceil_log2(u64 n) {
+  int b = 0;
+  while (b < 63 && (1ull << b) < n) b++;
+  return b;
+}
+
+// Decide which relations of stratum k run as dense cells, and build their
+// column domains.  A head column takes its values from head constants, the
+// relation's own facts and the body columns that bind its variable: a fixed
+// column (EDB, or a relation of a lower stratum) bounds it by that column's
+// distinct codes (the smallest such column is taken); otherwise it inherits
+// the domain of a column of the stratum (the union over everything reachable
+// in this provenance graph).  A domain covering more than half of the codes
+// becomes the identity.  Relations whose cells would exceed a quarter of the
+// memory budget (at most 16 GiB) keep the sorted-list path.
+void Engine::plan_dense(u32 k) {
+  dense_.clear();
+  dense_.resize(R_.size());
+  if (!dense_enabled_ || nparts_) return;
+  std::vector<bool> cand(R_.size(), false);
+  for (size_t i = 0; i < R_.size(); i++) {
+    if (!in_idb_[i] || R_[i].arity > kMaxCols) continue;
+    if (i < attached_.size() && attached_[i]) continue;
+    cand[i] = R_[i].sem == SK_TROP || R_[i].sem == SK_BOOL;
+  }
+  for (auto& r : P_.rules)
+    if (r.stratum == k && r.agg && r.head.rel >= 0) cand[r.head.rel] = false;
+  bool any = false;
+  for (size_t i = 0; i < R_.size(); i++) any = any || cand[i];
+  if (!any) return;
+
+  const u64 nvals = D_.nvals();
+  if (nvals > (1ull << 31)) return;  // domain bitmaps would not be modest
+  Exec x = ctx_->exec();
+  const u64 nw = (nvals + 31) / 32;
+
+  // distinct codes of fixed columns (and of the stratum relations' own facts)
+  std::map<std::pair<int, int>, int> col_id;
+  std::vector<DBuf<u32>> col_bm;
+  std::vector<u64> col_n;
+  auto col_codes = [&](int rel, int col) {
+    auto key = std::make_pair(rel, col);
+    auto it = col_id.find(key);
+    if (it != col_id.end()) return it->second;
+    int id = (int)col_bm.size();
+    col_bm.emplace_back(x.pool, nw);
+    memset_async(x, col_bm.back().get(), 0, nw * 4);
+    const RelData& d = R_[rel].full;
+    with_key(R_[rel].wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      cells_mark_col<K>(x, (const K*)d.k.ptr(), d.n, R_[rel].arity, R_[rel].bits, col, col_bm.back().get());
+    });
+    col_n.push_back(d.n ? popcount_bits(x, col_bm.back().get(), nw) : 0);
+    col_id[key] = id;
+    return id;
+  };
+
+  struct Node {
+    bool all = false;
+    std::vector<int> fixed;
+    std::vector<u32> consts;
+    std::vector<std::pair<int, int>> deps;
+  };
+  std::map<std::pair<int, int>, Node> g;
+  std::vector<bool> keep_full(R_.size(), false);
+  for (auto& r : P_.rules) {
+    if (r.stratum != k || r.agg || r.head.rel < 0) continue;
+    int nidb = 0;
+    for (auto& a : r.body)
+      if (!a.neg && a.rel >= 0 && in_idb_[a.rel]) nidb++;
+    if (nidb >= 2)
+      for (auto& a : r.body)
+        if (!a.neg && a.rel >= 0 && in_idb_[a.rel]) keep_full[a.rel] = true;
+    if (!cand[r.head.rel]) continue;
+    for (size_t j = 0; j < r.head.args.size(); j++) {
+      Node& nd = g[{r.head.rel, (int)j}];
+      const Term& t = r.head.args[j];
+      if (t.kind == SK_T_CONST || t.kind == SK_T_INT) {
+        bool never = false;
+        u32 c = code_or_never(t.kind, t.kind == SK_T_CONST ? (i64)t.a : t.b, &never);
+        if (never) nd.all = true;
+        else nd.consts.push_back(c);
+        continue;
+      }
+      if (t.kind != SK_T_VAR) {
+        nd.all = true;
+        continue;
+      }
+      int best = -1;
+      std::pair<int, int> dep{-1, -1};
+      for (auto& a : r.body) {
+        if (a.neg || a.rel < 0 || (int)a.args.size() != R_[a.rel].arity) continue;
+        for (size_t c = 0; c < a.args.size(); c++) {
+          if (a.args[c].kind != SK_T_VAR || a.args[c].a != t.a) continue;
+          if (in_idb_[a.rel]) {
+            if (dep.first < 0) dep = {a.rel, (int)c};
+            continue;
+          }
+          int id = col_codes(a.rel, (int)c);
+          if (best < 0 || col_n[id] < col_n[best]) best = id;
+        }
+      }
+      if (best >= 0) nd.fixed.push_back(best);
+      else if (dep.first >= 0) nd.deps.push_back(dep);
+      else nd.all = true;
+    }
+  }
+
+  const u64 cap_bytes = std::min<u64>(ctx_->pool->budget() / 4, 16ull << 30);
+  u64 used_bytes = 0;
+  for (size_t i = 0; i < R_.size(); i++) {
+    if (!cand[i]) continue;
+    DRel& R = R_[i];
+    const bool trop = R.sem == SK_TROP;
+    auto dn = std::make_unique<DenseRel>();
+    dn->keep_full = keep_full[i];
+    CellDesc& cd = dn->cd;
+    cd.trop = trop;
+    cd.ncols = R.arity;
+    cd.ncodes = nvals;
+    int tot_bits = 0;
+    bool ok = true;
+    for (int j = 0; j < R.arity && ok; j++) {
+      std::vector<std::pair<int, int>> stack{{(int)i, j}};
+      std::set<std::pair<int, int>> seen{{(int)i, j}};
+      bool all = false;
+      std::vector<int> parts;  // column-code bitmaps to unite
+      std::vector<u32> consts;
+      while (!stack.empty() && !all) {
+        auto at = stack.back();
+        stack.pop_back();
+        auto it = g.find(at);
+        if (it == g.end() || !cand[at.first]) {
+          all = true;  // a stratum column whose domain is not tracked
+          break;
+        }
+        const Node& nd = it->second;
+        all = nd.all;
+        for (int f : nd.fixed) parts.push_back(f);
+        for (u32 c : nd.consts) consts.push_back(c);
+        if (R_[at.first].full.n) parts.push_back(col_codes(at.first, at.second));
+        for (auto& d : nd.deps)
+          if (seen.insert(d).second) stack.push_back(d);
+      }
+      if (all) {
+        cd.ebits[j] = R.bits;
+      } else {
+        DBuf<u32> bm(x.pool, nw);
+        memset_async(x, bm.get(), 0, nw * 4);
+        for (int pidx : parts) cells_or(x, bm.get(), col_bm[pidx].get(), nw);
+        for (u32 c : consts) cells_mark_code(x, bm.get(), c);
+        u64 cnt = popcount_bits(x, bm.get(), nw);
+        if (2 * cnt > nvals) {
+          cd.ebits[j] = R.bits;
+        } else {
+          dn->dom_pre[j].reset(x.pool, nw);
+          cells_domain(x, bm.get(), nw, dn->dom_pre[j].get(), &dn->dom_list[j]);
+          dn->dom_bits[j] = std::move(bm);
+          cd.dom_bits[j] = dn->dom_bits[j].get();
+          cd.dom_pre[j] = dn->dom_pre[j].get();
+          cd.dom_list[j] = dn->dom_list[j].get();
+          cd.ebits[j] = ceil_log2(cnt ? cnt : 1);
+        }
+      }
+      tot_bits += cd.ebits[j];
+      if (tot_bits > 40) ok = false;
+    }
+    if (!ok) continue;
+    for (int j = R.arity - 1, sh = 0; j >= 0; j--) {
+      cd.shift[j] = sh;
+      sh += cd.ebits[j];
+    }
+    dn->ncells = 1ull << tot_bits;
+    const u64 nc = std::max<u64>(dn->ncells, 32);
+    const u64 nwc = nc / 32;
+    const u64 bytes = trop ? nc * 8 + nwc * 4 * (has_inf_ ? 3 : 2) : nwc * 4 * 2;
+    if (used_bytes + bytes > cap_bytes) continue;
+    used_bytes += bytes;
+    if (trop) {
+      dn->cell.reset(x.pool, nc);
+      memset_async(x, dn->cell.get(), 0xFF, nc * 8);
+      dn->maxf.reset(x.pool, nwc);
+      memset_async(x, dn->maxf.get(), 0, nwc * 4);
+      if (has_inf_) {
+        dn->infb.reset(x.pool, nwc);
+        memset_async(x, dn->infb.get(), 0, nwc * 4);
+      }
+    } else {
+      dn->present.reset(x.pool, nwc);
+      memset_async(x, dn->present.get(), 0, nwc * 4);
+    }
+    dn->touched.reset(x.pool, nwc);
+    memset_async(x, dn->touched.get(), 0, nwc * 4);
+    dn->bad.reset(x.pool, 1);
+    memset_async(x, dn->bad.get(), 0, 4);
+    cd.cell = dn->cell.get();
+    cd.present = dn->present.get();
+    cd.maxf = dn->maxf.get();
+    cd.infb = dn->infb.get();
+    cd.touched = dn->touched.get();
+    cd.bad = dn->bad.get();
+    cd.on = 1;
+    if (R.full.n) {
+      with_key(R.wide, [&](auto* tag) {
+        using K = std::remove_pointer_t<decltype(tag)>;
+        cells_load<K>(x, cd, (const K*)R.full.k.ptr(), trop ? R.full.w.get() : nullptr,
+                      (trop && has_inf_) ? R.full.inf.get() : nullptr, R.full.n, R.arity, R.bits);
+      });
+    }
+    dense_[i] = std::move(dn);
+  }
+}
+
+// Materialise every dense relation of the stratum as its sorted full state
+// and drop the cells.
+void Engine::end_dense() {
+  Exec x = ctx_->exec();
+  for (size_t i = 0; i < dense_.size(); i++) {
+    DenseRel* dn = dense_[i].get();
+    if (!dn) continue;
+    DRel& R = R_[i];
+    u32 bad = 0;
+    SK_HIP(hipMemcpyAsync(&bad, dn->bad.get(), 4, hipMemcpyDeviceToHost, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    if (bad) throw mkerr(SK_ERR_DEVICE, P_.preds[i].name, "a derived value lies outside its dense column domain");
+    if (dn->keep_full) continue;
+    const bool trop = R.sem == SK_TROP;
+    const bool hf = trop && has_inf_;
+    with_key(R.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      DBuf<K> fk;
+      DBuf<i64> fw;
+      DBuf<u8> ff;
+      u64 n = cells_compact<K>(x, dn->cd, dn->ncells, false, R.bits, &fk, &fw, &ff, hf);
+      RelData nf;
+      nf.n = n;
+      nf.k.wide = sizeof(K) == 16;
+      if constexpr (sizeof(K) == 16) nf.k.b = std::move(fk);
+      else nf.k.a = std::move(fk);
+      nf.w = std::move(fw);
+      nf.inf = std::move(ff);
+      R.full = std::move(nf);
+    });
+    R.version++;
+  }
+  dense_.clear();
+}
+
 void Engine::run_stratum(u32 k) {
   // aggregate rules first, once (seminaive.rs:79-84)
   in_idb_.assign(R_.size(), false);
@@ -1695,6 +2000,7 @@ void Engine::run_stratum(u32 k) {
       R_[i].delta.release();
       R_[i].dversion++;
     }
+  plan_dense(k);
   const u64 cap = universe_ + 2;
   u64 iters = 0;
   std::vector<u8> active_prev;  // batched mode: program had a non-empty delta after the last round
@@ -1728,9 +2034,12 @@ void Engine::run_stratum(u32 k) {
     for (size_t i = 0; i < R_.size(); i++) {
       if (!in_idb_[i]) continue;
       auto it = chunks.find((int)i);
-      if (it == chunks.end()) {
+      if (it == chunks.end() && !dense_of((int)i)) {
         R_[i].delta.release();
         R_[i].dversion++;
+      } else if (it == chunks.end()) {
+        std::vector<Chunk> none;
+        if (finalize((int)i, none) > 0) any = true;
       } else if (finalize((int)i, it->second) > 0) {
         any = true;
       }
@@ -1750,6 +2059,7 @@ void Engine::run_stratum(u32 k) {
     if (!any) break;
   }
   iters_[k] = iters;
+  end_dense();
   for (size_t i = 0; i < R_.size(); i++)
     if (in_idb_[i]) {
       R_[i].delta.release();
@@ -1759,6 +2069,7 @@ void Engine::run_stratum(u32 k) {
 
 void Engine::execute() {
   if (!loaded) load();
+  dense_.clear();
   iters_.assign(P_.num_strata, 0);
   auto t0 = std::chrono::steady_clock::now();
   for (u32 k = 0; k < P_.num_strata; k++) run_stratum(k);

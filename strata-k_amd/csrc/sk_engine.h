@@ -145,6 +145,19 @@ struct DRel {
   u64 st_rows = 0;
 };
 
+// Dense cell state of one relation while its stratum runs (see sk_cells.h).
+struct DenseRel {
+  bool keep_full = false;  // some rule of the stratum reads the relation's full
+                           // state: each round's delta is also merged into it
+  u64 ncells = 0;
+  CellDesc cd;
+  DBuf<u64> cell;
+  DBuf<u32> present, maxf, infb, touched, bad;
+  DBuf<u32> dom_bits[kMaxCols];
+  DBuf<u64> dom_pre[kMaxCols];
+  DBuf<u32> dom_list[kMaxCols];
+};
+
 // ------------------------------------------------------------- plans
 
 struct StepPlan {
@@ -248,6 +261,8 @@ class Engine {
               u64 max_sym, u64 universe_syms);
   // Use a given universe instead of scanning the relations at load.
   void set_universe(u64 u) { universe_override_ = u; }
+  // Keep eligible relations as dense cells during their stratum (default on).
+  void set_dense(bool on) { dense_enabled_ = on; }
 
  private:
   std::vector<RelData> snap_;  // post-load state of relations written by rules
@@ -276,12 +291,17 @@ class Engine {
   bool has_inf_ = false;
   std::vector<bool> in_idb_;  // relation changes in the stratum being run
   DBuf<i64> int_vals_;        // device copy of Dict::ints
+  std::vector<std::unique_ptr<DenseRel>> dense_;  // per relation, while its stratum runs
+  bool dense_enabled_ = true;
 
   void build_dict();
   void upload_rel(int r);
   void compute_universe();
   void grow_dict(const Dict& nd, u32 stratum);
   void run_stratum(u32 k);
+  void plan_dense(u32 k);
+  void end_dense();
+  DenseRel* dense_of(int rel) { return rel >= 0 && (size_t)rel < dense_.size() ? dense_[rel].get() : nullptr; }
   void eval_aggregate(const Rule& r);
   RulePlan plan_rule(int ri, int dpos);
   void exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out);

@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "sk_cells.h"
 #include "sk_dev.h"
 
 namespace sk {
@@ -60,6 +61,7 @@ struct OutDesc {
   i64* w = nullptr;
   u8* inf = nullptr;
   Table* table = nullptr;  // host side: destination table of a non-final join
+  CellDesc cd;             // cd.on: head rows are ⊕-applied to the head's cells, not written
 };
 
 using HeadAlloc = void (*)(void* ctx, OutDesc* o, u64 n);

@@ -207,6 +207,20 @@ __device__ __forceinline__ void emit_one(const Table& in, const ProbeDesc& p, co
                                          const u8* __restrict__ sf, u64 oi, DevErr* err) {
   K sk = skeys[s];
   if constexpr (FINAL) {
+    if (o.cd.on) {
+      u64 c;
+      if (!cell_of(o.cd, [&](int j) { return out_val<K>(o, j, in, r, sk, p); }, &c)) return;
+      i64 res = 0;
+      u8 rf = 0;
+      if constexpr (TROP) {
+        u8 af = in.inf ? in.inf[r] : (u8)0;
+        u8 bf = sf ? sf[s] : (u8)0;
+        DevErr* e = in.part_col >= 0 ? &in.perr[in.col[in.part_col][r]] : err;
+        trop_mul(in.w[r], af, sw[s], bf, &res, &rf, e);
+      }
+      cell_insert(o.cd, c, res, rf);
+      return;
+    }
     KO key = 0;
     for (int c = 0; c < o.ncols; c++) key = (key << o.bits) | (KO)out_val<K>(o, c, in, r, sk, p);
     ((KO*)o.keys)[oi] = key;
@@ -353,7 +367,10 @@ u64 join(const Exec& x, const Table& in, const ProbeDesc& p, const K* skeys, con
   // algorithmic bytes: every emitted row reads one relation record and
   // writes one output record
   double rec_in = sizeof(K) + (trop ? 8.0 + (sf ? 1.0 : 0) : 0);
-  double rec_out = (final_step ? (k128_out ? 16.0 : 8.0) : 4.0 * o->ncols) + (trop ? 8.0 + (o->inf ? 1.0 : 0) : 0);
+  // (a head applied to cells reads one cell word per row instead of writing a record)
+  double rec_out = (final_step && o->cd.on)
+                       ? (trop ? 8.0 : 4.0)
+                       : (final_step ? (k128_out ? 16.0 : 8.0) : 4.0 * o->ncols) + (trop ? 8.0 + (o->inf ? 1.0 : 0) : 0);
   Scope sc(x.timer, "join_emit", KC_JOIN, (double)total * (rec_in + rec_out), total);
   if (final_step) {
     if (k128_out) {
@@ -436,6 +453,12 @@ template <class KO>
 __global__ void k_project(Table in, OutDesc o) {
   u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= in.n) return;
+  if (o.cd.on) {
+    u64 cell;
+    if (!cell_of(o.cd, [&](int j) { return o.kind[j] == 0 ? in.col[o.idx[j]][r] : o.cnst[j]; }, &cell)) return;
+    cell_insert(o.cd, cell, in.w ? in.w[r] : 0, in.inf ? in.inf[r] : (u8)0);
+    return;
+  }
   KO key = 0;
   for (int c = 0; c < o.ncols; c++) {
     u32 v = o.kind[c] == 0 ? in.col[o.idx[c]][r] : o.cnst[c];

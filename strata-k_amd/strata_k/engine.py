@@ -141,6 +141,7 @@ SIGNATURES = [
     ("sk_run_load", C.c_int, [_VP]),
     ("sk_run_execute", C.c_int, [_VP]),
     ("sk_run_reset", C.c_int, [_VP]),
+    ("sk_run_set_dense", C.c_int, [_VP, C.c_int]),
     ("sk_run_error", None, [_VP, C.POINTER(SkErrorInfo)]),
     ("sk_result_size", _I64, [_VP, C.c_char_p]),
     ("sk_result_arity", _I64, [_VP, C.c_char_p]),
@@ -376,14 +377,16 @@ class Context:
         return Run(self, prog)
 
     def run_semi_naive(self, prog, edb: Iterable = (), bulk: Optional[dict] = None,
-                       want: Optional[Sequence[str]] = None) -> Db:
+                       want: Optional[Sequence[str]] = None, dense: bool = True) -> Db:
         """``run_semi_naive(prog, edb)`` (crates/strata-eval/src/seminaive.rs:29).
 
         ``edb``: iterable of ``(pred, tuple-of-Sym/IntV, ann)``; ``bulk``: pred ->
         ``(columns, weights, inf)`` numpy arrays of Sym ids for large inputs.
+        ``dense=False`` keeps every relation on the sorted-list path.
         """
         run = self.prepare(prog)
         try:
+            run.set_dense(dense)
             run.add_facts(edb)
             for p, (cols, w, inf) in (bulk or {}).items():
                 run.add_sym(p, cols, w, inf)
@@ -490,6 +493,11 @@ class Run:
     def reset(self):
         """Restore the post-load relation state (re-run over the resident EDB)."""
         self._check(self.L.sk_run_reset(self.h))
+
+    def set_dense(self, on: bool):
+        """Dense cell arrays for eligible relations (True, the default) or the
+        sorted-list path for every relation (False)."""
+        self._check(self.L.sk_run_set_dense(self.h, 1 if on else 0))
 
     @property
     def execute_ms(self) -> float:

@@ -32,20 +32,26 @@ def engine_rows(rel):
 
 
 def compare(ctx, prog, edb=(), bulk=None, label=""):
-    """Run both; assert identical outcome, relations, weights and round counts."""
+    """Run the oracle and the engine on both evaluation paths (dense cells and
+    sorted lists); assert identical outcome, relations, weights and round counts."""
     ref = O.run(prog, edb, bulk=bulk)
-    try:
-        db = ctx.run_semi_naive(prog, edb, bulk=bulk)
-    except sk.EngineError as e:
-        assert not ref.ok, f"{label}: engine {e.kind} but oracle ok"
-        assert e.kind == ERR[ref.status], f"{label}: {e.kind} vs {ref.status}"
-        return None, ref
-    assert ref.ok, f"{label}: oracle {ref.status} but engine ok"
-    assert db.iterations == ref.iterations, f"{label}: iterations {db.iterations} vs {ref.iterations}"
-    for p in prog.predicates:
-        g, r = engine_rows(db.relation(p.name)), ref.relations[p.name].rows()
-        assert g == r, f"{label}: relation {p.name} differs ({len(g)} vs {len(r)})"
-    return db, ref
+    out = None
+    for dense in (True, False):
+        lab = f"{label}[{'dense' if dense else 'sorted'}]"
+        try:
+            db = ctx.run_semi_naive(prog, edb, bulk=bulk, dense=dense)
+        except sk.EngineError as e:
+            assert not ref.ok, f"{lab}: engine {e.kind} but oracle ok"
+            assert e.kind == ERR[ref.status], f"{lab}: {e.kind} vs {ref.status}"
+            continue
+        assert ref.ok, f"{lab}: oracle {ref.status} but engine ok"
+        assert db.iterations == ref.iterations, f"{lab}: iterations {db.iterations} vs {ref.iterations}"
+        for p in prog.predicates:
+            g, r = engine_rows(db.relation(p.name)), ref.relations[p.name].rows()
+            assert g == r, f"{lab}: relation {p.name} differs ({len(g)} vs {len(r)})"
+        if dense:
+            out = db
+    return out, ref
 
 
 # ------------------------------------------------------ reference vectors

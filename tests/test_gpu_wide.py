@@ -23,15 +23,16 @@ def wide(x):
 def check(ctx, prog, bulk, outs):
     ref = O.run(prog, bulk=bulk, want=outs)
     assert ref.ok, ref.status
-    db = ctx.run_semi_naive(prog, bulk=bulk, want=outs)
-    for o in outs:
-        g, r = db.relation(o), ref.relations[o]
-        assert len(g) == len(r), (o, len(g), len(r))
-        assert (g.vals == r.vals).all(), o
-        if g.w is not None:
-            bad = np.nonzero(g.w != r.w)[0]
-            assert len(bad) == 0, (o, bad[:5], g.w[bad[:5]], r.w[bad[:5]])
-    assert db.iterations == ref.iterations
+    for dense in (True, False):
+        db = ctx.run_semi_naive(prog, bulk=bulk, want=outs, dense=dense)
+        for o in outs:
+            g, r = db.relation(o), ref.relations[o]
+            assert len(g) == len(r), (dense, o, len(g), len(r))
+            assert (g.vals == r.vals).all(), (dense, o)
+            if g.w is not None:
+                bad = np.nonzero(g.w != r.w)[0]
+                assert len(bad) == 0, (dense, o, bad[:5], g.w[bad[:5]], r.w[bad[:5]])
+        assert db.iterations == ref.iterations, dense
 
 
 def test_wide_edb_min_load(ctx):
