@@ -64,34 +64,45 @@ __device__ __forceinline__ void cell_touch(const CellDesc& cd, u64 c) {
   atomicOr(&cd.touched[c >> 5], 1u << (c & 31));
 }
 
-// ⊕-apply one derivation to cell c.  Plain loads may be stale within the
-// kernel, but cell states only decrease, so a stale value can only send a
-// derivation to the atomic, which decides.
-__device__ __forceinline__ void cell_insert(const CellDesc& cd, u64 c, i64 w, u8 inf) {
+// ⊕-application of one derivation to cell c, split so that callers can issue
+// many cell reads before acting on any: `cell_peek` is a plain read (the Trop
+// code, or the Bool presence word), `cell_apply` decides with it.  Plain
+// reads may be stale within the kernel, but cell states only decrease, so a
+// stale value can only send a derivation to the atomic, which decides.
+__device__ __forceinline__ u64 cell_peek(const CellDesc& cd, u64 c) {
+  return cd.trop ? cd.cell[c] : (u64)cd.present[c >> 5];
+}
+
+template <bool TROP>
+__device__ __forceinline__ void cell_apply_t(const CellDesc& cd, u64 c, i64 w, u8 inf, u64 seen) {
   const u32 m = 1u << (c & 31);
-  if (!cd.trop) {
-    u32* p = &cd.present[c >> 5];
-    if (*p & m) return;
-    if (!(atomicOr(p, m) & m)) cell_touch(cd, c);
+  if constexpr (!TROP) {
+    if ((u32)seen & m) return;
+    if (!(atomicOr(&cd.present[c >> 5], m) & m)) cell_touch(cd, c);
     return;
   }
   unsigned long long* p = (unsigned long long*)&cd.cell[c];
   if (!inf) {
     const u64 e = cell_enc(w);
     if (e != kCellTop) {
-      if (e >= (u64)*p) return;
+      if (e >= seen) return;
       if (e < (u64)atomicMin(p, (unsigned long long)e)) cell_touch(cd, c);
       return;
     }
     // finite INT64_MAX: better only than absent or +inf
-    if ((u64)*p != kCellTop) return;
+    if (seen != kCellTop) return;
     if (!(atomicOr(&cd.maxf[c >> 5], m) & m)) cell_touch(cd, c);
     return;
   }
   // +inf: better only than absent
-  if ((u64)*p != kCellTop) return;
+  if (seen != kCellTop) return;
   if (cd.maxf[c >> 5] & m) return;
   if (!(atomicOr(&cd.infb[c >> 5], m) & m)) cell_touch(cd, c);
+}
+
+__device__ __forceinline__ void cell_insert(const CellDesc& cd, u64 c, i64 w, u8 inf) {
+  if (cd.trop) cell_apply_t<true>(cd, c, w, inf, cell_peek(cd, c));
+  else cell_apply_t<false>(cd, c, w, inf, cell_peek(cd, c));
 }
 
 // Cell index of a tuple given its column codes; false (and `bad` set) when a

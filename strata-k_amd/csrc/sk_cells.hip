@@ -138,6 +138,13 @@ void cells_insert(const Exec& x, const CellDesc& cd, const K* keys, const i64* w
 }
 
 // ----------------------------------------------------------------- compact
+//
+// One block per kBlock bitmap words (kBlock * 32 cells).  Pass 1 counts the
+// selected cells per block; after a scan over blocks, pass 2 scans the
+// per-word counts inside the block, stages the selected cells' in-block
+// indices in LDS (in cell order) and writes the packed rows coalesced.
+
+constexpr u32 kCellsPerBlock = (u32)kBlock * 32;
 
 __device__ __forceinline__ u32 cells_word(const CellDesc& cd, u64 wi, bool touched_only) {
   if (touched_only) return cd.touched[wi];
@@ -150,41 +157,77 @@ __device__ __forceinline__ u32 cells_word(const CellDesc& cd, u64 wi, bool touch
   return b;
 }
 
-__global__ void k_cells_count(CellDesc cd, u64 nwords, int touched_only, u32* __restrict__ cnt) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nwords) cnt[i] = (u32)__popc(cells_word(cd, i, touched_only != 0));
+// Exclusive block scan of one u32 per thread (kBlock = 4 waves of 64).
+__device__ __forceinline__ u32 cells_block_scan(u32 v, u32* tmp, u32* total) {
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  u32 inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    u32 y = __shfl_up(inc, o, 64);
+    if (lane >= (u32)o) inc += y;
+  }
+  if (lane == 63) tmp[wave] = inc;
+  __syncthreads();
+  u32 base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; w++) {
+    u32 t = tmp[w];
+    if ((u32)w < wave) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cells_count(CellDesc cd, u64 nwords, int touched_only,
+                                                        u32* __restrict__ cnt) {
+  __shared__ u32 tmp[kBlock / 64];
+  const u64 wi = (u64)blockIdx.x * kBlock + threadIdx.x;
+  u32 c = wi < nwords ? (u32)__popc(cells_word(cd, wi, touched_only != 0)) : 0u;
+  u32 tot;
+  cells_block_scan(c, tmp, &tot);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
 template <class K>
-__global__ void k_cells_write(CellDesc cd, u64 nwords, int touched_only, const u64* __restrict__ off, int bits,
-                              K* __restrict__ keys, i64* __restrict__ w, u8* __restrict__ inf) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nwords) return;
-  u32 b = cells_word(cd, i, touched_only != 0);
-  if (touched_only && b) cd.touched[i] = 0;
-  u64 o = off[i];
+__global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 nwords, int touched_only,
+                                                        const u64* __restrict__ off, int bits, K* __restrict__ keys,
+                                                        i64* __restrict__ w, u8* __restrict__ inf) {
+  __shared__ u32 tmp[kBlock / 64];
+  __shared__ unsigned short sc[kCellsPerBlock];
+  const u64 wi = (u64)blockIdx.x * kBlock + threadIdx.x;
+  u32 b = wi < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
+  if (touched_only && b) cd.touched[wi] = 0;
+  u32 tot;
+  u32 o = cells_block_scan((u32)__popc(b), tmp, &tot);
   while (b) {
     int bit = __ffs(b) - 1;
     b &= b - 1;
-    u64 c = i * 32 + (u64)bit;
+    sc[o++] = (unsigned short)(threadIdx.x * 32 + bit);
+  }
+  __syncthreads();
+  const u64 base = off[blockIdx.x];
+  const u64 cbase = (u64)blockIdx.x * kCellsPerBlock;
+  for (u32 j = threadIdx.x; j < tot; j += kBlock) {
+    const u64 c = cbase + sc[j];
     K key = 0;
-    for (int j = 0; j < cd.ncols; j++) {
-      u64 r = (c >> cd.shift[j]) & ((1ull << cd.ebits[j]) - 1);
-      u32 code = cd.dom_list[j] ? cd.dom_list[j][r] : (u32)r;
+    for (int q = 0; q < cd.ncols; q++) {
+      u64 r = (c >> cd.shift[q]) & ((1ull << cd.ebits[q]) - 1);
+      u32 code = cd.dom_list[q] ? cd.dom_list[q][r] : (u32)r;
       key = (key << bits) | (K)code;
     }
-    keys[o] = key;
+    keys[base + j] = key;
     if (cd.trop) {
       u64 e = cd.cell[c];
       i64 wv = 0;
       u8 f = 0;
       if (e != kCellTop) wv = cell_dec(e);
-      else if (cd.maxf[i] & (1u << bit)) wv = LLONG_MAX;
+      else if (cd.maxf[c >> 5] & (1u << (c & 31))) wv = LLONG_MAX;
       else f = 1;
-      w[o] = wv;
-      if (inf) inf[o] = f;
+      w[base + j] = wv;
+      if (inf) inf[base + j] = f;
     }
-    o++;
   }
 }
 
@@ -192,16 +235,17 @@ template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
                   DBuf<i64>* w, DBuf<u8>* inf, bool want_inf) {
   const u64 nwords = (ncells + 31) / 32;
-  DBuf<u32> cnt(x.pool, nwords);
-  DBuf<u64> off(x.pool, nwords);
+  const u64 nblocks = (nwords + kBlock - 1) / kBlock;
+  DBuf<u32> cnt(x.pool, nblocks);
+  DBuf<u64> off(x.pool, nblocks);
   DBuf<u64> tot(x.pool, 1);
   const double scan_bytes = touched_only ? 4.0 * nwords : (cd.trop ? 8.0 * ncells + 8.0 * nwords : 4.0 * nwords);
   {
     Scope sc(x.timer, "cells_count", KC_DEDUP, scan_bytes, ncells);
-    k_cells_count<<<gridc(nwords), kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, cnt.get());
+    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, cnt.get());
     SK_HIP(hipGetLastError());
   }
-  scan_u32(x, cnt.get(), off.get(), nwords, tot.get());
+  scan_u32(x, cnt.get(), off.get(), nblocks, tot.get());
   u64 n = read_u64(x, tot.get());
   if (n == 0) return 0;
   keys->reset(x.pool, n);
@@ -211,9 +255,9 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   }
   const double rec = sizeof(K) + (cd.trop ? 16.0 + (want_inf ? 1.0 : 0) : 0);
   Scope sc(x.timer, "cells_write", KC_DEDUP, scan_bytes + rec * n, ncells);
-  k_cells_write<K><<<gridc(nwords), kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, off.get(), bits,
-                                                           keys->get(), cd.trop ? w->get() : nullptr,
-                                                           (cd.trop && want_inf) ? inf->get() : nullptr);
+  k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, off.get(), bits,
+                                                               keys->get(), cd.trop ? w->get() : nullptr,
+                                                               (cd.trop && want_inf) ? inf->get() : nullptr);
   SK_HIP(hipGetLastError());
   return n;
 }

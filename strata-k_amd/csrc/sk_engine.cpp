@@ -508,6 +508,7 @@ void Engine::reset() {
   for (size_t i = 0; i < R_.size(); i++) {
     DRel& R = R_[i];
     R.delta.release();
+    R.delta_order.clear();
     R.dversion++;
     if (!written[i]) continue;
     R.bits = snap_bits_[i];
@@ -609,12 +610,16 @@ Index& Engine::get_index(int rel, bool delta, const std::vector<int>& perm, bool
   bool identity = true;
   for (size_t j = 0; j < perm.size(); j++)
     if (perm[j] != (int)j) identity = false;
+  // the list's own column order: key order, or a dense relation's cell order
+  const bool key_sorted = !delta || R.delta_order.empty();
+  const bool in_order = key_sorted ? identity : perm == R.delta_order;
   const RelData& src = source(rel, delta);
   if (ix.version != ver) {
     ix.perm = perm;
     ix.csr_version = ~0ull;
     ix.adj_version = ~0ull;
-    if (!identity) {
+    ix.direct = identity && key_sorted;
+    if (!ix.direct) {
       bool trop = R.sem == SK_TROP;
       u64 n = src.n;
       ix.data.n = n;
@@ -639,6 +644,7 @@ Index& Engine::get_index(int rel, bool delta, const std::vector<int>& perm, bool
       with_key(R.wide, [&](auto* tag) {
         using K = std::remove_pointer_t<decltype(tag)>;
         repack<K, K>(x, (const K*)src.k.ptr(), n, R.arity, bits, perm.data(), (K*)ix.data.k.ptr());
+        if (in_order) return;  // already sorted by the index's column order
         K* sk;
         i64* sw;
         u8* sf;
@@ -656,7 +662,7 @@ Index& Engine::get_index(int rel, bool delta, const std::vector<int>& perm, bool
     }
     ix.version = ver;
   }
-  const RelData& d = identity ? src : ix.data;
+  const RelData& d = ix.direct ? src : ix.data;
   if (want_csr && ix.csr_version != ver) {
     u64 nv = D_.nvals();
     ix.csr.ensure(x.pool, nv + 1);
@@ -1177,10 +1183,7 @@ void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& ou
       } else {
         bool want_csr = static_rel && st.nprobe == 1 && pd.nfilt == 0;
         Index& ix = get_index(st.rel, st.delta, st.perm, want_csr, false);
-        bool identity = true;
-        for (size_t q = 0; q < st.perm.size(); q++)
-          if (st.perm[q] != (int)q) identity = false;
-        const RelData& d = identity ? src : ix.data;
+        const RelData& d = ix.direct ? src : ix.data;
         pd.sn = d.n;
         Table tin = cur.t;
         if (nparts_ && trop) {
@@ -1258,6 +1261,7 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
   u64 total = 0;
   for (auto& c : chunks) total += c.n;
   R.delta.release();
+  R.delta_order.clear();
   R.dversion++;
   if (DenseRel* dn = dense_of(rel)) {
     // rows not applied by their emitting kernel (leapfrog output) go in now;
@@ -1280,6 +1284,7 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
       else R.delta.k.a = std::move(dk);
       R.delta.w = std::move(dw);
       R.delta.inf = std::move(df);
+      if (!dn->key_order) R.delta_order = dn->order;
       if (dn->keep_full) {
         DBuf<K> u_k, f_k;
         DBuf<i64> u_w, f_w;
@@ -1773,30 +1778,28 @@ void Engine::plan_dense(u32 k) {
   Exec x = ctx_->exec();
   const u64 nw = (nvals + 31) / 32;
 
-  // distinct codes of fixed columns (and of the stratum relations' own facts)
-  std::map<std::pair<int, int>, int> col_id;
-  std::vector<DBuf<u32>> col_bm;
-  std::vector<u64> col_n;
-  auto col_codes = [&](int rel, int col) {
-    auto key = std::make_pair(rel, col);
-    auto it = col_id.find(key);
-    if (it != col_id.end()) return it->second;
-    int id = (int)col_bm.size();
-    col_bm.emplace_back(x.pool, nw);
-    memset_async(x, col_bm.back().get(), 0, nw * 4);
-    const RelData& d = R_[rel].full;
-    with_key(R_[rel].wide, [&](auto* tag) {
+  // distinct codes of a column (kept across executes while the relation's
+  // version and the dictionary are unchanged, like the relation indexes)
+  auto col_codes = [&](int rel, int col) -> const ColCodes* {
+    ColCodes& cc = colcodes_[{rel, col}];
+    const DRel& Rr = R_[rel];
+    if (cc.version == Rr.version && cc.nvals == nvals && cc.bm.get()) return &cc;
+    cc.bm.reset(x.pool, nw);
+    memset_async(x, cc.bm.get(), 0, nw * 4);
+    const RelData& d = Rr.full;
+    with_key(Rr.wide, [&](auto* tag) {
       using K = std::remove_pointer_t<decltype(tag)>;
-      cells_mark_col<K>(x, (const K*)d.k.ptr(), d.n, R_[rel].arity, R_[rel].bits, col, col_bm.back().get());
+      cells_mark_col<K>(x, (const K*)d.k.ptr(), d.n, Rr.arity, Rr.bits, col, cc.bm.get());
     });
-    col_n.push_back(d.n ? popcount_bits(x, col_bm.back().get(), nw) : 0);
-    col_id[key] = id;
-    return id;
+    cc.count = d.n ? popcount_bits(x, cc.bm.get(), nw) : 0;
+    cc.version = Rr.version;
+    cc.nvals = nvals;
+    return &cc;
   };
 
   struct Node {
     bool all = false;
-    std::vector<int> fixed;
+    std::vector<const ColCodes*> fixed;
     std::vector<u32> consts;
     std::vector<std::pair<int, int>> deps;
   };
@@ -1825,7 +1828,9 @@ void Engine::plan_dense(u32 k) {
         nd.all = true;
         continue;
       }
-      int best = -1;
+      // fixed occurrences by row count: a column with no more rows than the
+      // best distinct count so far cannot beat it and is not scanned
+      std::vector<std::pair<u64, std::pair<int, int>>> occ;
       std::pair<int, int> dep{-1, -1};
       for (auto& a : r.body) {
         if (a.neg || a.rel < 0 || (int)a.args.size() != R_[a.rel].arity) continue;
@@ -1835,11 +1840,17 @@ void Engine::plan_dense(u32 k) {
             if (dep.first < 0) dep = {a.rel, (int)c};
             continue;
           }
-          int id = col_codes(a.rel, (int)c);
-          if (best < 0 || col_n[id] < col_n[best]) best = id;
+          occ.push_back({R_[a.rel].full.n, {a.rel, (int)c}});
         }
       }
-      if (best >= 0) nd.fixed.push_back(best);
+      std::sort(occ.begin(), occ.end());
+      const ColCodes* best = nullptr;
+      for (auto& o : occ) {
+        if (best && o.first >= best->count) break;
+        const ColCodes* cc = col_codes(o.second.first, o.second.second);
+        if (!best || cc->count < best->count) best = cc;
+      }
+      if (best) nd.fixed.push_back(best);
       else if (dep.first >= 0) nd.deps.push_back(dep);
       else nd.all = true;
     }
@@ -1863,7 +1874,7 @@ void Engine::plan_dense(u32 k) {
       std::vector<std::pair<int, int>> stack{{(int)i, j}};
       std::set<std::pair<int, int>> seen{{(int)i, j}};
       bool all = false;
-      std::vector<int> parts;  // column-code bitmaps to unite
+      std::vector<const ColCodes*> parts;  // column-code bitmaps to unite
       std::vector<u32> consts;
       while (!stack.empty() && !all) {
         auto at = stack.back();
@@ -1875,7 +1886,7 @@ void Engine::plan_dense(u32 k) {
         }
         const Node& nd = it->second;
         all = nd.all;
-        for (int f : nd.fixed) parts.push_back(f);
+        for (const ColCodes* f : nd.fixed) parts.push_back(f);
         for (u32 c : nd.consts) consts.push_back(c);
         if (R_[at.first].full.n) parts.push_back(col_codes(at.first, at.second));
         for (auto& d : nd.deps)
@@ -1886,7 +1897,7 @@ void Engine::plan_dense(u32 k) {
       } else {
         DBuf<u32> bm(x.pool, nw);
         memset_async(x, bm.get(), 0, nw * 4);
-        for (int pidx : parts) cells_or(x, bm.get(), col_bm[pidx].get(), nw);
+        for (const ColCodes* pc : parts) cells_or(x, bm.get(), pc->bm.get(), nw);
         for (u32 c : consts) cells_mark_code(x, bm.get(), c);
         u64 cnt = popcount_bits(x, bm.get(), nw);
         if (2 * cnt > nvals) {
@@ -1905,9 +1916,41 @@ void Engine::plan_dense(u32 k) {
       if (tot_bits > 40) ok = false;
     }
     if (!ok) continue;
-    for (int j = R.arity - 1, sh = 0; j >= 0; j--) {
-      cd.shift[j] = sh;
-      sh += cd.ebits[j];
+    // Cell order: when a rule scans this relation's delta and joins it on a
+    // variable held in column c, column c goes first, so the next delta (the
+    // touched cells, compacted in cell order) groups the join's driver rows by
+    // that column: rows sharing it read the same relation range and, with
+    // this layout, touch neighbouring cells.  Otherwise key order.
+    dn->order.resize(R.arity);
+    for (int j = 0; j < R.arity; j++) dn->order[j] = j;
+    if (!dn->keep_full && R.arity >= 2) {
+      int lead = -1;
+      for (auto& r : P_.rules) {
+        if (r.stratum != k || r.agg || lead >= 0) continue;
+        for (size_t j = 0; j < r.body.size() && lead < 0; j++) {
+          const Atom& a = r.body[j];
+          if (a.neg || a.rel != (int)i) continue;
+          for (size_t c = 0; c < a.args.size() && lead < 0; c++) {
+            if (a.args[c].kind != SK_T_VAR) continue;
+            for (size_t q = 0; q < r.body.size() && lead < 0; q++) {
+              if (q == j || r.body[q].neg) continue;
+              for (auto& t : r.body[q].args)
+                if (t.kind == SK_T_VAR && t.a == a.args[c].a) lead = (int)c;
+            }
+          }
+        }
+      }
+      if (lead > 0) {
+        dn->order.erase(dn->order.begin() + lead);
+        dn->order.insert(dn->order.begin(), lead);
+      }
+    }
+    dn->key_order = true;
+    for (int j = 0; j < R.arity; j++)
+      if (dn->order[j] != j) dn->key_order = false;
+    for (int q = R.arity - 1, sh = 0; q >= 0; q--) {
+      cd.shift[dn->order[q]] = sh;
+      sh += cd.ebits[dn->order[q]];
     }
     dn->ncells = 1ull << tot_bits;
     const u64 nc = std::max<u64>(dn->ncells, 32);
@@ -1971,6 +2014,24 @@ void Engine::end_dense() {
       DBuf<i64> fw;
       DBuf<u8> ff;
       u64 n = cells_compact<K>(x, dn->cd, dn->ncells, false, R.bits, &fk, &fw, &ff, hf);
+      if (!dn->key_order && n) {
+        // cell order differs from key order: one sort of the compacted state
+        DBuf<K> ak(x.pool, n);
+        DBuf<i64> aw;
+        DBuf<u8> af;
+        if (trop) aw.reset(x.pool, n);
+        if (hf) af.reset(x.pool, n);
+        K* sk;
+        i64* sw;
+        u8* sf;
+        radix_sort<K>(x, fk.get(), ak.get(), trop ? fw.get() : nullptr, trop ? aw.get() : nullptr,
+                      hf ? ff.get() : nullptr, hf ? af.get() : nullptr, n, key_bits_of(R.arity, R.bits), &sk, &sw, &sf);
+        if (sk != fk.get()) {
+          std::swap(fk, ak);
+          if (trop) std::swap(fw, aw);
+          if (hf) std::swap(ff, af);
+        }
+      }
       RelData nf;
       nf.n = n;
       nf.k.wide = sizeof(K) == 16;
@@ -1998,6 +2059,7 @@ void Engine::run_stratum(u32 k) {
   for (size_t i = 0; i < R_.size(); i++)
     if (in_idb_[i]) {
       R_[i].delta.release();
+      R_[i].delta_order.clear();
       R_[i].dversion++;
     }
   plan_dense(k);
@@ -2036,6 +2098,7 @@ void Engine::run_stratum(u32 k) {
       auto it = chunks.find((int)i);
       if (it == chunks.end() && !dense_of((int)i)) {
         R_[i].delta.release();
+        R_[i].delta_order.clear();
         R_[i].dversion++;
       } else if (it == chunks.end()) {
         std::vector<Chunk> none;
@@ -2063,6 +2126,7 @@ void Engine::run_stratum(u32 k) {
   for (size_t i = 0; i < R_.size(); i++)
     if (in_idb_[i]) {
       R_[i].delta.release();
+      R_[i].delta_order.clear();
       R_[i].dversion++;
     }
 }

@@ -120,6 +120,7 @@ struct RelData {
 struct Index {
   u64 version = ~0ull;
   std::vector<int> perm;
+  bool direct = false;  // the relation state itself is in index order (data unused)
   RelData data;
   DBuf<u64> csr;
   u64 csr_version = ~0ull;
@@ -133,6 +134,9 @@ struct DRel {
   int bits = 1;       // bits per packed column (codes < 2^bits)
   bool wide = false;  // arity * bits > 64: 128-bit keys
   RelData full, delta;
+  // column order the delta list is sorted by (empty = key order); a dense
+  // relation emits its delta in cell order
+  std::vector<int> delta_order;
   u64 version = 1, dversion = 1;
   std::map<std::string, Index> idx;
   // staged EDB facts (host, before sk_run_load)
@@ -149,6 +153,8 @@ struct DRel {
 struct DenseRel {
   bool keep_full = false;  // some rule of the stratum reads the relation's full
                            // state: each round's delta is also merged into it
+  std::vector<int> order;  // cell column order, most significant first
+  bool key_order = true;   // order is the key's column order
   u64 ncells = 0;
   CellDesc cd;
   DBuf<u64> cell;
@@ -156,6 +162,12 @@ struct DenseRel {
   DBuf<u32> dom_bits[kMaxCols];
   DBuf<u64> dom_pre[kMaxCols];
   DBuf<u32> dom_list[kMaxCols];
+};
+
+// Distinct codes of one relation column, as a bitmap over the dictionary.
+struct ColCodes {
+  u64 version = ~0ull, nvals = 0, count = 0;
+  DBuf<u32> bm;
 };
 
 // ------------------------------------------------------------- plans
@@ -292,6 +304,7 @@ class Engine {
   std::vector<bool> in_idb_;  // relation changes in the stratum being run
   DBuf<i64> int_vals_;        // device copy of Dict::ints
   std::vector<std::unique_ptr<DenseRel>> dense_;  // per relation, while its stratum runs
+  std::map<std::pair<int, int>, ColCodes> colcodes_;  // column code sets, by (relation, column)
   bool dense_enabled_ = true;
 
   void build_dict();

@@ -322,10 +322,210 @@ __global__ void k_join_emit_rows(Table in, ProbeDesc p, OutDesc o, const K* __re
   }
 }
 
+// Dense-head emit (the head rows are ⊕-applied to cells): the same output
+// tiling as k_join_emit_lb (output k*kBlock + t of a tile goes to thread t,
+// so relation reads stay coalesced), but each thread runs its kEmitItems
+// outputs in phases — all row searches interleaved, then all relation and
+// table reads, then all cell reads, then the applications — so that every
+// lane has kEmitItems independent memory requests in flight instead of one
+// dependent chain per output.
+template <class K, bool TROP>
+__global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
+                                                       const i64* __restrict__ sw, const u8* __restrict__ sf,
+                                                       const u64* __restrict__ lo, const u64* __restrict__ off,
+                                                       u64 total, DevErr* err) {
+  __shared__ u64 s_off[kEmitLds];
+  __shared__ u64 s_r[2];
+  const u64 o0 = (u64)blockIdx.x * kEmitTile;
+  const u64 o1 = min(o0 + kEmitTile, total);
+  const u64 nrows = in.n;
+  if (threadIdx.x < 2) {
+    u64 q = threadIdx.x == 0 ? o0 : o1 - 1;
+    u64 a = 0, b = nrows;
+    while (a < b) {
+      u64 m = (a + b) >> 1;
+      if (off[m] <= q) a = m + 1;
+      else b = m;
+    }
+    s_r[threadIdx.x] = a - 1;
+  }
+  __syncthreads();
+  const u64 r0 = s_r[0], r1 = s_r[1];
+  const u64 span = r1 - r0 + 1;
+  const bool staged = span <= (u64)kEmitLds;
+  if (staged)
+    for (u64 q = threadIdx.x; q < span; q += kBlock) s_off[q] = off[r0 + q];
+  __syncthreads();
+
+  // phase 1: the row of every output — interleaved, branch-free binary
+  // searches (one LDS read per output per step, all issued together)
+  u64 q[kEmitItems], row[kEmitItems], rbeg[kEmitItems];
+  bool ok[kEmitItems];
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) {
+    q[k] = o0 + (u64)k * kBlock + threadIdx.x;
+    ok[k] = q[k] < o1;
+    q[k] = min(q[k], o1 - 1);
+  }
+  if (staged) {
+    u32 a[kEmitItems], b[kEmitItems];
+#pragma unroll
+    for (int k = 0; k < kEmitItems; k++) {
+      a[k] = 0;
+      b[k] = (u32)span;
+    }
+    for (u64 wdt = span; wdt; wdt >>= 1) {
+      u64 v[kEmitItems];
+      u32 m[kEmitItems];
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) {
+        m[k] = min((a[k] + b[k]) >> 1, (u32)span - 1);
+        v[k] = s_off[m[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) {
+        const bool live = a[k] < b[k];
+        const bool right = v[k] <= q[k];
+        a[k] = (live && right) ? m[k] + 1 : a[k];
+        b[k] = (live && !right) ? m[k] : b[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kEmitItems; k++) {
+      const u32 ai = a[k] - 1;  // s_off[0] <= o0 <= q[k], so a[k] >= 1
+      row[k] = r0 + ai;
+      rbeg[k] = s_off[ai];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kEmitItems; k++) {
+      u64 lo2 = r0, hi2 = r1 + 1;
+      while (lo2 < hi2) {
+        u64 m = (lo2 + hi2) >> 1;
+        if (off[m] <= q[k]) lo2 = m + 1;
+        else hi2 = m;
+      }
+      row[k] = lo2 - 1;
+      rbeg[k] = off[lo2 - 1];
+    }
+  }
+
+  // phase 2: relation records, then head columns one at a time across all
+  // outputs (uniform branches outside, independent reads inside)
+  u64 sidx[kEmitItems];
+  K key[kEmitItems];
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) sidx[k] = lo[row[k]] + (q[k] - rbeg[k]);
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) key[k] = skeys[sidx[k]];
+  u64 cell[kEmitItems];
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) cell[k] = 0;
+  for (int j = 0; j < o.ncols; j++) {
+    u32 code[kEmitItems];
+    const int kind = o.kind[j];
+    if (kind == 0) {
+      const u32* col = in.col[o.idx[j]];
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) code[k] = col[row[k]];
+    } else if (kind == 1) {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) code[k] = col_of<K>(key[k], p.arity, p.bits, o.idx[j]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) code[k] = o.cnst[j];
+    }
+    const u32* db = o.cd.dom_bits[j];
+    const int sh = o.cd.shift[j], eb = o.cd.ebits[j];
+    if (!db) {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) {
+        const bool in_dom = code[k] < o.cd.ncodes && !((u64)code[k] >> eb);
+        ok[k] = ok[k] && in_dom;
+        cell[k] |= (u64)code[k] << sh;
+      }
+    } else {
+      const u64* dp = o.cd.dom_pre[j];
+      u32 wbits[kEmitItems];
+      u64 pre[kEmitItems];
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) {
+        const u32 wi = min(code[k], (u32)(o.cd.ncodes - 1)) >> 5;
+        wbits[k] = db[wi];
+        pre[k] = dp[wi];
+      }
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) {
+        const u32 m = 1u << (code[k] & 31);
+        const u64 rk = pre[k] + (u64)__popc(wbits[k] & (m - 1));
+        const bool in_dom = code[k] < o.cd.ncodes && (wbits[k] & m) && !(rk >> eb);
+        ok[k] = ok[k] && in_dom;
+        cell[k] |= rk << sh;
+      }
+    }
+  }
+  // a valid output whose value lies outside the domain is a planning bug
+  {
+    bool badv = false;
+#pragma unroll
+    for (int k = 0; k < kEmitItems; k++) badv |= (q[k] == o0 + (u64)k * kBlock + threadIdx.x) && !ok[k] &&
+                                                 (o0 + (u64)k * kBlock + threadIdx.x < o1);
+    if (badv) atomicOr(o.cd.bad, 1u);
+  }
+  i64 res[kEmitItems];
+  u8 rf[kEmitItems];
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) {
+    res[k] = 0;
+    rf[k] = 0;
+  }
+  if constexpr (TROP) {
+    i64 aw[kEmitItems], bw[kEmitItems];
+    u8 af[kEmitItems], bf[kEmitItems];
+#pragma unroll
+    for (int k = 0; k < kEmitItems; k++) {
+      aw[k] = in.w[row[k]];
+      bw[k] = sw[sidx[k]];
+      af[k] = 0;
+      bf[k] = 0;
+    }
+    if (in.inf) {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) af[k] = in.inf[row[k]];
+    }
+    if (sf) {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) bf[k] = sf[sidx[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kEmitItems; k++) {
+      if (!ok[k]) continue;
+      DevErr* e = in.part_col >= 0 ? &in.perr[in.col[in.part_col][row[k]]] : err;
+      trop_mul(aw[k], af[k], bw[k], bf[k], &res[k], &rf[k], e);
+    }
+  }
+  // phase 3: every cell read before any application (unconditional: an
+  // output that is not applied reads cell 0 and ignores it)
+  u64 seen[kEmitItems];
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) {
+    const u64 c = ok[k] ? cell[k] : 0;
+    if constexpr (TROP) seen[k] = o.cd.cell[c];
+    else seen[k] = o.cd.present[c >> 5];
+  }
+  // phase 4: apply
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++)
+    if (ok[k]) cell_apply_t<TROP>(o.cd, cell[k], res[k], rf[k], seen[k]);
+}
+
 template <class K, class KO, bool FINAL, bool TROP>
 static void emit_launch(const Exec& x, const Table& in, const ProbeDesc& p, const OutDesc& o, const K* skeys,
                         const i64* sw, const u8* sf, const u64* lo, const u32* cnt, const u64* off, u64 total) {
-  if (p.nfilt == 0) {
+  if (FINAL && o.cd.on && p.nfilt == 0) {
+    u64 nb = (total + kEmitTile - 1) / kEmitTile;
+    k_join_apply<K, TROP><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total, x.err);
+  } else if (p.nfilt == 0) {
     u64 nb = (total + kEmitTile - 1) / kEmitTile;
     k_join_emit_lb<K, KO, FINAL, TROP><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total,
                                                                                x.err);

@@ -107,6 +107,15 @@ def test_ranked_domains_and_head_constants(ctx):
     both(ctx, prog, bulk=bulk, label="ranked")
 
 
+def test_rows_without_outputs_outside_domain(ctx):
+    """Sources with no edges: cost's first column is ranked over edge's first
+    column (fewer rows than src), so those sources lie outside the domain; the
+    rows that emit nothing must not be decoded."""
+    edb = [("edge", (Sym(1), Sym(7)), 17), ("edge", (Sym(7), Sym(3)), 27), ("edge", (Sym(4), Sym(0)), 28)]
+    edb += [("src", (Sym(s),), 0) for s in (1, 5, 6, 7)]
+    both(ctx, cost_program(), edb, label="emptyrows")
+
+
 def test_empty_and_single_cell(ctx):
     prog = CoreProgram([pred("e", 2, BOOL), pred("z", 0, BOOL), pred("p", 2, BOOL)],
                        [rule(atom("z"), [pos("e", v(0), v(1))], 2),
