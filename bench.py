@@ -20,6 +20,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "strata-k_amd"))
 sys.path.insert(0, ROOT)
@@ -195,6 +197,134 @@ def bench_cfg3(args):
                       "roofline": roofline_of(top)}), flush=True)
 
 
+def bench_cfg4(args):
+    """Batched: 512 independent programs (256 Bool TC, 256 Trop cost) through
+    one batch, against 512 sequential plain runs of the same programs.  Both
+    sides are timed end to end (host EDB in, host results out); every batched
+    result is compared with its plain run."""
+    import strata_k as sk
+    from strata_k import workloads as wl
+    t = time.time()
+    progs = wl.batch_programs(seed=args.seed if args.seed != 2 else 4, count=args.programs, cost_sources=1,
+                              tc_node_cap=2000, tc_edge_cap=10_000)
+    log(f"generated {len(progs)} programs in {time.time() - t:.1f}s")
+    ctx = sk.Context(0)
+    # plain: one prepared run per program
+    plain, plain_exec_ms = [], 0.0
+    t0 = time.perf_counter()
+    for p in progs:
+        run = ctx.prepare(p.program)
+        for pred, (cols, w) in p.bulk.items():
+            run.add_sym(pred, cols, w, None)
+        run.load()
+        run.execute()
+        plain_exec_ms += run.execute_ms
+        plain.append((run.relation_sym(p.output), run.iterations()))
+        run.free()
+    ctx.synchronize()
+    plain_s = time.perf_counter() - t0
+    # batched
+    t0 = time.perf_counter()
+    b = sk.Batch(ctx)
+    for p in progs:
+        b.add(p.program, p.bulk)
+    t_add = time.perf_counter()
+    b.load()
+    t_load = time.perf_counter()
+    b.execute()
+    t_exec = time.perf_counter()
+    batched = [(b.relation_sym(i, p.output), b.iterations(i)) for i, p in enumerate(progs)]
+    ctx.synchronize()
+    t_read = time.perf_counter()
+    batch_s = t_read - t0
+    phases = {"add_s": round(t_add - t0, 3), "load_s": round(t_load - t_add, 3), "execute_s": round(t_exec - t_load, 3),
+              "read_s": round(t_read - t_exec, 3)}
+    batch_exec_ms = b.execute_ms
+    b.free()
+    exact = True
+    n_out = 0
+    for (pa, pi), (ba, bi) in zip(plain, batched):
+        (pc, pw, _), (bc, bw) = pa, ba
+        n_out += len(pc[0])
+        same = pi == bi and len(pc[0]) == len(bc[0]) and all((x == y).all() for x, y in zip(pc, bc))
+        if pw is not None:
+            same = same and (pw == bw).all()
+        exact = exact and same
+    ctx.close()
+    print(json.dumps({
+        "metric": "batched-vs-plain speedup", "value": round(plain_s / batch_s, 2), "unit": "x",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "int64", "data": "synthetic (seeded ER / RMAT)",
+        "config": {"workload": f"{len(progs)} programs: {len(progs) // 2} Bool TC (n <= 2000, m <= 10000) + "
+                               f"{len(progs) - len(progs) // 2} Trop cost (n in [1e3, 2e5], m in [4e3, 2e6], 1 source)",
+                   "derived_tuples": n_out},
+        "plain_s": round(plain_s, 3), "batched_s": round(batch_s, 3),
+        "plain_tuples_per_s": round(n_out / plain_s, 1), "batched_tuples_per_s": round(n_out / batch_s, 1),
+        "plain_device_ms": round(plain_exec_ms, 1), "batched_device_ms": round(batch_exec_ms, 1),
+        "device_speedup": round(plain_exec_ms / batch_exec_ms, 2) if batch_exec_ms else None,
+        "batched_phases": phases,
+        "bit_exact_vs_plain": bool(exact)}), flush=True)
+
+
+def bench_cfg5(args):
+    """Served: resident RMAT graph, Poisson query stream (arrival times from
+    the workload generator), continuous batching; p50/p99 latency and
+    queries/s.  Capacity: the same queries offered at once, batched versus
+    one query at a time on the same resident graph."""
+    import strata_k as sk
+    from strata_k import workloads as wl
+    t = time.time()
+    scale = args.scale if args.scale != 20 else 24
+    s, d, w, n = wl.config5_graph(seed=5, scale=scale)
+    log(f"graph scale {scale}: {len(s)} edges in {time.time() - t:.1f}s")
+    qs = wl.config5_queries(seed=5, n_queries=args.queries, n_nodes=n, max_sources=args.max_sources,
+                            n_targets=64, filter_frac=0.1, rate_per_s=args.rate)
+    ctx = sk.Context(0)
+
+    def serve(max_batch, open_loop):
+        srv = sk.Server(ctx, s, d, w, max_batch=max_batch)
+        pending = list(range(len(qs)))
+        tickets, done = {}, {}
+        t0 = time.perf_counter()
+        while len(done) < len(qs):
+            now = time.perf_counter() - t0
+            while pending and (not open_loop or qs[pending[0]]["arrival_s"] <= now):
+                i = pending.pop(0)
+                tickets[i] = srv.submit(qs[i]["sources"], qs[i]["targets"])
+            active, waiting = srv.step()
+            for i, tk in list(tickets.items()):
+                if i in done:
+                    continue
+                r = srv.poll(tk)
+                if r is not None:
+                    done[i] = r
+                    srv.release(tk)
+            if active == 0 and waiting == 0 and pending and open_loop:
+                time.sleep(max(0.0, min(0.002, qs[pending[0]]["arrival_s"] - (time.perf_counter() - t0))))
+        total = time.perf_counter() - t0
+        srv.free()
+        return total, done
+
+    t_open, res = serve(args.max_batch, True)
+    lat = np.array([res[i]["latency_ms"] for i in range(len(qs))])
+    rows = sum(res[i]["rows"] for i in range(len(qs)))
+    t_cap, _ = serve(args.max_batch, False)
+    t_seq, res1 = serve(1, False)
+    exact = all(res1[i]["rows"] == res[i]["rows"] and res1[i]["iterations"] == res[i]["iterations"]
+                for i in range(len(qs)))
+    ctx.close()
+    print(json.dumps({
+        "metric": "p99 query ms", "value": round(float(np.percentile(lat, 99)), 2), "unit": "ms",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "int64", "data": "synthetic (seeded RMAT, Poisson arrivals)",
+        "config": {"workload": f"served cheapest cost, RMAT scale {scale} ({n} nodes, {len(s)} edges), "
+                               f"{len(qs)} queries, sources log-uniform [1, {args.max_sources}], 10% filtered",
+                   "rate_per_s": args.rate, "max_batch": args.max_batch, "answer_rows": int(rows)},
+        "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+        "served_qps": round(len(qs) / t_open, 2),
+        "capacity_qps_batched": round(len(qs) / t_cap, 2), "capacity_qps_one_at_a_time": round(len(qs) / t_seq, 2),
+        "batched_vs_one_at_a_time": round(t_seq / t_cap, 2),
+        "same_rows_and_rounds_alone_and_batched": bool(exact)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +335,11 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels", action="store_true")
+    ap.add_argument("--programs", type=int, default=512, help="config 4: number of programs")
+    ap.add_argument("--queries", type=int, default=64, help="config 5: number of queries")
+    ap.add_argument("--max-sources", type=int, default=16, help="config 5: largest source set")
+    ap.add_argument("--rate", type=float, default=20.0, help="config 5: arrivals per second")
+    ap.add_argument("--max-batch", type=int, default=16, help="config 5: queries in flight")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one known-size copy kernel (PMC counter calibration)")
     args = ap.parse_args()
@@ -214,7 +349,7 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
-    {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3}[args.config](args)
+    {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3, 4: bench_cfg4, 5: bench_cfg5}[args.config](args)
 
 
 if __name__ == "__main__":

@@ -152,18 +152,37 @@ sk_status sk_batch_load(sk_batch* b) {
       g.eng->validate();
       g.eng->set_partitions((u32)g.members.size());
       // stage every member's facts with its program id as column 0
+      std::map<std::string, u64> rows;
+      u64 max_n = 0;
+      for (int m : g.members) {
+        BProgram& bp = b->progs[m];
+        for (size_t e = 0; e < bp.edb_preds.size(); e++) {
+          rows[p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name] += bp.edb_n[e];
+          max_n = std::max<u64>(max_n, bp.edb_n[e]);
+        }
+      }
+      for (auto& kv : rows) g.eng->reserve_staging(kv.first, kv.second);
+      std::vector<u32> pid(max_n);
       for (int m = 0; m < (int)g.members.size(); m++) {
         BProgram& bp = b->progs[g.members[m]];
-        // per-program universe: distinct Sym values of its EDB
-        std::set<u32> syms;
+        // per-program universe: distinct Sym values of its EDB (a bitmap
+        // over the program's largest id)
+        u32 mx = 0;
         for (auto& cols : bp.edb_cols)
-          for (auto& c : cols) syms.insert(c.begin(), c.end());
-        bp.universe = syms.size();
+          for (auto& c : cols)
+            for (u32 v : c) mx = std::max(mx, v);
+        std::vector<u64> seen(((u64)mx >> 6) + 1, 0);
+        for (auto& cols : bp.edb_cols)
+          for (auto& c : cols)
+            for (u32 v : c) seen[v >> 6] |= 1ull << (v & 63);
+        u64 distinct = 0;
+        for (u64 wd : seen) distinct += (u64)__builtin_popcountll(wd);
+        bp.universe = distinct;
         g.eng->set_partition_universe((u32)m, bp.universe);
         for (size_t e = 0; e < bp.edb_preds.size(); e++) {
           const std::string& pname = p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name;
           u64 n = bp.edb_n[e];
-          std::vector<u32> pid(n, (u32)m);
+          std::fill(pid.begin(), pid.begin() + n, (u32)m);
           std::vector<const u32*> cols;
           cols.push_back(pid.data());
           for (auto& c : bp.edb_cols[e]) cols.push_back(c.data());
@@ -250,9 +269,19 @@ int64_t sk_batch_result_size(sk_batch* b, uint32_t index, const char* pred) {
   const std::string& fname = g.fused.preds[r].name;
   try {
     return (int64_t)g.eng->partition_size(fname, (u32)bp.member);
-  } catch (...) {
-    return -1;
+  } catch (const EvalErr& e) {
+    bp.err = e;
+  } catch (const DeviceError& d) {
+    bp.err.code = SK_ERR_DEVICE;
+    bp.err.msg = d.where;
+  } catch (const BudgetError&) {
+    bp.err.code = SK_ERR_OUT_OF_BUDGET;
+    bp.err.msg = "device memory budget exceeded while reading the result";
+  } catch (const std::exception& e) {
+    bp.err.code = SK_ERR_DEVICE;
+    bp.err.msg = e.what();
   }
+  return -1;
 }
 
 sk_status sk_batch_result_read_sym(sk_batch* b, uint32_t index, const char* pred, uint32_t* const* cols,

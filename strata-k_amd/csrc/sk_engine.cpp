@@ -247,13 +247,21 @@ void Engine::stage_rows(const std::string& pred, u64 n, const sk_val* rows, cons
   }
   // keep the generic rows aligned with their weights: generic rows are
   // appended after any all-Sym stage of the same relation at load time
-  if (weighted)
+  if (weighted) {
+    if (R.st_reserve > R.st_w.capacity()) {
+      R.st_w.reserve(R.st_reserve);
+      R.st_inf.reserve(R.st_reserve);
+    }
+    const u64 base = R.st_w.size();
+    R.st_w.resize(base + n);
+    R.st_inf.resize(base + n);
     for (u64 i = 0; i < n; i++) {
       u8 f = inf ? inf[i] : 0;
-      R.st_w.push_back(f ? 0 : w[i]);  // +inf rows carry weight 0, like every derived +inf
-      R.st_inf.push_back(f);
+      R.st_w[base + i] = f ? 0 : w[i];  // +inf rows carry weight 0, like every derived +inf
+      R.st_inf[base + i] = f;
       if (f) has_inf_ = true;
     }
+  }
   R.st_weighted = weighted;
   R.st_rows += n;
 }
@@ -285,7 +293,10 @@ void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* con
       }
   } else {
     R.st_sym.resize(arity);
-    for (u32 j = 0; j < arity; j++) R.st_sym[j].insert(R.st_sym[j].end(), cols[j], cols[j] + n);
+    for (u32 j = 0; j < arity; j++) {
+      if (R.st_reserve > R.st_sym[j].capacity()) R.st_sym[j].reserve(R.st_reserve);
+      R.st_sym[j].insert(R.st_sym[j].end(), cols[j], cols[j] + n);
+    }
   }
   if (weighted)
     for (u64 i = 0; i < n; i++) {
@@ -441,8 +452,9 @@ void Engine::load() {
   build_dict();
   for (size_t r = 0; r < R_.size(); r++) upload_rel((int)r);
   // distinct Sym values after the EDB load (crates/strata-eval/src/naive.rs:143-155)
+  // (batched runs cap each program by its own universe instead)
   if (universe_override_ != ~0ull) universe_ = universe_override_;
-  else compute_universe();
+  else if (!nparts_) compute_universe();
   for (size_t r = 0; r < R_.size(); r++)
     if (!attached_.empty() && attached_[r] && R_[r].bits < D_.bits())
       throw mkerr(SK_ERR_INVALID_ARGUMENT, P_.preds[r].name, "attached relation packed narrower than the dictionary");
@@ -1760,7 +1772,7 @@ static int ceil_log2(u64 n) {
 void Engine::plan_dense(u32 k) {
   dense_.clear();
   dense_.resize(R_.size());
-  if (!dense_enabled_ || nparts_) return;
+  if (!dense_enabled_) return;
   std::vector<bool> cand(R_.size(), false);
   for (size_t i = 0; i < R_.size(); i++) {
     if (!in_idb_[i] || R_[i].arity > kMaxCols) continue;
@@ -1923,7 +1935,9 @@ void Engine::plan_dense(u32 k) {
     // this layout, touch neighbouring cells.  Otherwise key order.
     dn->order.resize(R.arity);
     for (int j = 0; j < R.arity; j++) dn->order[j] = j;
-    if (!dn->keep_full && R.arity >= 2) {
+    // (batched runs keep key order: their deltas are split by the program id
+    // in column 0)
+    if (!dn->keep_full && R.arity >= 2 && !nparts_) {
       int lead = -1;
       for (auto& r : P_.rules) {
         if (r.stratum != k || r.agg || lead >= 0) continue;
@@ -2160,6 +2174,10 @@ void Engine::set_partitions(u32 np) {
 std::vector<u64> Engine::part_counts(int rel, bool delta) {
   DRel& R = R_[rel];
   const RelData& d = delta ? R.delta : R.full;
+  // the full state's counts are read once per partition when results are
+  // taken: keep them while the relation is unchanged
+  auto cached = part_full_counts_.find(rel);
+  if (!delta && cached != part_full_counts_.end() && cached->second.first == R.version) return cached->second.second;
   std::vector<u64> cnt(nparts_, 0);
   if (d.n == 0) return cnt;
   Exec x = ctx_->exec();
@@ -2172,6 +2190,7 @@ std::vector<u64> Engine::part_counts(int rel, bool delta) {
   SK_HIP(hipMemcpyAsync(h.data(), csr.get(), (nparts_ + 1) * 8, hipMemcpyDeviceToHost, x.stream));
   SK_HIP(hipStreamSynchronize(x.stream));
   for (u32 p = 0; p < nparts_; p++) cnt[p] = h[p + 1] - h[p];
+  if (!delta) part_full_counts_[rel] = {R.version, cnt};
   return cnt;
 }
 

@@ -147,6 +147,7 @@ struct DRel {
   std::vector<u8> st_inf;
   bool st_weighted = false;
   u64 st_rows = 0;
+  u64 st_reserve = 0;  // expected staged rows (capacity hint)
 };
 
 // Dense cell state of one relation while its stratum runs (see sk_cells.h).
@@ -273,6 +274,11 @@ class Engine {
               u64 max_sym, u64 universe_syms);
   // Use a given universe instead of scanning the relations at load.
   void set_universe(u64 u) { universe_override_ = u; }
+  // Expected number of staged rows of a relation (avoids regrowth).
+  void reserve_staging(const std::string& pred, u64 rows) {
+    int r = P_.rel_of(pred);
+    if (r >= 0) R_[r].st_reserve = rows;
+  }
   // Keep eligible relations as dense cells during their stratum (default on).
   void set_dense(bool on) { dense_enabled_ = on; }
 
@@ -286,6 +292,7 @@ class Engine {
   std::vector<std::vector<u64>> part_iters_;
   std::vector<u8> part_dead_;
   DBuf<DevErr> part_derr_;
+  std::map<int, std::pair<u64, std::vector<u64>>> part_full_counts_;  // by relation: (version, counts)
   DBuf<u8> part_dead_dev_;
   u64 attached_max_sym_ = 0;
   u64 universe_override_ = ~0ull;

@@ -945,7 +945,8 @@ __global__ void k_mark(const u32* __restrict__ c, u64 n, u32 limit, u32* __restr
   u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   u32 v = c[i];
-  if (v < limit) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+  // repeated values are common (hub ids): skip the atomic when the bit is visible
+  if (v < limit && !(bitmap[v >> 5] & (1u << (v & 31)))) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
 }
 
 __global__ void k_popcount(const u32* __restrict__ b, u64 n, unsigned long long* acc) {

@@ -65,10 +65,12 @@ __global__ void k_fill_u8(u8* p, u8 v, u64 n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-static unsigned grid_for(u64 n, u64 per = kBlock, unsigned cap = 65535u * 4) {
+// One thread per element: the exact block count (the kernels launched with it
+// have no grid-stride loop, so the grid is never clipped).
+static unsigned grid_for(u64 n, u64 per = kBlock) {
   u64 g = (n + per - 1) / per;
   if (g == 0) g = 1;
-  if (g > cap) g = cap;
+  if (g > 0x7fffffffull) throw BudgetError{n};
   return (unsigned)g;
 }
 

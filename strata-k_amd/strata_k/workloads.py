@@ -137,12 +137,13 @@ class BatchProgram:
 
 def batch_programs(seed: int = 4, count: int = 512, n_lo: int = 1000, n_hi: int = 200_000, m_lo: int = 4000,
                    m_hi: int = 2_000_000, cost_sources: int = 1, tc_node_cap: Optional[int] = None,
-                   ) -> List[BatchProgram]:
+                   tc_edge_cap: Optional[int] = None) -> List[BatchProgram]:
     """Independent programs: half Bool TC, half Trop cheapest cost, each on its
     own seeded ER or RMAT graph with log-uniform node and edge counts.
 
-    ``tc_node_cap`` / ``cost_sources`` bound the output volume (the Bool
-    closures of the uncapped configuration total ~10^11 tuples)."""
+    ``tc_node_cap`` / ``tc_edge_cap`` / ``cost_sources`` bound the output
+    volume (the Bool closures of the uncapped configuration total ~10^11
+    tuples)."""
     out = []
     for i in range(count):
         ps = seed * 1_000_003 + i
@@ -152,6 +153,8 @@ def batch_programs(seed: int = 4, count: int = 512, n_lo: int = 1000, n_hi: int 
         if kind == "tc" and tc_node_cap:
             n = min(n, tc_node_cap)
         m = log_uniform(ps, 2, m_lo, m_hi)
+        if kind == "tc" and tc_edge_cap:
+            m = min(m, tc_edge_cap)
         m = min(m, n * (n - 1) // 2)
         if gen == "er":
             s, d = uniform_graph(ps, n, m)

@@ -116,3 +116,30 @@ def test_config3_full_count(ctx):
     assert n == want
     assert ntri == [((sk.IntV(want),), None)]
     print("config3 full:", n, "triangles")
+
+
+def test_relation_beyond_2_26_rows(ctx):
+    """Load, ⊕-normalise and read back a relation of ~70M distinct rows (more
+    than 2^26, the size past which a clipped launch grid would drop rows), and
+    join over it (CSR index over all rows)."""
+    from strata_k.core_ir import BOOL, atom, pos, rule, v
+    rng = np.random.default_rng(11)
+    n = 72_000_000
+    a = rng.integers(0, 1 << 20, n, dtype=np.uint32)
+    b = rng.integers(0, 1 << 20, n, dtype=np.uint32)
+    key = np.unique(a.astype(np.uint64) << np.uint64(32) | b)
+    prog = CoreProgram([pred("e", 2, BOOL), pred("f", 1, BOOL), pred("h", 2, BOOL)],
+                       [rule(atom("h", v(0), v(1)), [pos("f", v(0)), pos("e", v(0), v(1))], 2)], 1)
+    fsel = np.array([7, 99_999, (1 << 20) - 1], dtype=np.uint32)
+    run = ctx.prepare(prog)
+    run.add_sym("e", [a, b])
+    run.add_sym("f", [fsel])
+    run.load()
+    run.execute()
+    (ga, gb), _, _ = run.relation_sym("e")
+    (ha, hb), _, _ = run.relation_sym("h")
+    run.free()
+    assert len(ga) == len(key)
+    assert (ga == (key >> np.uint64(32)).astype(np.uint32)).all() and (gb == (key & np.uint64(0xffffffff))).all()
+    want = key[np.isin((key >> np.uint64(32)).astype(np.uint32), fsel)]
+    assert (ha.astype(np.uint64) << np.uint64(32) | hb).tolist() == want.tolist()
