@@ -140,10 +140,11 @@ void cells_load(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, 
 template <class K>
 void cells_insert(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, const u8* inf, u64 n, int arity,
                   int bits);
-// Compact the touched cells (clearing them) or all present cells into a
-// sorted packed relation; returns its size.  The outputs are (re)allocated.
+// Compact the touched cells (clearing them) or all present cells of
+// [cell0, cell0 + ncells) into a packed relation in cell order; returns its
+// size.  cell0 is a multiple of 32.  The outputs are (re)allocated.
 template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
-                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf);
+                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0 = 0);
 
 }  // namespace sk

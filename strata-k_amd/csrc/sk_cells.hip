@@ -180,24 +180,25 @@ __device__ __forceinline__ u32 cells_block_scan(u32 v, u32* tmp, u32* total) {
   return base + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_cells_count(CellDesc cd, u64 nwords, int touched_only,
+__global__ __launch_bounds__(kBlock) void k_cells_count(CellDesc cd, u64 w0, u64 nwords, int touched_only,
                                                         u32* __restrict__ cnt) {
   __shared__ u32 tmp[kBlock / 64];
-  const u64 wi = (u64)blockIdx.x * kBlock + threadIdx.x;
-  u32 c = wi < nwords ? (u32)__popc(cells_word(cd, wi, touched_only != 0)) : 0u;
+  const u64 wl = (u64)blockIdx.x * kBlock + threadIdx.x;
+  u32 c = wl < nwords ? (u32)__popc(cells_word(cd, w0 + wl, touched_only != 0)) : 0u;
   u32 tot;
   cells_block_scan(c, tmp, &tot);
   if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
 template <class K>
-__global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 nwords, int touched_only,
+__global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64 nwords, int touched_only,
                                                         const u64* __restrict__ off, int bits, K* __restrict__ keys,
                                                         i64* __restrict__ w, u8* __restrict__ inf) {
   __shared__ u32 tmp[kBlock / 64];
   __shared__ unsigned short sc[kCellsPerBlock];
-  const u64 wi = (u64)blockIdx.x * kBlock + threadIdx.x;
-  u32 b = wi < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
+  const u64 wl = (u64)blockIdx.x * kBlock + threadIdx.x;
+  const u64 wi = w0 + wl;
+  u32 b = wl < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
   if (touched_only && b) cd.touched[wi] = 0;
   u32 tot;
   u32 o = cells_block_scan((u32)__popc(b), tmp, &tot);
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 nwords,
   }
   __syncthreads();
   const u64 base = off[blockIdx.x];
-  const u64 cbase = (u64)blockIdx.x * kCellsPerBlock;
+  const u64 cbase = w0 * 32 + (u64)blockIdx.x * kCellsPerBlock;
   for (u32 j = threadIdx.x; j < tot; j += kBlock) {
     const u64 c = cbase + sc[j];
     K key = 0;
@@ -233,7 +234,8 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 nwords,
 
 template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
-                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf) {
+                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0) {
+  const u64 w0 = cell0 / 32;  // cell0 is a multiple of 32
   const u64 nwords = (ncells + 31) / 32;
   const u64 nblocks = (nwords + kBlock - 1) / kBlock;
   DBuf<u32> cnt(x.pool, nblocks);
@@ -242,7 +244,7 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   const double scan_bytes = touched_only ? 4.0 * nwords : (cd.trop ? 8.0 * ncells + 8.0 * nwords : 4.0 * nwords);
   {
     Scope sc(x.timer, "cells_count", KC_DEDUP, scan_bytes, ncells);
-    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, cnt.get());
+    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, w0, nwords, touched_only ? 1 : 0, cnt.get());
     SK_HIP(hipGetLastError());
   }
   scan_u32(x, cnt.get(), off.get(), nblocks, tot.get());
@@ -255,7 +257,7 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   }
   const double rec = sizeof(K) + (cd.trop ? 16.0 + (want_inf ? 1.0 : 0) : 0);
   Scope sc(x.timer, "cells_write", KC_DEDUP, scan_bytes + rec * n, ncells);
-  k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, nwords, touched_only ? 1 : 0, off.get(), bits,
+  k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, w0, nwords, touched_only ? 1 : 0, off.get(), bits,
                                                                keys->get(), cd.trop ? w->get() : nullptr,
                                                                (cd.trop && want_inf) ? inf->get() : nullptr);
   SK_HIP(hipGetLastError());
@@ -266,7 +268,8 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   template void cells_mark_col<K>(const Exec&, const K*, u64, int, int, int, u32*);                          \
   template void cells_load<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int);  \
   template void cells_insert<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int); \
-  template u64 cells_compact<K>(const Exec&, const CellDesc&, u64, bool, int, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, bool);
+  template u64 cells_compact<K>(const Exec&, const CellDesc&, u64, bool, int, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, bool, \
+                                u64);
 SK_CELLS_INST(u64)
 SK_CELLS_INST(u128)
 
