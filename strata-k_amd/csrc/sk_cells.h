@@ -147,4 +147,8 @@ template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
                   DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0 = 0);
 
+// State of arbitrary cells: present[i] and the weight of cell idx[i]
+// (+inf reads as present with weight 0; check infb where it matters).
+void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present);
+
 }  // namespace sk

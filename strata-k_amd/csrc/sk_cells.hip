@@ -264,6 +264,35 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   return n;
 }
 
+// ------------------------------------------------------------------ gather
+
+__global__ void k_cells_gather(CellDesc cd, const u64* __restrict__ idx, u64 n, i64* __restrict__ w,
+                               u8* __restrict__ present) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u64 c = idx[i];
+  const u32 m = 1u << (c & 31);
+  if (!cd.trop) {
+    present[i] = (cd.present[c >> 5] & m) ? 1 : 0;
+    if (w) w[i] = 0;
+    return;
+  }
+  const u64 e = cd.cell[c];
+  i64 wv = 0;
+  u8 pr = 1;
+  if (e != kCellTop) wv = cell_dec(e);
+  else if (cd.maxf[c >> 5] & m) wv = LLONG_MAX;
+  else if (!(cd.infb && (cd.infb[c >> 5] & m))) pr = 0;  // +inf: present with weight 0
+  w[i] = wv;
+  present[i] = pr;
+}
+
+void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present) {
+  if (!n) return;
+  k_cells_gather<<<gridc(n), kBlock, 0, x.stream>>>(cd, idx, n, w, present);
+  SK_HIP(hipGetLastError());
+}
+
 #define SK_CELLS_INST(K)                                                                                      \
   template void cells_mark_col<K>(const Exec&, const K*, u64, int, int, int, u32*);                          \
   template void cells_load<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int);  \

@@ -7,16 +7,27 @@
 //   cost(S,Z) :- cost(S,Y), edge(Y,Z).     (stratum 0)
 // optionally followed by a target filter
 //   ans(S,Z) :- cost(S,Z), tgt(Z).         (stratum 1)
-// Active queries occupy slots; their relations live in one fused relation
-// cost(slot, S, Z).  Every sk_server_step is one semi-naive round for all
-// active queries together: newly admitted queries contribute their bootstrap
-// round (src ⋈ edge), running ones their delta round (Δcost ⋈ edge), and
-// the union is sorted, min-collapsed and merged in one pass.  A query whose
-// delta becomes empty is retired: its rows move to an answer buffer, its slot
-// and memory are freed, and waiting queries are admitted while slots and the
-// HBM budget allow.  Each query keeps its own round count and Bellman-Ford
-// cap (crates/strata-eval/src/seminaive.rs:94-104), so an answer is the same
-// alone and in any mix.
+//
+// State.  Every source of an admitted query gets an *instance*: one row of a
+// resident cell array, cost(instance, Z) for every node Z, holding the
+// order-preserving code of the least weight found so far (sk_cells.h).  The
+// number of instance rows is fixed at creation from the byte budget, and
+// queries are admitted while slots and free rows allow; a row is reset when
+// it is handed out, so nothing is allocated per query on the device except
+// its answer.
+//
+// Rounds.  Every sk_server_step is one semi-naive round for all active
+// queries together: newly admitted queries contribute their bootstrap
+// (src ⋈ edge, weight 0 ⊗ w), running ones their delta round (Δcost ⋈ edge).
+// Both are joins whose emitting kernel ⊕-applies each derivation to its
+// instance's cell and marks strict improvements; one compaction of the marks
+// gives every query's next delta.  Rounds are Jacobi: the joins read the
+// previous delta list, never the cells.  Each query keeps its own round
+// count, Bellman-Ford cap (crates/strata-eval/src/seminaive.rs:94-104) and
+// overflow record (charged per instance), so an answer is the same alone and
+// in any mix.  A query whose delta is empty retires: its rows are compacted
+// (or, with a target filter, only the target cells are read) into its answer,
+// and its rows and slot are freed.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -40,49 +51,18 @@ struct Query {
   Clock::time_point arrival;
   int state = 0;  // 0 waiting, 1 active, 2 done, 3 error
   int slot = -1;
-  bool ran_bootstrap = false;
+  std::vector<u32> inst;  // instance row per source (sources sorted)
   bool had_delta = false;  // non-empty delta after the last round
   u64 iters = 0;
   u64 universe = 0;
-  u64 rows = 0;  // current full rows (admission accounting)
   EvalErr err;
   double latency_ms = 0;
-  // answer (device, sorted by (S, Z))
+  // answer: rows sorted by (S, Z); S is expanded from per-source counts
   u64 n = 0;
-  DBuf<u128> keys;
+  std::vector<u64> per_source;
+  DBuf<u32> z;
   DBuf<i64> w;
 };
-
-}  // namespace
-
-struct sk_server {
-  sk_ctx* ctx = nullptr;
-  u64 n_edges = 0;
-  u64 n_nodes = 0;  // node ids are < n_nodes
-  int bits = 1;
-  DBuf<u64> ekeys;
-  DBuf<i64> ew;
-  DBuf<u64> ecsr;
-  std::vector<u8> edge_node;  // host bitmap (byte per node) of ids used by the EDB
-  u64 edge_universe = 0;
-  u64 budget = 0;
-  u32 max_batch = 64;
-  int slot_bits = 6;
-  std::vector<int> slot_q;  // slot -> query index, -1 free
-  std::vector<std::unique_ptr<Query>> qs;
-  std::map<u64, size_t> by_ticket;
-  std::deque<size_t> waiting;
-  u64 next_ticket = 1;
-  // fused state: cost(slot, S, Z)
-  RelData full, delta;
-  DBuf<DevErr> perr;
-  DBuf<u8> dead_dev;
-  double avg_rows_per_source = 0;
-  u64 finished_sources = 0, finished_rows = 0;
-  EvalErr err;
-};
-
-namespace {
 
 int bits_for(u64 nvals) {
   int b = 1;
@@ -90,7 +70,40 @@ int bits_for(u64 nvals) {
   return b;
 }
 
-void table_alloc_srv(void* ctx, Table* t, u64 n);
+}  // namespace
+
+struct sk_server {
+  sk_ctx* ctx = nullptr;
+  u64 n_edges = 0;
+  u64 n_nodes = 0;  // node ids are < n_nodes
+  int bits = 1;     // packed edge key width per column
+  DBuf<u64> ekeys;
+  DBuf<i64> ew;
+  DBuf<u64> ecsr;
+  std::vector<u8> edge_node;  // host bitmap (byte per node) of ids used by the EDB
+  u64 edge_universe = 0;
+  u32 max_batch = 64;
+  std::vector<int> slot_q;  // slot -> query index, -1 free
+  std::vector<std::unique_ptr<Query>> qs;
+  std::map<u64, size_t> by_ticket;
+  std::deque<size_t> waiting;
+  u64 next_ticket = 1;
+  // instance cells: cost(instance, Z)
+  int zbits = 5, ibits = 1, kbits = 1;
+  u32 inst_cap = 0;
+  std::vector<int> inst_owner;  // instance -> query index, -1 free
+  DBuf<u64> cell;
+  DBuf<u32> maxf, touched, bad;
+  CellDesc cd;
+  DBuf<DevErr> perr;     // per instance
+  DBuf<u8> dead_dev;     // per instance
+  // delta list: (instance, Z) keys packed with kbits, in cell order
+  u64 dn = 0;
+  DBuf<u64> dkeys;
+  DBuf<i64> dw;
+};
+
+namespace {
 
 struct SrvTable {
   Pool* pool;
@@ -112,102 +125,114 @@ void table_alloc_srv(void* ctx, Table* t, u64 n) {
   t->n = n;
 }
 
-struct SrvHead {
-  Pool* pool;
-  DBuf<u128> k;
-  DBuf<i64> w;
-  u64 n = 0;
-};
-
-void head_alloc_srv(void* ctx, OutDesc* o, u64 n) {
-  SrvHead* h = (SrvHead*)ctx;
-  h->k.ensure(h->pool, n ? n : 1);
-  h->w.ensure(h->pool, n ? n : 1);
-  o->keys = h->k.get();
-  o->w = h->w.get();
+// The head of both rules, cost(instance, Z), goes to the cells: nothing to allocate.
+void head_none(void*, OutDesc* o, u64) {
+  o->keys = nullptr;
+  o->w = nullptr;
   o->inf = nullptr;
-  h->n = n;
 }
 
-// Rows per slot of a fused relation state (slot is column 0).
-std::vector<u64> slot_counts(sk_server* s, const RelData& d) {
-  std::vector<u64> cnt(s->max_batch, 0);
-  if (!d.n) return cnt;
-  Exec x = s->ctx->c.exec();
-  DBuf<u64> csr(x.pool, s->max_batch + 1);
-  build_csr<u128>(x, (const u128*)d.k.ptr(), d.n, s->bits * 2, s->max_batch, csr.get());
-  std::vector<u64> h(s->max_batch + 1);
-  SK_HIP(hipMemcpyAsync(h.data(), csr.get(), (s->max_batch + 1) * 8, hipMemcpyDeviceToHost, x.stream));
-  SK_HIP(hipStreamSynchronize(x.stream));
-  for (u32 p = 0; p < s->max_batch; p++) cnt[p] = h[p + 1] - h[p];
-  return cnt;
+// One join round: table rows (instance, node, acc) ⋈ edge(node, Z) ⊕-applied
+// to cost(instance, Z).
+void apply_join(sk_server* s, const Exec& x, SrvTable& t) {
+  t.t.part_col = 0;  // an overflow is charged to the row's instance
+  t.t.perr = s->perr.get();
+  ProbeDesc pd;
+  pd.arity = 2;
+  pd.bits = s->bits;
+  pd.sn = s->n_edges;
+  pd.nprobe = 1;
+  pd.psrc[0] = 1;
+  OutDesc o;
+  o.ncols = 2;
+  o.bits = s->kbits;
+  o.kind[0] = 0;
+  o.idx[0] = 0;
+  o.kind[1] = 1;
+  o.idx[1] = 1;
+  o.cd = s->cd;
+  join<u64>(x, t.t, pd, s->ekeys.get(), s->ew.get(), nullptr, s->ecsr.get(), &o, true, false, true, nullptr, nullptr,
+            head_none, nullptr);
 }
 
-void drop_slots(sk_server* s, RelData& d, const std::vector<u8>& dead) {
-  if (!d.n) return;
-  Exec x = s->ctx->c.exec();
-  SK_HIP(hipMemcpyAsync(s->dead_dev.get(), dead.data(), dead.size(), hipMemcpyHostToDevice, x.stream));
-  RelData nd;
-  nd.k.ensure(x.pool, d.n, true);
-  nd.w.reset(x.pool, d.n);
-  nd.n = drop_partitions<u128>(x, (const u128*)d.k.ptr(), d.w.get(), nullptr, d.n, 3, s->bits, s->dead_dev.get(),
-                               (u128*)nd.k.ptr(), nd.w.get(), nullptr);
-  d = std::move(nd);
-}
-
-// Membership filter of answer rows on Z ∈ targets (ans(S,Z) :- cost(S,Z), tgt(Z)).
-void filter_targets(sk_server* s, Query& q, u128* keys, i64* w, u64 n) {
-  Exec x = s->ctx->c.exec();
-  // host-side filter of a device range: targets are few (tens)
-  std::vector<u128> hk(n);
-  std::vector<i64> hw(n);
-  SK_HIP(hipMemcpyAsync(hk.data(), keys, n * 16, hipMemcpyDeviceToHost, x.stream));
-  SK_HIP(hipMemcpyAsync(hw.data(), w, n * 8, hipMemcpyDeviceToHost, x.stream));
-  SK_HIP(hipStreamSynchronize(x.stream));
-  u128 mask = (((u128)1) << s->bits) - 1;
-  std::vector<u128> ok;
-  std::vector<i64> ow;
-  for (u64 i = 0; i < n; i++) {
-    u32 z = (u32)(hk[i] & mask);
-    if (std::binary_search(q.targets.begin(), q.targets.end(), z)) {
-      ok.push_back(hk[i]);
-      ow.push_back(hw[i]);
-    }
-  }
-  q.n = ok.size();
-  q.keys.reset(x.pool, q.n ? q.n : 1);
-  q.w.reset(x.pool, q.n ? q.n : 1);
-  if (q.n) {
-    SK_HIP(hipMemcpyAsync(q.keys.get(), ok.data(), q.n * 16, hipMemcpyHostToDevice, x.stream));
-    SK_HIP(hipMemcpyAsync(q.w.get(), ow.data(), q.n * 8, hipMemcpyHostToDevice, x.stream));
-    SK_HIP(hipStreamSynchronize(x.stream));
+void reset_rows(sk_server* s, const Exec& x, const std::vector<u32>& inst) {
+  const u64 row = 1ull << s->zbits;
+  for (u32 i : inst) {
+    memset_async(x, s->cell.get() + (u64)i * row, 0xFF, row * 8);
+    memset_async(x, s->maxf.get() + (u64)i * row / 32, 0, row / 8);
   }
 }
 
-void retire(sk_server* s, Query& q, u64 lo, u64 n) {
+void release_rows(sk_server* s, Query& q) {
+  for (u32 i : q.inst) s->inst_owner[i] = -1;
+  q.inst.clear();
+}
+
+// Answer of a finished query: its rows in source order (sorted by (S, Z)).
+void retire(sk_server* s, Query& q) {
   Exec x = s->ctx->c.exec();
+  const u64 row = 1ull << s->zbits;
+  q.per_source.assign(q.sources.size(), 0);
   if (q.targets.empty()) {
-    q.n = n;
-    q.keys.reset(x.pool, n ? n : 1);
-    q.w.reset(x.pool, n ? n : 1);
-    if (n) {
-      SK_HIP(hipMemcpyAsync(q.keys.get(), (const u128*)s->full.k.ptr() + lo, n * 16, hipMemcpyDeviceToDevice, x.stream));
-      SK_HIP(hipMemcpyAsync(q.w.get(), s->full.w.get() + lo, n * 8, hipMemcpyDeviceToDevice, x.stream));
+    std::vector<DBuf<u64>> keys(q.inst.size());
+    std::vector<DBuf<i64>> ws(q.inst.size());
+    DBuf<u8> none;
+    u64 total = 0;
+    for (size_t j = 0; j < q.inst.size(); j++) {
+      q.per_source[j] =
+          cells_compact<u64>(x, s->cd, row, false, s->kbits, &keys[j], &ws[j], &none, false, (u64)q.inst[j] * row);
+      total += q.per_source[j];
+    }
+    q.n = total;
+    q.z.reset(x.pool, total ? total : 1);
+    q.w.reset(x.pool, total ? total : 1);
+    DBuf<u32> scratch(x.pool, total ? total : 1);
+    u64 off = 0;
+    for (size_t j = 0; j < q.inst.size(); j++) {
+      u64 m = q.per_source[j];
+      if (!m) continue;
+      u32* cp[2] = {scratch.get() + off, q.z.get() + off};
+      unpack_cols<u64>(x, keys[j].get(), m, 2, s->kbits, cp);
+      SK_HIP(hipMemcpyAsync(q.w.get() + off, ws[j].get(), m * 8, hipMemcpyDeviceToDevice, x.stream));
+      off += m;
     }
   } else {
-    filter_targets(s, q, (u128*)s->full.k.ptr() + lo, s->full.w.get() + lo, n);
+    // ans(S,Z) :- cost(S,Z), tgt(Z): read only the target cells
+    const u64 nt = q.targets.size();
+    std::vector<u64> idx;
+    idx.reserve(q.inst.size() * nt);
+    for (u32 i : q.inst)
+      for (u32 t : q.targets) idx.push_back((u64)i * row + t);
+    DBuf<u64> didx(x.pool, idx.size());
+    DBuf<i64> gw(x.pool, idx.size());
+    DBuf<u8> gp(x.pool, idx.size());
+    SK_HIP(hipMemcpyAsync(didx.get(), idx.data(), idx.size() * 8, hipMemcpyHostToDevice, x.stream));
+    cells_gather(x, s->cd, didx.get(), idx.size(), gw.get(), gp.get());
+    std::vector<i64> hw(idx.size());
+    std::vector<u8> hp(idx.size());
+    SK_HIP(hipMemcpyAsync(hw.data(), gw.get(), idx.size() * 8, hipMemcpyDeviceToHost, x.stream));
+    SK_HIP(hipMemcpyAsync(hp.data(), gp.get(), idx.size(), hipMemcpyDeviceToHost, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    std::vector<u32> oz;
+    std::vector<i64> ow;
+    for (size_t j = 0; j < q.inst.size(); j++)
+      for (u64 k = 0; k < nt; k++)
+        if (hp[j * nt + k]) {
+          oz.push_back(q.targets[k]);
+          ow.push_back(hw[j * nt + k]);
+          q.per_source[j]++;
+        }
+    q.n = oz.size();
+    q.z.reset(x.pool, q.n ? q.n : 1);
+    q.w.reset(x.pool, q.n ? q.n : 1);
+    if (q.n) {
+      SK_HIP(hipMemcpyAsync(q.z.get(), oz.data(), q.n * 4, hipMemcpyHostToDevice, x.stream));
+      SK_HIP(hipMemcpyAsync(q.w.get(), ow.data(), q.n * 8, hipMemcpyHostToDevice, x.stream));
+    }
   }
   SK_HIP(hipStreamSynchronize(x.stream));
   q.state = 2;
   q.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - q.arrival).count();
-  s->finished_sources += q.sources.size();
-  s->finished_rows += n;
-}
-
-u64 estimate_bytes(sk_server* s, const Query& q) {
-  double per_src = s->finished_sources ? (double)s->finished_rows / s->finished_sources : (double)s->n_nodes / 4;
-  // full + delta + sorted candidates, 24 bytes a row
-  return (u64)(q.sources.size() * per_src * 24.0 * 3.0);
 }
 
 }  // namespace
@@ -222,8 +247,6 @@ sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst
   s->ctx = ctx;
   s->max_batch = max_batch;
   s->slot_q.assign(max_batch, -1);
-  s->budget = query_budget_bytes;
-  EvalErr e;
   try {
     Ctx& c = ctx->c;
     (void)hipSetDevice(c.device);
@@ -231,8 +254,7 @@ sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst
     u32 maxv = 0;
     for (u64 i = 0; i < n_edges; i++) maxv = std::max(maxv, std::max(src[i], dst[i]));
     s->n_nodes = (u64)maxv + 1;
-    s->bits = bits_for(std::max<u64>(s->n_nodes, max_batch));
-    s->slot_bits = bits_for(max_batch);
+    s->bits = bits_for(s->n_nodes);
     s->edge_node.assign(s->n_nodes, 0);
     for (u64 i = 0; i < n_edges; i++) s->edge_node[src[i]] = s->edge_node[dst[i]] = 1;
     for (u8 b : s->edge_node) s->edge_universe += b;
@@ -256,11 +278,47 @@ sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst
     s->n_edges = unique_runs<u64>(x, sk, sw, nullptr, n_edges, s->ekeys.get(), s->ew.get(), nullptr);
     s->ecsr.reset(c.pool.get(), s->n_nodes + 1);
     build_csr<u64>(x, s->ekeys.get(), s->n_edges, s->bits, s->n_nodes, s->ecsr.get());
-    s->perr.reset(c.pool.get(), max_batch);
-    s->dead_dev.reset(c.pool.get(), max_batch);
-    memset_async(x, s->perr.get(), 0, sizeof(DevErr) * max_batch);
     SK_HIP(hipStreamSynchronize(c.stream));
-    if (s->bits * 3 > 128) throw EvalErr{SK_ERR_UNSUPPORTED_SHAPE};
+    a.release(); b.release(); k1.release(); k2.release(); w1.release(); w2.release();
+    // instance rows from the byte budget (default: half the pool's budget)
+    s->zbits = std::max(s->bits, 5);
+    const u64 row = 1ull << s->zbits;
+    const u64 row_bytes = row * 8 + row / 8 * 2;
+    const u64 budget = query_budget_bytes ? query_budget_bytes : c.pool->budget() / 2;
+    u64 cap = budget / row_bytes;
+    cap = std::min<u64>(cap, 4096);
+    if (cap == 0) throw BudgetError{row_bytes};
+    s->inst_cap = (u32)cap;
+    s->ibits = bits_for(std::max<u64>(cap, 2));
+    s->kbits = std::max(s->ibits, s->zbits);
+    if (2 * s->kbits > 64) throw EvalErr{SK_ERR_UNSUPPORTED_SHAPE};
+    s->inst_owner.assign(cap, -1);
+    const u64 ncells = cap * row;
+    s->cell.reset(c.pool.get(), ncells);
+    s->maxf.reset(c.pool.get(), ncells / 32);
+    s->touched.reset(c.pool.get(), ncells / 32);
+    s->bad.reset(c.pool.get(), 1);
+    memset_async(x, s->cell.get(), 0xFF, ncells * 8);
+    memset_async(x, s->maxf.get(), 0, ncells / 8);
+    memset_async(x, s->touched.get(), 0, ncells / 8);
+    memset_async(x, s->bad.get(), 0, 4);
+    CellDesc& cd = s->cd;
+    cd.on = 1;
+    cd.trop = 1;
+    cd.ncols = 2;
+    cd.ncodes = std::max<u64>(s->n_nodes, cap);
+    cd.ebits[0] = s->ibits;
+    cd.ebits[1] = s->zbits;
+    cd.shift[0] = s->zbits;
+    cd.shift[1] = 0;
+    cd.cell = s->cell.get();
+    cd.maxf = s->maxf.get();
+    cd.touched = s->touched.get();
+    cd.bad = s->bad.get();
+    s->perr.reset(c.pool.get(), cap);
+    s->dead_dev.reset(c.pool.get(), cap);
+    memset_async(x, s->perr.get(), 0, sizeof(DevErr) * cap);
+    SK_HIP(hipStreamSynchronize(c.stream));
   } catch (const DeviceError&) {
     return SK_ERR_DEVICE;
   } catch (const BudgetError&) {
@@ -268,7 +326,6 @@ sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst
   } catch (const EvalErr& ee) {
     return (sk_status)ee.code;
   }
-  if (s->budget == 0) s->budget = (u64)(ctx->c.pool->budget() * 0.6);
   *out = s.release();
   return SK_OK;
 }
@@ -294,6 +351,8 @@ sk_status sk_server_submit(sk_server* s, const uint32_t* sources, uint32_t n_sou
     if (v >= s->n_nodes) return SK_ERR_INVALID_ARGUMENT;
   for (u32 v : q->targets)
     if (v >= s->n_nodes) return SK_ERR_INVALID_ARGUMENT;
+  // a query needs one instance row per source: larger ones can never run
+  if (q->sources.size() > s->inst_cap) return SK_ERR_OUT_OF_BUDGET;
   // universe: distinct Sym values over edge ∪ src ∪ tgt (crates/strata-eval/src/naive.rs:143-155)
   u64 u = s->edge_universe;
   std::vector<u32> extra;
@@ -318,8 +377,9 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
   (void)hipSetDevice(c.device);
   try {
     Exec x = c.exec();
-    // ---- admission: free slots, within the HBM budget
-    u64 in_use = (s->full.n + s->delta.n) * 24;
+    // ---- admission: a free slot and a free instance row per source
+    u64 free_rows = 0;
+    for (int o : s->inst_owner) free_rows += o < 0;
     std::vector<size_t> admitted;
     while (!s->waiting.empty()) {
       int slot = -1;
@@ -329,165 +389,97 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
           break;
         }
       if (slot < 0) break;
-      Query& q = *s->qs[s->waiting.front()];
-      u64 est = estimate_bytes(s, q);
-      bool idle = s->full.n == 0 && s->delta.n == 0 && admitted.empty();
-      if (!idle && in_use + est > s->budget) break;
-      in_use += est;
+      const size_t qi = s->waiting.front();
+      Query& q = *s->qs[qi];
+      if (q.sources.size() > free_rows) break;  // in arrival order: wait for rows to free up
+      q.inst.clear();
+      for (u32 i = 0; i < s->inst_cap && q.inst.size() < q.sources.size(); i++)
+        if (s->inst_owner[i] < 0) {
+          s->inst_owner[i] = (int)qi;
+          q.inst.push_back(i);
+        }
+      free_rows -= q.sources.size();
+      reset_rows(s, x, q.inst);
       q.slot = slot;
       q.state = 1;
-      s->slot_q[slot] = (int)s->waiting.front();
-      admitted.push_back(s->waiting.front());
+      s->slot_q[slot] = (int)qi;
+      admitted.push_back(qi);
       s->waiting.pop_front();
     }
-    // ---- candidates of this round
-    std::vector<SrvHead> heads;
+    // ---- this round's derivations, applied to the cells
     if (!admitted.empty()) {
-      // bootstrap of the new queries: cost(S,Y) :- src(S), edge(S,Y)
-      std::vector<u32> qc, sc;
+      // bootstrap: cost(S,Y) :- src(S), edge(S,Y); the src fact carries weight 0
+      std::vector<u32> ic, sc;
       for (size_t qi : admitted) {
         Query& q = *s->qs[qi];
-        q.ran_bootstrap = true;
-        for (u32 v : q.sources) {
-          qc.push_back((u32)q.slot);
-          sc.push_back(v);
+        for (size_t j = 0; j < q.sources.size(); j++) {
+          ic.push_back(q.inst[j]);
+          sc.push_back(q.sources[j]);
         }
       }
       SrvTable t;
       t.pool = x.pool;
       t.t.ncols = 2;
-      table_alloc_srv(&t, &t.t, qc.size());
-      SK_HIP(hipMemcpyAsync(t.t.col[0], qc.data(), qc.size() * 4, hipMemcpyHostToDevice, x.stream));
+      table_alloc_srv(&t, &t.t, ic.size());
+      SK_HIP(hipMemcpyAsync(t.t.col[0], ic.data(), ic.size() * 4, hipMemcpyHostToDevice, x.stream));
       SK_HIP(hipMemcpyAsync(t.t.col[1], sc.data(), sc.size() * 4, hipMemcpyHostToDevice, x.stream));
-      memset_async(x, t.t.w, 0, qc.size() * 8);  // src facts carry weight 0 (otimes_id)
-      t.t.part_col = 0;
-      t.t.perr = s->perr.get();
-      ProbeDesc pd;
-      pd.arity = 2;
-      pd.bits = s->bits;
-      pd.sn = s->n_edges;
-      pd.nprobe = 1;
-      pd.psrc[0] = 1;
-      OutDesc o;
-      o.ncols = 3;
-      o.bits = s->bits;
-      o.kind[0] = 0; o.idx[0] = 0;
-      o.kind[1] = 0; o.idx[1] = 1;
-      o.kind[2] = 1; o.idx[2] = 1;
-      heads.emplace_back();
-      heads.back().pool = x.pool;
-      join<u64>(x, t.t, pd, s->ekeys.get(), s->ew.get(), nullptr, s->ecsr.get(), &o, true, true, true, nullptr,
-                nullptr, head_alloc_srv, &heads.back());
+      memset_async(x, t.t.w, 0, ic.size() * 8);
+      apply_join(s, x, t);
     }
-    if (s->delta.n) {
-      // delta round: cost(S,Z) :- cost(S,Y), edge(Y,Z) with cost from the delta
+    if (s->dn) {
+      // delta round: cost(S,Z) :- cost(S,Y), edge(Y,Z) over last round's delta
       SrvTable t;
       t.pool = x.pool;
-      t.t.ncols = 3;
-      table_alloc_srv(&t, &t.t, s->delta.n);
-      u32* cp[3] = {t.t.col[0], t.t.col[1], t.t.col[2]};
-      unpack_cols<u128>(x, (const u128*)s->delta.k.ptr(), s->delta.n, 3, s->bits, cp);
-      SK_HIP(hipMemcpyAsync(t.t.w, s->delta.w.get(), s->delta.n * 8, hipMemcpyDeviceToDevice, x.stream));
-      t.t.part_col = 0;
-      t.t.perr = s->perr.get();
-      ProbeDesc pd;
-      pd.arity = 2;
-      pd.bits = s->bits;
-      pd.sn = s->n_edges;
-      pd.nprobe = 1;
-      pd.psrc[0] = 2;
-      OutDesc o;
-      o.ncols = 3;
-      o.bits = s->bits;
-      o.kind[0] = 0; o.idx[0] = 0;
-      o.kind[1] = 0; o.idx[1] = 1;
-      o.kind[2] = 1; o.idx[2] = 1;
-      heads.emplace_back();
-      heads.back().pool = x.pool;
-      join<u64>(x, t.t, pd, s->ekeys.get(), s->ew.get(), nullptr, s->ecsr.get(), &o, true, true, true, nullptr,
-                nullptr, head_alloc_srv, &heads.back());
+      t.t.ncols = 2;
+      table_alloc_srv(&t, &t.t, s->dn);
+      u32* cp[2] = {t.t.col[0], t.t.col[1]};
+      unpack_cols<u64>(x, s->dkeys.get(), s->dn, 2, s->kbits, cp);
+      SK_HIP(hipMemcpyAsync(t.t.w, s->dw.get(), s->dn * 8, hipMemcpyDeviceToDevice, x.stream));
+      apply_join(s, x, t);
     }
-    u64 total = 0;
-    for (auto& h : heads) total += h.n;
-    // ---- ⊕-combine (sort, min per key, merge with full)
-    RelData fresh;
-    fresh.n = 0;
-    if (total) {
-      DBuf<u128> ck(x.pool, total), ck2(x.pool, total);
-      DBuf<i64> cw(x.pool, total), cw2(x.pool, total);
-      u64 off = 0;
-      for (auto& h : heads) {
-        if (!h.n) continue;
-        SK_HIP(hipMemcpyAsync(ck.get() + off, h.k.get(), h.n * 16, hipMemcpyDeviceToDevice, x.stream));
-        SK_HIP(hipMemcpyAsync(cw.get() + off, h.w.get(), h.n * 8, hipMemcpyDeviceToDevice, x.stream));
-        off += h.n;
-      }
-      heads.clear();
-      u128* sk;
-      i64* sw;
-      int kb = 2 * s->bits + s->slot_bits;
-      radix_sort<u128>(x, ck.get(), ck2.get(), cw.get(), cw2.get(), nullptr, nullptr, total, kb, &sk, &sw, nullptr);
-      DBuf<u128> uk(x.pool, total);
-      DBuf<i64> uw(x.pool, total);
-      u64 nu = unique_runs<u128>(x, sk, sw, nullptr, total, uk.get(), uw.get(), nullptr);
-      if (s->full.n == 0) {
-        s->full.n = nu;
-        s->full.k.ensure(x.pool, nu, true);
-        s->full.w.reset(x.pool, nu);
-        SK_HIP(hipMemcpyAsync(s->full.k.ptr(), uk.get(), nu * 16, hipMemcpyDeviceToDevice, x.stream));
-        SK_HIP(hipMemcpyAsync(s->full.w.get(), uw.get(), nu * 8, hipMemcpyDeviceToDevice, x.stream));
-        fresh.n = nu;
-        fresh.k.wide = true;
-        fresh.k.b = std::move(uk);
-        fresh.w = std::move(uw);
-      } else {
-        DBuf<u128> u_k, f_k;
-        DBuf<i64> u_w, f_w;
-        DBuf<u8> u_f, f_f;
-        u64 n_union = 0, n_fresh = 0;
-        merge_diff<u128>(x, uk.get(), uw.get(), nullptr, nu, (const u128*)s->full.k.ptr(), s->full.w.get(), nullptr,
-                         s->full.n, &u_k, &u_w, &u_f, &n_union, &f_k, &f_w, &f_f, &n_fresh, true, false);
-        if (n_fresh) {
-          RelData nf;
-          nf.n = n_union;
-          nf.k.wide = true;
-          nf.k.b = std::move(u_k);
-          nf.w = std::move(u_w);
-          s->full = std::move(nf);
-          fresh.n = n_fresh;
-          fresh.k.wide = true;
-          fresh.k.b = std::move(f_k);
-          fresh.w = std::move(f_w);
-        }
-      }
+    // ---- next delta: the improved cells (instance-major, so grouped by instance)
+    u32 hi = 0;
+    for (u32 i = 0; i < s->inst_cap; i++)
+      if (s->inst_owner[i] >= 0) hi = i + 1;
+    {
+      DBuf<u64> nk;
+      DBuf<i64> nw;
+      DBuf<u8> nf;
+      const u64 row = 1ull << s->zbits;
+      s->dn = hi ? cells_compact<u64>(x, s->cd, (u64)hi * row, true, s->kbits, &nk, &nw, &nf, false, 0) : 0;
+      s->dkeys = std::move(nk);
+      s->dw = std::move(nw);
     }
-    s->delta = std::move(fresh);
     // ---- per-query bookkeeping: overflow, round count, cap, retirement
-    std::vector<DevErr> herr(s->max_batch);
-    SK_HIP(hipMemcpyAsync(herr.data(), s->perr.get(), s->max_batch * sizeof(DevErr), hipMemcpyDeviceToHost, x.stream));
-    SK_HIP(hipStreamSynchronize(x.stream));
-    bool clear_err = false;
-    std::vector<u64> dcnt = slot_counts(s, s->delta);
-    std::vector<u64> fcnt = slot_counts(s, s->full);
-    std::vector<u8> dead(s->max_batch, 0);
-    bool any_dead = false;
-    u64 lo = 0;
-    std::vector<u64> start(s->max_batch, 0);
-    for (u32 p = 0; p < s->max_batch; p++) {
-      start[p] = lo;
-      lo += fcnt[p];
+    std::vector<DevErr> herr(s->inst_cap);
+    SK_HIP(hipMemcpyAsync(herr.data(), s->perr.get(), s->inst_cap * sizeof(DevErr), hipMemcpyDeviceToHost, x.stream));
+    std::vector<u64> dcnt(s->inst_cap, 0);
+    if (s->dn) {
+      DBuf<u64> csr(x.pool, s->inst_cap + 1);
+      build_csr<u64>(x, s->dkeys.get(), s->dn, s->kbits, s->inst_cap, csr.get());
+      std::vector<u64> h(s->inst_cap + 1);
+      SK_HIP(hipMemcpyAsync(h.data(), csr.get(), (s->inst_cap + 1) * 8, hipMemcpyDeviceToHost, x.stream));
+      SK_HIP(hipStreamSynchronize(x.stream));
+      for (u32 i = 0; i < s->inst_cap; i++) dcnt[i] = h[i + 1] - h[i];
     }
+    SK_HIP(hipStreamSynchronize(x.stream));
+    bool clear_err = false, any_dead = false;
+    std::vector<u8> dead(s->inst_cap, 0);
     for (u32 p = 0; p < s->max_batch; p++) {
       int qi = s->slot_q[p];
       if (qi < 0) continue;
       Query& q = *s->qs[qi];
-      bool newly = std::find(admitted.begin(), admitted.end(), (size_t)qi) != admitted.end();
-      if (herr[p].code) {
-        clear_err = true;
-        q.err.code = SK_ERR_OVERFLOW;
-        q.err.lhs = herr[p].lhs;
-        q.err.rhs = herr[p].rhs;
-        q.state = 3;
+      const bool newly = std::find(admitted.begin(), admitted.end(), (size_t)qi) != admitted.end();
+      u64 qd = 0;
+      for (u32 i : q.inst) {
+        qd += dcnt[i];
+        if (herr[i].code && q.err.code == 0) {
+          q.err.code = SK_ERR_OVERFLOW;
+          q.err.lhs = herr[i].lhs;
+          q.err.rhs = herr[i].rhs;
+          q.state = 3;
+        }
+        if (herr[i].code) clear_err = true;
       }
       if (q.state == 1 && !newly && q.had_delta) {
         q.iters++;
@@ -496,22 +488,33 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
           q.state = 3;
         }
       }
-      q.had_delta = dcnt[p] > 0;
-      q.rows = fcnt[p];
-      if (q.state == 1 && !q.had_delta) retire(s, q, start[p], fcnt[p]);
+      q.had_delta = qd > 0;
+      if (q.state == 1 && !q.had_delta) retire(s, q);
       if (q.state != 1) {
-        if (q.state == 3) q.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - q.arrival).count();
-        dead[p] = 1;
-        any_dead = true;
+        if (q.state == 3) {
+          q.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - q.arrival).count();
+          for (u32 i : q.inst) dead[i] = 1;
+          any_dead = any_dead || qd > 0;
+        }
+        release_rows(s, q);
         s->slot_q[p] = -1;
       }
     }
-    if (clear_err) memset_async(x, s->perr.get(), 0, sizeof(DevErr) * s->max_batch);
+    if (clear_err) memset_async(x, s->perr.get(), 0, sizeof(DevErr) * s->inst_cap);
     if (any_dead) {
-      drop_slots(s, s->full, dead);
-      drop_slots(s, s->delta, dead);
+      // a failed query's delta rows must not feed the next round
+      SK_HIP(hipMemcpyAsync(s->dead_dev.get(), dead.data(), s->inst_cap, hipMemcpyHostToDevice, x.stream));
+      DBuf<u64> nk(x.pool, s->dn);
+      DBuf<i64> nw(x.pool, s->dn);
+      s->dn = drop_partitions<u64>(x, s->dkeys.get(), s->dw.get(), nullptr, s->dn, 2, s->kbits, s->dead_dev.get(),
+                                   nk.get(), nw.get(), nullptr);
+      s->dkeys = std::move(nk);
+      s->dw = std::move(nw);
     }
+    u32 bad = 0;
+    SK_HIP(hipMemcpyAsync(&bad, s->bad.get(), 4, hipMemcpyDeviceToHost, x.stream));
     SK_HIP(hipStreamSynchronize(x.stream));
+    if (bad) return SK_ERR_DEVICE;
   } catch (const DeviceError&) {
     return SK_ERR_DEVICE;
   } catch (const BudgetError&) {
@@ -546,18 +549,21 @@ sk_status sk_server_take(sk_server* s, uint64_t ticket, uint32_t* src, uint32_t*
   try {
     Exec x = s->ctx->c.exec();
     if (q.n) {
-      DBuf<u32> c0(x.pool, q.n), c1(x.pool, q.n), c2(x.pool, q.n);
-      u32* cp[3] = {c0.get(), c1.get(), c2.get()};
-      unpack_cols<u128>(x, q.keys.get(), q.n, 3, s->bits, cp);
-      if (src) SK_HIP(hipMemcpyAsync(src, c1.get(), q.n * 4, hipMemcpyDeviceToHost, x.stream));
-      if (dst) SK_HIP(hipMemcpyAsync(dst, c2.get(), q.n * 4, hipMemcpyDeviceToHost, x.stream));
+      if (dst) SK_HIP(hipMemcpyAsync(dst, q.z.get(), q.n * 4, hipMemcpyDeviceToHost, x.stream));
       if (w) SK_HIP(hipMemcpyAsync(w, q.w.get(), q.n * 8, hipMemcpyDeviceToHost, x.stream));
+      if (src) {
+        u64 off = 0;
+        for (size_t j = 0; j < q.sources.size(); j++) {
+          std::fill(src + off, src + off + q.per_source[j], q.sources[j]);
+          off += q.per_source[j];
+        }
+      }
       SK_HIP(hipStreamSynchronize(x.stream));
     }
   } catch (const DeviceError&) {
     return SK_ERR_DEVICE;
   }
-  q.keys.release();
+  q.z.release();
   q.w.release();
   return SK_OK;
 }
@@ -567,7 +573,7 @@ sk_status sk_server_release(sk_server* s, uint64_t ticket) {
   auto it = s->by_ticket.find(ticket);
   if (it == s->by_ticket.end()) return SK_ERR_INVALID_ARGUMENT;
   Query& q = *s->qs[it->second];
-  q.keys.release();
+  q.z.release();
   q.w.release();
   return SK_OK;
 }

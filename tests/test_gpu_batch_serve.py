@@ -169,3 +169,34 @@ def test_server_answers_exact_alone_and_in_mix(ctx):
         a, b, ww = srv.take(t, res["rows"])
         assert (a == m[0]).all() and (b == m[1]).all() and (ww == m[2]).all() and res["iterations"] == m[3]
         srv.free()
+
+
+def test_server_failures_stay_per_query(ctx):
+    """A query reaching a negative cycle fails with NegativeWeightCycle and one
+    reaching an overflowing path fails with Overflow, while queries running
+    beside them still match the oracle (each query has its own cap and
+    overflow record)."""
+    # nodes 0..5: a normal chain, a negative cycle 10<->11, an overflow path 20->21->22
+    src = np.array([0, 1, 2, 3, 10, 11, 20, 21], np.uint32)
+    dst = np.array([1, 2, 3, 4, 11, 10, 21, 22], np.uint32)
+    w = np.array([5, 7, 1, 2, 1, -3, 2**63 - 1, 1], np.int64)
+    srv = sk.Server(ctx, src, dst, w, max_batch=4)
+    good = srv.submit(np.array([0, 2], np.uint32))
+    negc = srv.submit(np.array([10], np.uint32))
+    ovf = srv.submit(np.array([20], np.uint32))
+    for _ in range(1000):
+        a, wt = srv.step()
+        if a == 0 and wt == 0:
+            break
+    with pytest.raises(sk.EvalError) as e1:
+        srv.poll(negc)
+    assert e1.value.kind == "NegativeWeightCycle"
+    with pytest.raises(sk.EvalError) as e2:
+        srv.poll(ovf)
+    assert e2.value.kind == "Overflow"
+    res = srv.poll(good)
+    a, b, ww = srv.take(good, res["rows"])
+    q = dict(sources=np.array([0, 2], np.uint32), targets=np.empty(0, np.uint32))
+    oa, ob, ow, oit = _oracle_answer(src, dst, w, q)
+    assert (a == oa).all() and (b == ob).all() and (ww == ow).all() and res["iterations"] == oit
+    srv.free()
