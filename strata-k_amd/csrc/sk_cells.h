@@ -25,6 +25,7 @@
 namespace sk {
 
 constexpr u64 kCellTop = ~0ull;
+constexpr u32 kCellTop32 = ~0u;
 
 struct CellDesc {
   int on = 0;
@@ -37,6 +38,15 @@ struct CellDesc {
   const u64* dom_pre[kMaxCols] = {};   // per-word count of domain codes in earlier words
   const u32* dom_list[kMaxCols] = {};  // rank -> code (null = identity)
   u64* cell = nullptr;     // Trop weight codes
+  // Narrow Trop cells: when every weight the run can hold is a non-negative
+  // value below 2^32-1, cells hold the weight itself as u32 (kCellTop32 =
+  // absent), halving cell traffic.  A derivation outside that range sets
+  // *widen and is not applied; the engine then widens the cells and runs the
+  // round again (⊕ is idempotent, so re-applying the round changes nothing
+  // already applied).
+  int narrow = 0;
+  u32* cell32 = nullptr;
+  u32* widen = nullptr;
   u32* present = nullptr;  // Bool presence bits
   u32* maxf = nullptr;     // Trop: present with finite INT64_MAX
   u32* infb = nullptr;     // Trop: present with +inf
@@ -70,7 +80,16 @@ __device__ __forceinline__ void cell_touch(const CellDesc& cd, u64 c) {
 // reads may be stale within the kernel, but cell states only decrease, so a
 // stale value can only send a derivation to the atomic, which decides.
 __device__ __forceinline__ u64 cell_peek(const CellDesc& cd, u64 c) {
-  return cd.trop ? cd.cell[c] : (u64)cd.present[c >> 5];
+  if (!cd.trop) return (u64)cd.present[c >> 5];
+  return cd.narrow ? (u64)cd.cell32[c] : cd.cell[c];
+}
+
+// Raw Trop cell code -> (absent?, weight); maxf / infb decide the absent case.
+__device__ __forceinline__ bool cell_code_absent(const CellDesc& cd, u64 e) {
+  return cd.narrow ? e == (u64)kCellTop32 : e == kCellTop;
+}
+__device__ __forceinline__ i64 cell_code_value(const CellDesc& cd, u64 e) {
+  return cd.narrow ? (i64)e : cell_dec(e);
 }
 
 template <bool TROP>
@@ -79,6 +98,21 @@ __device__ __forceinline__ void cell_apply_t(const CellDesc& cd, u64 c, i64 w, u
   if constexpr (!TROP) {
     if ((u32)seen & m) return;
     if (!(atomicOr(&cd.present[c >> 5], m) & m)) cell_touch(cd, c);
+    return;
+  }
+  if (cd.narrow) {
+    if (!inf) {
+      if ((u64)w >= (u64)kCellTop32) {  // negative, or too large for a narrow cell
+        atomicOr(cd.widen, 1u);
+        return;
+      }
+      const u32 e = (u32)w;
+      if ((u64)e >= seen) return;
+      if (e < atomicMin(&cd.cell32[c], e)) cell_touch(cd, c);
+      return;
+    }
+    if (seen != (u64)kCellTop32) return;
+    if (!(atomicOr(&cd.infb[c >> 5], m) & m)) cell_touch(cd, c);
     return;
   }
   unsigned long long* p = (unsigned long long*)&cd.cell[c];
@@ -150,5 +184,7 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
 // State of arbitrary cells: present[i] and the weight of cell idx[i]
 // (+inf reads as present with weight 0; check infb where it matters).
 void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present);
+// Narrow cell codes -> wide codes (same states).
+void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n);
 
 }  // namespace sk

@@ -101,10 +101,14 @@ __global__ void k_cells_load(CellDesc cd, const K* __restrict__ keys, const i64*
     atomicOr(&cd.present[c >> 5], m);
   } else if (inf && inf[i]) {
     atomicOr(&cd.infb[c >> 5], m);
+  } else if (cd.narrow) {
+    // keys are unique: one writer per cell
+    if ((u64)w[i] >= (u64)kCellTop32) atomicOr(cd.widen, 1u);
+    else cd.cell32[c] = (u32)w[i];
   } else if (w[i] == LLONG_MAX) {
     atomicOr(&cd.maxf[c >> 5], m);
   } else {
-    cd.cell[c] = cell_enc(w[i]);  // keys are unique: one writer per cell
+    cd.cell[c] = cell_enc(w[i]);
   }
 }
 
@@ -149,7 +153,15 @@ constexpr u32 kCellsPerBlock = (u32)kBlock * 32;
 __device__ __forceinline__ u32 cells_word(const CellDesc& cd, u64 wi, bool touched_only) {
   if (touched_only) return cd.touched[wi];
   if (!cd.trop) return cd.present[wi];
-  u32 b = cd.maxf[wi] | (cd.infb ? cd.infb[wi] : 0u);
+  u32 b = cd.infb ? cd.infb[wi] : 0u;
+  if (cd.narrow) {
+    const u32* p = cd.cell32 + wi * 32;
+#pragma unroll 8
+    for (int k = 0; k < 32; k++)
+      if (p[k] != kCellTop32) b |= 1u << k;
+    return b;
+  }
+  b |= cd.maxf[wi];
   const u64* p = cd.cell + wi * 32;
 #pragma unroll 8
   for (int k = 0; k < 32; k++)
@@ -220,11 +232,11 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
     }
     keys[base + j] = key;
     if (cd.trop) {
-      u64 e = cd.cell[c];
+      const u64 e = cell_peek(cd, c);
       i64 wv = 0;
       u8 f = 0;
-      if (e != kCellTop) wv = cell_dec(e);
-      else if (cd.maxf[c >> 5] & (1u << (c & 31))) wv = LLONG_MAX;
+      if (!cell_code_absent(cd, e)) wv = cell_code_value(cd, e);
+      else if (!cd.narrow && (cd.maxf[c >> 5] & (1u << (c & 31)))) wv = LLONG_MAX;
       else f = 1;
       w[base + j] = wv;
       if (inf) inf[base + j] = f;
@@ -277,11 +289,11 @@ __global__ void k_cells_gather(CellDesc cd, const u64* __restrict__ idx, u64 n, 
     if (w) w[i] = 0;
     return;
   }
-  const u64 e = cd.cell[c];
+  const u64 e = cell_peek(cd, c);
   i64 wv = 0;
   u8 pr = 1;
-  if (e != kCellTop) wv = cell_dec(e);
-  else if (cd.maxf[c >> 5] & m) wv = LLONG_MAX;
+  if (!cell_code_absent(cd, e)) wv = cell_code_value(cd, e);
+  else if (!cd.narrow && (cd.maxf[c >> 5] & m)) wv = LLONG_MAX;
   else if (!(cd.infb && (cd.infb[c >> 5] & m))) pr = 0;  // +inf: present with weight 0
   w[i] = wv;
   present[i] = pr;
@@ -290,6 +302,20 @@ __global__ void k_cells_gather(CellDesc cd, const u64* __restrict__ idx, u64 n, 
 void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present) {
   if (!n) return;
   k_cells_gather<<<gridc(n), kBlock, 0, x.stream>>>(cd, idx, n, w, present);
+  SK_HIP(hipGetLastError());
+}
+
+__global__ void k_cells_widen(const u32* __restrict__ c32, u64* __restrict__ c64, u64 n) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const u32 e = c32[i];
+  c64[i] = e == kCellTop32 ? kCellTop : cell_enc((i64)e);
+}
+
+void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
+  if (!n) return;
+  Scope sc(x.timer, "cells_widen", KC_MISC, (double)n * 12.0, n);
+  k_cells_widen<<<gridc(n), kBlock, 0, x.stream>>>(c32, c64, n);
   SK_HIP(hipGetLastError());
 }
 

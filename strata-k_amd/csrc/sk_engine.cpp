@@ -298,13 +298,21 @@ void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* con
       R.st_sym[j].insert(R.st_sym[j].end(), cols[j], cols[j] + n);
     }
   }
-  if (weighted)
+  if (weighted) {
+    if (R.st_reserve > R.st_w.capacity()) {
+      R.st_w.reserve(R.st_reserve);
+      R.st_inf.reserve(R.st_reserve);
+    }
+    const u64 base = R.st_w.size();
+    R.st_w.resize(base + n);
+    R.st_inf.resize(base + n);
     for (u64 i = 0; i < n; i++) {
       u8 f = inf ? inf[i] : 0;
-      R.st_w.push_back(f ? 0 : w[i]);  // +inf rows carry weight 0, like every derived +inf
-      R.st_inf.push_back(f);
+      R.st_w[base + i] = f ? 0 : w[i];  // +inf rows carry weight 0, like every derived +inf
+      R.st_inf[base + i] = f;
       if (f) has_inf_ = true;
     }
+  }
   R.st_weighted = weighted;
   R.st_rows += n;
 }
@@ -358,6 +366,9 @@ void Engine::upload_rel(int r) {
     return;
   }
   bool trop = R.sem == SK_TROP;
+  if (trop)
+    for (u64 i = 0; i < n && narrow_ok_; i++)
+      if (!R.st_inf[i] && (u64)R.st_w[i] >= 0xFFFFFFFFull) narrow_ok_ = false;
   // encode to codes, one column at a time
   std::vector<DBuf<u32>> dcols(R.arity);
   Cols cols{};
@@ -449,6 +460,7 @@ void Engine::compute_universe() {
 
 void Engine::load() {
   if (pending.code) throw pending;
+  narrow_ok_ = true;
   build_dict();
   for (size_t r = 0; r < R_.size(); r++) upload_rel((int)r);
   // distinct Sym values after the EDB load (crates/strata-eval/src/naive.rs:143-155)
@@ -1969,12 +1981,23 @@ void Engine::plan_dense(u32 k) {
     dn->ncells = 1ull << tot_bits;
     const u64 nc = std::max<u64>(dn->ncells, 32);
     const u64 nwc = nc / 32;
-    const u64 bytes = trop ? nc * 8 + nwc * 4 * (has_inf_ ? 3 : 2) : nwc * 4 * 2;
+    const bool narrow = trop && narrow_ok_;
+    const u64 bytes = trop ? nc * (narrow ? 4 : 8) + nwc * 4 * (has_inf_ ? 3 : 2) : nwc * 4 * 2;
     if (used_bytes + bytes > cap_bytes) continue;
     used_bytes += bytes;
-    if (trop) {
+    if (narrow) {
+      if (!widen_flag_.get()) widen_flag_.reset(x.pool, 1);
+      memset_async(x, widen_flag_.get(), 0, 4);
+      dn->cell32.reset(x.pool, nc);
+      memset_async(x, dn->cell32.get(), 0xFF, nc * 4);
+      cd.narrow = 1;
+      cd.cell32 = dn->cell32.get();
+      cd.widen = widen_flag_.get();
+    } else if (trop) {
       dn->cell.reset(x.pool, nc);
       memset_async(x, dn->cell.get(), 0xFF, nc * 8);
+    }
+    if (trop) {
       dn->maxf.reset(x.pool, nwc);
       memset_async(x, dn->maxf.get(), 0, nwc * 4);
       if (has_inf_) {
@@ -2005,6 +2028,33 @@ void Engine::plan_dense(u32 k) {
     }
     dense_[i] = std::move(dn);
   }
+}
+
+bool Engine::widen_needed() {
+  bool any = false;
+  for (auto& d : dense_)
+    if (d && d->cd.narrow) any = true;
+  if (!any) return false;
+  Exec x = ctx_->exec();
+  u32 f = 0;
+  SK_HIP(hipMemcpyAsync(&f, widen_flag_.get(), 4, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  return f != 0;
+}
+
+void Engine::widen_dense() {
+  Exec x = ctx_->exec();
+  for (auto& d : dense_) {
+    if (!d || !d->cd.narrow) continue;
+    const u64 nc = std::max<u64>(d->ncells, 32);
+    d->cell.reset(x.pool, nc);
+    cells_widen(x, d->cell32.get(), d->cell.get(), nc);
+    d->cell32.release();
+    d->cd.narrow = 0;
+    d->cd.cell32 = nullptr;
+    d->cd.cell = d->cell.get();
+  }
+  memset_async(x, widen_flag_.get(), 0, 4);
 }
 
 // Materialise every dense relation of the stratum as its sorted full state
@@ -2090,20 +2140,30 @@ void Engine::run_stratum(u32 k) {
   bool bootstrap = true;
   for (;;) {
     std::map<int, std::vector<Chunk>> chunks;
-    for (size_t ri = 0; ri < P_.rules.size(); ri++) {
-      const Rule& r = P_.rules[ri];
-      if (r.stratum != k || r.agg) continue;
-      if (r.head.rel < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, r.head.pred, "unknown predicate");
-      if (bootstrap) {
-        exec_plan(plan_of((int)ri, -1), chunks);
-        continue;
+    auto run_rules = [&]() {
+      for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+        const Rule& r = P_.rules[ri];
+        if (r.stratum != k || r.agg) continue;
+        if (r.head.rel < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, r.head.pred, "unknown predicate");
+        if (bootstrap) {
+          exec_plan(plan_of((int)ri, -1), chunks);
+          continue;
+        }
+        for (size_t j = 0; j < r.body.size(); j++) {
+          const Atom& a = r.body[j];
+          if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
+          if (R_[a.rel].delta.n == 0) continue;
+          exec_plan(plan_of((int)ri, (int)j), chunks);
+        }
       }
-      for (size_t j = 0; j < r.body.size(); j++) {
-        const Atom& a = r.body[j];
-        if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
-        if (R_[a.rel].delta.n == 0) continue;
-        exec_plan(plan_of((int)ri, (int)j), chunks);
-      }
+    };
+    run_rules();
+    if (widen_needed()) {
+      // a derivation did not fit a narrow cell: widen every narrow relation
+      // and apply the whole round again (re-applying is a no-op for ⊕)
+      widen_dense();
+      chunks.clear();
+      run_rules();
     }
     if (is_trop && !nparts_) check_device_error();
     bool any = false;

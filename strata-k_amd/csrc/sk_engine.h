@@ -159,6 +159,7 @@ struct DenseRel {
   u64 ncells = 0;
   CellDesc cd;
   DBuf<u64> cell;
+  DBuf<u32> cell32;  // narrow Trop cells
   DBuf<u32> present, maxf, infb, touched, bad;
   DBuf<u32> dom_bits[kMaxCols];
   DBuf<u64> dom_pre[kMaxCols];
@@ -313,6 +314,8 @@ class Engine {
   std::vector<std::unique_ptr<DenseRel>> dense_;  // per relation, while its stratum runs
   std::map<std::pair<int, int>, ColCodes> colcodes_;  // column code sets, by (relation, column)
   bool dense_enabled_ = true;
+  bool narrow_ok_ = true;   // every staged Trop weight is finite-or-inf in [0, 2^32-1)
+  DBuf<u32> widen_flag_;    // set by a derivation that does not fit a narrow cell
 
   void build_dict();
   void upload_rel(int r);
@@ -321,6 +324,8 @@ class Engine {
   void run_stratum(u32 k);
   void plan_dense(u32 k);
   void end_dense();
+  bool widen_needed();
+  void widen_dense();
   DenseRel* dense_of(int rel) { return rel >= 0 && (size_t)rel < dense_.size() ? dense_[rel].get() : nullptr; }
   void eval_aggregate(const Rule& r);
   RulePlan plan_rule(int ri, int dpos);

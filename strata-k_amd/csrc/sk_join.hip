@@ -510,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, Ou
 #pragma unroll
   for (int k = 0; k < kEmitItems; k++) {
     const u64 c = ok[k] ? cell[k] : 0;
-    if constexpr (TROP) seen[k] = o.cd.cell[c];
+    if constexpr (TROP) seen[k] = o.cd.narrow ? (u64)o.cd.cell32[c] : o.cd.cell[c];
     else seen[k] = o.cd.present[c >> 5];
   }
   // phase 4: apply
@@ -569,7 +569,7 @@ u64 join(const Exec& x, const Table& in, const ProbeDesc& p, const K* skeys, con
   double rec_in = sizeof(K) + (trop ? 8.0 + (sf ? 1.0 : 0) : 0);
   // (a head applied to cells reads one cell word per row instead of writing a record)
   double rec_out = (final_step && o->cd.on)
-                       ? (trop ? 8.0 : 4.0)
+                       ? (trop ? (o->cd.narrow ? 4.0 : 8.0) : 4.0)
                        : (final_step ? (k128_out ? 16.0 : 8.0) : 4.0 * o->ncols) + (trop ? 8.0 + (o->inf ? 1.0 : 0) : 0);
   Scope sc(x.timer, "join_emit", KC_JOIN, (double)total * (rec_in + rec_out), total);
   if (final_step) {

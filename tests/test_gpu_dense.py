@@ -61,6 +61,18 @@ def test_weight_extremes(ctx, case):
     both(ctx, tc_program(TROP), edb, label=f"extreme{case}")
 
 
+@pytest.mark.parametrize("wt", [2**31 - 1, 2**32 - 2, 2**32 - 1, 2**32 + 5])
+def test_narrow_cells_widen(ctx, wt):
+    """Non-negative weights start on narrow (u32) cells when every fact fits;
+    a chain whose costs pass 2^32 mid-fixpoint forces the cells wide and the
+    round to be applied again; a fact at or past 2^32-1 starts wide."""
+    edb = [("edge", (Sym(i), Sym(i + 1)), wt) for i in range(12)]
+    edb += [("edge", (Sym(3), Sym(9)), 1), ("edge", (Sym(5), Sym(2)), INF), ("edge", (Sym(2), Sym(0)), 7)]
+    both(ctx, tc_program(TROP), edb, label=f"narrow{wt}")
+    src_edb = [e for e in edb] + [("src", (Sym(0),), 0), ("src", (Sym(4),), 0)]
+    both(ctx, cost_program(), src_edb, label=f"narrowcost{wt}")
+
+
 def test_idb_with_max_and_inf_facts(ctx):
     """A relation that starts with INT64_MAX and +inf facts: derivations below
     them are fresh, equal ones are not."""
