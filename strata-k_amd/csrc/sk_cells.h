@@ -51,6 +51,7 @@ struct CellDesc {
   u32* maxf = nullptr;     // Trop: present with finite INT64_MAX
   u32* infb = nullptr;     // Trop: present with +inf
   u32* touched = nullptr;  // strictly improved in the current round
+  u32* prev = nullptr;     // when set: the compaction also keeps the round's touched set here
   u32* bad = nullptr;      // set when a value falls outside its column domain
 };
 
@@ -186,5 +187,8 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
 void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present);
 // Narrow cell codes -> wide codes (same states).
 void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n);
+// Exclusive prefix over rows of the set bits in `bits` (rows of 2^lbits
+// cells): row_off[r] = number of set cells in rows before r; returns the total.
+u64 cells_row_offsets(const Exec& x, const u32* bits, u64 nrows, int lbits, u64* row_off);
 
 }  // namespace sk

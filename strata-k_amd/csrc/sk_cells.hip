@@ -212,6 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
   const u64 wi = w0 + wl;
   u32 b = wl < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
   if (touched_only && b) cd.touched[wi] = 0;
+  if (touched_only && cd.prev && wl < nwords) cd.prev[wi] = b;
   u32 tot;
   u32 o = cells_block_scan((u32)__popc(b), tmp, &tot);
   while (b) {
@@ -317,6 +318,31 @@ void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
   Scope sc(x.timer, "cells_widen", KC_MISC, (double)n * 12.0, n);
   k_cells_widen<<<gridc(n), kBlock, 0, x.stream>>>(c32, c64, n);
   SK_HIP(hipGetLastError());
+}
+
+__global__ void k_cells_row_popc(const u32* __restrict__ bits, u64 nrows, int lbits, u32* __restrict__ cnt) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  u32 c = 0;
+  if (lbits >= 5) {
+    const u64 w0 = r << (lbits - 5), w1 = (r + 1) << (lbits - 5);
+    for (u64 w = w0; w < w1; w++) c += (u32)__popc(bits[w]);
+  } else {
+    const u64 b0 = r << lbits;
+    const u32 m = (lbits == 0) ? 1u : ((1u << (1u << lbits)) - 1u);
+    c = (u32)__popc((bits[b0 >> 5] >> (b0 & 31)) & m);
+  }
+  cnt[r] = c;
+}
+
+u64 cells_row_offsets(const Exec& x, const u32* bits, u64 nrows, int lbits, u64* row_off) {
+  DBuf<u32> cnt(x.pool, nrows ? nrows : 1);
+  DBuf<u64> tot(x.pool, 1);
+  if (!nrows) return 0;
+  k_cells_row_popc<<<gridc(nrows), kBlock, 0, x.stream>>>(bits, nrows, lbits, cnt.get());
+  SK_HIP(hipGetLastError());
+  scan_u32(x, cnt.get(), row_off, nrows, tot.get());
+  return read_u64(x, tot.get());
 }
 
 #define SK_CELLS_INST(K)                                                                                      \

@@ -1017,7 +1017,58 @@ static int col_index(const std::vector<int>& slots, int slot) {
   return -1;
 }
 
+// Destination-side evaluation of a delta variant, when the rule has the
+// pull shape and this round's push join would produce at least a quarter
+// of |E| outputs.  Returns false to run the push join instead.
+bool Engine::try_pull(const RulePlan& pl) {
+  if (pl.dpos < 0) return false;
+  const int hr = P_.rules[pl.rule].head.rel;
+  DenseRel* dn = dense_of(hr);
+  if (!dn) return false;
+  PullRule* pr = nullptr;
+  for (auto& p : dn->pulls)
+    if (p.rule == pl.rule && p.dpos == pl.dpos) pr = &p;
+  if (!pr) return false;
+  DRel& H = R_[hr];
+  DRel& E = R_[pr->E];
+  if (H.delta.n == 0 || E.full.n == 0) return false;
+  Exec x = ctx_->exec();
+  const bool trop = H.sem == SK_TROP;
+  Index& ixj = get_index(pr->E, false, {pr->eJ, pr->eR}, true, false);
+  u64 cand = 0;
+  with_key(H.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    cand = push_cost<K>(x, (const K*)H.delta.k.ptr(), H.delta.n, 2, H.bits, pr->R, ixj.csr.get());
+  });
+  if (cand * 4 < E.full.n) return false;
+  Index& ixr = get_index(pr->E, false, {pr->eR, pr->eJ}, true, true);
+  const u64 nv = D_.nvals();
+  if (pr->e_version != E.version) {
+    pr->nchunks = pull_chunks(x, ixr.csr.get(), nv, 256, &pr->crow, &pr->clo, &pr->chi);
+    pr->e_version = E.version;
+  }
+  PullDesc pd;
+  pd.nchunks = pr->nchunks;
+  pd.chunk_row = pr->crow.get();
+  pd.chunk_lo = pr->clo.get();
+  pd.chunk_hi = pr->chi.get();
+  pd.adj = ixr.adj.get();
+  pd.w = trop ? (ixr.direct ? E.full.w.get() : ixr.data.w.get()) : nullptr;
+  pd.prev = dn->prev.get();
+  pd.prev_off = dn->prev_off.get();
+  pd.dw = trop ? H.delta.w.get() : nullptr;
+  pd.lbits = dn->lbits;
+  pd.delta_first = pr->delta_first ? 1 : 0;
+  pd.nrows = dn->nrows;
+  // algorithmic bytes: every index row of E (j, weight, mask word) once,
+  // every candidate's delta weight once
+  const double bytes = (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 8.0) + (double)cand * (trop ? 8.0 : 0.0);
+  pull_apply(x, pd, dn->cd, trop, bytes, cand);
+  return true;
+}
+
 void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out) {
+  if (try_pull(pl)) return;
   const Rule& r = P_.rules[pl.rule];
   Exec x = ctx_->exec();
   const int hr = r.head.rel;
@@ -1302,6 +1353,7 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
       DBuf<u8> df;
       nd = cells_compact<K>(x, dn->cd, dn->ncells, true, bits, &dk, &dw, &df, hf);
       if (!nd) return;
+      if (!dn->pulls.empty()) cells_row_offsets(x, dn->prev.get(), dn->nrows, dn->lbits, dn->prev_off.get());
       R.delta.n = nd;
       R.delta.k.wide = sizeof(K) == 16;
       if constexpr (sizeof(K) == 16) R.delta.k.b = std::move(dk);
@@ -2025,6 +2077,59 @@ void Engine::plan_dense(u32 k) {
         cells_load<K>(x, cd, (const K*)R.full.k.ptr(), trop ? R.full.w.get() : nullptr,
                       (trop && has_inf_) ? R.full.inf.get() : nullptr, R.full.n, R.arity, R.bits);
       });
+    }
+    // Rules a round may evaluate from the destination side: two positive
+    // literals, this relation's delta and a fixed binary relation E of the
+    // same semiring, in the shape H(r, x) :- H(j, x), E(j, r); the head's row
+    // column must be the cells' major column with an identity domain.
+    if (R.arity == 2 && !nparts_ && !(trop && has_inf_)) {
+      for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+        const Rule& ru = P_.rules[ri];
+        if (ru.stratum != k || ru.agg || ru.head.rel != (int)i || ru.body.size() != 2) continue;
+        for (int dpos = 0; dpos < 2; dpos++) {
+          const Atom& A = ru.body[dpos];
+          const Atom& B = ru.body[1 - dpos];
+          if (A.neg || B.neg || A.rel != (int)i || B.rel < 0 || B.rel == (int)i || in_idb_[B.rel]) continue;
+          if (R_[B.rel].arity != 2 || R_[B.rel].sem != R.sem || A.args.size() != 2 || B.args.size() != 2) continue;
+          bool vars = ru.head.args.size() == 2;
+          for (auto* at : {&ru.head, &A, &B})
+            for (auto& t : at->args) vars = vars && t.kind == SK_T_VAR;
+          if (!vars) continue;
+          auto hv = [&](int c) { return ru.head.args[c].a; };
+          auto av = [&](int c) { return A.args[c].a; };
+          auto bv = [&](int c) { return B.args[c].a; };
+          for (int Lc = 0; Lc < 2; Lc++) {
+            const int Rc = 1 - Lc;
+            if (hv(Lc) != av(Lc) || hv(Rc) == av(Rc)) continue;
+            const u32 J = av(Rc);
+            if (J == hv(Lc) || hv(Rc) == hv(Lc)) continue;
+            for (int eJ = 0; eJ < 2; eJ++) {
+              const int eR = 1 - eJ;
+              if (bv(eJ) != J || bv(eR) != hv(Rc)) continue;
+              if (cd.dom_bits[Rc] || cd.shift[Lc] != 0 || cd.shift[Rc] != cd.ebits[Lc]) continue;
+              if (cd.ebits[Lc] > 6) continue;  // rows of at most one wave of lanes
+              PullRule pr;
+              pr.rule = (int)ri;
+              pr.dpos = dpos;
+              pr.E = B.rel;
+              pr.eR = eR;
+              pr.eJ = eJ;
+              pr.R = Rc;
+              pr.L = Lc;
+              pr.delta_first = dpos == 0;
+              dn->pulls.push_back(std::move(pr));
+              dn->lbits = cd.ebits[Lc];
+            }
+          }
+        }
+      }
+      if (!dn->pulls.empty()) {
+        dn->nrows = dn->ncells >> dn->lbits;
+        dn->prev.reset(x.pool, nwc);
+        memset_async(x, dn->prev.get(), 0, nwc * 4);
+        dn->prev_off.reset(x.pool, dn->nrows ? dn->nrows : 1);
+        cd.prev = dn->prev.get();
+      }
     }
     dense_[i] = std::move(dn);
   }

@@ -150,6 +150,19 @@ struct DRel {
   u64 st_reserve = 0;  // expected staged rows (capacity hint)
 };
 
+// A rule `H(r, x) :- H(j, x), E(j, r)` (any body order, either column
+// order) that a round may evaluate from the destination side (k_pull).
+struct PullRule {
+  int rule = -1, dpos = -1;  // the rule and its delta literal
+  int E = -1, eR = 0, eJ = 1;  // the fixed relation and its row / join columns
+  int R = 0, L = 1;          // the head's row column and lane column
+  bool delta_first = true;   // ⊗ order in the body
+  u64 e_version = ~0ull;     // chunk list built for this version of E
+  u64 nchunks = 0;
+  DBuf<u32> crow;
+  DBuf<u64> clo, chi;
+};
+
 // Dense cell state of one relation while its stratum runs (see sk_cells.h).
 struct DenseRel {
   bool keep_full = false;  // some rule of the stratum reads the relation's full
@@ -164,6 +177,12 @@ struct DenseRel {
   DBuf<u32> dom_bits[kMaxCols];
   DBuf<u64> dom_pre[kMaxCols];
   DBuf<u32> dom_list[kMaxCols];
+  // pull evaluation: the last delta as a bitmap and its per-row offsets
+  std::vector<PullRule> pulls;
+  DBuf<u32> prev;
+  DBuf<u64> prev_off;
+  u64 nrows = 0;
+  int lbits = 0;
 };
 
 // Distinct codes of one relation column, as a bitmap over the dictionary.
@@ -330,6 +349,7 @@ class Engine {
   void eval_aggregate(const Rule& r);
   RulePlan plan_rule(int ri, int dpos);
   void exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out);
+  bool try_pull(const RulePlan& pl);
   u64 finalize(int rel, std::vector<Chunk>& chunks);
   const RelData& source(int rel, bool delta) { return delta ? R_[rel].delta : R_[rel].full; }
   Index& get_index(int rel, bool delta, const std::vector<int>& perm, bool want_csr, bool want_adj);

@@ -66,6 +66,24 @@ struct OutDesc {
 
 using HeadAlloc = void (*)(void* ctx, OutDesc* o, u64 n);
 
+// Pull evaluation of a linear recursive rule over dense cells (see
+// k_pull): head H(r, x) ⊕= Δ(j, x) ⊗ E(j → r), rows r of the head taken from
+// E's index on its r column, lanes x passed through from the delta.
+struct PullDesc {
+  u64 nchunks = 0;
+  const u32* chunk_row = nullptr;  // head row of each chunk
+  const u64* chunk_lo = nullptr;   // index rows [lo, hi) of E in that row
+  const u64* chunk_hi = nullptr;
+  const u32* adj = nullptr;        // per index row: the j value (code = rank)
+  const i64* w = nullptr;          // per index row: the weight (Trop)
+  const u32* prev = nullptr;       // last round's delta, as a bitmap in cell layout
+  const u64* prev_off = nullptr;   // per row: position of the row's first delta cell in the delta list
+  const i64* dw = nullptr;         // delta weights, in cell order
+  int lbits = 0;                   // lanes per row = 2^lbits
+  int delta_first = 1;             // ⊗ operand order: the delta's weight on the left
+  u64 nrows = 0;                   // rows of the cell layout (j values at or above are absent)
+};
+
 struct AntiDesc {
   int arity = 0, bits = 0;
   int src[kMaxCols] = {};  // table column, or -1 = constant
@@ -130,5 +148,13 @@ template <class K>
 u64 drop_partitions(const Exec& x, const K* keys, const i64* w, const u8* inf, u64 n, int arity, int bits,
                     const u8* dead, K* ok, i64* ow, u8* of);
 u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords);
+
+// Chunks of at most `per` index rows per head row, from a CSR over the head
+// row column (nrows + 1 offsets).  Returns the chunk count.
+u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, u32 per, DBuf<u32>* row, DBuf<u64>* lo, DBuf<u64>* hi);
+// Σ over the delta rows of the push join's output count (the push cost).
+template <class K>
+u64 push_cost(const Exec& x, const K* dkeys, u64 n, int arity, int bits, int col, const u64* csr);
+void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop, double bytes, u64 items);
 
 }  // namespace sk

@@ -11,6 +11,8 @@
 //   leapfrog  sorted-list intersection for triangle bodies
 // Trop ⊗ is checked i64 addition in body order with +inf absorbing; the
 // first overflow is recorded in the device error record (crates/strata-ir/src/trop.rs:56-62).
+#include <climits>
+
 #include "sk_join.h"
 
 namespace sk {
@@ -877,18 +879,25 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
 
 // Σ over driver rows of |adj2(v)| + |adj3(u)| (the leapfrog's list reads).
 template <class K>
-__global__ void k_tri_listlen(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
-                              const u64* __restrict__ csr3, unsigned long long* __restrict__ acc) {
-  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_tri_listlen(const K* __restrict__ d_keys, u64 nd, TriDesc t,
+                                                        const u64* __restrict__ csr2, const u64* __restrict__ csr3,
+                                                        unsigned long long* __restrict__ acc) {
+  __shared__ u64 part[kBlock / 64];
   u64 s = 0;
-  if (e < nd) {
+  for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < nd; e += (u64)gridDim.x * blockDim.x) {
     K k = d_keys[e];
     u32 u = col_of<K>(k, 2, t.bits, t.u_col);
     u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
-    s = (csr2[v + 1] - csr2[v]) + (csr3[u + 1] - csr3[u]);
+    s += (csr2[v + 1] - csr2[v]) + (csr3[u + 1] - csr3[u]);
   }
   s = wave_incl_scan_j(s);
-  if (lane_id_j() == 63 && s) atomicAdd(acc, (unsigned long long)s);
+  if (lane_id_j() == 63) part[wave_id_j()] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 tt = 0;
+    for (int w = 0; w < kBlock / 64; w++) tt += part[w];
+    if (tt) atomicAdd(acc, (unsigned long long)tt);
+  }
 }
 
 template <class K>
@@ -896,9 +905,214 @@ double tri_list_len(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, co
   if (!nd) return 0;
   DBuf<u64> acc(x.pool, 1);
   memset_async(x, acc.get(), 0, 8);
-  k_tri_listlen<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, csr3, (unsigned long long*)acc.get());
+  k_tri_listlen<K><<<(unsigned)std::min<u64>(gridj(nd), 2048), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, csr3,
+                                                                                     (unsigned long long*)acc.get());
   SK_HIP(hipGetLastError());
   return (double)read_u64(x, acc.get());
+}
+
+// ================================================================== pull
+//
+// A round of `H(r, x) :- H(j, x), E(j, r)` (or the mirrored column order)
+// evaluated from the destination side.  One wave owns a chunk of E's index
+// rows for one head row r; lane l stands for the lane value x = g + l.  For
+// each index row (j, w) of the chunk, lanes first fetch 64 index rows at
+// once, read each j's bits of the last round's delta, and then every edge
+// with a non-empty mask is taken in turn: lanes whose bit is set read the
+// delta weight at (row offset of j + rank of the bit), which is contiguous
+// across lanes, and fold w ⊗ Δ into their running minimum (Bool: presence).
+// Only the final minimum per lane goes to the cell.  The candidates are
+// exactly the push join's (every (delta row, edge) pair), so the round's
+// result, fresh set and errors are the same; what changes is that each head
+// cell is touched once per chunk instead of once per candidate.
+
+// Position of the k-th (0-based) set bit of m (k < popcount(m)).
+__device__ __forceinline__ u32 select_bit(u64 m, u32 k) {
+  u32 pos = 0, c;
+  c = (u32)__popc((u32)m);
+  if (k >= c) { k -= c; m >>= 32; pos += 32; }
+  c = (u32)__popc((u32)(m & 0xFFFFu));
+  if (k >= c) { k -= c; m >>= 16; pos += 16; }
+  c = (u32)__popc((u32)(m & 0xFFu));
+  if (k >= c) { k -= c; m >>= 8; pos += 8; }
+  c = (u32)__popc((u32)(m & 0xFu));
+  if (k >= c) { k -= c; m >>= 4; pos += 4; }
+  c = (u32)__popc((u32)(m & 0x3u));
+  if (k >= c) { k -= c; m >>= 2; pos += 2; }
+  if (k >= (u32)(m & 1u)) pos += 1;
+  return pos;
+}
+
+// Per wave of the pull kernel: one batch of 64 index rows.
+struct PullLds {
+  u64 m[64];     // delta lanes of each edge's source row
+  u64 base[64];  // delta-list position of that row's first delta cell
+  i64 w[64];     // edge weight
+  u32 incl[64];  // inclusive prefix of popcount(m)
+  i64 acc[64];   // running minimum per lane value
+  u32 have[64];  // lane value has a candidate (Bool: is derived)
+};
+
+// Rows of at most 64 lanes (lbits <= 6).  A wave takes its chunk in batches
+// of 64 index rows: lane l loads row l's source, weight and delta mask, a
+// wave scan counts the (edge, delta lane) pairs, and the pairs are dealt
+// round-robin to lanes, so every lane does work whatever the masks' density.
+// A lane finds its pair's edge by binary search over the scan, its lane value
+// by selecting the set bit, and its delta weight at base + rank (pairs of one
+// edge read consecutive delta positions).  Candidates fold into the wave's
+// LDS minimum; each lane value's minimum is applied to its cell once per
+// chunk.
+template <bool TROP>
+__global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevErr* err) {
+  __shared__ PullLds sh[kBlock / 64];
+  const u32 lane = threadIdx.x & 63;
+  const u64 ch = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
+  PullLds& L = sh[threadIdx.x >> 6];
+  const u64 r = pd.chunk_row[ch];
+  const u64 lo = pd.chunk_lo[ch], hi = pd.chunk_hi[ch];
+  const u32 nl = 1u << pd.lbits;  // lanes per row, <= 64
+  L.acc[lane] = LLONG_MAX;
+  L.have[lane] = 0;
+  for (u64 e0 = lo; e0 < hi; e0 += 64) {
+    const u64 e = e0 + lane;
+    u64 m = 0, base = 0;
+    i64 we = 0;
+    if (e < hi) {
+      const u32 q = pd.adj[e];
+      if constexpr (TROP) we = pd.w[e];
+      if (q < pd.nrows) {
+        const u64 b0 = (u64)q << pd.lbits;
+        if (nl == 64) m = (u64)pd.prev[b0 >> 5] | ((u64)pd.prev[(b0 >> 5) + 1] << 32);
+        else m = ((u64)pd.prev[b0 >> 5] >> (b0 & 31)) & ((1ull << nl) - 1);
+        if (m) base = pd.prev_off[q];
+      }
+    }
+    const u32 c = (u32)__popcll(m);
+    const u32 incl = (u32)wave_incl_scan_j(c);
+    const u32 total = __shfl(incl, 63);
+    L.m[lane] = m;
+    L.base[lane] = base;
+    L.w[lane] = we;
+    L.incl[lane] = incl;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (u32 pidx = lane; pidx < total; pidx += 64) {
+      // the edge holding pair pidx: the first j with incl[j] > pidx
+      u32 lo2 = 0, hi2 = 64;
+      while (lo2 < hi2) {
+        const u32 mid = (lo2 + hi2) >> 1;
+        if (L.incl[mid] <= pidx) lo2 = mid + 1;
+        else hi2 = mid;
+      }
+      const u32 j = lo2;
+      const u32 k = pidx - (L.incl[j] - (u32)__popcll(L.m[j]));
+      const u32 xl = select_bit(L.m[j], k);
+      if constexpr (TROP) {
+        const i64 v = pd.dw[L.base[j] + k];
+        i64 res;
+        u8 rf;
+        if (pd.delta_first) trop_mul(v, 0, L.w[j], 0, &res, &rf, err);
+        else trop_mul(L.w[j], 0, v, 0, &res, &rf, err);
+        atomicMin((long long*)&L.acc[xl], (long long)res);
+        L.have[xl] = 1u;
+      } else {
+        L.have[xl] = 1u;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane < nl && L.have[lane]) {
+    const u64 cl = (r << pd.lbits) + lane;
+    cell_apply_t<TROP>(cd, cl, TROP ? L.acc[lane] : 0, 0, cell_peek(cd, cl));
+  }
+}
+
+void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop, double bytes, u64 items) {
+  if (!p.nchunks) return;
+  Scope sc(x.timer, "join_pull", KC_JOIN, bytes, items);
+  const u64 waves_per_block = kBlock / 64;
+  const unsigned nb = (unsigned)((p.nchunks + waves_per_block - 1) / waves_per_block);
+  if (trop) k_pull<true><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
+  else k_pull<false><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
+  SK_HIP(hipGetLastError());
+}
+
+__global__ void k_pull_chunk_count(const u64* __restrict__ csr, u64 nrows, u32 per, u32* __restrict__ cnt) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const u64 d = csr[r + 1] - csr[r];
+  cnt[r] = (u32)((d + per - 1) / per);
+}
+
+__global__ void k_pull_chunk_write(const u64* __restrict__ csr, u64 nrows, u32 per, const u64* __restrict__ off,
+                                   u32* __restrict__ row, u64* __restrict__ lo, u64* __restrict__ hi) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const u64 a = csr[r], b = csr[r + 1];
+  u64 o = off[r];
+  for (u64 s = a; s < b; s += per, o++) {
+    row[o] = (u32)r;
+    lo[o] = s;
+    hi[o] = min(s + per, b);
+  }
+}
+
+u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, u32 per, DBuf<u32>* row, DBuf<u64>* lo, DBuf<u64>* hi) {
+  DBuf<u32> cnt(x.pool, nrows ? nrows : 1);
+  DBuf<u64> off(x.pool, nrows ? nrows : 1);
+  DBuf<u64> tot(x.pool, 1);
+  if (!nrows) return 0;
+  k_pull_chunk_count<<<gridj(nrows), kBlock, 0, x.stream>>>(csr, nrows, per, cnt.get());
+  SK_HIP(hipGetLastError());
+  scan_u32(x, cnt.get(), off.get(), nrows, tot.get());
+  const u64 n = read_u64(x, tot.get());
+  row->reset(x.pool, n ? n : 1);
+  lo->reset(x.pool, n ? n : 1);
+  hi->reset(x.pool, n ? n : 1);
+  if (n) {
+    k_pull_chunk_write<<<gridj(nrows), kBlock, 0, x.stream>>>(csr, nrows, per, off.get(), row->get(), lo->get(),
+                                                               hi->get());
+    SK_HIP(hipGetLastError());
+  }
+  return n;
+}
+
+// Grid-stride sum; one atomic per block (a single counter hit once per wave
+// serialises the whole launch).
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_push_cost(const K* __restrict__ dkeys, u64 n, int arity, int bits, int col,
+                                                      const u64* __restrict__ csr,
+                                                      unsigned long long* __restrict__ acc) {
+  __shared__ u64 part[kBlock / 64];
+  u64 s = 0;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) {
+    const u32 v = col_of<K>(dkeys[i], arity, bits, col);
+    s += csr[v + 1] - csr[v];
+  }
+  s = wave_incl_scan_j(s);
+  if (lane_id_j() == 63) part[wave_id_j()] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t = 0;
+    for (int w = 0; w < kBlock / 64; w++) t += part[w];
+    if (t) atomicAdd(acc, (unsigned long long)t);
+  }
+}
+
+template <class K>
+u64 push_cost(const Exec& x, const K* dkeys, u64 n, int arity, int bits, int col, const u64* csr) {
+  if (!n) return 0;
+  DBuf<u64> acc(x.pool, 1);
+  memset_async(x, acc.get(), 0, 8);
+  {
+    Scope sc(x.timer, "push_cost", KC_MISC, (double)n * (sizeof(K) + 16.0), n);
+    const unsigned nb = (unsigned)std::min<u64>((n + kBlock - 1) / kBlock, 2048);
+    k_push_cost<K><<<nb, kBlock, 0, x.stream>>>(dkeys, n, arity, bits, col, csr, (unsigned long long*)acc.get());
+    SK_HIP(hipGetLastError());
+  }
+  return read_u64(x, acc.get());
 }
 
 // ================================================================ remap
@@ -949,11 +1163,19 @@ __global__ void k_mark(const u32* __restrict__ c, u64 n, u32 limit, u32* __restr
   if (v < limit && !(bitmap[v >> 5] & (1u << (v & 31)))) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
 }
 
-__global__ void k_popcount(const u32* __restrict__ b, u64 n, unsigned long long* acc) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  u64 s = i < n ? (u64)__popc(b[i]) : 0;
+__global__ __launch_bounds__(kBlock) void k_popcount(const u32* __restrict__ b, u64 n, unsigned long long* acc) {
+  __shared__ u64 part[kBlock / 64];
+  u64 s = 0;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+    s += (u64)__popc(b[i]);
   s = wave_incl_scan_j(s);
-  if (lane_id_j() == 63 && s) atomicAdd(acc, (unsigned long long)s);
+  if (lane_id_j() == 63) part[wave_id_j()] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t = 0;
+    for (int w = 0; w < kBlock / 64; w++) t += part[w];
+    if (t) atomicAdd(acc, (unsigned long long)t);
+  }
 }
 
 void mark_codes(const Exec& x, const u32* c, u64 n, u32 limit, u32* bitmap) {
@@ -965,7 +1187,9 @@ void mark_codes(const Exec& x, const u32* c, u64 n, u32 limit, u32* bitmap) {
 u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords) {
   DBuf<u64> acc(x.pool, 1);
   memset_async(x, acc.get(), 0, 8);
-  if (nwords) k_popcount<<<gridj(nwords), kBlock, 0, x.stream>>>(bitmap, nwords, (unsigned long long*)acc.get());
+  if (nwords)
+    k_popcount<<<(unsigned)std::min<u64>(gridj(nwords), 2048), kBlock, 0, x.stream>>>(bitmap, nwords,
+                                                                                (unsigned long long*)acc.get());
   SK_HIP(hipGetLastError());
   return read_u64(x, acc.get());
 }
@@ -980,6 +1204,7 @@ u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords) {
   template u64 anti_join<K>(const Exec&, const Table&, const AntiDesc&, const K*, u64, Table*, TableAlloc, void*); \
   template u64 aggregate<K>(const Exec&, const Table&, const AggDesc&, const i64*, int, std::vector<K>*,         \
                             std::vector<i64>*, bool*);                                                          \
+  template u64 push_cost<K>(const Exec&, const K*, u64, int, int, int, const u64*);                              \
   template u64 triangles<K>(const Exec&, const K*, u64, const TriDesc&, const u64*, const u32*, const u64*,       \
                             const u32*, OutDesc*, bool, bool, HeadAlloc, void*, double*);                        \
   template double tri_list_len<K>(const Exec&, const K*, u64, const TriDesc&, const u64*, const u64*);           \
