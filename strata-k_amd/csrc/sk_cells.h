@@ -51,7 +51,17 @@ struct CellDesc {
   u32* maxf = nullptr;     // Trop: present with finite INT64_MAX
   u32* infb = nullptr;     // Trop: present with +inf
   u32* touched = nullptr;  // strictly improved in the current round
-  u32* prev = nullptr;     // when set: the compaction also keeps the round's touched set here
+  // When set, a compaction of the touched cells (from cell 0) also writes one
+  // record per row of 2^rlbits <= 64 cells: {the row's touched lanes, the
+  // position of the row's first touched cell in the compacted list}.
+  u64* rowrec = nullptr;
+  u64 rnrows = 0;
+  int rlbits = 0;
+  // When set with rowrec, the counting pass of that compaction also sums,
+  // over the touched cells, pc_csr[row+1] - pc_csr[row] (row < pc_nv): the
+  // output count of the push join of the new delta against that CSR.
+  const u64* pc_csr = nullptr;
+  u64 pc_nv = 0;
   u32* bad = nullptr;      // set when a value falls outside its column domain
 };
 
@@ -177,18 +187,17 @@ void cells_insert(const Exec& x, const CellDesc& cd, const K* keys, const i64* w
                   int bits);
 // Compact the touched cells (clearing them) or all present cells of
 // [cell0, cell0 + ncells) into a packed relation in cell order; returns its
-// size.  cell0 is a multiple of 32.  The outputs are (re)allocated.
+// size.  cell0 is a multiple of 32.  The outputs are (re)allocated.  With
+// `push_cost` given and cd.pc_csr set, *push_cost receives the sum described
+// at CellDesc::pc_csr, or ~0 when it was not computed.
 template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
-                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0 = 0);
+                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0 = 0, u64* push_cost = nullptr);
 
 // State of arbitrary cells: present[i] and the weight of cell idx[i]
 // (+inf reads as present with weight 0; check infb where it matters).
 void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present);
 // Narrow cell codes -> wide codes (same states).
 void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n);
-// Exclusive prefix over rows of the set bits in `bits` (rows of 2^lbits
-// cells): row_off[r] = number of set cells in rows before r; returns the total.
-u64 cells_row_offsets(const Exec& x, const u32* bits, u64 nrows, int lbits, u64* row_off);
 
 }  // namespace sk

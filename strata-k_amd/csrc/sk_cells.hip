@@ -193,13 +193,45 @@ __device__ __forceinline__ u32 cells_block_scan(u32 v, u32* tmp, u32* total) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_cells_count(CellDesc cd, u64 w0, u64 nwords, int touched_only,
-                                                        u32* __restrict__ cnt) {
+                                                        u32* __restrict__ cnt, u64* __restrict__ pcb) {
   __shared__ u32 tmp[kBlock / 64];
+  __shared__ u64 ptmp[kBlock / 64];
   const u64 wl = (u64)blockIdx.x * kBlock + threadIdx.x;
-  u32 c = wl < nwords ? (u32)__popc(cells_word(cd, w0 + wl, touched_only != 0)) : 0u;
+  const u32 b = wl < nwords ? cells_word(cd, w0 + wl, touched_only != 0) : 0u;
   u32 tot;
-  cells_block_scan(c, tmp, &tot);
+  cells_block_scan((u32)__popc(b), tmp, &tot);
   if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+  if (pcb) {
+    // push cost of the touched cells: each costs its row's degree in pc_csr
+    // (rows of 2^rlbits cells; w0 == 0 here)
+    u64 pc = 0;
+    if (b) {
+      const u64 wi = w0 + wl;
+      const int lb = cd.rlbits;
+      if (lb >= 5) {
+        const u64 r = wi >> (lb - 5);
+        if (r < cd.pc_nv) pc = (u64)__popc(b) * (cd.pc_csr[r + 1] - cd.pc_csr[r]);
+      } else {
+        const int per = 32 >> lb;
+        const u32 m = (1u << (1u << lb)) - 1u;
+        for (int q = 0; q < per; q++) {
+          const u32 lanes = (b >> (q << lb)) & m;
+          const u64 r = wi * (u64)per + q;
+          if (lanes && r < cd.pc_nv) pc += (u64)__popc(lanes) * (cd.pc_csr[r + 1] - cd.pc_csr[r]);
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o, 64);
+    if ((threadIdx.x & 63) == 0) ptmp[threadIdx.x >> 6] = pc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u64 s = 0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; w++) s += ptmp[w];
+      pcb[blockIdx.x] = s;
+    }
+  }
 }
 
 template <class K>
@@ -212,9 +244,36 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
   const u64 wi = w0 + wl;
   u32 b = wl < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
   if (touched_only && b) cd.touched[wi] = 0;
-  if (touched_only && cd.prev && wl < nwords) cd.prev[wi] = b;
   u32 tot;
   u32 o = cells_block_scan((u32)__popc(b), tmp, &tot);
+  if (touched_only && cd.rowrec) {
+    // row records (w0 == 0, so word wi sits in lane wi % 64 of its wave)
+    const u64 g = off[blockIdx.x] + o;
+    const u32 hi = __shfl_down(b, 1, 64);
+    if (wl < nwords) {
+      if (cd.rlbits == 6) {
+        const u64 r = wi >> 1;
+        if (!(wi & 1) && r < cd.rnrows) {
+          ulonglong2 v;
+          v.x = (u64)b | ((u64)hi << 32);
+          v.y = g;
+          *(ulonglong2*)(cd.rowrec + 2 * r) = v;
+        }
+      } else {
+        const int lb = cd.rlbits, per = 32 >> lb;
+        const u32 m = (lb == 5) ? ~0u : ((1u << (1u << lb)) - 1u);
+        for (int q = 0; q < per; q++) {
+          const u64 r = wi * (u64)per + q;
+          if (r >= cd.rnrows) break;
+          const int sh = q << lb;
+          ulonglong2 v;
+          v.x = (b >> sh) & m;
+          v.y = g + (u32)__popc(b & ((1u << sh) - 1u));
+          *(ulonglong2*)(cd.rowrec + 2 * r) = v;
+        }
+      }
+    }
+  }
   while (b) {
     int bit = __ffs(b) - 1;
     b &= b - 1;
@@ -247,21 +306,37 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
 
 template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
-                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0) {
+                  DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0, u64* push_cost) {
   const u64 w0 = cell0 / 32;  // cell0 is a multiple of 32
+  if (cd.rowrec && touched_only && (w0 || cd.rlbits < 0 || cd.rlbits > 6))
+    throw std::runtime_error("cells_compact: row records need a touched compaction from cell 0, rows <= 64 cells");
+  if (push_cost) *push_cost = ~0ull;
   const u64 nwords = (ncells + 31) / 32;
   const u64 nblocks = (nwords + kBlock - 1) / kBlock;
   DBuf<u32> cnt(x.pool, nblocks);
   DBuf<u64> off(x.pool, nblocks);
-  DBuf<u64> tot(x.pool, 1);
+  DBuf<u64> tot(x.pool, 2);
+  // the push cost rides along with the count when the block counts fit one
+  // scan launch
+  const bool pc = push_cost && cd.pc_csr && cd.rowrec && touched_only && nblocks <= kScanOneMax;
+  DBuf<u64> pcb(x.pool, pc ? nblocks : 1);
   const double scan_bytes = touched_only ? 4.0 * nwords : (cd.trop ? 8.0 * ncells + 8.0 * nwords : 4.0 * nwords);
   {
     Scope sc(x.timer, "cells_count", KC_DEDUP, scan_bytes, ncells);
-    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, w0, nwords, touched_only ? 1 : 0, cnt.get());
+    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, w0, nwords, touched_only ? 1 : 0, cnt.get(),
+                                                              pc ? pcb.get() : nullptr);
     SK_HIP(hipGetLastError());
   }
-  scan_u32(x, cnt.get(), off.get(), nblocks, tot.get());
-  u64 n = read_u64(x, tot.get());
+  u64 hv[2] = {0, 0};
+  if (nblocks <= kScanOneMax) {
+    scan_u32_one(x, cnt.get(), off.get(), nblocks, tot.get(), pc ? pcb.get() : nullptr);
+    read_u64s(x, tot.get(), hv, pc ? 2 : 1);
+    if (pc) *push_cost = hv[1];
+  } else {
+    scan_u32(x, cnt.get(), off.get(), nblocks, tot.get());
+    hv[0] = read_u64(x, tot.get());
+  }
+  const u64 n = hv[0];
   if (n == 0) return 0;
   keys->reset(x.pool, n);
   if (cd.trop) {
@@ -320,37 +395,12 @@ void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
   SK_HIP(hipGetLastError());
 }
 
-__global__ void k_cells_row_popc(const u32* __restrict__ bits, u64 nrows, int lbits, u32* __restrict__ cnt) {
-  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  u32 c = 0;
-  if (lbits >= 5) {
-    const u64 w0 = r << (lbits - 5), w1 = (r + 1) << (lbits - 5);
-    for (u64 w = w0; w < w1; w++) c += (u32)__popc(bits[w]);
-  } else {
-    const u64 b0 = r << lbits;
-    const u32 m = (lbits == 0) ? 1u : ((1u << (1u << lbits)) - 1u);
-    c = (u32)__popc((bits[b0 >> 5] >> (b0 & 31)) & m);
-  }
-  cnt[r] = c;
-}
-
-u64 cells_row_offsets(const Exec& x, const u32* bits, u64 nrows, int lbits, u64* row_off) {
-  DBuf<u32> cnt(x.pool, nrows ? nrows : 1);
-  DBuf<u64> tot(x.pool, 1);
-  if (!nrows) return 0;
-  k_cells_row_popc<<<gridc(nrows), kBlock, 0, x.stream>>>(bits, nrows, lbits, cnt.get());
-  SK_HIP(hipGetLastError());
-  scan_u32(x, cnt.get(), row_off, nrows, tot.get());
-  return read_u64(x, tot.get());
-}
-
 #define SK_CELLS_INST(K)                                                                                      \
   template void cells_mark_col<K>(const Exec&, const K*, u64, int, int, int, u32*);                          \
   template void cells_load<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int);  \
   template void cells_insert<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int); \
   template u64 cells_compact<K>(const Exec&, const CellDesc&, u64, bool, int, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, bool, \
-                                u64);
+                                u64, u64*);
 SK_CELLS_INST(u64)
 SK_CELLS_INST(u128)
 

@@ -39,10 +39,12 @@ struct BudgetError {
 };
 
 // Device error record written by kernels: the first overflow of a checked
-// Trop addition wins (atomicCAS on `code`).
+// Trop addition wins (atomicCAS on `code`).  `widen` is set (atomicOr) by a
+// derivation that does not fit a narrow cell; keeping it in the same record
+// lets the host read both with one copy per round.
 struct DevErr {
   int32_t code;  // 0 none, 2 overflow
-  int32_t pad;
+  uint32_t widen;
   i64 lhs, rhs;
 };
 
@@ -182,6 +184,10 @@ struct Exec {
 void scan_u32(const Exec& x, const u32* in, u64* out, u64 n, u64* d_total);
 void scan_u64(const Exec& x, const u64* in, u64* out, u64 n, u64* d_total);
 void scan_u8(const Exec& x, const u8* in, u64* out, u64 n, u64* d_total);
+// Scan of n <= kScanOneMax values in one launch: total[0] = the sum and, when
+// `side` is given, total[1] = the sum of side[0..n).
+constexpr u64 kScanOneMax = (u64)kBlock * 64;
+void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, const u64* side);
 
 // LSD radix sort of keys (stable) on the low `key_bits` bits, optionally
 // carrying i64 weights and u8 +inf flags.  Result in (keys, w, inf); the
@@ -231,5 +237,7 @@ void memset_async(const Exec& x, void* p, int v, u64 bytes);
 void fill_u8(const Exec& x, u8* p, u8 v, u64 n);
 
 u64 read_u64(const Exec& x, const u64* d);
+// n <= 8 device words to the host with one copy and one synchronisation.
+void read_u64s(const Exec& x, const u64* d, u64* h, int n);
 
 }  // namespace sk

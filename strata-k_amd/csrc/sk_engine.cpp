@@ -1036,26 +1036,28 @@ bool Engine::try_pull(const RulePlan& pl) {
   const bool trop = H.sem == SK_TROP;
   Index& ixj = get_index(pr->E, false, {pr->eJ, pr->eR}, true, false);
   u64 cand = 0;
-  with_key(H.wide, [&](auto* tag) {
-    using K = std::remove_pointer_t<decltype(tag)>;
-    cand = push_cost<K>(x, (const K*)H.delta.k.ptr(), H.delta.n, 2, H.bits, pr->R, ixj.csr.get());
-  });
+  if (pr->cost_dv == H.dversion) {
+    cand = pr->cost;
+  } else {
+    with_key(H.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      cand = push_cost<K>(x, (const K*)H.delta.k.ptr(), H.delta.n, 2, H.bits, pr->R, ixj.csr.get());
+    });
+  }
   if (cand * 4 < E.full.n) return false;
   Index& ixr = get_index(pr->E, false, {pr->eR, pr->eJ}, true, true);
   const u64 nv = D_.nvals();
-  if (pr->e_version != E.version) {
-    pr->nchunks = pull_chunks(x, ixr.csr.get(), nv, 256, &pr->crow, &pr->clo, &pr->chi);
-    pr->e_version = E.version;
+  // the chunk list is an index of E, kept with it across runs
+  if (ixr.pull_version != ixr.csr_version) {
+    ixr.pull_n = pull_chunks(x, ixr.csr.get(), nv, &ixr.pull);
+    ixr.pull_version = ixr.csr_version;
   }
   PullDesc pd;
-  pd.nchunks = pr->nchunks;
-  pd.chunk_row = pr->crow.get();
-  pd.chunk_lo = pr->clo.get();
-  pd.chunk_hi = pr->chi.get();
+  pd.nchunks = ixr.pull_n;
+  pd.chunks = ixr.pull.get();
   pd.adj = ixr.adj.get();
   pd.w = trop ? (ixr.direct ? E.full.w.get() : ixr.data.w.get()) : nullptr;
-  pd.prev = dn->prev.get();
-  pd.prev_off = dn->prev_off.get();
+  pd.rowrec = dn->rowrec.get();
   pd.dw = trop ? H.delta.w.get() : nullptr;
   pd.lbits = dn->lbits;
   pd.delta_first = pr->delta_first ? 1 : 0;
@@ -1351,9 +1353,20 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
       DBuf<K> dk;
       DBuf<i64> dw;
       DBuf<u8> df;
-      nd = cells_compact<K>(x, dn->cd, dn->ncells, true, bits, &dk, &dw, &df, hf);
+      // the first pull rule's push cost for the new delta comes with the count
+      u64 pcost = ~0ull;
+      if (!dn->pulls.empty()) {
+        PullRule& p0 = dn->pulls[0];
+        Index& ixj = get_index(p0.E, false, {p0.eJ, p0.eR}, true, false);
+        dn->cd.pc_csr = ixj.csr.get();
+        dn->cd.pc_nv = D_.nvals();
+      }
+      nd = cells_compact<K>(x, dn->cd, dn->ncells, true, bits, &dk, &dw, &df, hf, 0, &pcost);
+      if (!dn->pulls.empty() && pcost != ~0ull) {
+        dn->pulls[0].cost = pcost;
+        dn->pulls[0].cost_dv = R.dversion;
+      }
       if (!nd) return;
-      if (!dn->pulls.empty()) cells_row_offsets(x, dn->prev.get(), dn->nrows, dn->lbits, dn->prev_off.get());
       R.delta.n = nd;
       R.delta.k.wide = sizeof(K) == 16;
       if constexpr (sizeof(K) == 16) R.delta.k.b = std::move(dk);
@@ -1500,11 +1513,18 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
   return nfresh;
 }
 
-void Engine::check_device_error() {
+DevErr Engine::read_device_status() {
   Exec x = ctx_->exec();
   DevErr h{};
   SK_HIP(hipMemcpyAsync(&h, ctx_->derr, sizeof(DevErr), hipMemcpyDeviceToHost, x.stream));
   SK_HIP(hipStreamSynchronize(x.stream));
+  return h;
+}
+
+void Engine::check_device_error() { raise_device_error(read_device_status()); }
+
+void Engine::raise_device_error(const DevErr& h) {
+  Exec x = ctx_->exec();
   if (h.code == SK_ERR_OVERFLOW) {
     DevErr z{};
     SK_HIP(hipMemcpyAsync(ctx_->derr, &z, sizeof(DevErr), hipMemcpyHostToDevice, x.stream));
@@ -2038,13 +2058,12 @@ void Engine::plan_dense(u32 k) {
     if (used_bytes + bytes > cap_bytes) continue;
     used_bytes += bytes;
     if (narrow) {
-      if (!widen_flag_.get()) widen_flag_.reset(x.pool, 1);
-      memset_async(x, widen_flag_.get(), 0, 4);
+      memset_async(x, &ctx_->derr->widen, 0, 4);
       dn->cell32.reset(x.pool, nc);
       memset_async(x, dn->cell32.get(), 0xFF, nc * 4);
       cd.narrow = 1;
       cd.cell32 = dn->cell32.get();
-      cd.widen = widen_flag_.get();
+      cd.widen = &ctx_->derr->widen;
     } else if (trop) {
       dn->cell.reset(x.pool, nc);
       memset_async(x, dn->cell.get(), 0xFF, nc * 8);
@@ -2125,26 +2144,20 @@ void Engine::plan_dense(u32 k) {
       }
       if (!dn->pulls.empty()) {
         dn->nrows = dn->ncells >> dn->lbits;
-        dn->prev.reset(x.pool, nwc);
-        memset_async(x, dn->prev.get(), 0, nwc * 4);
-        dn->prev_off.reset(x.pool, dn->nrows ? dn->nrows : 1);
-        cd.prev = dn->prev.get();
+        dn->rowrec.reset(x.pool, 2 * (dn->nrows ? dn->nrows : 1));
+        cd.rowrec = dn->rowrec.get();
+        cd.rnrows = dn->nrows;
+        cd.rlbits = dn->lbits;
       }
     }
     dense_[i] = std::move(dn);
   }
 }
 
-bool Engine::widen_needed() {
-  bool any = false;
+bool Engine::any_narrow() const {
   for (auto& d : dense_)
-    if (d && d->cd.narrow) any = true;
-  if (!any) return false;
-  Exec x = ctx_->exec();
-  u32 f = 0;
-  SK_HIP(hipMemcpyAsync(&f, widen_flag_.get(), 4, hipMemcpyDeviceToHost, x.stream));
-  SK_HIP(hipStreamSynchronize(x.stream));
-  return f != 0;
+    if (d && d->cd.narrow) return true;
+  return false;
 }
 
 void Engine::widen_dense() {
@@ -2159,7 +2172,7 @@ void Engine::widen_dense() {
     d->cd.cell32 = nullptr;
     d->cd.cell = d->cell.get();
   }
-  memset_async(x, widen_flag_.get(), 0, 4);
+  memset_async(x, &ctx_->derr->widen, 0, 4);
 }
 
 // Materialise every dense relation of the stratum as its sorted full state
@@ -2263,14 +2276,19 @@ void Engine::run_stratum(u32 k) {
       }
     };
     run_rules();
-    if (widen_needed()) {
-      // a derivation did not fit a narrow cell: widen every narrow relation
-      // and apply the whole round again (re-applying is a no-op for ⊕)
-      widen_dense();
-      chunks.clear();
-      run_rules();
+    const bool check_err = is_trop && !nparts_;
+    if (check_err || any_narrow()) {
+      DevErr st = read_device_status();
+      if (st.widen) {
+        // a derivation did not fit a narrow cell: widen every narrow relation
+        // and apply the whole round again (re-applying is a no-op for ⊕)
+        widen_dense();
+        chunks.clear();
+        run_rules();
+        if (check_err) st = read_device_status();
+      }
+      if (check_err) raise_device_error(st);
     }
-    if (is_trop && !nparts_) check_device_error();
     bool any = false;
     for (size_t i = 0; i < R_.size(); i++) {
       if (!in_idb_[i]) continue;

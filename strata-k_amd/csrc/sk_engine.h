@@ -126,6 +126,8 @@ struct Index {
   u64 csr_version = ~0ull;
   DBuf<u32> adj;  // second index column, for leapfrog lists
   u64 adj_version = ~0ull;
+  DBuf<PullChunk> pull;  // pull-round chunk list over the CSR's rows
+  u64 pull_n = 0, pull_version = ~0ull;
 };
 
 struct DRel {
@@ -157,10 +159,8 @@ struct PullRule {
   int E = -1, eR = 0, eJ = 1;  // the fixed relation and its row / join columns
   int R = 0, L = 1;          // the head's row column and lane column
   bool delta_first = true;   // ⊗ order in the body
-  u64 e_version = ~0ull;     // chunk list built for this version of E
-  u64 nchunks = 0;
-  DBuf<u32> crow;
-  DBuf<u64> clo, chi;
+  u64 cost = ~0ull;          // push join output count of the delta of version cost_dv
+  u64 cost_dv = ~0ull;
 };
 
 // Dense cell state of one relation while its stratum runs (see sk_cells.h).
@@ -177,10 +177,9 @@ struct DenseRel {
   DBuf<u32> dom_bits[kMaxCols];
   DBuf<u64> dom_pre[kMaxCols];
   DBuf<u32> dom_list[kMaxCols];
-  // pull evaluation: the last delta as a bitmap and its per-row offsets
+  // pull evaluation: per row, the last delta's lanes and list position
   std::vector<PullRule> pulls;
-  DBuf<u32> prev;
-  DBuf<u64> prev_off;
+  DBuf<u64> rowrec;  // per row: {delta lanes, delta-list position}
   u64 nrows = 0;
   int lbits = 0;
 };
@@ -334,7 +333,6 @@ class Engine {
   std::map<std::pair<int, int>, ColCodes> colcodes_;  // column code sets, by (relation, column)
   bool dense_enabled_ = true;
   bool narrow_ok_ = true;   // every staged Trop weight is finite-or-inf in [0, 2^32-1)
-  DBuf<u32> widen_flag_;    // set by a derivation that does not fit a narrow cell
 
   void build_dict();
   void upload_rel(int r);
@@ -343,7 +341,7 @@ class Engine {
   void run_stratum(u32 k);
   void plan_dense(u32 k);
   void end_dense();
-  bool widen_needed();
+  bool any_narrow() const;
   void widen_dense();
   DenseRel* dense_of(int rel) { return rel >= 0 && (size_t)rel < dense_.size() ? dense_[rel].get() : nullptr; }
   void eval_aggregate(const Rule& r);
@@ -355,6 +353,8 @@ class Engine {
   Index& get_index(int rel, bool delta, const std::vector<int>& perm, bool want_csr, bool want_adj);
   u32 code_or_never(u32 kind, i64 v, bool* never) const;
   void check_device_error();
+  DevErr read_device_status();  // one copy: overflow record + narrow widen flag
+  void raise_device_error(const DevErr& h);
 };
 
 }  // namespace sk

@@ -69,15 +69,28 @@ using HeadAlloc = void (*)(void* ctx, OutDesc* o, u64 n);
 // Pull evaluation of a linear recursive rule over dense cells (see
 // k_pull): head H(r, x) ⊕= Δ(j, x) ⊗ E(j → r), rows r of the head taken from
 // E's index on its r column, lanes x passed through from the delta.
+// One wave's share of a pull round: index rows [lo, lo + n) of E's R-index,
+// which belong to nr <= kPullRows consecutive non-empty destination rows;
+// destination row s owns index rows from lo + off(s) (off(0) = 0).  A row
+// with more than kPullPack index rows is split into chunks of at most 256,
+// each with nr = 1.
+constexpr u32 kPullRows = 4;
+constexpr u32 kPullPack = 64;
+struct PullChunk {
+  u64 lo;
+  u32 n;
+  u32 nr_off;  // bits 0-7: nr; bits 8s..8s+7: off(s) for s = 1..3
+  u32 row[kPullRows];
+};
+static_assert(sizeof(PullChunk) == 32, "PullChunk is two 16-byte loads");
+
 struct PullDesc {
   u64 nchunks = 0;
-  const u32* chunk_row = nullptr;  // head row of each chunk
-  const u64* chunk_lo = nullptr;   // index rows [lo, hi) of E in that row
-  const u64* chunk_hi = nullptr;
+  const PullChunk* chunks = nullptr;
   const u32* adj = nullptr;        // per index row: the j value (code = rank)
   const i64* w = nullptr;          // per index row: the weight (Trop)
-  const u32* prev = nullptr;       // last round's delta, as a bitmap in cell layout
-  const u64* prev_off = nullptr;   // per row: position of the row's first delta cell in the delta list
+  const u64* rowrec = nullptr;     // per row: {last round's delta lanes, position of the row's first
+                                   //           delta cell in the delta list}
   const i64* dw = nullptr;         // delta weights, in cell order
   int lbits = 0;                   // lanes per row = 2^lbits
   int delta_first = 1;             // ⊗ operand order: the delta's weight on the left
@@ -151,7 +164,8 @@ u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords);
 
 // Chunks of at most `per` index rows per head row, from a CSR over the head
 // row column (nrows + 1 offsets).  Returns the chunk count.
-u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, u32 per, DBuf<u32>* row, DBuf<u64>* lo, DBuf<u64>* hi);
+// Chunk list of a CSR over nrows destination rows (host-side greedy packing).
+u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out);
 // Σ over the delta rows of the push join's output count (the push cost).
 template <class K>
 u64 push_cost(const Exec& x, const K* dkeys, u64 n, int arity, int bits, int col, const u64* csr);

@@ -945,87 +945,129 @@ __device__ __forceinline__ u32 select_bit(u64 m, u32 k) {
 
 // Per wave of the pull kernel: one batch of 64 index rows.
 struct PullLds {
-  u64 m[64];     // delta lanes of each edge's source row
-  u64 base[64];  // delta-list position of that row's first delta cell
-  i64 w[64];     // edge weight
-  u32 incl[64];  // inclusive prefix of popcount(m)
-  i64 acc[64];   // running minimum per lane value
-  u32 have[64];  // lane value has a candidate (Bool: is derived)
+  u64 m[64];                  // delta lanes of each index row's source row
+  u64 base[64];               // delta-list position of that row's first delta cell
+  i64 w[64];                  // edge weight
+  u32 incl[64];               // inclusive prefix of popcount(m)
+  u32 slot[64];               // which of the chunk's destination rows the index row feeds
+  i64 acc[kPullRows * 64];    // running minimum per (destination row, lane value)
+  u8 have[kPullRows * 64];    // (destination row, lane value) has a candidate
 };
 
-// Rows of at most 64 lanes (lbits <= 6).  A wave takes its chunk in batches
-// of 64 index rows: lane l loads row l's source, weight and delta mask, a
-// wave scan counts the (edge, delta lane) pairs, and the pairs are dealt
-// round-robin to lanes, so every lane does work whatever the masks' density.
-// A lane finds its pair's edge by binary search over the scan, its lane value
-// by selecting the set bit, and its delta weight at base + rank (pairs of one
-// edge read consecutive delta positions).  Candidates fold into the wave's
-// LDS minimum; each lane value's minimum is applied to its cell once per
-// chunk.
+// Rows of at most 64 lanes (lbits <= 6).  A wave takes one chunk (up to
+// kPullRows destination rows) in batches of 64 index rows: lane l loads row
+// l's source, weight and the source's delta record, a wave scan counts the
+// (edge, delta lane) pairs, and the pairs are dealt round-robin to lanes, so
+// every lane does work whatever the masks' density.  A lane finds its pair's
+// edge by a branch-free search over the scan, its lane value by selecting
+// the set bit, and its delta weight at base + rank (pairs of one edge read
+// consecutive delta positions).  Candidates fold into the wave's LDS minimum
+// per (destination row, lane value), applied to the cells once per chunk.
+// The destination cells are peeked at the start, overlapping the loads.
+constexpr int kPullPairs = 2;
+
 template <bool TROP>
 __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevErr* err) {
   __shared__ PullLds sh[kBlock / 64];
   const u32 lane = threadIdx.x & 63;
-  const u64 ch = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
   PullLds& L = sh[threadIdx.x >> 6];
-  const u64 r = pd.chunk_row[ch];
-  const u64 lo = pd.chunk_lo[ch], hi = pd.chunk_hi[ch];
-  const u32 nl = 1u << pd.lbits;  // lanes per row, <= 64
-  L.acc[lane] = LLONG_MAX;
-  L.have[lane] = 0;
+  const PullChunk c = pd.chunks[ch];
+  const u32 nr = c.nr_off & 0xFFu;
+  const u64 lo = c.lo, hi = c.lo + c.n;
+  const int lb = pd.lbits;
+  const u32 nl = 1u << lb;
+  const u32 ncell = nr << lb;  // <= kPullRows * 64
+  for (u32 i = lane; i < kPullRows * 64; i += 64) {
+    L.acc[i] = LLONG_MAX;
+    L.have[i] = 0;
+  }
+  u64 seen[kPullRows];
+#pragma unroll
+  for (u32 u = 0; u < kPullRows; u++) {
+    const u32 i = lane + 64 * u;
+    seen[u] = 0;
+    if (i < ncell) seen[u] = cell_peek(cd, ((u64)c.row[i >> lb] << lb) + (i & (nl - 1)));
+  }
   for (u64 e0 = lo; e0 < hi; e0 += 64) {
     const u64 e = e0 + lane;
     u64 m = 0, base = 0;
     i64 we = 0;
+    u32 slot = 0;
     if (e < hi) {
       const u32 q = pd.adj[e];
       if constexpr (TROP) we = pd.w[e];
       if (q < pd.nrows) {
-        const u64 b0 = (u64)q << pd.lbits;
-        if (nl == 64) m = (u64)pd.prev[b0 >> 5] | ((u64)pd.prev[(b0 >> 5) + 1] << 32);
-        else m = ((u64)pd.prev[b0 >> 5] >> (b0 & 31)) & ((1ull << nl) - 1);
-        if (m) base = pd.prev_off[q];
+        // one 16-byte record per source row: its delta lanes and delta-list position
+        const ulonglong2 rec = *(const ulonglong2*)(pd.rowrec + 2 * (u64)q);
+        m = rec.x;
+        base = rec.y;
       }
+      const u32 el = (u32)(e - lo);
+      for (u32 s = 1; s < nr; s++) slot += el >= ((c.nr_off >> (8 * s)) & 0xFFu) ? 1u : 0u;
     }
-    const u32 c = (u32)__popcll(m);
-    const u32 incl = (u32)wave_incl_scan_j(c);
+    const u32 cnt = (u32)__popcll(m);
+    const u32 incl = (u32)wave_incl_scan_j(cnt);
     const u32 total = __shfl(incl, 63);
     L.m[lane] = m;
     L.base[lane] = base;
     L.w[lane] = we;
     L.incl[lane] = incl;
+    L.slot[lane] = slot;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (u32 pidx = lane; pidx < total; pidx += 64) {
-      // the edge holding pair pidx: the first j with incl[j] > pidx
-      u32 lo2 = 0, hi2 = 64;
-      while (lo2 < hi2) {
-        const u32 mid = (lo2 + hi2) >> 1;
-        if (L.incl[mid] <= pidx) lo2 = mid + 1;
-        else hi2 = mid;
+    // kPullPairs pairs per lane per step: their searches and delta-weight
+    // loads are independent, so they overlap instead of queueing
+    for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairs) {
+      u32 jj[kPullPairs], kk[kPullPairs];
+      i64 vv[kPullPairs];
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) {
+        // the edge holding pair pidx: the number of edges whose inclusive
+        // pair count is <= pidx (branch-free search over 64 entries)
+        const u32 pidx = p0 + (u32)u * 64 + lane;
+        u32 j = 0;
+#pragma unroll
+        for (u32 st = 32; st; st >>= 1)
+          if (L.incl[j + st - 1] <= pidx) j += st;
+        jj[u] = j;
+        kk[u] = pidx - (L.incl[j] - (u32)__popcll(L.m[j]));
       }
-      const u32 j = lo2;
-      const u32 k = pidx - (L.incl[j] - (u32)__popcll(L.m[j]));
-      const u32 xl = select_bit(L.m[j], k);
-      if constexpr (TROP) {
-        const i64 v = pd.dw[L.base[j] + k];
-        i64 res;
-        u8 rf;
-        if (pd.delta_first) trop_mul(v, 0, L.w[j], 0, &res, &rf, err);
-        else trop_mul(L.w[j], 0, v, 0, &res, &rf, err);
-        atomicMin((long long*)&L.acc[xl], (long long)res);
-        L.have[xl] = 1u;
-      } else {
-        L.have[xl] = 1u;
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) {
+        vv[u] = 0;
+        if constexpr (TROP)
+          if (p0 + (u32)u * 64 + lane < total) vv[u] = pd.dw[L.base[jj[u]] + kk[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) {
+        if (p0 + (u32)u * 64 + lane >= total) continue;
+        const u32 j = jj[u];
+        const u32 xl = select_bit(L.m[j], kk[u]);
+        const u32 a = L.slot[j] * 64 + xl;
+        if constexpr (TROP) {
+          i64 res;
+          u8 rf;
+          if (pd.delta_first) trop_mul(vv[u], 0, L.w[j], 0, &res, &rf, err);
+          else trop_mul(L.w[j], 0, vv[u], 0, &res, &rf, err);
+          atomicMin((long long*)&L.acc[a], (long long)res);
+        }
+        L.have[a] = 1;  // plain byte stores: racing writers all store 1
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  if (lane < nl && L.have[lane]) {
-    const u64 cl = (r << pd.lbits) + lane;
-    cell_apply_t<TROP>(cd, cl, TROP ? L.acc[lane] : 0, 0, cell_peek(cd, cl));
+#pragma unroll
+  for (u32 u = 0; u < kPullRows; u++) {
+    const u32 i = lane + 64 * u;
+    if (i >= ncell) continue;
+    const u32 sl = i >> lb, xl = i & (nl - 1);
+    const u32 a = sl * 64 + xl;
+    if (!L.have[a]) continue;
+    const u64 cl = ((u64)c.row[sl] << lb) + xl;
+    cell_apply_t<TROP>(cd, cl, TROP ? L.acc[a] : 0, 0, seen[u]);
   }
 }
 
@@ -1039,43 +1081,49 @@ void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop,
   SK_HIP(hipGetLastError());
 }
 
-__global__ void k_pull_chunk_count(const u64* __restrict__ csr, u64 nrows, u32 per, u32* __restrict__ cnt) {
-  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const u64 d = csr[r + 1] - csr[r];
-  cnt[r] = (u32)((d + per - 1) / per);
-}
-
-__global__ void k_pull_chunk_write(const u64* __restrict__ csr, u64 nrows, u32 per, const u64* __restrict__ off,
-                                   u32* __restrict__ row, u64* __restrict__ lo, u64* __restrict__ hi) {
-  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  const u64 a = csr[r], b = csr[r + 1];
-  u64 o = off[r];
-  for (u64 s = a; s < b; s += per, o++) {
-    row[o] = (u32)r;
-    lo[o] = s;
-    hi[o] = min(s + per, b);
+u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out) {
+  std::vector<u64> h(nrows + 1);
+  SK_HIP(hipMemcpyAsync(h.data(), csr, (nrows + 1) * sizeof(u64), hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  std::vector<PullChunk> v;
+  PullChunk cur{};
+  bool open = false;
+  auto flush = [&] {
+    if (open) v.push_back(cur);
+    open = false;
+  };
+  for (u64 r = 0; r < nrows; r++) {
+    const u64 a = h[r], d = h[r + 1] - a;
+    if (!d) continue;
+    if (d > kPullPack) {
+      flush();
+      for (u64 s = a; s < a + d; s += 256) {
+        PullChunk c{};
+        c.lo = s;
+        c.n = (u32)std::min<u64>(256, a + d - s);
+        c.nr_off = 1;
+        c.row[0] = (u32)r;
+        v.push_back(c);
+      }
+      continue;
+    }
+    if (open && (cur.n + d > kPullPack || (cur.nr_off & 0xFFu) == kPullRows)) flush();
+    if (!open) {
+      cur = PullChunk{};
+      cur.lo = a;
+      open = true;
+    }
+    const u32 s = cur.nr_off & 0xFFu;
+    if (s) cur.nr_off |= cur.n << (8 * s);  // cur.n < kPullPack fits a byte
+    cur.row[s] = (u32)r;
+    cur.nr_off = (cur.nr_off & ~0xFFu) | (s + 1);
+    cur.n += (u32)d;
   }
-}
-
-u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, u32 per, DBuf<u32>* row, DBuf<u64>* lo, DBuf<u64>* hi) {
-  DBuf<u32> cnt(x.pool, nrows ? nrows : 1);
-  DBuf<u64> off(x.pool, nrows ? nrows : 1);
-  DBuf<u64> tot(x.pool, 1);
-  if (!nrows) return 0;
-  k_pull_chunk_count<<<gridj(nrows), kBlock, 0, x.stream>>>(csr, nrows, per, cnt.get());
-  SK_HIP(hipGetLastError());
-  scan_u32(x, cnt.get(), off.get(), nrows, tot.get());
-  const u64 n = read_u64(x, tot.get());
-  row->reset(x.pool, n ? n : 1);
-  lo->reset(x.pool, n ? n : 1);
-  hi->reset(x.pool, n ? n : 1);
-  if (n) {
-    k_pull_chunk_write<<<gridj(nrows), kBlock, 0, x.stream>>>(csr, nrows, per, off.get(), row->get(), lo->get(),
-                                                               hi->get());
-    SK_HIP(hipGetLastError());
-  }
+  flush();
+  const u64 n = v.size();
+  out->reset(x.pool, n ? n : 1);
+  if (n) SK_HIP(hipMemcpyAsync(out->get(), v.data(), n * sizeof(PullChunk), hipMemcpyHostToDevice, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
   return n;
 }
 

@@ -82,12 +82,24 @@ void fill_u8(const Exec& x, u8* p, u8 v, u64 n) {
 
 void memset_async(const Exec& x, void* p, int v, u64 bytes) { fill_u8(x, (u8*)p, (u8)v, bytes); }
 
-u64 read_u64(const Exec& x, const u64* d) {
+static u64* pinned_words() {
   static thread_local u64* host = nullptr;
   if (!host) SK_HIP(hipHostMalloc((void**)&host, 64, hipHostMallocDefault));
-  SK_HIP(hipMemcpyAsync(host, d, sizeof(u64), hipMemcpyDeviceToHost, x.stream));
+  return host;
+}
+
+u64 read_u64(const Exec& x, const u64* d) {
+  u64 h;
+  read_u64s(x, d, &h, 1);
+  return h;
+}
+
+void read_u64s(const Exec& x, const u64* d, u64* h, int n) {
+  if (n < 1 || n > 8) throw std::runtime_error("read_u64s: 1..8 words");
+  u64* host = pinned_words();
+  SK_HIP(hipMemcpyAsync(host, d, sizeof(u64) * n, hipMemcpyDeviceToHost, x.stream));
   SK_HIP(hipStreamSynchronize(x.stream));
-  return *host;
+  for (int i = 0; i < n; i++) h[i] = host[i];
 }
 
 // ===================================================================== scan
@@ -139,10 +151,51 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, 
   }
 }
 
+// Small inputs (n <= kScanOneMax): one block, each thread scanning a
+// contiguous run of at most 64 elements around one block scan -- one launch
+// instead of three.  Safe in place (each element is read before it is written).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scan_one(const T* in, u64* out, u64 n, u64* total, const u64* side) {
+  __shared__ u64 tmp[kWaves];
+  const u64 per = (n + kBlock - 1) / kBlock;
+  const u64 lo = min(n, (u64)threadIdx.x * per), hi = min(n, lo + per);
+  u64 acc = 0, sacc = 0;
+  for (u64 i = lo; i < hi; i++) acc += (u64)in[i];
+  if (side)
+    for (u64 i = lo; i < hi; i++) sacc += side[i];
+  u64 tot;
+  u64 c = block_excl_scan(acc, tmp, &tot);
+  for (u64 i = lo; i < hi; i++) {
+    const u64 v = (u64)in[i];
+    out[i] = c;
+    c += v;
+  }
+  if (side) {
+    __syncthreads();
+    u64 stot;
+    block_excl_scan(sacc, tmp, &stot);
+    if (threadIdx.x == 0) total[1] = stot;
+  }
+  if (threadIdx.x == 0 && total) *total = tot;
+}
+
+void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, const u64* side) {
+  if (n == 0 || n > kScanOneMax) throw std::runtime_error("scan_u32_one: 1..kScanOneMax values");
+  Scope sc(x.timer, "scan_one", KC_MISC, (double)n * (4 + 8 + (side ? 8 : 0)), n);
+  k_scan_one<u32><<<1, kBlock, 0, x.stream>>>(in, out, n, total, side);
+  SK_HIP(hipGetLastError());
+}
+
 template <class T>
 static void scan_impl(const Exec& x, const T* in, u64* out, u64 n, u64* d_total) {
   if (n == 0) {
     if (d_total) memset_async(x, d_total, 0, sizeof(u64));
+    return;
+  }
+  if (n <= kScanOneMax) {
+    Scope sc(x.timer, "scan_one", KC_MISC, (double)n * (2 * sizeof(T) + 8), n);
+    k_scan_one<T><<<1, kBlock, 0, x.stream>>>(in, out, n, d_total, nullptr);
+    SK_HIP(hipGetLastError());
     return;
   }
   u64 nb = (n + kScanTile - 1) / kScanTile;

@@ -173,3 +173,40 @@ def test_config2_full_dense_equals_sorted(ctx):
     assert len(s1) == len(s2) and i1 == i2
     assert (s1 == s2).all() and (z1 == z2).all() and (w1 == w2).all()
     print("config2 full:", len(s1), "tuples,", i1[0], "rounds, dense == sorted")
+
+
+def reach_program() -> CoreProgram:
+    """Bool multi-source reachability, the shape of ``cost_program`` over Bool."""
+    return CoreProgram(
+        [pred("src", 1, BOOL), pred("edge", 2, BOOL), pred("cost", 2, BOOL)],
+        [
+            rule(atom("cost", v(0), v(1)), [pos("src", v(0)), pos("edge", v(0), v(1))], 2),
+            rule(atom("cost", v(0), v(2)), [pos("cost", v(0), v(1)), pos("edge", v(1), v(2))], 3),
+        ],
+        1,
+    )
+
+
+@pytest.mark.parametrize("sem", [BOOL, TROP])
+@pytest.mark.parametrize("nsrc", [1, 3, 17, 40, 64])
+def test_pull_rounds_all_row_widths(ctx, sem, nsrc):
+    """Destination-side (pull) rounds at every row width: the lane column
+    holds nsrc sources, so a cell row has 1, 4, 32 or 64 lanes.  Tuples,
+    weights and round counts equal the oracle's and the sorted path's, and
+    the pull kernel must have run in at least one round."""
+    w = wl.config2(seed=5 + nsrc, scale=8, edge_factor=16, n_sources=nsrc)
+    prog = cost_program() if sem == TROP else reach_program()
+    bulk = w.bulk if sem == TROP else {p: (cols, None, None) for p, (cols, _, _) in w.bulk.items()}
+    both(ctx, prog, bulk=bulk, label=f"pull-{sem}-{nsrc}")
+    run = ctx.prepare(prog)
+    for p, (cols, wt, inf) in bulk.items():
+        run.add_sym(p, cols, wt, inf)
+    run.load()
+    ctx.set_profiling(True)
+    try:
+        run.execute()
+        names = {r["name"] for r in run.kernel_records() if r["launches"]}
+    finally:
+        ctx.set_profiling(False)
+        run.free()
+    assert "join_pull" in names, sorted(names)
