@@ -194,6 +194,20 @@ template <class K>
 u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_only, int bits, DBuf<K>* keys,
                   DBuf<i64>* w, DBuf<u8>* inf, bool want_inf, u64 cell0 = 0, u64* push_cost = nullptr);
 
+// Touched compaction (from cell 0) into caller buffers whose capacity is at
+// least the number of touched cells (ncells bounds it), with no host read
+// between the passes: one read at the end returns the count, the push cost
+// (as cells_compact, ~0 when not computed) and, when `status` is given, a
+// snapshot of that device record taken after every earlier launch.  Needs
+// ncells / (kBlock * 32) <= kScanOneMax blocks (cells_into_fits).
+bool cells_into_fits(u64 ncells);
+template <class K>
+u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
+                       u64* push_cost, const DevErr* status, DevErr* status_out);
+// Mark the cells of n packed keys touched (the keys' cells are in the layout).
+template <class K>
+void cells_mark_touched(const Exec& x, const CellDesc& cd, const K* keys, u64 n, int arity, int bits);
+
 // State of arbitrary cells: present[i] and the weight of cell idx[i]
 // (+inf reads as present with weight 0; check infb where it matters).
 void cells_gather(const Exec& x, const CellDesc& cd, const u64* idx, u64 n, i64* w, u8* present);

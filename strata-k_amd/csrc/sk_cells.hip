@@ -2,6 +2,7 @@
 // relation into cells, ⊕-applying derivations, and compacting touched or
 // present cells back into sorted packed relations.
 #include <climits>
+#include <cstring>
 
 #include "sk_cells.h"
 
@@ -352,6 +353,66 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   return n;
 }
 
+bool cells_into_fits(u64 ncells) {
+  const u64 nwords = (ncells + 31) / 32;
+  return (nwords + kBlock - 1) / kBlock <= kScanOneMax;
+}
+
+template <class K>
+u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
+                       u64* push_cost, const DevErr* status, DevErr* status_out) {
+  static_assert(sizeof(DevErr) == 3 * sizeof(u64), "status record is three words");
+  const u64 nwords = (ncells + 31) / 32;
+  const u64 nblocks = (nwords + kBlock - 1) / kBlock;
+  if (!cells_into_fits(ncells) || nblocks == 0) throw std::runtime_error("cells_compact_into: too many cells");
+  if (cd.rowrec && (cd.rlbits < 0 || cd.rlbits > 6)) throw std::runtime_error("cells_compact_into: row records");
+  if (push_cost) *push_cost = ~0ull;
+  const bool pc = push_cost && cd.pc_csr && cd.rowrec;
+  DBuf<u32> cnt(x.pool, nblocks);
+  DBuf<u64> off(x.pool, nblocks);
+  DBuf<u64> tot(x.pool, 8);
+  DBuf<u64> pcb(x.pool, pc ? nblocks : 1);
+  const double scan_bytes = 4.0 * nwords;
+  {
+    Scope sc(x.timer, "cells_count", KC_DEDUP, scan_bytes, ncells);
+    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, cnt.get(), pc ? pcb.get() : nullptr);
+    SK_HIP(hipGetLastError());
+  }
+  scan_u32_one(x, cnt.get(), off.get(), nblocks, tot.get(), pc ? pcb.get() : nullptr, (const u64*)status,
+               status ? 3 : 0);
+  {
+    // the records' bytes are credited once the count is known
+    Scope sc(x.timer, "cells_write", KC_DEDUP, scan_bytes, ncells);
+    k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, off.get(), bits, keys,
+                                                                 cd.trop ? w : nullptr, cd.trop ? inf : nullptr);
+    SK_HIP(hipGetLastError());
+  }
+  u64 hv[5] = {0, 0, 0, 0, 0};
+  read_u64s(x, tot.get(), hv, status ? 5 : (pc ? 2 : 1));
+  if (pc) *push_cost = hv[1];
+  if (status) std::memcpy((void*)status_out, &hv[2], sizeof(DevErr));
+  const u64 n = hv[0];
+  if (x.timer) x.timer->credit("cells_write", KC_DEDUP, (sizeof(K) + (cd.trop ? 16.0 + (inf ? 1.0 : 0) : 0)) * n, 0);
+  return n;
+}
+
+template <class K>
+__global__ void k_cells_mark_touched(CellDesc cd, const K* __restrict__ keys, u64 n, int arity, int bits) {
+  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const K key = keys[i];
+  u64 c;
+  if (!cell_of(cd, [&](int j) { return code_at<K>(key, arity, bits, j); }, &c)) return;
+  cell_touch(cd, c);
+}
+
+template <class K>
+void cells_mark_touched(const Exec& x, const CellDesc& cd, const K* keys, u64 n, int arity, int bits) {
+  if (!n) return;
+  k_cells_mark_touched<K><<<gridc(n), kBlock, 0, x.stream>>>(cd, keys, n, arity, bits);
+  SK_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ gather
 
 __global__ void k_cells_gather(CellDesc cd, const u64* __restrict__ idx, u64 n, i64* __restrict__ w,
@@ -400,7 +461,10 @@ void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
   template void cells_load<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int);  \
   template void cells_insert<K>(const Exec&, const CellDesc&, const K*, const i64*, const u8*, u64, int, int); \
   template u64 cells_compact<K>(const Exec&, const CellDesc&, u64, bool, int, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, bool, \
-                                u64, u64*);
+                                u64, u64*);                                                                \
+  template u64 cells_compact_into<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u8*, u64*, const DevErr*,   \
+                                     DevErr*);                                                                    \
+  template void cells_mark_touched<K>(const Exec&, const CellDesc&, const K*, u64, int, int);
 SK_CELLS_INST(u64)
 SK_CELLS_INST(u128)
 

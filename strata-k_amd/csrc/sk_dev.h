@@ -139,6 +139,9 @@ class Timer {
   hipStream_t stream = nullptr;
   void begin(const char* name, int cls, double bytes, u64 items);
   void end();
+  // Credit bytes / items to a kernel record without a launch (work whose
+  // size is known only after the launch was timed).
+  void credit(const char* name, int cls, double bytes, u64 items);
   void collect();  // synchronise and fold pending event pairs into stats
   void reset();
   KStat stats[KC_N];
@@ -185,16 +188,21 @@ void scan_u32(const Exec& x, const u32* in, u64* out, u64 n, u64* d_total);
 void scan_u64(const Exec& x, const u64* in, u64* out, u64 n, u64* d_total);
 void scan_u8(const Exec& x, const u8* in, u64* out, u64 n, u64* d_total);
 // Scan of n <= kScanOneMax values in one launch: total[0] = the sum and, when
-// `side` is given, total[1] = the sum of side[0..n).
+// `side` is given, total[1] = the sum of side[0..n); total[2 + i] =
+// extra[i] for i < nextra (a snapshot taken after every earlier launch on
+// the stream, so one host read can fetch it with the sums).
 constexpr u64 kScanOneMax = (u64)kBlock * 64;
-void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, const u64* side);
+void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, const u64* side,
+                  const u64* extra = nullptr, int nextra = 0);
 
 // LSD radix sort of keys (stable) on the low `key_bits` bits, optionally
 // carrying i64 weights and u8 +inf flags.  Result in (keys, w, inf); the
-// alt buffers are scratch of the same length.
+// alt buffers are scratch of the same length.  With lo_bit > 0 the sort
+// uses bits [lo_bit, lo_bit + key_bits) only; the keys must have no set bit
+// at or above lo_bit + key_bits (the sort reads whole 8-bit digits).
 template <class K>
 void radix_sort(const Exec& x, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt, u64 n,
-                int key_bits, K** out_keys, i64** out_w, u8** out_inf);
+                int key_bits, K** out_keys, i64** out_w, u8** out_inf, int lo_bit = 0);
 
 // Collapse runs of equal keys of a sorted array.  Bool: keep one.  With
 // weights: keep the minimum (+inf above every finite).  Returns the count.

@@ -1337,9 +1337,16 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
   const bool hf = trop && has_inf_;
   u64 total = 0;
   for (auto& c : chunks) total += c.n;
+  // the last round's delta; a dense relation compacting into capacity
+  // buffers reuses its buffers for the new one
+  RelData old = std::move(R.delta);
   R.delta.release();
   R.delta_order.clear();
   R.dversion++;
+  {
+    DenseRel* d0 = dense_of(rel);
+    if (!(d0 && d0->into)) old.release();
+  }
   if (DenseRel* dn = dense_of(rel)) {
     // rows not applied by their emitting kernel (leapfrog output) go in now;
     // the delta is the set of cells this round improved, in key order
@@ -1361,18 +1368,45 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
         dn->cd.pc_csr = ixj.csr.get();
         dn->cd.pc_nv = D_.nvals();
       }
-      nd = cells_compact<K>(x, dn->cd, dn->ncells, true, bits, &dk, &dw, &df, hf, 0, &pcost);
-      if (!dn->pulls.empty() && pcost != ~0ull) {
-        dn->pulls[0].cost = pcost;
-        dn->pulls[0].cost_dv = R.dversion;
+      if (dn->into) {
+        // no host read between the compaction's passes; the round's device
+        // status may ride along with its one read
+        RelData buf = old.k.ptr() ? std::move(old) : std::move(dn->spare);
+        buf.k.ensure(x.pool, dn->ncells, R.wide);
+        if (trop) buf.w.ensure(x.pool, dn->ncells);
+        if (hf) buf.inf.ensure(x.pool, dn->ncells);
+        const bool fold = fold_req_ && !fold_done_;
+        DevErr st{};
+        nd = cells_compact_into<K>(x, dn->cd, dn->ncells, bits, (K*)buf.k.ptr(), trop ? buf.w.get() : nullptr,
+                                   hf ? buf.inf.get() : nullptr, &pcost, fold ? ctx_->derr : nullptr, &st);
+        if (fold) {
+          fold_done_ = true;
+          folded_ = st;
+        }
+        if (!dn->pulls.empty() && pcost != ~0ull) {
+          dn->pulls[0].cost = pcost;
+          dn->pulls[0].cost_dv = R.dversion;
+        }
+        if (!nd) {
+          dn->spare = std::move(buf);
+          return;
+        }
+        buf.n = nd;
+        R.delta = std::move(buf);
+      } else {
+        nd = cells_compact<K>(x, dn->cd, dn->ncells, true, bits, &dk, &dw, &df, hf, 0, &pcost);
+        if (!dn->pulls.empty() && pcost != ~0ull) {
+          dn->pulls[0].cost = pcost;
+          dn->pulls[0].cost_dv = R.dversion;
+        }
+        if (!nd) return;
+        R.delta.n = nd;
+        R.delta.k.wide = sizeof(K) == 16;
+        if constexpr (sizeof(K) == 16) R.delta.k.b = std::move(dk);
+        else R.delta.k.a = std::move(dk);
+        R.delta.w = std::move(dw);
+        R.delta.inf = std::move(df);
       }
-      if (!nd) return;
-      R.delta.n = nd;
-      R.delta.k.wide = sizeof(K) == 16;
-      if constexpr (sizeof(K) == 16) R.delta.k.b = std::move(dk);
-      else R.delta.k.a = std::move(dk);
-      R.delta.w = std::move(dw);
-      R.delta.inf = std::move(df);
       if (!dn->key_order) R.delta_order = dn->order;
       if (dn->keep_full) {
         DBuf<K> u_k, f_k;
@@ -1954,6 +1988,8 @@ void Engine::plan_dense(u32 k) {
 
   const u64 cap_bytes = std::min<u64>(ctx_->pool->budget() / 4, 16ull << 30);
   u64 used_bytes = 0;
+  const u64 into_cap = std::min<u64>(ctx_->pool->budget() / 8, 8ull << 30);
+  u64 into_used = 0;
   for (size_t i = 0; i < R_.size(); i++) {
     if (!cand[i]) continue;
     DRel& R = R_[i];
@@ -2053,10 +2089,16 @@ void Engine::plan_dense(u32 k) {
     dn->ncells = 1ull << tot_bits;
     const u64 nc = std::max<u64>(dn->ncells, 32);
     const u64 nwc = nc / 32;
-    const bool narrow = trop && narrow_ok_;
+    const bool narrow = trop && narrow_ok_ && !narrow_off_;
     const u64 bytes = trop ? nc * (narrow ? 4 : 8) + nwc * 4 * (has_inf_ ? 3 : 2) : nwc * 4 * 2;
     if (used_bytes + bytes > cap_bytes) continue;
     used_bytes += bytes;
+    // capacity buffers for the delta (ncells records) when they fit too
+    const u64 into_b = dn->ncells * ((R.wide ? 16 : 8) + (trop ? 8 : 0) + (trop && has_inf_ ? 1 : 0));
+    if (cells_into_fits(dn->ncells) && into_used + into_b <= into_cap) {
+      dn->into = true;
+      into_used += into_b;
+    }
     if (narrow) {
       memset_async(x, &ctx_->derr->widen, 0, 4);
       dn->cell32.reset(x.pool, nc);
@@ -2197,7 +2239,20 @@ void Engine::end_dense() {
       DBuf<u8> ff;
       u64 n = cells_compact<K>(x, dn->cd, dn->ncells, false, R.bits, &fk, &fw, &ff, hf);
       if (!dn->key_order && n) {
-        // cell order differs from key order: one sort of the compacted state
+        // cell order differs from key order: a stable sort of the compacted
+        // state by its leading key columns 0..k-1, where k is the least count
+        // such that cell order already ranks the columns k.. in key order
+        const int a = R.arity;
+        int k = a;
+        for (int kk = 0; kk < a && k == a; kk++) {
+          std::vector<int> rest;
+          for (int c : dn->order)
+            if (c >= kk) rest.push_back(c);
+          bool inc = true;
+          for (size_t q = 1; q < rest.size(); q++)
+            if (rest[q] < rest[q - 1]) inc = false;
+          if (inc) k = kk;
+        }
         DBuf<K> ak(x.pool, n);
         DBuf<i64> aw;
         DBuf<u8> af;
@@ -2207,7 +2262,8 @@ void Engine::end_dense() {
         i64* sw;
         u8* sf;
         radix_sort<K>(x, fk.get(), ak.get(), trop ? fw.get() : nullptr, trop ? aw.get() : nullptr,
-                      hf ? ff.get() : nullptr, hf ? af.get() : nullptr, n, key_bits_of(R.arity, R.bits), &sk, &sw, &sf);
+                      hf ? ff.get() : nullptr, hf ? af.get() : nullptr, n, R.bits * k, &sk, &sw, &sf,
+                      R.bits * (a - k));
         if (sk != fk.get()) {
           std::swap(fk, ak);
           if (trop) std::swap(fw, aw);
@@ -2244,11 +2300,27 @@ void Engine::run_stratum(u32 k) {
       R_[i].delta_order.clear();
       R_[i].dversion++;
     }
+  narrow_off_ = false;
   plan_dense(k);
   const u64 cap = universe_ + 2;
   u64 iters = 0;
   std::vector<u8> active_prev;  // batched mode: program had a non-empty delta after the last round
   std::map<std::pair<int, int>, RulePlan> plans;
+  // The round's device status (overflow record, narrow widen flag) is read
+  // with the first compaction's count when every relation the stratum
+  // derives is dense with capacity compaction and none merges its deltas
+  // into its full state; a widen seen that way restarts the stratum with
+  // wide cells (the compaction has consumed the round's touched set).
+  auto fold_ok = [&]() {
+    if (nparts_) return false;
+    for (size_t i = 0; i < R_.size(); i++) {
+      if (!in_idb_[i]) continue;
+      DenseRel* d = dense_of((int)i);
+      if (!d || !d->into || d->keep_full) return false;
+    }
+    return true;
+  };
+  bool fold = fold_ok();
   auto plan_of = [&](int ri, int dpos) -> const RulePlan& {
     auto key = std::make_pair(ri, dpos);
     auto it = plans.find(key);
@@ -2277,7 +2349,8 @@ void Engine::run_stratum(u32 k) {
     };
     run_rules();
     const bool check_err = is_trop && !nparts_;
-    if (check_err || any_narrow()) {
+    const bool need_status = check_err || any_narrow();
+    if (need_status && !fold) {
       DevErr st = read_device_status();
       if (st.widen) {
         // a derivation did not fit a narrow cell: widen every narrow relation
@@ -2290,6 +2363,8 @@ void Engine::run_stratum(u32 k) {
       if (check_err) raise_device_error(st);
     }
     bool any = false;
+    fold_req_ = need_status && fold;
+    fold_done_ = false;
     for (size_t i = 0; i < R_.size(); i++) {
       if (!in_idb_[i]) continue;
       auto it = chunks.find((int)i);
@@ -2303,6 +2378,31 @@ void Engine::run_stratum(u32 k) {
       } else if (finalize((int)i, it->second) > 0) {
         any = true;
       }
+    }
+    fold_req_ = false;
+    if (need_status && fold) {
+      if (!fold_done_) folded_ = read_device_status();
+      if (folded_.widen) {
+        // restart the stratum from its initial state with wide cells: the
+        // relations it derives are dense and never merged into their full
+        // state, so that state is still the stratum's input
+        dense_.clear();
+        for (size_t i = 0; i < R_.size(); i++)
+          if (in_idb_[i]) {
+            R_[i].delta.release();
+            R_[i].delta_order.clear();
+            R_[i].dversion++;
+          }
+        memset_async(ctx_->exec(), &ctx_->derr->widen, 0, 4);
+        narrow_off_ = true;
+        plan_dense(k);
+        fold = fold_ok();
+        plans.clear();
+        iters = 0;
+        bootstrap = true;
+        continue;
+      }
+      if (check_err) raise_device_error(folded_);
     }
     if (nparts_) {
       part_round_bookkeeping(k, is_trop, active_prev, !bootstrap);

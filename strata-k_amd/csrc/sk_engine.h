@@ -180,6 +180,10 @@ struct DenseRel {
   // pull evaluation: per row, the last delta's lanes and list position
   std::vector<PullRule> pulls;
   DBuf<u64> rowrec;  // per row: {delta lanes, delta-list position}
+  // the delta is compacted into capacity buffers (ncells records) with no
+  // host read between passes; `spare` keeps them while the delta is empty
+  bool into = false;
+  RelData spare;
   u64 nrows = 0;
   int lbits = 0;
 };
@@ -342,6 +346,9 @@ class Engine {
   void plan_dense(u32 k);
   void end_dense();
   bool any_narrow() const;
+  bool narrow_off_ = false;  // this stratum runs with wide cells only
+  bool fold_req_ = false, fold_done_ = false;  // read the status with the next capacity compaction
+  DevErr folded_{};
   void widen_dense();
   DenseRel* dense_of(int rel) { return rel >= 0 && (size_t)rel < dense_.size() ? dense_[rel].get() : nullptr; }
   void eval_aggregate(const Rule& r);

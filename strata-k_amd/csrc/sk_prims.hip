@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 
 namespace sk {
 
@@ -94,11 +95,25 @@ u64 read_u64(const Exec& x, const u64* d) {
   return h;
 }
 
+// The host polls an event behind the copy instead of sleeping in a stream
+// synchronisation: these reads sit between dependent launches, where the
+// wake-up latency of a blocking wait shows up directly as idle device time.
 void read_u64s(const Exec& x, const u64* d, u64* h, int n) {
   if (n < 1 || n > 8) throw std::runtime_error("read_u64s: 1..8 words");
+  // one event per device (an event records only on its own device's streams)
+  static thread_local std::map<int, hipEvent_t> evs;
+  int dev = 0;
+  SK_HIP(hipGetDevice(&dev));
+  hipEvent_t& ev = evs[dev];
+  if (!ev) SK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   u64* host = pinned_words();
   SK_HIP(hipMemcpyAsync(host, d, sizeof(u64) * n, hipMemcpyDeviceToHost, x.stream));
-  SK_HIP(hipStreamSynchronize(x.stream));
+  SK_HIP(hipEventRecord(ev, x.stream));
+  for (;;) {
+    const hipError_t q = hipEventQuery(ev);
+    if (q == hipSuccess) break;
+    if (q != hipErrorNotReady) SK_HIP(q);
+  }
   for (int i = 0; i < n; i++) h[i] = host[i];
 }
 
@@ -151,38 +166,63 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, 
   }
 }
 
-// Small inputs (n <= kScanOneMax): one block, each thread scanning a
-// contiguous run of at most 64 elements around one block scan -- one launch
-// instead of three.  Safe in place (each element is read before it is written).
+// Small inputs (n <= kScanOneMax): one block of 1024 threads, each scanning
+// a contiguous run of at most 16 elements around one block scan -- one
+// launch instead of three (neighbouring threads' runs are adjacent, so the
+// wave's loads stay within a few cache lines).  Safe in place (each element
+// is read before it is written).
+constexpr int kScanOneThreads = 1024;
+static_assert(kScanOneMax <= (u64)kScanOneThreads * 16, "k_scan_one runs");
+
+__device__ __forceinline__ u64 scan_one_block(u64 v, u64* tmp, u64* total) {
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const u64 inc = wave_incl_scan(v);
+  if (lane == 63) tmp[wave] = inc;
+  __syncthreads();
+  u64 off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kScanOneThreads / 64; w++) {
+    const u64 s = tmp[w];
+    if ((u32)w < wave) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_scan_one(const T* in, u64* out, u64 n, u64* total, const u64* side) {
-  __shared__ u64 tmp[kWaves];
-  const u64 per = (n + kBlock - 1) / kBlock;
+__global__ __launch_bounds__(kScanOneThreads) void k_scan_one(const T* in, u64* out, u64 n, u64* total,
+                                                              const u64* side, const u64* extra, int nextra) {
+  __shared__ u64 tmp[kScanOneThreads / 64];
+  const u64 per = (n + kScanOneThreads - 1) / kScanOneThreads;
   const u64 lo = min(n, (u64)threadIdx.x * per), hi = min(n, lo + per);
   u64 acc = 0, sacc = 0;
   for (u64 i = lo; i < hi; i++) acc += (u64)in[i];
   if (side)
     for (u64 i = lo; i < hi; i++) sacc += side[i];
   u64 tot;
-  u64 c = block_excl_scan(acc, tmp, &tot);
+  u64 c = scan_one_block(acc, tmp, &tot);
   for (u64 i = lo; i < hi; i++) {
     const u64 v = (u64)in[i];
     out[i] = c;
     c += v;
   }
   if (side) {
-    __syncthreads();
     u64 stot;
-    block_excl_scan(sacc, tmp, &stot);
+    scan_one_block(sacc, tmp, &stot);
     if (threadIdx.x == 0) total[1] = stot;
   }
   if (threadIdx.x == 0 && total) *total = tot;
+  if ((int)threadIdx.x < nextra) total[2 + threadIdx.x] = extra[threadIdx.x];
 }
 
-void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, const u64* side) {
+void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, const u64* side, const u64* extra,
+                  int nextra) {
   if (n == 0 || n > kScanOneMax) throw std::runtime_error("scan_u32_one: 1..kScanOneMax values");
+  if (nextra < 0 || nextra > 6 || (nextra && !total)) throw std::runtime_error("scan_u32_one: extra words");
   Scope sc(x.timer, "scan_one", KC_MISC, (double)n * (4 + 8 + (side ? 8 : 0)), n);
-  k_scan_one<u32><<<1, kBlock, 0, x.stream>>>(in, out, n, total, side);
+  k_scan_one<u32><<<1, kScanOneThreads, 0, x.stream>>>(in, out, n, total, side, extra, nextra);
   SK_HIP(hipGetLastError());
 }
 
@@ -194,7 +234,7 @@ static void scan_impl(const Exec& x, const T* in, u64* out, u64 n, u64* d_total)
   }
   if (n <= kScanOneMax) {
     Scope sc(x.timer, "scan_one", KC_MISC, (double)n * (2 * sizeof(T) + 8), n);
-    k_scan_one<T><<<1, kBlock, 0, x.stream>>>(in, out, n, d_total, nullptr);
+    k_scan_one<T><<<1, kScanOneThreads, 0, x.stream>>>(in, out, n, d_total, nullptr, nullptr, 0);
     SK_HIP(hipGetLastError());
     return;
   }
@@ -380,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 
 template <class K>
 void radix_sort(const Exec& x, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt, u64 n,
-                int key_bits, K** out_keys, i64** out_w, u8** out_inf) {
+                int key_bits, K** out_keys, i64** out_w, u8** out_inf, int lo_bit) {
   *out_keys = keys;
   if (out_w) *out_w = w;
   if (out_inf) *out_inf = inf;
@@ -404,7 +444,7 @@ void radix_sort(const Exec& x, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf
   u8* fa = inf;
   u8* fb = inf_alt;
   for (int p = 0; p < passes; p++) {
-    int shift = 8 * p;
+    int shift = lo_bit + 8 * p;
     {
       Scope sc(x.timer, "radix_hist", KC_SORT, (double)n * sizeof(K), n);
       k_radix_hist<K><<<(unsigned)G, kBlock, 0, x.stream>>>(a, n, chunk, shift, counts.get(), (u32)G);
@@ -1061,7 +1101,7 @@ void member(const Exec& x, const K* s, u64 ns, const K* q, u64 nq, u8* flags) {
 // ======================================================= instantiations
 
 #define SK_INST_K(K)                                                                                     \
-  template void radix_sort<K>(const Exec&, K*, K*, i64*, i64*, u8*, u8*, u64, int, K**, i64**, u8**);  \
+  template void radix_sort<K>(const Exec&, K*, K*, i64*, i64*, u8*, u8*, u64, int, K**, i64**, u8**, int);  \
   template u64 unique_runs<K>(const Exec&, const K*, const i64*, const u8*, u64, K*, i64*, u8*);        \
   template void merge_diff<K>(const Exec&, const K*, const i64*, const u8*, u64, const K*, const i64*, \
                               const u8*, u64, DBuf<K>*, DBuf<i64>*, DBuf<u8>*, u64*, DBuf<K>*,          \
@@ -1168,6 +1208,17 @@ void Timer::end() {
   pending_.push_back(cur_);
   open_ = false;
   if (pending_.size() > 4096) collect();
+}
+
+void Timer::credit(const char* name, int cls, double bytes, u64 items) {
+  if (!enabled) return;
+  stats[cls].bytes += bytes;
+  stats[cls].items += items;
+  KRecord& r = kernels[name];
+  r.name = name;
+  r.cls = cls;
+  r.bytes += bytes;
+  r.items += items;
 }
 
 void Timer::collect() {
