@@ -57,6 +57,9 @@ struct CellDesc {
   u64* rowrec = nullptr;
   u64 rnrows = 0;
   int rlbits = 0;
+  // With rowrec: that compaction also writes each listed cell's lane
+  // (cell mod 2^rlbits), in list order.
+  u8* dlane = nullptr;
   // When set with rowrec, the counting pass of that compaction also sums,
   // over the touched cells, pc_csr[row+1] - pc_csr[row] (row < pc_nv): the
   // output count of the push join of the new delta against that CSR.

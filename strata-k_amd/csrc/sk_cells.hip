@@ -292,6 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
       key = (key << bits) | (K)code;
     }
     keys[base + j] = key;
+    if (touched_only && cd.dlane) cd.dlane[base + j] = (u8)(c & ((1u << cd.rlbits) - 1u));
     if (cd.trop) {
       const u64 e = cell_peek(cd, c);
       i64 wv = 0;

@@ -1059,12 +1059,14 @@ bool Engine::try_pull(const RulePlan& pl) {
   pd.w = trop ? (ixr.direct ? E.full.w.get() : ixr.data.w.get()) : nullptr;
   pd.rowrec = dn->rowrec.get();
   pd.dw = trop ? H.delta.w.get() : nullptr;
+  pd.dlane = dn->dlane.get();
   pd.lbits = dn->lbits;
   pd.delta_first = pr->delta_first ? 1 : 0;
   pd.nrows = dn->nrows;
-  // algorithmic bytes: every index row of E (j, weight, mask word) once,
-  // every candidate's delta weight once
-  const double bytes = (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 8.0) + (double)cand * (trop ? 8.0 : 0.0);
+  // algorithmic bytes: every index row of E (j, weight, its source's 16-byte
+  // row record) once, every candidate's delta cell (lane byte, weight) once
+  const double bytes =
+      (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 16.0) + (double)cand * (1.0 + (trop ? 8.0 : 0.0));
   pull_apply(x, pd, dn->cd, trop, bytes, cand);
   return true;
 }
@@ -2187,7 +2189,9 @@ void Engine::plan_dense(u32 k) {
       if (!dn->pulls.empty()) {
         dn->nrows = dn->ncells >> dn->lbits;
         dn->rowrec.reset(x.pool, 2 * (dn->nrows ? dn->nrows : 1));
+        dn->dlane.reset(x.pool, std::max<u64>(dn->ncells, 1));
         cd.rowrec = dn->rowrec.get();
+        cd.dlane = dn->dlane.get();
         cd.rnrows = dn->nrows;
         cd.rlbits = dn->lbits;
       }

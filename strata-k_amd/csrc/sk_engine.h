@@ -180,6 +180,7 @@ struct DenseRel {
   // pull evaluation: per row, the last delta's lanes and list position
   std::vector<PullRule> pulls;
   DBuf<u64> rowrec;  // per row: {delta lanes, delta-list position}
+  DBuf<u8> dlane;    // per delta entry: its cell's lane
   // the delta is compacted into capacity buffers (ncells records) with no
   // host read between passes; `spare` keeps them while the delta is empty
   bool into = false;

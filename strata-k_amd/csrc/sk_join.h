@@ -92,6 +92,7 @@ struct PullDesc {
   const u64* rowrec = nullptr;     // per row: {last round's delta lanes, position of the row's first
                                    //           delta cell in the delta list}
   const i64* dw = nullptr;         // delta weights, in cell order
+  const u8* dlane = nullptr;       // delta cells' lanes, in cell order
   int lbits = 0;                   // lanes per row = 2^lbits
   int delta_first = 1;             // ⊗ operand order: the delta's weight on the left
   u64 nrows = 0;                   // rows of the cell layout (j values at or above are absent)

@@ -914,34 +914,13 @@ double tri_list_len(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, co
 // ================================================================== pull
 //
 // A round of `H(r, x) :- H(j, x), E(j, r)` (or the mirrored column order)
-// evaluated from the destination side.  One wave owns a chunk of E's index
-// rows for one head row r; lane l stands for the lane value x = g + l.  For
-// each index row (j, w) of the chunk, lanes first fetch 64 index rows at
-// once, read each j's bits of the last round's delta, and then every edge
-// with a non-empty mask is taken in turn: lanes whose bit is set read the
-// delta weight at (row offset of j + rank of the bit), which is contiguous
-// across lanes, and fold w ⊗ Δ into their running minimum (Bool: presence).
-// Only the final minimum per lane goes to the cell.  The candidates are
-// exactly the push join's (every (delta row, edge) pair), so the round's
-// result, fresh set and errors are the same; what changes is that each head
-// cell is touched once per chunk instead of once per candidate.
-
-// Position of the k-th (0-based) set bit of m (k < popcount(m)).
-__device__ __forceinline__ u32 select_bit(u64 m, u32 k) {
-  u32 pos = 0, c;
-  c = (u32)__popc((u32)m);
-  if (k >= c) { k -= c; m >>= 32; pos += 32; }
-  c = (u32)__popc((u32)(m & 0xFFFFu));
-  if (k >= c) { k -= c; m >>= 16; pos += 16; }
-  c = (u32)__popc((u32)(m & 0xFFu));
-  if (k >= c) { k -= c; m >>= 8; pos += 8; }
-  c = (u32)__popc((u32)(m & 0xFu));
-  if (k >= c) { k -= c; m >>= 4; pos += 4; }
-  c = (u32)__popc((u32)(m & 0x3u));
-  if (k >= c) { k -= c; m >>= 2; pos += 2; }
-  if (k >= (u32)(m & 1u)) pos += 1;
-  return pos;
-}
+// evaluated from the destination side: E's index by r lists each head row's
+// in-edges (j, w); the last round's delta of H is read through per-row
+// records {lanes of row j in the delta, position of row j's first delta
+// cell}.  The candidates are exactly the push join's (every (delta cell,
+// edge) pair), so the round's result, fresh set and errors are the same;
+// what changes is that each head cell is written once per chunk instead of
+// once per candidate.
 
 // Per wave of the pull kernel: one batch of 64 index rows.
 struct PullLds {
@@ -959,9 +938,9 @@ struct PullLds {
 // l's source, weight and the source's delta record, a wave scan counts the
 // (edge, delta lane) pairs, and the pairs are dealt round-robin to lanes, so
 // every lane does work whatever the masks' density.  A lane finds its pair's
-// edge by a branch-free search over the scan, its lane value by selecting
-// the set bit, and its delta weight at base + rank (pairs of one edge read
-// consecutive delta positions).  Candidates fold into the wave's LDS minimum
+// edge by a branch-free search over the scan; the pair's delta cell sits at
+// base + rank in the delta list, where it reads the cell's weight and lane
+// value (pairs of one edge read consecutive positions).  Candidates fold into the wave's LDS minimum
 // per (destination row, lane value), applied to the cells once per chunk.
 // The destination cells are peeked at the start, overlapping the loads.
 constexpr int kPullPairs = 2;
@@ -1020,7 +999,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
     // kPullPairs pairs per lane per step: their searches and delta-weight
     // loads are independent, so they overlap instead of queueing
     for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairs) {
-      u32 jj[kPullPairs], kk[kPullPairs];
+      u32 jj[kPullPairs], kk[kPullPairs], xx[kPullPairs];
       i64 vv[kPullPairs];
 #pragma unroll
       for (int u = 0; u < kPullPairs; u++) {
@@ -1036,15 +1015,20 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
       }
 #pragma unroll
       for (int u = 0; u < kPullPairs; u++) {
+        // the pair's delta cell: its weight and its lane value
         vv[u] = 0;
-        if constexpr (TROP)
-          if (p0 + (u32)u * 64 + lane < total) vv[u] = pd.dw[L.base[jj[u]] + kk[u]];
+        xx[u] = 0;
+        if (p0 + (u32)u * 64 + lane < total) {
+          const u64 di = L.base[jj[u]] + kk[u];
+          xx[u] = pd.dlane[di];
+          if constexpr (TROP) vv[u] = pd.dw[di];
+        }
       }
 #pragma unroll
       for (int u = 0; u < kPullPairs; u++) {
         if (p0 + (u32)u * 64 + lane >= total) continue;
         const u32 j = jj[u];
-        const u32 xl = select_bit(L.m[j], kk[u]);
+        const u32 xl = xx[u];
         const u32 a = L.slot[j] * 64 + xl;
         if constexpr (TROP) {
           i64 res;
