@@ -10,12 +10,14 @@
 // was non-empty), its Bellman-Ford cap (its own universe + 2) and its errors;
 // a program that fails stops contributing without disturbing the others.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sk_engine.h"
@@ -60,8 +62,12 @@ std::string signature(const Program& p) {
 struct sk_batch {
   sk_ctx* ctx = nullptr;
   std::vector<BProgram> progs;
+  // Groups run concurrently, each from its own host thread on its own
+  // context (stream, pool, error record): their kernels overlap on the
+  // device.  The context outlives the engine that uses it.
   struct Group {
     std::vector<int> members;
+    std::unique_ptr<Ctx> gc;
     std::unique_ptr<Engine> eng;
     Program fused;
     EvalErr err;  // group-wide error (shape, device, budget)
@@ -89,6 +95,46 @@ Program fuse(const Program& p) {
   return f;
 }
 
+// Runs f(group index) for every group, groups spread over up to 8 host
+// threads (one group per thread at a time); each thread makes the device
+// current first.
+template <class F>
+void for_groups(sk_batch* b, F f) {
+  const size_t ng = b->groups.size();
+  const int dev = b->ctx->c.device;
+  const size_t nt = std::min<size_t>(ng, 8);
+  if (nt <= 1) {
+    for (size_t i = 0; i < ng; i++) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> ts;
+  for (size_t t = 0; t < nt; t++)
+    ts.emplace_back([&] {
+      (void)hipSetDevice(dev);
+      for (size_t i; (i = next.fetch_add(1)) < ng;) f(i);
+    });
+  for (auto& t : ts) t.join();
+}
+
+// Group-wide failure of a load or execute step.
+template <class F>
+void guarded_group(sk_batch::Group& g, F f) {
+  try {
+    f();
+  } catch (const EvalErr& e) {
+    g.err = e;
+  } catch (const DeviceError& d) {
+    g.err.code = SK_ERR_DEVICE;
+    g.err.msg = d.where;
+  } catch (const BudgetError&) {
+    g.err.code = SK_ERR_OUT_OF_BUDGET;
+  } catch (const std::exception& e) {
+    g.err.code = SK_ERR_DEVICE;
+    g.err.msg = e.what();
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -102,7 +148,11 @@ sk_status sk_batch_create(sk_ctx* ctx, sk_batch** out) {
 
 void sk_batch_free(sk_batch* b) {
   if (!b) return;
-  (void)hipStreamSynchronize(b->ctx->c.stream);
+  (void)hipSetDevice(b->ctx->c.device);
+  for (auto& g : b->groups) {
+    g.eng.reset();
+    if (g.gc) g.gc->release();
+  }
   delete b;
 }
 
@@ -144,11 +194,16 @@ sk_status sk_batch_load(sk_batch* b) {
     b->progs[i].member = (int)b->groups[it->second].members.size();
     b->groups[it->second].members.push_back((int)i);
   }
-  for (auto& g : b->groups) {
+  // each group gets its own context, with an even share of the budget
+  const u64 share = b->ctx->c.pool->budget() / std::max<size_t>(1, b->groups.size());
+  for_groups(b, [&](size_t gi) {
+    sk_batch::Group& g = b->groups[gi];
     const Program& p0 = b->progs[g.members[0]].prog;
     g.fused = fuse(p0);
-    try {
-      g.eng = std::make_unique<Engine>(&b->ctx->c, g.fused);
+    guarded_group(g, [&] {
+      g.gc = std::make_unique<Ctx>();
+      g.gc->init(b->ctx->c.device, share);
+      g.eng = std::make_unique<Engine>(g.gc.get(), g.fused);
       g.eng->validate();
       g.eng->set_partitions((u32)g.members.size());
       // stage every member's facts with its program id as column 0
@@ -193,15 +248,8 @@ sk_status sk_batch_load(sk_batch* b) {
         std::vector<std::vector<i64>>().swap(bp.edb_w);
       }
       g.eng->load();
-    } catch (const EvalErr& e) {
-      g.err = e;
-    } catch (const DeviceError& d) {
-      g.err.code = SK_ERR_DEVICE;
-      g.err.msg = d.where;
-    } catch (const BudgetError&) {
-      g.err.code = SK_ERR_OUT_OF_BUDGET;
-    }
-  }
+    });
+  });
   b->loaded = true;
   return SK_OK;
 }
@@ -210,20 +258,14 @@ sk_status sk_batch_execute(sk_batch* b) {
   if (!b || !b->loaded) return SK_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(b->ctx->c.device);
   auto t0 = std::chrono::steady_clock::now();
-  for (auto& g : b->groups) {
-    if (g.err.code) continue;
-    try {
+  for_groups(b, [&](size_t gi) {
+    sk_batch::Group& g = b->groups[gi];
+    if (g.err.code || !g.eng) return;
+    guarded_group(g, [&] {
       g.eng->execute();
-    } catch (const EvalErr& e) {
-      g.err = e;
-    } catch (const DeviceError& d) {
-      g.err.code = SK_ERR_DEVICE;
-      g.err.msg = d.where;
-    } catch (const BudgetError&) {
-      g.err.code = SK_ERR_OUT_OF_BUDGET;
-    }
-  }
-  (void)hipStreamSynchronize(b->ctx->c.stream);
+      SK_HIP(hipStreamSynchronize(g.gc->stream));
+    });
+  });
   b->exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (auto& bp : b->progs) {
     auto& g = b->groups[bp.group];

@@ -93,16 +93,10 @@ sk_status sk_ctx_create(int device, uint64_t budget, sk_ctx** out) {
     size_t freeb = 0, total = 0;
     SK_HIP(hipMemGetInfo(&freeb, &total));
     u64 b = budget ? budget : (u64)(freeb * 0.9);
-    c->c.device = device;
-    c->c.pool = std::make_unique<Pool>(b);
-    // default-priority, non-blocking stream
-    SK_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
-    c->c.timer.stream = c->c.stream;
-    SK_HIP(hipMalloc((void**)&c->c.derr, sizeof(DevErr)));
-    SK_HIP(hipMemsetAsync(c->c.derr, 0, sizeof(DevErr), c->c.stream));
-    SK_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.init(device, b);
   });
   if (s != SK_OK) {
+    c->c.release();
     delete c;
     return s;
   }
@@ -112,12 +106,7 @@ sk_status sk_ctx_create(int device, uint64_t budget, sk_ctx** out) {
 
 void sk_ctx_destroy(sk_ctx* ctx) {
   if (!ctx) return;
-  (void)hipSetDevice(ctx->c.device);
-  (void)hipStreamSynchronize(ctx->c.stream);
-  ctx->c.timer.reset();
-  ctx->c.pool.reset();
-  if (ctx->c.derr) (void)hipFree(ctx->c.derr);
-  if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+  ctx->c.release();
   delete ctx;
 }
 

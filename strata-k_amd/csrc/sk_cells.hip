@@ -20,12 +20,9 @@ __device__ __forceinline__ u32 code_at(K key, int arity, int bits, int j) {
 
 template <class K>
 __global__ void k_cells_mark_col(const K* __restrict__ keys, u64 n, int arity, int bits, int j, u32* bitmap) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  u32 v = code_at<K>(keys[i], arity, bits, j);
-  u32 m = 1u << (v & 31);
-  // hub values repeat: skip the atomic when the bit is already visible
-  if (!(bitmap[v >> 5] & m)) atomicOr(&bitmap[v >> 5], m);
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 v = i < n ? code_at<K>(keys[i], arity, bits, j) : 0u;
+  mark_bit_once(bitmap, v, i < n);
 }
 
 __global__ void k_cells_mark_code(u32* bitmap, u32 code) {

@@ -112,6 +112,21 @@ struct DBuf {
   T* get() const { return p; }
 };
 
+// Set bit v of a device bitmap, for a value that may repeat heavily across
+// the launch (a sorted column's runs, hub ids).  A lane whose wave neighbour
+// below holds the same value leaves it to that lane; the others read the
+// word at device scope (not a stale L1 copy) and skip the atomic when the
+// bit is already set.  Must be reached by every lane of the wave.
+__device__ __forceinline__ void mark_bit_once(u32* bitmap, u32 v, bool on) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 pv = __shfl_up(v, 1, 64);
+  const bool pon = __shfl_up(on ? 1 : 0, 1, 64) != 0;
+  if (!on || (lane > 0 && pon && pv == v)) return;
+  const u32 m = 1u << (v & 31);
+  if (__hip_atomic_load(&bitmap[v >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) return;
+  atomicOr(&bitmap[v >> 5], m);
+}
+
 // ------------------------------------------------------------- profiling
 
 enum KClass { KC_SORT = 0, KC_DEDUP = 1, KC_MERGE = 2, KC_JOIN = 3, KC_LEAPFROG = 4, KC_MISC = 5, KC_N = 6 };
@@ -179,6 +194,8 @@ struct Exec {
   hipStream_t stream;
   Timer* timer;
   DevErr* err;  // device error record
+  u64* host = nullptr;          // pinned words for small device-to-host reads
+  hipEvent_t read_ev = nullptr;  // polled behind those reads
 };
 
 // ------------------------------------------------------ primitive wrappers

@@ -109,6 +109,35 @@ static EvalErr mkerr(int code, const std::string& pred = "", const std::string& 
   return e;
 }
 
+void Ctx::init(int dev, u64 budget) {
+  device = dev;
+  SK_HIP(hipSetDevice(dev));
+  pool = std::make_unique<Pool>(budget);
+  // default-priority, non-blocking stream
+  SK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  timer.stream = stream;
+  SK_HIP(hipMalloc((void**)&derr, sizeof(DevErr)));
+  SK_HIP(hipMemsetAsync(derr, 0, sizeof(DevErr), stream));
+  SK_HIP(hipHostMalloc((void**)&host_words, 64, hipHostMallocDefault));
+  SK_HIP(hipEventCreateWithFlags(&read_ev, hipEventDisableTiming));
+  SK_HIP(hipStreamSynchronize(stream));
+}
+
+void Ctx::release() {
+  (void)hipSetDevice(device);
+  if (stream) (void)hipStreamSynchronize(stream);
+  timer.reset();
+  pool.reset();
+  if (derr) (void)hipFree(derr);
+  if (host_words) (void)hipHostFree(host_words);
+  if (read_ev) (void)hipEventDestroy(read_ev);
+  if (stream) (void)hipStreamDestroy(stream);
+  derr = nullptr;
+  host_words = nullptr;
+  read_ev = nullptr;
+  stream = nullptr;
+}
+
 static int key_bits_of(int arity, int bits) { return arity * bits; }
 
 template <class F>
@@ -2057,16 +2086,17 @@ void Engine::plan_dense(u32 k) {
     // this layout, touch neighbouring cells.  Otherwise key order.
     dn->order.resize(R.arity);
     for (int j = 0; j < R.arity; j++) dn->order[j] = j;
-    // (batched runs keep key order: their deltas are split by the program id
-    // in column 0)
-    if (!dn->keep_full && R.arity >= 2 && !nparts_) {
+    // (batched runs keep the program id, column 0, first: their deltas are
+    // split by it; the join column goes second)
+    const size_t first = nparts_ ? 1 : 0;
+    if (!dn->keep_full && R.arity >= (int)first + 2) {
       int lead = -1;
       for (auto& r : P_.rules) {
         if (r.stratum != k || r.agg || lead >= 0) continue;
         for (size_t j = 0; j < r.body.size() && lead < 0; j++) {
           const Atom& a = r.body[j];
           if (a.neg || a.rel != (int)i) continue;
-          for (size_t c = 0; c < a.args.size() && lead < 0; c++) {
+          for (size_t c = first; c < a.args.size() && lead < 0; c++) {
             if (a.args[c].kind != SK_T_VAR) continue;
             for (size_t q = 0; q < r.body.size() && lead < 0; q++) {
               if (q == j || r.body[q].neg) continue;
@@ -2076,9 +2106,9 @@ void Engine::plan_dense(u32 k) {
           }
         }
       }
-      if (lead > 0) {
+      if (lead > (int)first) {
         dn->order.erase(dn->order.begin() + lead);
-        dn->order.insert(dn->order.begin(), lead);
+        dn->order.insert(dn->order.begin() + first, lead);
       }
     }
     dn->key_order = true;

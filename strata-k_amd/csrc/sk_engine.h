@@ -242,14 +242,21 @@ struct RulePlan {
 
 // ------------------------------------------------------------ run state
 
+// One device, one stream and everything that lives on it: a caching pool
+// (blocks are reused in stream order, so a pool serves one stream only), the
+// device error record and pinned words for small reads.
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::unique_ptr<Pool> pool;
   Timer timer;
   DevErr* derr = nullptr;
+  u64* host_words = nullptr;
+  hipEvent_t read_ev = nullptr;
   EvalErr last;
-  Exec exec() { return Exec{pool.get(), stream, &timer, derr}; }
+  Exec exec() { return Exec{pool.get(), stream, &timer, derr, host_words, read_ev}; }
+  void init(int dev, u64 budget);  // throws DeviceError
+  void release();
 };
 
 struct Chunk {

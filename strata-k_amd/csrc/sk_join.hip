@@ -1188,11 +1188,9 @@ void pack_cols(const Exec& x, const Cols& cols, int arity, int bits, u64 n, K* o
 
 // Mark codes present (universe bitmap).
 __global__ void k_mark(const u32* __restrict__ c, u64 n, u32 limit, u32* __restrict__ bitmap) {
-  u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  u32 v = c[i];
-  // repeated values are common (hub ids): skip the atomic when the bit is visible
-  if (v < limit && !(bitmap[v >> 5] & (1u << (v & 31)))) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 v = i < n ? c[i] : 0u;
+  mark_bit_once(bitmap, v, i < n && v < limit);
 }
 
 __global__ __launch_bounds__(kBlock) void k_popcount(const u32* __restrict__ b, u64 n, unsigned long long* acc) {

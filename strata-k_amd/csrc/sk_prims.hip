@@ -9,7 +9,6 @@
 
 #include <algorithm>
 #include <cstring>
-#include <map>
 
 namespace sk {
 
@@ -83,12 +82,6 @@ void fill_u8(const Exec& x, u8* p, u8 v, u64 n) {
 
 void memset_async(const Exec& x, void* p, int v, u64 bytes) { fill_u8(x, (u8*)p, (u8)v, bytes); }
 
-static u64* pinned_words() {
-  static thread_local u64* host = nullptr;
-  if (!host) SK_HIP(hipHostMalloc((void**)&host, 64, hipHostMallocDefault));
-  return host;
-}
-
 u64 read_u64(const Exec& x, const u64* d) {
   u64 h;
   read_u64s(x, d, &h, 1);
@@ -100,21 +93,15 @@ u64 read_u64(const Exec& x, const u64* d) {
 // wake-up latency of a blocking wait shows up directly as idle device time.
 void read_u64s(const Exec& x, const u64* d, u64* h, int n) {
   if (n < 1 || n > 8) throw std::runtime_error("read_u64s: 1..8 words");
-  // one event per device (an event records only on its own device's streams)
-  static thread_local std::map<int, hipEvent_t> evs;
-  int dev = 0;
-  SK_HIP(hipGetDevice(&dev));
-  hipEvent_t& ev = evs[dev];
-  if (!ev) SK_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  u64* host = pinned_words();
-  SK_HIP(hipMemcpyAsync(host, d, sizeof(u64) * n, hipMemcpyDeviceToHost, x.stream));
-  SK_HIP(hipEventRecord(ev, x.stream));
+  if (!x.host || !x.read_ev) throw std::runtime_error("read_u64s: execution context without read buffers");
+  SK_HIP(hipMemcpyAsync(x.host, d, sizeof(u64) * n, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipEventRecord(x.read_ev, x.stream));
   for (;;) {
-    const hipError_t q = hipEventQuery(ev);
+    const hipError_t q = hipEventQuery(x.read_ev);
     if (q == hipSuccess) break;
     if (q != hipErrorNotReady) SK_HIP(q);
   }
-  for (int i = 0; i < n; i++) h[i] = host[i];
+  for (int i = 0; i < n; i++) h[i] = x.host[i];
 }
 
 // ===================================================================== scan
