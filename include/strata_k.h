@@ -193,6 +193,12 @@ sk_status sk_run_reset(sk_run* run);
  * enabled == 0: every relation uses the sorted-list path (sort, ⊕-unique,
  * merge).  Results, round counts and errors are identical either way. */
 sk_status sk_run_set_dense(sk_run* run, int enabled);
+/* Destination-side (pull) rounds for rules of the linear recursive shape over
+ * dense cells: 0 never, 1 when the cost model prefers them (default), 2 on
+ * every delta round the shape allows.  Results are identical in every mode.
+ * No reference counterpart (an evaluation strategy switch, like
+ * sk_run_set_dense). */
+sk_status sk_run_set_pull(sk_run* run, int mode);
 void sk_run_error(const sk_run* run, sk_error_info* out);
 
 /* Results: rows in BTreeMap order (sorted by GroundVal tuple). */
@@ -292,6 +298,11 @@ int64_t sk_batch_result_size(sk_batch* b, uint32_t index, const char* pred);
 sk_status sk_batch_result_read_sym(sk_batch* b, uint32_t index, const char* pred,
                                    uint32_t* const* cols, int64_t* weights, uint8_t* inf);
 double sk_batch_execute_ms(const sk_batch* b);
+/* Per-kernel records of the last sk_batch_execute summed over the batch's
+ * groups (profiling on in the batch's context), as sk_run_kernel_record. */
+uint32_t sk_batch_kernel_count(const sk_batch* b);
+sk_status sk_batch_kernel_record(const sk_batch* b, uint32_t i, char* name, size_t name_len,
+                                 uint32_t* kernel_class, sk_kernel_stats* out);
 
 /* -------------------------------------------------------------- serving */
 

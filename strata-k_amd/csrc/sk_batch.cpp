@@ -76,6 +76,9 @@ struct sk_batch {
   bool loaded = false;
   double exec_ms = 0;
   EvalErr err;
+  // per-kernel records of the last execute, summed over the groups (when
+  // the parent context's profiling is on)
+  std::map<std::string, KRecord> kernels;
 };
 
 namespace {
@@ -257,16 +260,34 @@ sk_status sk_batch_load(sk_batch* b) {
 sk_status sk_batch_execute(sk_batch* b) {
   if (!b || !b->loaded) return SK_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(b->ctx->c.device);
+  const bool prof = b->ctx->c.timer.enabled;
   auto t0 = std::chrono::steady_clock::now();
   for_groups(b, [&](size_t gi) {
     sk_batch::Group& g = b->groups[gi];
     if (g.err.code || !g.eng) return;
     guarded_group(g, [&] {
+      g.gc->timer.reset();
+      g.gc->timer.enabled = prof;
       g.eng->execute();
       SK_HIP(hipStreamSynchronize(g.gc->stream));
     });
   });
   b->exec_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  b->kernels.clear();
+  for (auto& g : b->groups) {
+    if (!g.gc) continue;
+    guarded_group(g, [&] { g.gc->timer.collect(); });
+    g.gc->timer.enabled = false;
+    for (auto& kv : g.gc->timer.kernels) {
+      KRecord& r = b->kernels[kv.first];
+      r.name = kv.second.name;
+      r.cls = kv.second.cls;
+      r.launches += kv.second.launches;
+      r.ms += kv.second.ms;
+      r.bytes += kv.second.bytes;
+      r.items += kv.second.items;
+    }
+  }
   for (auto& bp : b->progs) {
     auto& g = b->groups[bp.group];
     if (g.err.code) {
@@ -345,5 +366,24 @@ sk_status sk_batch_result_read_sym(sk_batch* b, uint32_t index, const char* pred
 }
 
 double sk_batch_execute_ms(const sk_batch* b) { return b ? b->exec_ms : 0; }
+
+uint32_t sk_batch_kernel_count(const sk_batch* b) { return b ? (uint32_t)b->kernels.size() : 0; }
+
+sk_status sk_batch_kernel_record(const sk_batch* b, uint32_t i, char* name, size_t name_len, uint32_t* kernel_class,
+                                 sk_kernel_stats* out) {
+  if (!b || i >= b->kernels.size()) return SK_ERR_INVALID_ARGUMENT;
+  auto it = b->kernels.begin();
+  std::advance(it, i);
+  const KRecord& r = it->second;
+  if (name && name_len) std::snprintf(name, name_len, "%s", r.name.c_str());
+  if (kernel_class) *kernel_class = (uint32_t)r.cls;
+  if (out) {
+    out->launches = r.launches;
+    out->ms = r.ms;
+    out->bytes = r.bytes;
+    out->items = r.items;
+  }
+  return SK_OK;
+}
 
 }  // extern "C"

@@ -200,6 +200,12 @@ sk_status sk_run_set_dense(sk_run* run, int enabled) {
   return SK_OK;
 }
 
+sk_status sk_run_set_pull(sk_run* run, int mode) {
+  if (!run || mode < 0 || mode > 2) return SK_ERR_INVALID_ARGUMENT;
+  run->eng->set_pull(mode);
+  return SK_OK;
+}
+
 void sk_run_error(const sk_run* run, sk_error_info* out) {
   if (run) set_info(run->err, out);
 }

@@ -65,6 +65,14 @@ struct CellDesc {
   // output count of the push join of the new delta against that CSR.
   const u64* pc_csr = nullptr;
   u64 pc_nv = 0;
+  // Bool relations with a word-parallel pull rule: a touched compaction
+  // (from cell 0) also copies every touched word into `dbits` (the delta as
+  // a bitmap) and stamps each row of 2^(5 + dwshift) cells that holds a
+  // delta cell with `stamp` in `rstamp` (rows are 1 << dwshift words).
+  u32* dbits = nullptr;
+  u32* rstamp = nullptr;
+  u32 stamp = 0;
+  int dwshift = 0;
   u32* bad = nullptr;      // set when a value falls outside its column domain
 };
 

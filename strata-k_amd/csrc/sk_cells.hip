@@ -242,6 +242,15 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
   const u64 wi = w0 + wl;
   u32 b = wl < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
   if (touched_only && b) cd.touched[wi] = 0;
+  if (touched_only && cd.dbits && wl < nwords) {
+    // the delta as a bitmap, every word rewritten; rows holding a delta cell
+    // carry this compaction's stamp (racing writers store the same value)
+    cd.dbits[wi] = b;
+    if (b) {
+      const u64 r = wi >> cd.dwshift;
+      if (cd.rstamp[r] != cd.stamp) cd.rstamp[r] = cd.stamp;
+    }
+  }
   u32 tot;
   u32 o = cells_block_scan((u32)__popc(b), tmp, &tot);
   if (touched_only && cd.rowrec) {

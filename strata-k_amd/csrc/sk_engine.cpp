@@ -1050,7 +1050,7 @@ static int col_index(const std::vector<int>& slots, int slot) {
 // pull shape and this round's push join would produce at least a quarter
 // of |E| outputs.  Returns false to run the push join instead.
 bool Engine::try_pull(const RulePlan& pl) {
-  if (pl.dpos < 0) return false;
+  if (pl.dpos < 0 || pull_mode_ == 0) return false;
   const int hr = P_.rules[pl.rule].head.rel;
   DenseRel* dn = dense_of(hr);
   if (!dn) return false;
@@ -1073,7 +1073,7 @@ bool Engine::try_pull(const RulePlan& pl) {
       cand = push_cost<K>(x, (const K*)H.delta.k.ptr(), H.delta.n, 2, H.bits, pr->R, ixj.csr.get());
     });
   }
-  if (cand * 4 < E.full.n) return false;
+  if (pull_mode_ == 1 && cand * 4 < E.full.n) return false;
   Index& ixr = get_index(pr->E, false, {pr->eR, pr->eJ}, true, true);
   const u64 nv = D_.nvals();
   // the chunk list is an index of E, kept with it across runs
@@ -1100,8 +1100,57 @@ bool Engine::try_pull(const RulePlan& pl) {
   return true;
 }
 
+// Word-parallel destination-side evaluation of a Bool delta variant (see
+// BitPullRule), when the rule has that shape and the pull's bytes (every
+// row-graph entry, plus the delta rows of the active share of them) are
+// below the push join's (each output a random cell access), both estimated
+// from the delta's size.  Returns false to run the push join instead.
+bool Engine::try_bitpull(const RulePlan& pl) {
+  if (pl.dpos < 0 || pull_mode_ == 0) return false;
+  const int hr = P_.rules[pl.rule].head.rel;
+  DenseRel* dn = dense_of(hr);
+  if (!dn || dn->bpulls.empty()) return false;
+  BitPullRule* bp = nullptr;
+  for (auto& p : dn->bpulls)
+    if (p.rule == pl.rule && p.dpos == pl.dpos) bp = &p;
+  if (!bp) return false;
+  DRel& H = R_[hr];
+  DRel& E = R_[bp->E];
+  if (H.delta.n == 0 || E.full.n == 0) return false;
+  Exec x = ctx_->exec();
+  if (bp->ev != E.version) {
+    with_key(E.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      bp->nedges = bitpull_graph<K>(x, (const K*)E.full.k.ptr(), E.full.n, E.arity, E.bits, dn->cd, H.arity, bp->eh,
+                                    bp->ed, dn->blbits, dn->brows, &bp->graph, &bp->csr);
+    });
+    bp->nchunks = bitpull_chunks(x, bp->csr.get(), dn->brows, &bp->chunks);
+    bp->ev = E.version;
+  }
+  if (bp->nedges == 0) return false;
+  const double W = (double)(1ull << (dn->blbits - 5));
+  const double nE = (double)bp->nedges, rows = (double)std::max<u64>(dn->brows, 1);
+  const double active = std::min(1.0, (double)H.delta.n / rows);
+  const double pull_b = nE * 12.0 + nE * active * 4.0 * W;
+  const double push_b = (double)H.delta.n * (nE / rows) * 16.0;
+  if (pull_mode_ == 1 && pull_b > push_b) return false;
+  BitPullDesc bd;
+  bd.nchunks = bp->nchunks;
+  bd.chunks = bp->chunks.get();
+  bd.graph = bp->graph.get();
+  bd.dbits = dn->dbits.get();
+  bd.rstamp = dn->rstamp.get();
+  bd.stamp = dn->stamp;
+  bd.wshift = dn->blbits - 5;
+  // algorithmic bytes: every graph entry (8) and its source row's stamp (4),
+  // the active entries' delta rows, and each chunk's destination row read
+  // and written once
+  bitpull_apply(x, bd, dn->cd, nE * 12.0 + nE * active * 4.0 * W + (double)bp->nchunks * 8.0 * W, bp->nedges);
+  return true;
+}
+
 void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out) {
-  if (try_pull(pl)) return;
+  if (try_bitpull(pl) || try_pull(pl)) return;
   const Rule& r = P_.rules[pl.rule];
   Exec x = ctx_->exec();
   const int hr = r.head.rel;
@@ -1391,6 +1440,8 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
       DBuf<K> dk;
       DBuf<i64> dw;
       DBuf<u8> df;
+      // a new stamp for the rows of the delta this compaction writes
+      if (dn->cd.dbits) dn->cd.stamp = ++dn->stamp;
       // the first pull rule's push cost for the new delta comes with the count
       u64 pcost = ~0ull;
       if (!dn->pulls.empty()) {
@@ -2201,6 +2252,7 @@ void Engine::plan_dense(u32 k) {
               if (bv(eJ) != J || bv(eR) != hv(Rc)) continue;
               if (cd.dom_bits[Rc] || cd.shift[Lc] != 0 || cd.shift[Rc] != cd.ebits[Lc]) continue;
               if (cd.ebits[Lc] > 6) continue;  // rows of at most one wave of lanes
+              if (!trop && cd.ebits[Lc] >= 5) continue;  // whole-word Bool rows: bit pull
               PullRule pr;
               pr.rule = (int)ri;
               pr.dpos = dpos;
@@ -2226,8 +2278,83 @@ void Engine::plan_dense(u32 k) {
         cd.rlbits = dn->lbits;
       }
     }
+    if (!trop && R.arity >= 2) plan_bitpulls(k, (int)i, *dn);
     dense_[i] = std::move(dn);
   }
+}
+
+// Bool rules of relation i that a round may evaluate word-parallel (see
+// BitPullRule): H(h..) :- H(d..), E(e..) in either body order, all terms
+// variables, no variable repeated within an atom, the lane variable in the
+// cells' least significant column of both H atoms and absent from E, every
+// other H column's variable found in E, and every E variable one of those.
+void Engine::plan_bitpulls(u32 k, int i, DenseRel& dn) {
+  DRel& R = R_[i];
+  CellDesc& cd = dn.cd;
+  const int Lc = dn.order.back();
+  const int lb = cd.ebits[Lc];
+  if (lb < 5 || cd.shift[Lc] != 0 || (dn.ncells >> lb) >= (1ull << 32)) return;
+  for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+    const Rule& ru = P_.rules[ri];
+    if (ru.stratum != k || ru.agg || ru.head.rel != i || ru.body.size() != 2) continue;
+    for (int dpos = 0; dpos < 2; dpos++) {
+      const Atom& A = ru.body[dpos];
+      const Atom& B = ru.body[1 - dpos];
+      if (A.neg || B.neg || A.rel != i || B.rel < 0 || B.rel == i || in_idb_[B.rel]) continue;
+      if (R_[B.rel].sem != SK_BOOL || (int)A.args.size() != R.arity || (int)ru.head.args.size() != R.arity ||
+          (int)B.args.size() != R_[B.rel].arity)
+        continue;
+      bool ok = true;
+      auto distinct_vars = [&](const Atom& at) {
+        for (size_t a = 0; a < at.args.size(); a++) {
+          if (at.args[a].kind != SK_T_VAR) return false;
+          for (size_t b = 0; b < a; b++)
+            if (at.args[b].a == at.args[a].a) return false;
+        }
+        return true;
+      };
+      ok = distinct_vars(ru.head) && distinct_vars(A) && distinct_vars(B);
+      if (!ok) continue;
+      const u32 xv = ru.head.args[Lc].a;
+      if (A.args[Lc].a != xv) continue;
+      auto in_e = [&](u32 v) {
+        for (size_t a = 0; a < B.args.size(); a++)
+          if (B.args[a].a == v) return (int)a;
+        return -1;
+      };
+      if (in_e(xv) >= 0) continue;
+      BitPullRule bp;
+      bp.rule = (int)ri;
+      bp.dpos = dpos;
+      bp.E = B.rel;
+      std::vector<bool> covered(B.args.size(), false);
+      for (int c = 0; c < R.arity && ok; c++) {
+        if (c == Lc) {
+          bp.eh[c] = bp.ed[c] = -1;
+          continue;
+        }
+        bp.eh[c] = in_e(ru.head.args[c].a);
+        bp.ed[c] = in_e(A.args[c].a);
+        ok = bp.eh[c] >= 0 && bp.ed[c] >= 0;
+        if (ok) covered[bp.eh[c]] = covered[bp.ed[c]] = true;
+      }
+      for (bool cv : covered) ok = ok && cv;
+      if (!ok) continue;
+      dn.bpulls.push_back(std::move(bp));
+    }
+  }
+  if (dn.bpulls.empty()) return;
+  Exec x = ctx_->exec();
+  dn.blbits = lb;
+  dn.brows = dn.ncells >> lb;
+  const u64 nwc = std::max<u64>(dn.ncells, 32) / 32;
+  dn.dbits.reset(x.pool, nwc);
+  dn.rstamp.reset(x.pool, dn.brows);
+  memset_async(x, dn.rstamp.get(), 0, dn.brows * 4);
+  dn.stamp = 0;
+  cd.dbits = dn.dbits.get();
+  cd.rstamp = dn.rstamp.get();
+  cd.dwshift = lb - 5;
 }
 
 bool Engine::any_narrow() const {

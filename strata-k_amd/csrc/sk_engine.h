@@ -163,6 +163,23 @@ struct PullRule {
   u64 cost_dv = ~0ull;
 };
 
+// A Bool rule `H(.., x) :- H(.., x), E(..)` (either body order) whose lane
+// variable x sits in the cells' least significant column, with rows of
+// whole words: a round may evaluate it word-parallel from the destination
+// side (k_bitpull).  Every other column of the head and of the delta literal
+// holds a distinct variable bound by E, and every variable of E is one of
+// those, so each tuple of E names one (head row, delta row) pair.
+struct BitPullRule {
+  int rule = -1, dpos = -1;
+  int E = -1;
+  int eh[kMaxCols] = {}, ed[kMaxCols] = {};  // per H column: E column of the head's / delta's
+                                             // variable; -1 = the lane column
+  DBuf<u64> graph, csr;  // the row graph, built once per version of E
+  DBuf<BitChunk> chunks;
+  u64 nchunks = 0, nedges = 0;
+  u64 ev = ~0ull;
+};
+
 // Dense cell state of one relation while its stratum runs (see sk_cells.h).
 struct DenseRel {
   bool keep_full = false;  // some rule of the stratum reads the relation's full
@@ -187,6 +204,12 @@ struct DenseRel {
   RelData spare;
   u64 nrows = 0;
   int lbits = 0;
+  // word-parallel Bool pulls: the last delta as a bitmap and per-row stamps
+  std::vector<BitPullRule> bpulls;
+  DBuf<u32> dbits, rstamp;
+  u64 brows = 0;  // rows of 2^blbits cells
+  int blbits = 0;
+  u32 stamp = 0;
 };
 
 // Distinct codes of one relation column, as a bitmap over the dictionary.
@@ -312,6 +335,9 @@ class Engine {
   }
   // Keep eligible relations as dense cells during their stratum (default on).
   void set_dense(bool on) { dense_enabled_ = on; }
+  // Destination-side (pull) rounds: 0 never, 1 when the cost model prefers
+  // them (default), 2 whenever the rule's shape allows.
+  void set_pull(int mode) { pull_mode_ = mode; }
 
  private:
   std::vector<RelData> snap_;  // post-load state of relations written by rules
@@ -344,6 +370,7 @@ class Engine {
   std::vector<std::unique_ptr<DenseRel>> dense_;  // per relation, while its stratum runs
   std::map<std::pair<int, int>, ColCodes> colcodes_;  // column code sets, by (relation, column)
   bool dense_enabled_ = true;
+  int pull_mode_ = 1;
   bool narrow_ok_ = true;   // every staged Trop weight is finite-or-inf in [0, 2^32-1)
 
   void build_dict();
@@ -363,6 +390,8 @@ class Engine {
   RulePlan plan_rule(int ri, int dpos);
   void exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out);
   bool try_pull(const RulePlan& pl);
+  bool try_bitpull(const RulePlan& pl);
+  void plan_bitpulls(u32 k, int i, DenseRel& dn);
   u64 finalize(int rel, std::vector<Chunk>& chunks);
   const RelData& source(int rel, bool delta) { return delta ? R_[rel].delta : R_[rel].full; }
   Index& get_index(int rel, bool delta, const std::vector<int>& perm, bool want_csr, bool want_adj);

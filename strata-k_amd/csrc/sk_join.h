@@ -98,6 +98,40 @@ struct PullDesc {
   u64 nrows = 0;                   // rows of the cell layout (j values at or above are absent)
 };
 
+// Word-parallel pull of a Bool linear recursive rule over dense cells:
+// head row d ⊕= OR of the delta rows s with (d, s) in the rule's row graph
+// (one pair per tuple of the fixed relation E: d = the head's row, s = the
+// delta's row, both cell-row indices).  Rows are 2^wshift words of 32
+// lanes; the lane column passes through unchanged.  One wave takes a chunk:
+// up to 256 graph entries of one destination row.
+struct BitChunk {
+  u64 lo;
+  u32 n;
+  u32 row;
+};
+static_assert(sizeof(BitChunk) == 16, "BitChunk is one 16-byte load");
+
+struct BitPullDesc {
+  u64 nchunks = 0;
+  const BitChunk* chunks = nullptr;
+  const u64* graph = nullptr;  // (d << 32 | s), sorted by d
+  const u32* dbits = nullptr;  // last round's delta as a bitmap
+  const u32* rstamp = nullptr; // per row: stamp of the last compaction that found a delta cell in it
+  u32 stamp = 0;
+  int wshift = 0;
+};
+
+// Row graph of a bit-pull rule from E's packed tuples: for every tuple, the
+// head's and the delta's cell rows (column c of H takes the value of E
+// column eh[c] / ed[c]; -1 marks the lane column).  Invalid pairs (a value
+// outside H's domain) sort last as (nrows << 32).  Returns the valid count;
+// `graph` is sorted by destination row and `csr` (nrows + 1) indexes it.
+template <class K>
+u64 bitpull_graph(const Exec& x, const K* ekeys, u64 n, int arity, int bits, const CellDesc& cd, int hcols,
+                  const int* eh, const int* ed, int lbits, u64 nrows, DBuf<u64>* graph, DBuf<u64>* csr);
+u64 bitpull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<BitChunk>* out);
+void bitpull_apply(const Exec& x, const BitPullDesc& p, const CellDesc& cd, double bytes, u64 items);
+
 struct AntiDesc {
   int arity = 0, bits = 0;
   int src[kMaxCols] = {};  // table column, or -1 = constant

@@ -11,6 +11,7 @@
 //   leapfrog  sorted-list intersection for triangle bodies
 // Trop ⊗ is checked i64 addition in body order with +inf absorbing; the
 // first overflow is recorded in the device error record (crates/strata-ir/src/trop.rs:56-62).
+#include <algorithm>
 #include <climits>
 
 #include "sk_join.h"
@@ -1111,6 +1112,173 @@ u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out) 
   return n;
 }
 
+// ============================================================ bit pull
+//
+// Bool rules of the pull shape over rows of any width: a destination row is
+// the OR of the delta rows its row-graph entries name, 32 lanes per word
+// operation and no atomic per lane.  Per wave and batch of 64 graph
+// entries, lane l loads entry l's source row and its stamp; a ballot keeps
+// the rows holding delta cells, which are compacted into LDS, and the wave
+// then ORs those rows' delta words.  Rows of W >= 64 words go word block by
+// word block (one word per lane); narrower rows split the wave into 64 / W
+// slots that take the active rows in turn, folded by shuffles at the end.
+// The result is applied with one atomic OR per word that gains bits; the
+// bits it gained (and only those) are marked touched, exactly as the push
+// join marks each new cell.
+
+struct BitGraphDesc {
+  int arity = 0, bits = 0, hcols = 0, lbits = 0;
+  int eh[kMaxCols] = {}, ed[kMaxCols] = {};
+  u64 nrows = 0;
+};
+
+template <class K>
+__global__ void k_bitpull_graph(const K* __restrict__ ek, u64 n, BitGraphDesc g, CellDesc cd,
+                                u64* __restrict__ out) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const K key = ek[i];
+  u64 d = 0, s = 0;
+  bool ok = true;
+  for (int c = 0; c < g.hcols; c++) {
+    if (g.eh[c] < 0) continue;  // the lane column
+    u64 rd = 0, rs = 0;
+    ok = ok && cell_rank(cd, c, col_of<K>(key, g.arity, g.bits, g.eh[c]), &rd) && !(rd >> cd.ebits[c]);
+    ok = ok && cell_rank(cd, c, col_of<K>(key, g.arity, g.bits, g.ed[c]), &rs) && !(rs >> cd.ebits[c]);
+    d |= rd << (cd.shift[c] - g.lbits);
+    s |= rs << (cd.shift[c] - g.lbits);
+  }
+  out[i] = ok ? ((d << 32) | s) : (g.nrows << 32);
+}
+
+__device__ __forceinline__ void bit_apply(const CellDesc& cd, u64 wi, u32 acc) {
+  if (!acc) return;
+  const u32 nw = acc & ~cd.present[wi];
+  if (!nw) return;
+  const u32 old = atomicOr(&cd.present[wi], nw);
+  const u32 nb = nw & ~old;
+  if (nb) atomicOr(&cd.touched[wi], nb);
+}
+
+// Loads entry e0 + lane's source row; returns the number of entries of the
+// batch whose source row holds delta cells, their rows compacted in `act`.
+__device__ __forceinline__ u32 bitpull_batch(const BitPullDesc& bd, u64 e0, u64 e_end, u32 lane, u32* act) {
+  const u64 e = e0 + lane;
+  u32 s = 0;
+  bool a = false;
+  if (e < e_end) {
+    s = (u32)bd.graph[e];
+    a = bd.rstamp[s] == bd.stamp;
+  }
+  const u64 m = __ballot(a);
+  if (a) act[__popcll(m & ((1ull << lane) - 1ull))] = s;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return (u32)__popcll(m);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bitpull(BitPullDesc bd, CellDesc cd) {
+  __shared__ u32 s_act[kBlock];
+  const u32 lane = threadIdx.x & 63;
+  const u32 wv = threadIdx.x >> 6;
+  const u64 ch = (u64)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(wv);
+  if (ch >= bd.nchunks) return;  // whole waves leave; no block barrier below
+  u32* act = s_act + wv * 64;
+  const BitChunk c = bd.chunks[ch];
+  const u64 e_end = c.lo + c.n;
+  const u32 W = 1u << bd.wshift;
+  if (W >= 64) {
+    for (u32 wb = 0; wb < W; wb += 64) {
+      u32 acc = 0;
+      for (u64 e0 = c.lo; e0 < e_end; e0 += 64) {
+        const u32 na = bitpull_batch(bd, e0, e_end, lane, act);
+        u32 k = 0;
+        for (; k + 4 <= na; k += 4) {
+          const u32 v0 = bd.dbits[((u64)act[k] << bd.wshift) + wb + lane];
+          const u32 v1 = bd.dbits[((u64)act[k + 1] << bd.wshift) + wb + lane];
+          const u32 v2 = bd.dbits[((u64)act[k + 2] << bd.wshift) + wb + lane];
+          const u32 v3 = bd.dbits[((u64)act[k + 3] << bd.wshift) + wb + lane];
+          acc |= v0 | v1 | v2 | v3;
+        }
+        for (; k < na; k++) acc |= bd.dbits[((u64)act[k] << bd.wshift) + wb + lane];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      bit_apply(cd, ((u64)c.row << bd.wshift) + wb + lane, acc);
+    }
+  } else {
+    const u32 slot = lane >> bd.wshift, word = lane & (W - 1), nslots = 64u >> bd.wshift;
+    u32 acc = 0;
+    for (u64 e0 = c.lo; e0 < e_end; e0 += 64) {
+      const u32 na = bitpull_batch(bd, e0, e_end, lane, act);
+      for (u32 k = slot; k < na; k += nslots) acc |= bd.dbits[((u64)act[k] << bd.wshift) + word];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (u32 o = W; o < 64; o <<= 1) acc |= __shfl_xor(acc, o, 64);
+    if (lane < W) bit_apply(cd, ((u64)c.row << bd.wshift) + word, acc);
+  }
+}
+
+static int ceil_log2_j(u64 n) {
+  int b = 0;
+  while ((1ull << b) < n) b++;
+  return b;
+}
+
+template <class K>
+u64 bitpull_graph(const Exec& x, const K* ekeys, u64 n, int arity, int bits, const CellDesc& cd, int hcols,
+                  const int* eh, const int* ed, int lbits, u64 nrows, DBuf<u64>* graph, DBuf<u64>* csr) {
+  BitGraphDesc g;
+  g.arity = arity;
+  g.bits = bits;
+  g.hcols = hcols;
+  g.lbits = lbits;
+  g.nrows = nrows;
+  for (int c = 0; c < hcols; c++) {
+    g.eh[c] = eh[c];
+    g.ed[c] = ed[c];
+  }
+  DBuf<u64> a(x.pool, n ? n : 1), b(x.pool, n ? n : 1);
+  u64* sorted = a.get();
+  if (n) {
+    {
+      Scope sc(x.timer, "bitpull_graph", KC_JOIN, (double)n * (sizeof(K) + 8.0), n);
+      k_bitpull_graph<K><<<gridj(n), kBlock, 0, x.stream>>>(ekeys, n, g, cd, a.get());
+      SK_HIP(hipGetLastError());
+    }
+    radix_sort<u64>(x, a.get(), b.get(), nullptr, nullptr, nullptr, nullptr, n, 32 + ceil_log2_j(nrows + 1),
+                    &sorted, nullptr, nullptr);
+  }
+  csr->reset(x.pool, nrows + 1);
+  build_csr<u64>(x, sorted, n, 32, nrows, csr->get());
+  const u64 valid = read_u64(x, csr->get() + nrows);
+  *graph = (sorted == a.get()) ? std::move(a) : std::move(b);
+  return valid;
+}
+
+u64 bitpull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<BitChunk>* out) {
+  std::vector<u64> h(nrows + 1);
+  SK_HIP(hipMemcpyAsync(h.data(), csr, (nrows + 1) * sizeof(u64), hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  std::vector<BitChunk> v;
+  for (u64 r = 0; r < nrows; r++)
+    for (u64 s = h[r]; s < h[r + 1]; s += 256) v.push_back(BitChunk{s, (u32)std::min<u64>(256, h[r + 1] - s), (u32)r});
+  const u64 n = v.size();
+  out->reset(x.pool, n ? n : 1);
+  if (n) SK_HIP(hipMemcpyAsync(out->get(), v.data(), n * sizeof(BitChunk), hipMemcpyHostToDevice, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  return n;
+}
+
+void bitpull_apply(const Exec& x, const BitPullDesc& p, const CellDesc& cd, double bytes, u64 items) {
+  if (!p.nchunks) return;
+  Scope sc(x.timer, "join_bitpull", KC_JOIN, bytes, items);
+  const unsigned nb = (unsigned)((p.nchunks + kWaves - 1) / kWaves);
+  k_bitpull<<<nb, kBlock, 0, x.stream>>>(p, cd);
+  SK_HIP(hipGetLastError());
+}
+
 // Grid-stride sum; one atomic per block (a single counter hit once per wave
 // serialises the whole launch).
 template <class K>
@@ -1238,7 +1406,9 @@ u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords) {
   template u64 triangles<K>(const Exec&, const K*, u64, const TriDesc&, const u64*, const u32*, const u64*,       \
                             const u32*, OutDesc*, bool, bool, HeadAlloc, void*, double*);                        \
   template double tri_list_len<K>(const Exec&, const K*, u64, const TriDesc&, const u64*, const u64*);           \
-  template void pack_cols<K>(const Exec&, const Cols&, int, int, u64, K*);
+  template void pack_cols<K>(const Exec&, const Cols&, int, int, u64, K*);                                       \
+  template u64 bitpull_graph<K>(const Exec&, const K*, u64, int, int, const CellDesc&, int, const int*, const int*, \
+                                int, u64, DBuf<u64>*, DBuf<u64>*);
 
 SK_JOIN_INST(u64)
 SK_JOIN_INST(u128)

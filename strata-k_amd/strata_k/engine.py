@@ -142,6 +142,7 @@ SIGNATURES = [
     ("sk_run_execute", C.c_int, [_VP]),
     ("sk_run_reset", C.c_int, [_VP]),
     ("sk_run_set_dense", C.c_int, [_VP, C.c_int]),
+    ("sk_run_set_pull", C.c_int, [_VP, C.c_int]),
     ("sk_run_error", None, [_VP, C.POINTER(SkErrorInfo)]),
     ("sk_result_size", _I64, [_VP, C.c_char_p]),
     ("sk_result_arity", _I64, [_VP, C.c_char_p]),
@@ -176,6 +177,9 @@ SIGNATURES = [
     ("sk_batch_result_size", _I64, [_VP, _U32, C.c_char_p]),
     ("sk_batch_result_read_sym", C.c_int, [_VP, _U32, C.c_char_p, _VP, _VP, _VP]),
     ("sk_batch_execute_ms", C.c_double, [_VP]),
+    ("sk_batch_kernel_count", _U32, [_VP]),
+    ("sk_batch_kernel_record", C.c_int, [_VP, _U32, C.c_char_p, C.c_size_t, C.POINTER(_U32),
+                                        C.POINTER(SkKernelStats)]),
     ("sk_server_create", C.c_int, [_VP, _VP, _VP, _VP, _U64, _U64, _U32, C.POINTER(_VP)]),
     ("sk_server_free", None, [_VP]),
     ("sk_server_submit", C.c_int, [_VP, _VP, _U32, _VP, _U32, C.POINTER(_U64)]),
@@ -377,16 +381,18 @@ class Context:
         return Run(self, prog)
 
     def run_semi_naive(self, prog, edb: Iterable = (), bulk: Optional[dict] = None,
-                       want: Optional[Sequence[str]] = None, dense: bool = True) -> Db:
+                       want: Optional[Sequence[str]] = None, dense: bool = True, pull: int = 1) -> Db:
         """``run_semi_naive(prog, edb)`` (crates/strata-eval/src/seminaive.rs:29).
 
         ``edb``: iterable of ``(pred, tuple-of-Sym/IntV, ann)``; ``bulk``: pred ->
         ``(columns, weights, inf)`` numpy arrays of Sym ids for large inputs.
-        ``dense=False`` keeps every relation on the sorted-list path.
+        ``dense=False`` keeps every relation on the sorted-list path; ``pull``
+        selects destination-side rounds (see ``Run.set_pull``).
         """
         run = self.prepare(prog)
         try:
             run.set_dense(dense)
+            run.set_pull(pull)
             run.add_facts(edb)
             for p, (cols, w, inf) in (bulk or {}).items():
                 run.add_sym(p, cols, w, inf)
@@ -499,6 +505,11 @@ class Run:
         sorted-list path for every relation (False)."""
         self._check(self.L.sk_run_set_dense(self.h, 1 if on else 0))
 
+    def set_pull(self, mode: int):
+        """Pull rounds: 0 never, 1 by the cost model (default), 2 whenever
+        the rule's shape allows.  Results are identical in every mode."""
+        self._check(self.L.sk_run_set_pull(self.h, int(mode)))
+
     @property
     def execute_ms(self) -> float:
         return float(self.L.sk_run_execute_ms(self.h))
@@ -555,15 +566,7 @@ class Run:
         return out
 
     def kernel_records(self) -> List[dict]:
-        recs = []
-        for i in range(int(self.L.sk_run_kernel_count(self.h))):
-            nm = C.create_string_buffer(64)
-            cls = C.c_uint32()
-            s = SkKernelStats()
-            self.L.sk_run_kernel_record(self.h, i, nm, 64, C.byref(cls), C.byref(s))
-            recs.append(dict(name=nm.value.decode(), cls=KERNEL_CLASSES[cls.value], launches=int(s.launches),
-                             ms=float(s.ms), bytes=float(s.bytes), items=int(s.items)))
-        return recs
+        return _kernel_records(self.L, self.h, self.L.sk_run_kernel_count, self.L.sk_run_kernel_record)
 
     def db(self, want: Optional[Sequence[str]] = None) -> Db:
         names = want if want is not None else [p.name for p in self.prog.predicates]
@@ -574,6 +577,18 @@ class Run:
 # ------------------------------------------------- reference crate functions
 
 _default_ctx: Optional[Context] = None
+
+
+def _kernel_records(L, h, count_fn, record_fn) -> List[dict]:
+    recs = []
+    for i in range(int(count_fn(h))):
+        nm = C.create_string_buffer(64)
+        cls = C.c_uint32()
+        s = SkKernelStats()
+        record_fn(h, i, nm, 64, C.byref(cls), C.byref(s))
+        recs.append(dict(name=nm.value.decode(), cls=KERNEL_CLASSES[cls.value], launches=int(s.launches),
+                         ms=float(s.ms), bytes=float(s.bytes), items=int(s.items)))
+    return recs
 
 
 def default_context() -> Context:
@@ -733,6 +748,11 @@ class Batch:
     @property
     def execute_ms(self) -> float:
         return float(self.L.sk_batch_execute_ms(self.h))
+
+    def kernel_records(self) -> List[dict]:
+        """Per-kernel records of the last execute, summed over the groups
+        (profiling on in the context)."""
+        return _kernel_records(self.L, self.h, self.L.sk_batch_kernel_count, self.L.sk_batch_kernel_record)
 
     def status(self, i: int) -> Tuple[int, dict]:
         info = SkErrorInfo()
