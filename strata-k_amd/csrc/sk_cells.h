@@ -20,6 +20,8 @@
 // monotone in the code, so cell order equals key order.
 #pragma once
 
+#include <vector>
+
 #include "sk_dev.h"
 
 namespace sk {
@@ -37,6 +39,14 @@ struct CellDesc {
   const u32* dom_bits[kMaxCols] = {};  // domain bitmap over codes, or null = identity
   const u64* dom_pre[kMaxCols] = {};   // per-word count of domain codes in earlier words
   const u32* dom_list[kMaxCols] = {};  // rank -> code (null = identity)
+  // Batched runs (column 0 holds the program id p, whose code is p): a
+  // column may instead be ranked within its program — its codes for program
+  // p are pdom_codes[c][pdom_off[c][p] .. pdom_off[c][p+1]), sorted, and a
+  // code's rank is its position in that segment.  Program ids lead the cell
+  // order, so cell order is still key order.
+  const u64* pdom_off[kMaxCols] = {};
+  const u32* pdom_codes[kMaxCols] = {};
+  u64 nparts = 0;
   u64* cell = nullptr;     // Trop weight codes
   // Narrow Trop cells: when every weight the run can hold is a non-negative
   // value below 2^32-1, cells hold the weight itself as u32 (kCellTop32 =
@@ -79,8 +89,24 @@ struct CellDesc {
 __device__ __forceinline__ u64 cell_enc(i64 w) { return (u64)w ^ (1ull << 63); }
 __device__ __forceinline__ i64 cell_dec(u64 e) { return (i64)(e ^ (1ull << 63)); }
 
-__device__ __forceinline__ bool cell_rank(const CellDesc& cd, int c, u32 code, u64* r) {
+// Rank of `code` in column c's domain; `pid` is the row's column-0 code
+// (read only for a column ranked within its program).
+__device__ __forceinline__ bool cell_rank(const CellDesc& cd, int c, u32 code, u32 pid, u64* r) {
   if (code >= cd.ncodes) return false;
+  if (cd.pdom_off[c]) {
+    if (pid >= cd.nparts) return false;
+    const u64 a0 = cd.pdom_off[c][pid];
+    u64 a = a0, b = cd.pdom_off[c][pid + 1];
+    const u32* pc = cd.pdom_codes[c];
+    while (a < b) {
+      const u64 m = (a + b) >> 1;
+      if (pc[m] < code) a = m + 1;
+      else b = m;
+    }
+    if (a == cd.pdom_off[c][pid + 1] || pc[a] != code) return false;
+    *r = a - a0;
+    return true;
+  }
   const u32* b = cd.dom_bits[c];
   if (!b) {
     *r = code;
@@ -166,9 +192,10 @@ __device__ __forceinline__ void cell_insert(const CellDesc& cd, u64 c, i64 w, u8
 template <class GetCode>
 __device__ __forceinline__ bool cell_of(const CellDesc& cd, GetCode code_of, u64* cell) {
   u64 c = 0;
+  const u32 pid = cd.nparts ? code_of(0) : 0u;
   for (int j = 0; j < cd.ncols; j++) {
     u64 r;
-    if (!cell_rank(cd, j, code_of(j), &r) || (r >> cd.ebits[j])) {
+    if (!cell_rank(cd, j, code_of(j), pid, &r) || (r >> cd.ebits[j])) {
       atomicOr(cd.bad, 1u);
       return false;
     }
@@ -188,6 +215,17 @@ void cells_or(const Exec& x, u32* dst, const u32* src, u64 nwords);
 // Domain tables of a bitmap: per-word prefix counts and the rank -> code list.
 // Returns the number of codes in the domain.
 u64 cells_domain(const Exec& x, const u32* bitmap, u64 nwords, u64* pre, DBuf<u32>* list);
+// Per-program domain of a column in a batched run: the distinct (program,
+// code) pairs of the given relation columns (column 0 of each relation is the
+// program id, whose code is below nparts).  off (nparts + 1) and codes are
+// (re)allocated; returns the largest per-program count.
+struct PartCol {
+  const void* keys;
+  bool wide;
+  u64 n;
+  int arity, bits, col;
+};
+u64 cells_part_domain(const Exec& x, const std::vector<PartCol>& parts, u64 nparts, DBuf<u64>* off, DBuf<u32>* codes);
 // Load a relation's tuples into empty cells (no touched bits).
 template <class K>
 void cells_load(const Exec& x, const CellDesc& cd, const K* keys, const i64* w, const u8* inf, u64 n, int arity,

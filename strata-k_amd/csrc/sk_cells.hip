@@ -4,6 +4,8 @@
 #include <climits>
 #include <cstring>
 
+#include <algorithm>
+
 #include "sk_cells.h"
 
 namespace sk {
@@ -23,6 +25,30 @@ __global__ void k_cells_mark_col(const K* __restrict__ keys, u64 n, int arity, i
   const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   const u32 v = i < n ? code_at<K>(keys[i], arity, bits, j) : 0u;
   mark_bit_once(bitmap, v, i < n);
+}
+
+// Codes below 2^kMarkLdsBits: each block marks a private LDS bitmap over a
+// grid-stride share of the rows and ORs its non-zero words into the global
+// bitmap once (a column of few distinct codes otherwise sends every row's
+// atomic to the same few words).
+constexpr int kMarkLdsBits = 18;
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_cells_mark_col_lds(const K* __restrict__ keys, u64 n, int arity, int bits,
+                                                               int j, u32* bitmap) {
+  __shared__ u32 sb[(1u << kMarkLdsBits) / 32];
+  const u32 nw = (1u << bits) / 32 ? (1u << bits) / 32 : 1u;
+  for (u32 i = threadIdx.x; i < nw; i += kBlock) sb[i] = 0;
+  __syncthreads();
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (u64)gridDim.x * kBlock) {
+    const u32 v = code_at<K>(keys[i], arity, bits, j);
+    const u32 m = 1u << (v & 31);
+    if (!(sb[v >> 5] & m)) atomicOr(&sb[v >> 5], m);
+  }
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < nw; i += kBlock) {
+    const u32 w = sb[i];
+    if (w && (bitmap[i] & w) != w) atomicOr(&bitmap[i], w);
+  }
 }
 
 __global__ void k_cells_mark_code(u32* bitmap, u32 code) {
@@ -56,7 +82,13 @@ template <class K>
 void cells_mark_col(const Exec& x, const K* keys, u64 n, int arity, int bits, int j, u32* bitmap) {
   if (!n) return;
   Scope sc(x.timer, "cells_domain", KC_MISC, (double)n * sizeof(K), n);
-  k_cells_mark_col<K><<<gridc(n), kBlock, 0, x.stream>>>(keys, n, arity, bits, j, bitmap);
+  if (bits <= kMarkLdsBits) {
+    // enough blocks to fill the chip, each over at least 16 rows per thread
+    const u64 nb = std::min<u64>(1024, std::max<u64>(1, n / (kBlock * 16)));
+    k_cells_mark_col_lds<K><<<(unsigned)nb, kBlock, 0, x.stream>>>(keys, n, arity, bits, j, bitmap);
+  } else {
+    k_cells_mark_col<K><<<gridc(n), kBlock, 0, x.stream>>>(keys, n, arity, bits, j, bitmap);
+  }
   SK_HIP(hipGetLastError());
 }
 
@@ -82,6 +114,58 @@ u64 cells_domain(const Exec& x, const u32* bitmap, u64 nwords, u64* pre, DBuf<u3
   k_cells_list<<<gridc(nwords), kBlock, 0, x.stream>>>(bitmap, pre, nwords, list->get());
   SK_HIP(hipGetLastError());
   return n;
+}
+
+template <class K>
+__global__ void k_part_pairs(const K* __restrict__ keys, u64 n, int arity, int bits, int col, u64* __restrict__ out) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const K k = keys[i];
+  out[i] = ((u64)code_at<K>(k, arity, bits, 0) << 32) | code_at<K>(k, arity, bits, col);
+}
+
+__global__ void k_low32(const u64* __restrict__ in, u64 n, u32* __restrict__ out) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (u32)in[i];
+}
+
+u64 cells_part_domain(const Exec& x, const std::vector<PartCol>& parts, u64 nparts, DBuf<u64>* off, DBuf<u32>* codes) {
+  u64 total = 0;
+  for (auto& p : parts) total += p.n;
+  DBuf<u64> a(x.pool, total ? total : 1), b(x.pool, total ? total : 1), u(x.pool, total ? total : 1);
+  u64 at = 0;
+  for (auto& p : parts) {
+    if (!p.n) continue;
+    if (p.wide)
+      k_part_pairs<u128><<<gridc(p.n), kBlock, 0, x.stream>>>((const u128*)p.keys, p.n, p.arity, p.bits, p.col,
+                                                              a.get() + at);
+    else
+      k_part_pairs<u64><<<gridc(p.n), kBlock, 0, x.stream>>>((const u64*)p.keys, p.n, p.arity, p.bits, p.col,
+                                                             a.get() + at);
+    SK_HIP(hipGetLastError());
+    at += p.n;
+  }
+  int pb = 0;
+  while ((1ull << pb) < nparts) pb++;
+  u64* sorted = a.get();
+  u64 nu = 0;
+  if (total) {
+    radix_sort<u64>(x, a.get(), b.get(), nullptr, nullptr, nullptr, nullptr, total, 32 + pb, &sorted, nullptr, nullptr);
+    nu = unique_runs<u64>(x, sorted, nullptr, nullptr, total, u.get(), nullptr, nullptr);
+  }
+  off->reset(x.pool, nparts + 1);
+  build_csr<u64>(x, u.get(), nu, 32, nparts, off->get());
+  codes->reset(x.pool, nu ? nu : 1);
+  if (nu) {
+    k_low32<<<gridc(nu), kBlock, 0, x.stream>>>(u.get(), nu, codes->get());
+    SK_HIP(hipGetLastError());
+  }
+  std::vector<u64> h(nparts + 1);
+  SK_HIP(hipMemcpyAsync(h.data(), off->get(), (nparts + 1) * 8, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  u64 mx = 0;
+  for (u64 p = 0; p < nparts; p++) mx = std::max<u64>(mx, h[p + 1] - h[p]);
+  return mx;
 }
 
 // ------------------------------------------------------ load / ⊕-apply
@@ -292,9 +376,13 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
   for (u32 j = threadIdx.x; j < tot; j += kBlock) {
     const u64 c = cbase + sc[j];
     K key = 0;
+    u32 pid = 0;
     for (int q = 0; q < cd.ncols; q++) {
       u64 r = (c >> cd.shift[q]) & ((1ull << cd.ebits[q]) - 1);
-      u32 code = cd.dom_list[q] ? cd.dom_list[q][r] : (u32)r;
+      u32 code = cd.pdom_off[q]  ? cd.pdom_codes[q][cd.pdom_off[q][pid] + r]
+                 : cd.dom_list[q] ? cd.dom_list[q][r]
+                                  : (u32)r;
+      if (q == 0) pid = code;
       key = (key << bits) | (K)code;
     }
     keys[base + j] = key;

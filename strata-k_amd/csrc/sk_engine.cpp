@@ -2004,6 +2004,8 @@ void Engine::plan_dense(u32 k) {
       cells_mark_col<K>(x, (const K*)d.k.ptr(), d.n, Rr.arity, Rr.bits, col, cc.bm.get());
     });
     cc.count = d.n ? popcount_bits(x, cc.bm.get(), nw) : 0;
+    cc.rel = rel;
+    cc.col = col;
     cc.version = Rr.version;
     cc.nvals = nvals;
     return &cc;
@@ -2082,6 +2084,7 @@ void Engine::plan_dense(u32 k) {
     cd.trop = trop;
     cd.ncols = R.arity;
     cd.ncodes = nvals;
+    cd.nparts = nparts_;
     int tot_bits = 0;
     bool ok = true;
     for (int j = 0; j < R.arity && ok; j++) {
@@ -2124,6 +2127,33 @@ void Engine::plan_dense(u32 k) {
           cd.dom_pre[j] = dn->dom_pre[j].get();
           cd.dom_list[j] = dn->dom_list[j].get();
           cd.ebits[j] = ceil_log2(cnt ? cnt : 1);
+        }
+        // batched runs: a column whose values differ from program to
+        // program (each program's own sources, say) is ranked within its
+        // program when that needs fewer bits
+        if (nparts_ && j > 0 && consts.empty() && !parts.empty()) {
+          std::vector<PartCol> pcs;
+          for (const ColCodes* pc : parts) {
+            const DRel& F = R_[pc->rel];
+            pcs.push_back(PartCol{F.full.k.ptr(), F.wide, F.full.n, F.arity, F.bits, pc->col});
+          }
+          DBuf<u64> poff;
+          DBuf<u32> pcodes;
+          const u64 mx = cells_part_domain(x, pcs, nparts_, &poff, &pcodes);
+          const int pb = ceil_log2(mx ? mx : 1);
+          if (pb < cd.ebits[j]) {
+            dn->dom_bits[j].release();
+            dn->dom_pre[j].release();
+            dn->dom_list[j].release();
+            cd.dom_bits[j] = nullptr;
+            cd.dom_pre[j] = nullptr;
+            cd.dom_list[j] = nullptr;
+            dn->pdom_off[j] = std::move(poff);
+            dn->pdom_codes[j] = std::move(pcodes);
+            cd.pdom_off[j] = dn->pdom_off[j].get();
+            cd.pdom_codes[j] = dn->pdom_codes[j].get();
+            cd.ebits[j] = pb;
+          }
         }
       }
       tot_bits += cd.ebits[j];

@@ -194,6 +194,8 @@ struct DenseRel {
   DBuf<u32> dom_bits[kMaxCols];
   DBuf<u64> dom_pre[kMaxCols];
   DBuf<u32> dom_list[kMaxCols];
+  DBuf<u64> pdom_off[kMaxCols];    // batched: per-program domains (see CellDesc)
+  DBuf<u32> pdom_codes[kMaxCols];
   // pull evaluation: per row, the last delta's lanes and list position
   std::vector<PullRule> pulls;
   DBuf<u64> rowrec;  // per row: {delta lanes, delta-list position}
@@ -216,6 +218,7 @@ struct DenseRel {
 struct ColCodes {
   u64 version = ~0ull, nvals = 0, count = 0;
   DBuf<u32> bm;
+  int rel = -1, col = -1;
 };
 
 // ------------------------------------------------------------- plans

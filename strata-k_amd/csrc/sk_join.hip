@@ -424,6 +424,9 @@ __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, Ou
   u64 cell[kEmitItems];
 #pragma unroll
   for (int k = 0; k < kEmitItems; k++) cell[k] = 0;
+  u32 pidc[kEmitItems];  // column 0's codes (program ids in batched runs)
+#pragma unroll
+  for (int k = 0; k < kEmitItems; k++) pidc[k] = 0;
   for (int j = 0; j < o.ncols; j++) {
     u32 code[kEmitItems];
     const int kind = o.kind[j];
@@ -438,9 +441,21 @@ __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, Ou
 #pragma unroll
       for (int k = 0; k < kEmitItems; k++) code[k] = o.cnst[j];
     }
+    if (j == 0) {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) pidc[k] = code[k];
+    }
     const u32* db = o.cd.dom_bits[j];
     const int sh = o.cd.shift[j], eb = o.cd.ebits[j];
-    if (!db) {
+    if (o.cd.pdom_off[j]) {
+#pragma unroll
+      for (int k = 0; k < kEmitItems; k++) {
+        u64 rk = 0;
+        const bool in_dom = cell_rank(o.cd, j, code[k], pidc[k], &rk) && !(rk >> eb);
+        ok[k] = ok[k] && in_dom;
+        cell[k] |= rk << sh;
+      }
+    } else if (!db) {
 #pragma unroll
       for (int k = 0; k < kEmitItems; k++) {
         const bool in_dom = code[k] < o.cd.ncodes && !((u64)code[k] >> eb);
@@ -1143,8 +1158,10 @@ __global__ void k_bitpull_graph(const K* __restrict__ ek, u64 n, BitGraphDesc g,
   for (int c = 0; c < g.hcols; c++) {
     if (g.eh[c] < 0) continue;  // the lane column
     u64 rd = 0, rs = 0;
-    ok = ok && cell_rank(cd, c, col_of<K>(key, g.arity, g.bits, g.eh[c]), &rd) && !(rd >> cd.ebits[c]);
-    ok = ok && cell_rank(cd, c, col_of<K>(key, g.arity, g.bits, g.ed[c]), &rs) && !(rs >> cd.ebits[c]);
+    const u32 ph = col_of<K>(key, g.arity, g.bits, g.eh[0] < 0 ? 0 : g.eh[0]);
+    const u32 pd = col_of<K>(key, g.arity, g.bits, g.ed[0] < 0 ? 0 : g.ed[0]);
+    ok = ok && cell_rank(cd, c, col_of<K>(key, g.arity, g.bits, g.eh[c]), ph, &rd) && !(rd >> cd.ebits[c]);
+    ok = ok && cell_rank(cd, c, col_of<K>(key, g.arity, g.bits, g.ed[c]), pd, &rs) && !(rs >> cd.ebits[c]);
     d |= rd << (cd.shift[c] - g.lbits);
     s |= rs << (cd.shift[c] - g.lbits);
   }

@@ -1001,8 +1001,29 @@ __global__ void k_csr(const K* __restrict__ keys, u64 n, int shift, u64 nvals, u
   for (long long v = prev + 1; v <= cur; v++) csr[v] = i;
 }
 
+// Few values over many rows: csr[v] = the first row whose leading value is
+// >= v, by one binary search per value.
+template <class K>
+__global__ void k_csr_search(const K* __restrict__ keys, u64 n, int shift, u64 nvals, u64* __restrict__ csr) {
+  const u64 v = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nvals) return;
+  u64 lo = 0, hi = n;
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if ((u64)(keys[mid] >> shift) < v) lo = mid + 1;
+    else hi = mid;
+  }
+  csr[v] = lo;
+}
+
 template <class K>
 void build_csr(const Exec& x, const K* keys, u64 n, int shift, u64 nvals, u64* csr) {
+  if ((nvals + 1) * 32 < n) {
+    Scope sc(x.timer, "csr", KC_MISC, (nvals + 1) * 8.0, nvals + 1);
+    k_csr_search<K><<<grid_for(nvals + 1), kBlock, 0, x.stream>>>(keys, n, shift, nvals, csr);
+    SK_HIP(hipGetLastError());
+    return;
+  }
   Scope sc(x.timer, "csr", KC_MISC, (double)n * sizeof(K) + (nvals + 1) * 8.0, n);
   k_csr<K><<<grid_for(n + 1), kBlock, 0, x.stream>>>(keys, n, shift, nvals, csr);
   SK_HIP(hipGetLastError());

@@ -44,6 +44,39 @@ def test_batch_matches_plain_and_oracle(ctx):
     b.free()
 
 
+def test_batch_per_program_source_domains(ctx):
+    """Cost programs with 1 to 9 sources each, sources shared between some
+    programs and a source without out-edges: the source column is ranked
+    within its program (fewer cell bits than the union of all sources)."""
+    progs = []
+    for i in range(10):
+        n = 80 + 37 * i
+        s, d = wl.uniform_graph(900 + i, n, 3 * n)
+        w = wl.weights(900 + i, len(s))
+        k = 1 + (i * 4) % 9
+        src = np.sort(wl.sample_distinct(900 + i, n, k)).astype(np.uint32)
+        if i % 3 == 0:
+            src = np.unique(np.concatenate([src, np.array([n + 5], np.uint32)]))  # no out-edges
+        progs.append((s, d, w, src))
+    b = sk.Batch(ctx)
+    for s, d, w, src in progs:
+        b.add(cost_program(), {"edge": ([s, d], w), "src": ([src], np.zeros(len(src), np.int64))})
+    b.load()
+    b.execute()
+    for i, (s, d, w, src) in enumerate(progs):
+        code, info = b.status(i)
+        assert code == 0, (i, info)
+        bulk = {"edge": ([s, d], w), "src": ([src], np.zeros(len(src), np.int64))}
+        cols, bw = b.relation_sym(i, "cost")
+        ocols, ow, oit = oracle_cols(cost_program(), bulk, "cost")
+        assert len(cols[0]) == len(ocols[0]), i
+        for c, oc in zip(cols, ocols):
+            assert (c == oc).all(), i
+        assert (bw == ow).all(), i
+        assert b.iterations(i) == oit, (i, b.iterations(i), oit)
+    b.free()
+
+
 def test_batch_isolates_errors(ctx):
     """A program that fails (negative cycle, overflow) stops alone; the others
     are unaffected (each keeps its own cap universe+2, seminaive.rs:94-103)."""
