@@ -124,7 +124,7 @@ def bench_cfg2(args):
     ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n_cost = run.size("cost")
     iters = run.iterations()
-    top = max(recs, key=lambda r: r["ms"]) if recs else None
+    top = max((r for r in recs if r["bytes"] > 0), key=lambda r: r["ms"], default=None)
     roof = roofline_of(top, measured_traffic(top["name"]) if top else None)
     stats = {}
     for r in recs:
@@ -168,7 +168,7 @@ def bench_cfg1(args):
     run.load()
     ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n = run.size("path")
-    top = max(recs, key=lambda r: r["ms"]) if recs else None
+    top = max((r for r in recs if r["bytes"] > 0), key=lambda r: r["ms"], default=None)
     print(json.dumps({"config": "bool_tc_uniform_2000_10000", "derived_tuples": n, "rounds": run.iterations()[0],
                       "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
                       "kernel_ms_per_step": round(sum(r["ms"] for r in recs) / args.steps, 3),
@@ -189,7 +189,7 @@ def bench_cfg3(args):
     ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n = run.size("tri")
     ntri = run.relation("ntri").rows()
-    top = max(recs, key=lambda r: r["ms"]) if recs else None
+    top = max((r for r in recs if r["bytes"] > 0), key=lambda r: r["ms"], default=None)
     if args.kernels:
         log(json.dumps(sorted(recs, key=lambda r: -r["ms"]), indent=1))
     print(json.dumps({"config": "wcoj_triangles_rmat", "params": w.params, "tri": n, "ntri": str(ntri),

@@ -199,6 +199,12 @@ sk_status sk_run_set_dense(sk_run* run, int enabled);
  * No reference counterpart (an evaluation strategy switch, like
  * sk_run_set_dense). */
 sk_status sk_run_set_pull(sk_run* run, int mode);
+/* Device-driven rounds (enabled != 0, the default): once every delta variant
+ * of a stratum takes the word-parallel Bool pull, its remaining rounds run as
+ * a hipGraph of rounds relaunched until a device-side convergence flag is
+ * set, with no host read inside a launch.  Results and round counts are
+ * identical either way.  No reference counterpart. */
+sk_status sk_run_set_graph(sk_run* run, int enabled);
 void sk_run_error(const sk_run* run, sk_error_info* out);
 
 /* Results: rows in BTreeMap order (sorted by GroundVal tuple). */

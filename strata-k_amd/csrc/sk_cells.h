@@ -29,6 +29,18 @@ namespace sk {
 constexpr u64 kCellTop = ~0ull;
 constexpr u32 kCellTop32 = ~0u;
 
+// State of a device-driven run of delta rounds (a hipGraph of rounds that
+// the host relaunches until `done`): each round's kernels do nothing once
+// `done` is set; `stamp` is the last compaction's row stamp; `iters` counts
+// the rounds that ran.
+struct LoopState {
+  u32 done;
+  u32 stamp;
+  u64 iters;
+  u32 any;  // the round's compaction found a touched cell
+  u32 pad;
+};
+
 struct CellDesc {
   int on = 0;
   int trop = 0;
@@ -83,6 +95,12 @@ struct CellDesc {
   u32* rstamp = nullptr;
   u32 stamp = 0;
   int dwshift = 0;
+  // Device-driven rounds (cells_loop_round): with `pact` (batched runs) the
+  // round's compaction also marks the program of every row holding a delta
+  // cell: program rank = row >> pact_shift, id = pact_list[rank] (or rank).
+  u8* pact = nullptr;
+  int pact_shift = 0;
+  const u32* pact_list = nullptr;
   u32* bad = nullptr;      // set when a value falls outside its column domain
 };
 
@@ -250,6 +268,15 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
 // snapshot of that device record taken after every earlier launch.  Needs
 // ncells / (kBlock * 32) <= kScanOneMax blocks (cells_into_fits).
 bool cells_into_fits(u64 ncells);
+// One device-driven round's compaction (see LoopState): the touched cells
+// become the delta bitmap and row stamps (no delta list), then the round's
+// bookkeeping — iters += 1, stamp += 1, done when no cell was touched;
+// batched runs (nparts > 0) count the round for each program whose previous
+// delta was non-empty (active_prev) and move active_next (marked by the
+// compaction) into active_prev.  Two launches, no host read; capturable in
+// a graph.
+void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* ls, u8* active_prev,
+                      u8* active_next, u64* piters, u32 nparts);
 template <class K>
 u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                        u64* push_cost, const DevErr* status, DevErr* status_out);

@@ -448,6 +448,61 @@ u64 cells_compact(const Exec& x, const CellDesc& cd, u64 ncells, bool touched_on
   return n;
 }
 
+// Bitmap-only compaction of a device-driven round: every touched word
+// becomes the delta bitmap's word (zeros included) and is cleared; rows
+// holding delta cells get the new stamp and their program marked; one
+// atomic per wave that saw a touched word sets loop->any.
+__global__ __launch_bounds__(kBlock) void k_loop_compact(CellDesc cd, u64 nwords, LoopState* ls) {
+  if (ls->done) return;  // uniform over the launch
+  const u64 wi = (u64)blockIdx.x * kBlock + threadIdx.x;
+  u32 b = 0;
+  if (wi < nwords) {
+    b = cd.touched[wi];
+    if (b) cd.touched[wi] = 0;
+    cd.dbits[wi] = b;
+    if (b) {
+      const u32 st = ls->stamp + 1u;
+      const u64 r = wi >> cd.dwshift;
+      if (cd.rstamp[r] != st) cd.rstamp[r] = st;
+      if (cd.pact) {
+        const u64 pr = r >> cd.pact_shift;
+        const u32 pid = cd.pact_list ? cd.pact_list[pr] : (u32)pr;
+        if (!cd.pact[pid]) cd.pact[pid] = 1;
+      }
+    }
+  }
+  if (__ballot(b != 0) && (threadIdx.x & 63) == 0 && !ls->any) atomicOr(&ls->any, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_loop_book(LoopState* ls, u8* __restrict__ active_prev,
+                                                      u8* __restrict__ active_next, u64* __restrict__ piters,
+                                                      u32 nparts) {
+  if (ls->done) return;  // uniform
+  for (u32 p = threadIdx.x; p < nparts; p += kBlock) {
+    if (active_prev[p]) piters[p] += 1;
+    active_prev[p] = active_next[p];
+    active_next[p] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ls->iters += 1;
+    ls->stamp += 1;
+    if (!ls->any) ls->done = 1;
+    ls->any = 0;
+  }
+}
+
+void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* ls, u8* active_prev,
+                      u8* active_next, u64* piters, u32 nparts) {
+  const u64 nwords = (std::max<u64>(ncells, 32) + 31) / 32;
+  const u64 nblocks = (nwords + kBlock - 1) / kBlock;
+  if (!cd.dbits || nblocks > 0x7fffffffull) throw std::runtime_error("cells_loop_round: layout");
+  k_loop_compact<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, nwords, ls);
+  SK_HIP(hipGetLastError());
+  k_loop_book<<<1, kBlock, 0, x.stream>>>(ls, active_prev, active_next, piters, nparts);
+  SK_HIP(hipGetLastError());
+}
+
 bool cells_into_fits(u64 ncells) {
   const u64 nwords = (ncells + 31) / 32;
   return (nwords + kBlock - 1) / kBlock <= kScanOneMax;

@@ -1118,15 +1118,7 @@ bool Engine::try_bitpull(const RulePlan& pl) {
   DRel& E = R_[bp->E];
   if (H.delta.n == 0 || E.full.n == 0) return false;
   Exec x = ctx_->exec();
-  if (bp->ev != E.version) {
-    with_key(E.wide, [&](auto* tag) {
-      using K = std::remove_pointer_t<decltype(tag)>;
-      bp->nedges = bitpull_graph<K>(x, (const K*)E.full.k.ptr(), E.full.n, E.arity, E.bits, dn->cd, H.arity, bp->eh,
-                                    bp->ed, dn->blbits, dn->brows, &bp->graph, &bp->csr);
-    });
-    bp->nchunks = bitpull_chunks(x, bp->csr.get(), dn->brows, &bp->chunks);
-    bp->ev = E.version;
-  }
+  ensure_bitpull_graph(*bp, *dn);
   if (bp->nedges == 0) return false;
   const double W = (double)(1ull << (dn->blbits - 5));
   const double nE = (double)bp->nedges, rows = (double)std::max<u64>(dn->brows, 1);
@@ -1146,7 +1138,149 @@ bool Engine::try_bitpull(const RulePlan& pl) {
   // the active entries' delta rows, and each chunk's destination row read
   // and written once
   bitpull_apply(x, bd, dn->cd, nE * 12.0 + nE * active * 4.0 * W + (double)bp->nchunks * 8.0 * W, bp->nedges);
+  round_bitpulls_++;
   return true;
+}
+
+// The row graph and chunk list of a bit-pull rule, built once per version of
+// its fixed relation.
+void Engine::ensure_bitpull_graph(BitPullRule& bp, DenseRel& dn) {
+  DRel& E = R_[bp.E];
+  if (bp.ev == E.version) return;
+  Exec x = ctx_->exec();
+  const DRel& H = R_[P_.rules[bp.rule].head.rel];
+  with_key(E.wide, [&](auto* tag) {
+    using K = std::remove_pointer_t<decltype(tag)>;
+    bp.nedges = bitpull_graph<K>(x, (const K*)E.full.k.ptr(), E.full.n, E.arity, E.bits, dn.cd, H.arity, bp.eh, bp.ed,
+                                 dn.blbits, dn.brows, &bp.graph, &bp.csr);
+  });
+  bp.nchunks = bitpull_chunks(x, bp.csr.get(), dn.brows, &bp.chunks);
+  bp.ev = E.version;
+}
+
+// The relation whose remaining rounds may run device-driven, or -1: the
+// stratum derives one relation, kept as Bool cells, and every delta variant of the stratum's rules is one of its
+// bit-pull rules, all of which took the pull in the round just run.
+int Engine::graph_loop_rel(u32 k) {
+  if (!graph_enabled_ || pull_mode_ == 0 || round_variants_ == 0 || round_bitpulls_ != round_variants_) return -1;
+  int rel = -1;
+  for (size_t i = 0; i < R_.size(); i++)
+    if (in_idb_[i]) {
+      if (rel >= 0) return -1;
+      rel = (int)i;
+    }
+  if (rel < 0 || R_[rel].delta.n == 0) return -1;
+  DenseRel* dn = dense_of(rel);
+  if (!dn || dn->bpulls.empty() || dn->keep_full || dn->cd.trop) return -1;
+  for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+    const Rule& r = P_.rules[ri];
+    if (r.stratum != k || r.agg) continue;
+    for (size_t j = 0; j < r.body.size(); j++) {
+      const Atom& a = r.body[j];
+      if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
+      bool found = false;
+      for (auto& bp : dn->bpulls) found = found || (bp.rule == (int)ri && bp.dpos == (int)j && bp.nchunks);
+      if (!found) return -1;
+    }
+  }
+  return rel;
+}
+
+// Runs the stratum's remaining delta rounds on the device: a hipGraph of
+// kGraphRounds rounds (each: every bit pull, then the bitmap-only
+// compaction and the round's bookkeeping, see cells_loop_round) is launched until the
+// device's done flag is set, with one small read per launch.  Rounds after
+// convergence do nothing.  Round counts (per program in batched runs) are
+// the host loop's: every round that ran, including the final empty one.
+void Engine::run_graph_loop(u32 k, int rel, u64* iters, std::vector<u8>& active_prev) {
+  constexpr int kGraphRounds = 4;
+  DRel& H = R_[rel];
+  DenseRel& dn = *dense_of(rel);
+  Exec x = ctx_->exec();
+  std::vector<BitPullDesc> bds;
+  for (auto& bp : dn.bpulls) {
+    ensure_bitpull_graph(bp, dn);
+    if (!bp.nchunks) continue;
+    BitPullDesc bd;
+    bd.nchunks = bp.nchunks;
+    bd.chunks = bp.chunks.get();
+    bd.graph = bp.graph.get();
+    bd.dbits = dn.dbits.get();
+    bd.rstamp = dn.rstamp.get();
+    bd.wshift = dn.blbits - 5;
+    bds.push_back(bd);
+  }
+  const u32 np = nparts_;
+  DBuf<LoopState> ls(x.pool, 1);
+  DBuf<u8> ap(x.pool, np ? np : 1), an(x.pool, np ? np : 1);
+  DBuf<u64> pit(x.pool, np ? np : 1);
+  LoopState h0{0, dn.stamp, 0, 0, 0};
+  SK_HIP(hipMemcpyAsync(ls.get(), &h0, sizeof h0, hipMemcpyHostToDevice, x.stream));
+  if (np) {
+    if (active_prev.size() != np) active_prev.assign(np, 0);
+    SK_HIP(hipMemcpyAsync(ap.get(), active_prev.data(), np, hipMemcpyHostToDevice, x.stream));
+    memset_async(x, an.get(), 0, np);
+    memset_async(x, pit.get(), 0, np * 8);
+  }
+  SK_HIP(hipStreamSynchronize(x.stream));
+  CellDesc cdl = dn.cd;
+  if (np) {
+    cdl.pact = an.get();
+    cdl.pact_shift = cdl.shift[0] - dn.blbits;
+    cdl.pact_list = cdl.dom_list[0];
+  }
+  for (auto& bd : bds) bd.loop = ls.get();
+  Exec xg = x;
+  xg.timer = nullptr;  // no event records inside the capture
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  SK_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int r = 0; r < kGraphRounds; r++) {
+      for (auto& bd : bds) bitpull_apply(xg, bd, cdl, 0, 0);
+      cells_loop_round(xg, cdl, dn.ncells, ls.get(), ap.get(), an.get(), pit.get(), np);
+    }
+  } catch (...) {
+    hipGraph_t bad = nullptr;
+    (void)hipStreamEndCapture(x.stream, &bad);
+    if (bad) (void)hipGraphDestroy(bad);
+    throw;
+  }
+  SK_HIP(hipStreamEndCapture(x.stream, &g));
+  hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    SK_HIP(ie);
+  }
+  u64 hv[2] = {0, 0};
+  try {
+    for (;;) {
+      {
+        Scope sc(x.timer, "graph_rounds", KC_JOIN, 0.0, kGraphRounds);
+        SK_HIP(hipGraphLaunch(ge, x.stream));
+      }
+      read_u64s(x, (const u64*)ls.get(), hv, 2);
+      if ((u32)hv[0]) break;
+    }
+  } catch (...) {
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    throw;
+  }
+  SK_HIP(hipGraphExecDestroy(ge));
+  SK_HIP(hipGraphDestroy(g));
+  *iters += hv[1];
+  dn.stamp = (u32)(hv[0] >> 32);
+  if (np) {
+    std::vector<u64> pi(np);
+    SK_HIP(hipMemcpyAsync(pi.data(), pit.get(), np * 8, hipMemcpyDeviceToHost, x.stream));
+    SK_HIP(hipStreamSynchronize(x.stream));
+    for (u32 p = 0; p < np; p++) part_iters_[p][k] += pi[p];
+    active_prev.assign(np, 0);
+  }
+  H.delta.release();
+  H.delta_order.clear();
+  H.dversion++;
 }
 
 void Engine::exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out) {
@@ -2521,6 +2655,7 @@ void Engine::run_stratum(u32 k) {
   bool bootstrap = true;
   for (;;) {
     std::map<int, std::vector<Chunk>> chunks;
+    round_variants_ = round_bitpulls_ = 0;
     auto run_rules = [&]() {
       for (size_t ri = 0; ri < P_.rules.size(); ri++) {
         const Rule& r = P_.rules[ri];
@@ -2534,6 +2669,7 @@ void Engine::run_stratum(u32 k) {
           const Atom& a = r.body[j];
           if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
           if (R_[a.rel].delta.n == 0) continue;
+          round_variants_++;
           exec_plan(plan_of((int)ri, (int)j), chunks);
         }
       }
@@ -2608,6 +2744,12 @@ void Engine::run_stratum(u32 k) {
     }
     bootstrap = false;
     if (!any) break;
+    // the remaining rounds device-driven, when every delta variant pulled
+    const int grel = graph_loop_rel(k);
+    if (grel >= 0) {
+      run_graph_loop(k, grel, &iters, active_prev);
+      break;
+    }
   }
   iters_[k] = iters;
   end_dense();

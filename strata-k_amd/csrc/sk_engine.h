@@ -341,6 +341,9 @@ class Engine {
   // Destination-side (pull) rounds: 0 never, 1 when the cost model prefers
   // them (default), 2 whenever the rule's shape allows.
   void set_pull(int mode) { pull_mode_ = mode; }
+  // Device-driven rounds captured in a hipGraph for strata whose every delta
+  // variant takes the word-parallel pull (default on).
+  void set_graph(bool on) { graph_enabled_ = on; }
 
  private:
   std::vector<RelData> snap_;  // post-load state of relations written by rules
@@ -374,6 +377,8 @@ class Engine {
   std::map<std::pair<int, int>, ColCodes> colcodes_;  // column code sets, by (relation, column)
   bool dense_enabled_ = true;
   int pull_mode_ = 1;
+  bool graph_enabled_ = true;
+  int round_variants_ = 0, round_bitpulls_ = 0;  // delta variants of the current round, and those pulled
   bool narrow_ok_ = true;   // every staged Trop weight is finite-or-inf in [0, 2^32-1)
 
   void build_dict();
@@ -394,6 +399,9 @@ class Engine {
   void exec_plan(const RulePlan& pl, std::map<int, std::vector<Chunk>>& out);
   bool try_pull(const RulePlan& pl);
   bool try_bitpull(const RulePlan& pl);
+  void ensure_bitpull_graph(BitPullRule& bp, DenseRel& dn);
+  int graph_loop_rel(u32 k);
+  void run_graph_loop(u32 k, int rel, u64* iters, std::vector<u8>& active_prev);
   void plan_bitpulls(u32 k, int i, DenseRel& dn);
   u64 finalize(int rel, std::vector<Chunk>& chunks);
   const RelData& source(int rel, bool delta) { return delta ? R_[rel].delta : R_[rel].full; }

@@ -119,6 +119,7 @@ struct BitPullDesc {
   const u32* rstamp = nullptr; // per row: stamp of the last compaction that found a delta cell in it
   u32 stamp = 0;
   int wshift = 0;
+  const LoopState* loop = nullptr;  // device-driven rounds: stamp and done flag from here
 };
 
 // Row graph of a bit-pull rule from E's packed tuples: for every tuple, the

@@ -1179,13 +1179,14 @@ __device__ __forceinline__ void bit_apply(const CellDesc& cd, u64 wi, u32 acc) {
 
 // Loads entry e0 + lane's source row; returns the number of entries of the
 // batch whose source row holds delta cells, their rows compacted in `act`.
-__device__ __forceinline__ u32 bitpull_batch(const BitPullDesc& bd, u64 e0, u64 e_end, u32 lane, u32* act) {
+__device__ __forceinline__ u32 bitpull_batch(const BitPullDesc& bd, u32 stamp, u64 e0, u64 e_end, u32 lane,
+                                             u32* act) {
   const u64 e = e0 + lane;
   u32 s = 0;
   bool a = false;
   if (e < e_end) {
     s = (u32)bd.graph[e];
-    a = bd.rstamp[s] == bd.stamp;
+    a = bd.rstamp[s] == stamp;
   }
   const u64 m = __ballot(a);
   if (a) act[__popcll(m & ((1ull << lane) - 1ull))] = s;
@@ -1200,6 +1201,8 @@ __global__ __launch_bounds__(kBlock) void k_bitpull(BitPullDesc bd, CellDesc cd)
   const u32 wv = threadIdx.x >> 6;
   const u64 ch = (u64)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(wv);
   if (ch >= bd.nchunks) return;  // whole waves leave; no block barrier below
+  if (bd.loop && bd.loop->done) return;
+  const u32 stamp = bd.loop ? bd.loop->stamp : bd.stamp;
   u32* act = s_act + wv * 64;
   const BitChunk c = bd.chunks[ch];
   const u64 e_end = c.lo + c.n;
@@ -1208,7 +1211,7 @@ __global__ __launch_bounds__(kBlock) void k_bitpull(BitPullDesc bd, CellDesc cd)
     for (u32 wb = 0; wb < W; wb += 64) {
       u32 acc = 0;
       for (u64 e0 = c.lo; e0 < e_end; e0 += 64) {
-        const u32 na = bitpull_batch(bd, e0, e_end, lane, act);
+        const u32 na = bitpull_batch(bd, stamp, e0, e_end, lane, act);
         u32 k = 0;
         for (; k + 4 <= na; k += 4) {
           const u32 v0 = bd.dbits[((u64)act[k] << bd.wshift) + wb + lane];
@@ -1227,7 +1230,7 @@ __global__ __launch_bounds__(kBlock) void k_bitpull(BitPullDesc bd, CellDesc cd)
     const u32 slot = lane >> bd.wshift, word = lane & (W - 1), nslots = 64u >> bd.wshift;
     u32 acc = 0;
     for (u64 e0 = c.lo; e0 < e_end; e0 += 64) {
-      const u32 na = bitpull_batch(bd, e0, e_end, lane, act);
+      const u32 na = bitpull_batch(bd, stamp, e0, e_end, lane, act);
       for (u32 k = slot; k < na; k += nslots) acc |= bd.dbits[((u64)act[k] << bd.wshift) + word];
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();

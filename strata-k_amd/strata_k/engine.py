@@ -143,6 +143,7 @@ SIGNATURES = [
     ("sk_run_reset", C.c_int, [_VP]),
     ("sk_run_set_dense", C.c_int, [_VP, C.c_int]),
     ("sk_run_set_pull", C.c_int, [_VP, C.c_int]),
+    ("sk_run_set_graph", C.c_int, [_VP, C.c_int]),
     ("sk_run_error", None, [_VP, C.POINTER(SkErrorInfo)]),
     ("sk_result_size", _I64, [_VP, C.c_char_p]),
     ("sk_result_arity", _I64, [_VP, C.c_char_p]),
@@ -381,18 +382,21 @@ class Context:
         return Run(self, prog)
 
     def run_semi_naive(self, prog, edb: Iterable = (), bulk: Optional[dict] = None,
-                       want: Optional[Sequence[str]] = None, dense: bool = True, pull: int = 1) -> Db:
+                       want: Optional[Sequence[str]] = None, dense: bool = True, pull: int = 1,
+                       graph: bool = True) -> Db:
         """``run_semi_naive(prog, edb)`` (crates/strata-eval/src/seminaive.rs:29).
 
         ``edb``: iterable of ``(pred, tuple-of-Sym/IntV, ann)``; ``bulk``: pred ->
         ``(columns, weights, inf)`` numpy arrays of Sym ids for large inputs.
         ``dense=False`` keeps every relation on the sorted-list path; ``pull``
-        selects destination-side rounds (see ``Run.set_pull``).
+        selects destination-side rounds (see ``Run.set_pull``), ``graph``
+        device-driven rounds (see ``Run.set_graph``).
         """
         run = self.prepare(prog)
         try:
             run.set_dense(dense)
             run.set_pull(pull)
+            run.set_graph(graph)
             run.add_facts(edb)
             for p, (cols, w, inf) in (bulk or {}).items():
                 run.add_sym(p, cols, w, inf)
@@ -509,6 +513,11 @@ class Run:
         """Pull rounds: 0 never, 1 by the cost model (default), 2 whenever
         the rule's shape allows.  Results are identical in every mode."""
         self._check(self.L.sk_run_set_pull(self.h, int(mode)))
+
+    def set_graph(self, on: bool):
+        """Device-driven rounds captured in a hipGraph once every delta
+        variant takes the word-parallel pull (True, the default)."""
+        self._check(self.L.sk_run_set_graph(self.h, 1 if on else 0))
 
     @property
     def execute_ms(self) -> float:

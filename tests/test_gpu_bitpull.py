@@ -1,10 +1,13 @@
 """GPU: word-parallel Bool pull rounds (a destination row ORs the delta rows
 its fixed relation names, 32 lanes per word) against the oracle and against
 push rounds, with the pull forced off (0), chosen by the cost model (1) and
-forced on (2).  Covers rows narrower than a word (the lane-level pull), one
-word, several words and several word blocks, both linear forms and body
-orders, rows spanning two cell columns, a second recursive rule that must
-push in the same round, and batched programs (program id as a row column)."""
+forced on (2), with and without device-driven rounds (a hipGraph of
+rounds relaunched until a device-side flag says the delta is empty).
+Covers rows narrower than a word (the lane-level pull), one word, several
+words and several word blocks, both linear forms and body orders, rows
+spanning two cell columns, a second recursive rule that must push in the
+same round, and batched programs (program id as a row column) whose round
+counts differ."""
 import random
 
 import numpy as np
@@ -38,9 +41,9 @@ def random_edges(seed, n, m):
 def check_modes(ctx, prog, edb, label):
     ref = O.run(prog, edb)
     assert ref.ok, ref.status
-    for mode in (0, 1, 2):
-        db = ctx.run_semi_naive(prog, edb, pull=mode)
-        lab = f"{label}[pull={mode}]"
+    for mode, graph in ((0, True), (1, True), (1, False), (2, True), (2, False)):
+        db = ctx.run_semi_naive(prog, edb, pull=mode, graph=graph)
+        lab = f"{label}[pull={mode},graph={graph}]"
         assert db.iterations == ref.iterations, (lab, db.iterations, ref.iterations)
         for p in prog.predicates:
             assert rows(db.relation(p.name)) == oracle_rows(ref.relations[p.name]), (lab, p.name)
@@ -119,6 +122,8 @@ def test_batched_tc_matches_plain(ctx):
     for i in range(12):
         n = [20, 45, 130, 300][i % 4]
         progs.append(random_edges(500 + i, n, 2 * n + i))
+    progs.append([(i, i + 1) for i in range(150)])  # 150 rounds
+    progs.append([(i, i + 1) for i in range(60)] + [(60, 0)])  # a 61-cycle
     b = sk.Batch(ctx)
     for es in progs:
         s = np.array([a for a, _ in es], np.uint32)
