@@ -340,6 +340,59 @@ def bench_cfg5(args):
         "same_rows_and_rounds_alone_batched_and_plain": bool(exact)}), flush=True)
 
 
+def bench_sort(args):
+    """Device-resident LSD radix sort (the engine's sort primitive) of 2^26
+    packed two-column keys (44 significant bits: 22-bit ids), keys alone and
+    with i64 weights, under the context's autotuner.  Algorithmic bytes of a
+    pass: the histogram reads every key, the scatter reads and writes every
+    record."""
+    import torch
+    import strata_k as sk
+    n = 1 << args.sort_log2
+    ctx = sk.Context(0)
+    ctx.set_autotune(2)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(7)
+    out = []
+    for weighted in (False, True):
+        keys = torch.randint(0, 1 << 44, (n,), dtype=torch.int64, device="cuda:0", generator=g)
+        alt = torch.empty_like(keys)
+        w = torch.randint(1, 1000, (n,), dtype=torch.int64, device="cuda:0", generator=g) if weighted else None
+        walt = torch.empty_like(w) if weighted else None
+        src = keys.clone()
+        torch.cuda.synchronize()
+        choice = ctx.radix_choice(n, 44, weighted=weighted)
+        passes = -(-44 // choice["digit_bits"])
+        for _ in range(args.warmup):
+            keys.copy_(src)
+            torch.cuda.synchronize()
+            ctx.radix_sort_device(keys.data_ptr(), alt.data_ptr(), n, 44, w.data_ptr() if weighted else 0,
+                                  walt.data_ptr() if weighted else 0)
+            ctx.synchronize()
+        ms = []
+        for _ in range(args.steps):
+            keys.copy_(src)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            in_alt = ctx.radix_sort_device(keys.data_ptr(), alt.data_ptr(), n, 44, w.data_ptr() if weighted else 0,
+                                           walt.data_ptr() if weighted else 0)
+            ctx.synchronize()
+            ms.append((time.perf_counter() - t0) * 1e3)
+        res = alt if in_alt else keys
+        ok = bool(torch.equal(res, torch.sort(src).values))
+        t = float(np.median(ms))
+        rec = 8 + (8 if weighted else 0)
+        pass_bytes = n * 8 + 2 * rec * n
+        gbs = pass_bytes * passes / (t * 1e-3) / 1e9
+        out.append({"weighted": weighted, "n": n, "key_bits": 44, "digit_bits": choice["digit_bits"],
+                    "items_per_thread": choice["items_per_thread"], "passes": passes, "ms_per_sort": round(t, 3),
+                    "keys_per_s": round(n / (t * 1e-3), 1), "alg_gbps": round(gbs, 1),
+                    "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBPS, 4), "sorted_equals_torch": ok})
+        del keys, alt, w, walt, src
+    ctx.close()
+    print(json.dumps({"metric": "radix sort keys/s", "unit": "keys/s", "results": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,6 +409,8 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="config 5: arrivals per second")
     ap.add_argument("--max-batch", type=int, default=16, help="config 5: queries in flight")
     ap.add_argument("--plain-queries", type=int, default=8, help="config 5: queries timed as plain runs")
+    ap.add_argument("--sort", action="store_true", help="time the device radix sort primitive")
+    ap.add_argument("--sort-log2", type=int, default=26)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one known-size copy kernel (PMC counter calibration)")
     args = ap.parse_args()
@@ -365,6 +420,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+    if args.sort:
+        bench_sort(args)
+        return
     {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3, 4: bench_cfg4, 5: bench_cfg5}[args.config](args)
 
 

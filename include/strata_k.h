@@ -281,6 +281,47 @@ sk_status sk_intern_terms(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
 /* Σ_v d(v)^2 over the id-oriented simple graph (lib.rs:138-143). */
 uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint64_t n);
 
+/* ------------------------------------------------------- sort primitives */
+
+/* The reference crate's device sort helpers over host slices:
+ *   sk_sort_u64            gpu_sort_u64        (crates/strata-gpu/src/sort.rs:396)
+ *   sk_sort_unique_u64     gpu_sort_unique_u64 (sort.rs:409)
+ *   sk_sort_unique_pairs   sort_unique_pairs   (sort.rs:518)
+ *   sk_sort_min_pairs      sort_min_pairs      (sort.rs:565): per distinct
+ *                          (a, b) the minimum weight
+ * Outputs are caller buffers of n entries; *out_n receives the row count. */
+sk_status sk_sort_u64(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out);
+sk_status sk_sort_unique_u64(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out, uint64_t* out_n);
+sk_status sk_sort_unique_pairs(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n,
+                               uint32_t* out_a, uint32_t* out_b, uint64_t* out_n);
+sk_status sk_sort_min_pairs(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, const int64_t* w,
+                            uint64_t n, uint32_t* out_a, uint32_t* out_b, int64_t* out_w,
+                            uint64_t* out_n);
+/* Device-resident stable LSD radix sort of n u64 keys on their low key_bits
+ * bits, carrying optional i64 weights (w and w_alt both given or both
+ * NULL).  keys_alt / w_alt are scratch of n entries; on return the sorted
+ * data is in the alternate buffers when *result_in_alt is 1.  Runs on the
+ * context's stream and returns once it is enqueued (sk_ctx_synchronize to
+ * wait).  Counterpart of radix_sort_u64 (sort.rs:337-365). */
+sk_status sk_radix_sort_device(sk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, int64_t* w,
+                               int64_t* w_alt, uint64_t n, int key_bits, int* result_in_alt);
+
+/* Radix sort autotuner of a context (no reference counterpart).  Each sort
+ * of at least 2^16 records picks its digit width (6, 7 or 8 bits) and tile
+ * (items per thread) from one-pass timings of every candidate, measured on
+ * the device per key type, weights and size class (2^16 ... 2^24 records).
+ * mode 0: a fixed default (8-bit digits); 1 (the default): a class is timed
+ * on its first sort; 2: time every class now. */
+sk_status sk_ctx_set_autotune(sk_ctx* ctx, int mode);
+/* Pin every sort of the context to candidate c (0 .. 5: digit bits 8, 8, 7,
+ * 7, 6, 6 with 16 / 8 items per thread, half for 128-bit keys), or -1 to
+ * return to the tuner.  A testing knob. */
+sk_status sk_ctx_force_radix(sk_ctx* ctx, int candidate);
+/* The configuration a sort of n records with key_bits significant bits would
+ * take now; *est_ms the tuner's estimate (-1 when the class is untimed). */
+sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits,
+                              int* digit_bits, int* items, double* est_ms);
+
 /* ------------------------------------------------------------ batched runs */
 
 /* Many independent programs evaluated together.  Programs whose Core-IR is

@@ -215,6 +215,171 @@ sk_status sk_intern_terms(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_
   return SK_OK;
 }
 
+// ------------------------------------------------------- sort primitives
+//
+// The reference crate's device sort helpers over host slices
+// (crates/strata-gpu/src/sort.rs:396-413, 518-600), on the LSD radix sort,
+// run collapse (Bool: keep one; weighted: keep the minimum) and the
+// context's radix autotuner.
+
+}  // extern "C"
+
+namespace {
+template <class F>
+sk_status sort_guard(sk_ctx* ctx, F f) {
+  try {
+    (void)hipSetDevice(ctx->c.device);
+    f(ctx->c);
+  } catch (const DeviceError&) {
+    return SK_ERR_DEVICE;
+  } catch (const BudgetError&) {
+    return SK_ERR_OUT_OF_BUDGET;
+  } catch (const std::bad_alloc&) {
+    return SK_ERR_OUT_OF_BUDGET;
+  }
+  return SK_OK;
+}
+
+int bits_needed(const uint64_t* k, uint64_t n) {
+  uint64_t o = 0;
+  for (uint64_t i = 0; i < n; i++) o |= k[i];
+  int b = 0;
+  while (b < 64 && (o >> b)) b++;
+  return b;
+}
+
+// sort (+ unique) of packed keys, weights optional; returns the row count
+u64 sort_keys(Ctx& c, const uint64_t* keys, const int64_t* w, uint64_t n, bool unique, uint64_t* ok, int64_t* ow) {
+  Exec x = c.exec();
+  DBuf<u64> a(c.pool.get(), n), b(c.pool.get(), n), u(c.pool.get(), unique ? n : 1);
+  DBuf<i64> wa(c.pool.get(), w ? n : 1), wb(c.pool.get(), w ? n : 1), wu(c.pool.get(), (w && unique) ? n : 1);
+  SK_HIP(hipMemcpyAsync(a.get(), keys, n * 8, hipMemcpyHostToDevice, c.stream));
+  if (w) SK_HIP(hipMemcpyAsync(wa.get(), w, n * 8, hipMemcpyHostToDevice, c.stream));
+  u64* sk;
+  i64* sw;
+  u8* sf;
+  radix_sort<u64>(x, a.get(), b.get(), w ? wa.get() : nullptr, w ? wb.get() : nullptr, nullptr, nullptr, n,
+                  bits_needed(keys, n), &sk, &sw, &sf);
+  u64 m = n;
+  if (unique) {
+    m = unique_runs<u64>(x, sk, w ? sw : nullptr, nullptr, n, u.get(), w ? wu.get() : nullptr, nullptr);
+    sk = u.get();
+    sw = wu.get();
+  }
+  SK_HIP(hipMemcpyAsync(ok, sk, m * 8, hipMemcpyDeviceToHost, c.stream));
+  if (w && ow) SK_HIP(hipMemcpyAsync(ow, sw, m * 8, hipMemcpyDeviceToHost, c.stream));
+  SK_HIP(hipStreamSynchronize(c.stream));
+  return m;
+}
+}  // namespace
+
+extern "C" {
+
+sk_status sk_sort_u64(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out) {
+  if (!ctx || (n && (!keys || !out))) return SK_ERR_INVALID_ARGUMENT;
+  if (n == 0) return SK_OK;
+  return sort_guard(ctx, [&](Ctx& c) { sort_keys(c, keys, nullptr, n, false, out, nullptr); });
+}
+
+sk_status sk_sort_unique_u64(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint64_t* out, uint64_t* out_n) {
+  if (!ctx || !out_n || (n && (!keys || !out))) return SK_ERR_INVALID_ARGUMENT;
+  *out_n = 0;
+  if (n == 0) return SK_OK;
+  return sort_guard(ctx, [&](Ctx& c) { *out_n = sort_keys(c, keys, nullptr, n, true, out, nullptr); });
+}
+
+sk_status sk_sort_unique_pairs(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, uint32_t* out_a,
+                               uint32_t* out_b, uint64_t* out_n) {
+  if (!ctx || !out_n || (n && (!a || !b || !out_a || !out_b))) return SK_ERR_INVALID_ARGUMENT;
+  *out_n = 0;
+  if (n == 0) return SK_OK;
+  std::vector<uint64_t> k(n), o(n);
+  for (uint64_t i = 0; i < n; i++) k[i] = ((uint64_t)a[i] << 32) | b[i];
+  sk_status s = sort_guard(ctx, [&](Ctx& c) { *out_n = sort_keys(c, k.data(), nullptr, n, true, o.data(), nullptr); });
+  for (uint64_t i = 0; i < *out_n; i++) {
+    out_a[i] = (uint32_t)(o[i] >> 32);
+    out_b[i] = (uint32_t)o[i];
+  }
+  return s;
+}
+
+sk_status sk_sort_min_pairs(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, const int64_t* w, uint64_t n,
+                            uint32_t* out_a, uint32_t* out_b, int64_t* out_w, uint64_t* out_n) {
+  if (!ctx || !out_n || (n && (!a || !b || !w || !out_a || !out_b || !out_w))) return SK_ERR_INVALID_ARGUMENT;
+  *out_n = 0;
+  if (n == 0) return SK_OK;
+  std::vector<uint64_t> k(n), o(n);
+  for (uint64_t i = 0; i < n; i++) k[i] = ((uint64_t)a[i] << 32) | b[i];
+  sk_status s = sort_guard(ctx, [&](Ctx& c) { *out_n = sort_keys(c, k.data(), w, n, true, o.data(), out_w); });
+  for (uint64_t i = 0; i < *out_n; i++) {
+    out_a[i] = (uint32_t)(o[i] >> 32);
+    out_b[i] = (uint32_t)o[i];
+  }
+  return s;
+}
+
+sk_status sk_radix_sort_device(sk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, int64_t* w, int64_t* w_alt,
+                               uint64_t n, int key_bits, int* result_in_alt) {
+  if (!ctx || !result_in_alt || (n && (!keys || !keys_alt)) || (!w != !w_alt) || key_bits < 0 || key_bits > 64)
+    return SK_ERR_INVALID_ARGUMENT;
+  *result_in_alt = 0;
+  return sort_guard(ctx, [&](Ctx& c) {
+    Exec x = c.exec();
+    u64* sk;
+    i64* sw;
+    u8* sf;
+    radix_sort<u64>(x, keys, keys_alt, w, w_alt, nullptr, nullptr, n, key_bits, &sk, &sw, &sf);
+    *result_in_alt = sk == keys_alt ? 1 : 0;
+  });
+}
+
+sk_status sk_ctx_set_autotune(sk_ctx* ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return SK_ERR_INVALID_ARGUMENT;
+  RadixTune& t = ctx->c.tune;
+  t.forced = mode == 0 ? 0 : -1;
+  t.lazy = mode != 0;
+  if (mode == 2)
+    return sort_guard(ctx, [&](Ctx& c) {
+      Exec x = c.exec();
+      radix_autotune(x, &t);
+    });
+  return SK_OK;
+}
+
+sk_status sk_ctx_force_radix(sk_ctx* ctx, int candidate) {
+  if (!ctx || candidate < -1 || candidate >= radix_candidates()) return SK_ERR_INVALID_ARGUMENT;
+  ctx->c.tune.forced = candidate;
+  return SK_OK;
+}
+
+sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits, int* digit_bits,
+                              int* items, double* est_ms) {
+  if (!ctx || !digit_bits || !items || key_bits <= 0 || key_bits > 128) return SK_ERR_INVALID_ARGUMENT;
+  RadixTune& t = ctx->c.tune;
+  int cls = 0;
+  while (cls < RadixTune::kClasses - 1 && (n >> (RadixTune::kMinLog2 + 2 * cls + 1)) > 0) cls++;
+  const int kw = wide ? 1 : 0, hw = weighted ? 1 : 0;
+  int best = 0;
+  double bt = -1;
+  if (t.forced < 0 && t.tuned[kw][hw][cls] && n >= (1ull << RadixTune::kMinLog2)) {
+    for (int c = 0; c < radix_candidates(); c++) {
+      const int D = radix_candidate_digit(c);
+      const double tt = ((key_bits + D - 1) / D) * (double)t.pass_ms[kw][hw][cls][c] *
+                        ((double)n / (double)(1ull << (RadixTune::kMinLog2 + 2 * cls)));
+      if (bt < 0 || tt < bt) {
+        bt = tt;
+        best = c;
+      }
+    }
+  } else if (t.forced > 0) {
+    best = t.forced;
+  }
+  *digit_bits = radix_candidate_digit(best);
+  *items = radix_candidate_items(best, wide != 0);
+  if (est_ms) *est_ms = bt;
+  return SK_OK;
+}
+
 uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint64_t n) {
   std::vector<uint64_t> k = orient_by_id(a, b, n);
   uint64_t s = 0, run = 0;

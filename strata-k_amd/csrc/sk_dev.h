@@ -188,6 +188,21 @@ struct Scope {
   }
 };
 
+// Radix sort autotuner state of a context: per key type (u64 / u128), with
+// or without weights, and size class (2^16, 2^18, ..., 2^24 records), the
+// measured time of one pass of every candidate configuration (digit bits x
+// items per thread).  A sort takes the candidate with the least
+// passes x time for its key width; a class is timed on first use (`lazy`)
+// or all at once by radix_autotune.  `forced` >= 0 pins one candidate.
+struct RadixTune {
+  static constexpr int kClasses = 5, kMinLog2 = 16, kMaxCands = 8;
+  bool lazy = true;
+  int forced = -1;
+  bool tuned[2][2][kClasses] = {};
+  float pass_ms[2][2][kClasses][kMaxCands] = {};
+  int chosen[2][2][kClasses] = {};
+};
+
 // Execution context handed to every primitive.
 struct Exec {
   Pool* pool;
@@ -196,6 +211,7 @@ struct Exec {
   DevErr* err;  // device error record
   u64* host = nullptr;          // pinned words for small device-to-host reads
   hipEvent_t read_ev = nullptr;  // polled behind those reads
+  RadixTune* tune = nullptr;    // radix configuration choices (null: the default)
 };
 
 // ------------------------------------------------------ primitive wrappers
@@ -220,6 +236,12 @@ void scan_u32_one(const Exec& x, const u32* in, u64* out, u64 n, u64* total, con
 template <class K>
 void radix_sort(const Exec& x, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt, u64 n,
                 int key_bits, K** out_keys, i64** out_w, u8** out_inf, int lo_bit = 0);
+
+// Time every radix candidate for every size class now (see RadixTune).
+void radix_autotune(const Exec& x, RadixTune* t);
+int radix_candidates();
+int radix_candidate_digit(int c);
+int radix_candidate_items(int c, bool wide);
 
 // Collapse runs of equal keys of a sorted array.  Bool: keep one.  With
 // weights: keep the minimum (+inf above every finite).  Returns the count.

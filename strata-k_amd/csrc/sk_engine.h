@@ -280,7 +280,8 @@ struct Ctx {
   u64* host_words = nullptr;
   hipEvent_t read_ev = nullptr;
   EvalErr last;
-  Exec exec() { return Exec{pool.get(), stream, &timer, derr, host_words, read_ev}; }
+  RadixTune tune;
+  Exec exec() { return Exec{pool.get(), stream, &timer, derr, host_words, read_ev, &tune}; }
   void init(int dev, u64 budget);  // throws DeviceError
   void release();
 };

@@ -249,30 +249,23 @@ void scan_u8(const Exec& x, const u8* in, u64* out, u64 n, u64* t) { scan_impl<u
 
 // =============================================================== radix sort
 //
-// One LSD pass over 8-bit digit `shift`:
+// One LSD pass over a D-bit digit at `shift` (D = 6, 7 or 8; 2^D <= kBlock
+// buckets):
 //   k_radix_hist     per-block digit histogram (wave ballot match + LDS)
 //   scan             exclusive scan of counts in digit-major order
-//   k_radix_scatter  per tile: stable in-wave ranking by ballot match,
-//                    tile-local counting sort in LDS, then coalesced runs
-//                    written to the digit's global position.
-// Each block owns a contiguous chunk, so the pass is stable.
+//   k_radix_scatter  per tile of kBlock * IT records: stable in-wave ranking
+//                    by ballot match, tile-local counting sort in LDS, then
+//                    coalesced runs written to the digit's global position.
+// Each block owns a contiguous chunk of whole tiles, so the pass is stable.
+// The digit width, the items per thread and the grid cap of each sort come
+// from the context's autotuner (RadixTune), or a default when it has none.
 
-template <class K>
-struct RadixCfg;
-template <>
-struct RadixCfg<u64> {
-  static constexpr int kItems = 16;
-};
-template <>
-struct RadixCfg<u128> {
-  static constexpr int kItems = 8;
-};
-
-// Lanes of the wave (among `valid` lanes) holding the same 8-bit digit.
+// Lanes of the wave (among `valid` lanes) holding the same D-bit digit.
+template <int D>
 __device__ __forceinline__ u64 match_digit(u32 d, bool valid) {
   u64 m = __ballot(valid);
 #pragma unroll
-  for (int b = 0; b < 8; b++) {
+  for (int b = 0; b < D; b++) {
     bool bit = (d >> b) & 1u;
     u64 bb = __ballot(bit);
     m &= bit ? bb : ~bb;
@@ -280,28 +273,29 @@ __device__ __forceinline__ u64 match_digit(u32 d, bool valid) {
   return m;
 }
 
-template <class K>
+template <class K, int D>
 __device__ __forceinline__ u32 digit_of(K k, int shift) {
-  return (u32)(k >> shift) & 0xffu;
+  return (u32)(k >> shift) & ((1u << D) - 1u);
 }
 
-template <class K>
+template <class K, int D>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, u64 n, u64 chunk,
                                                        int shift, u32* __restrict__ counts, u32 G) {
-  __shared__ u32 h[kWaves][256];
-  for (u32 i = threadIdx.x; i < kWaves * 256; i += kBlock) (&h[0][0])[i] = 0;
+  constexpr u32 NB = 1u << D;
+  __shared__ u32 h[kWaves][NB];
+  for (u32 i = threadIdx.x; i < kWaves * NB; i += kBlock) (&h[0][0])[i] = 0;
   __syncthreads();
   u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
   u32 w = wave_id();
   for (u64 base = lo; base < hi; base += kBlock) {
     u64 i = base + threadIdx.x;
     bool valid = i < hi;
-    u32 d = valid ? digit_of<K>(keys[i], shift) : 0;
-    u64 m = match_digit(d, valid);
+    u32 d = valid ? digit_of<K, D>(keys[i], shift) : 0;
+    u64 m = match_digit<D>(d, valid);
     if (valid && (int)lane_id() == (m ? (int)__builtin_ctzll(m) : -1)) atomicAdd(&h[w][d], (u32)__popcll(m));
   }
   __syncthreads();
-  for (u32 d = threadIdx.x; d < 256; d += kBlock) {
+  for (u32 d = threadIdx.x; d < NB; d += kBlock) {
     u32 s = 0;
 #pragma unroll
     for (int q = 0; q < kWaves; q++) s += h[q][d];
@@ -309,29 +303,30 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
   }
 }
 
-template <class K, bool HASW>
+template <class K, bool HASW, int D, int IT>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const K* __restrict__ kin, K* __restrict__ kout, const i64* __restrict__ win, i64* __restrict__ wout,
     const u8* __restrict__ fin, u8* __restrict__ fout, u64 n, u64 chunk, int shift,
     const u64* __restrict__ offs, u32 G) {
-  constexpr int IT = RadixCfg<K>::kItems;
+  constexpr u32 NB = 1u << D;
+  static_assert(NB <= (u32)kBlock, "one thread per bucket");
   constexpr int TILE = kBlock * IT;
   __shared__ K s_key[TILE];
   __shared__ i64 s_w[HASW ? TILE : 1];
   __shared__ u8 s_f[HASW ? TILE : 1];
-  __shared__ u32 s_wcnt[kWaves][256];
-  __shared__ u32 s_start[256];
-  __shared__ u32 s_tot[256];
-  __shared__ u64 s_gbase[256];
+  __shared__ u32 s_wcnt[kWaves][NB];
+  __shared__ u32 s_start[NB];
+  __shared__ u32 s_tot[NB];
+  __shared__ u64 s_gbase[NB];
   __shared__ u64 tmp[kWaves];
 
   const u32 tid = threadIdx.x, lane = lane_id(), wv = wave_id();
   const u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
-  for (u32 d = tid; d < 256; d += kBlock) s_gbase[d] = offs[(u64)d * G + blockIdx.x];
+  for (u32 d = tid; d < NB; d += kBlock) s_gbase[d] = offs[(u64)d * G + blockIdx.x];
   const bool has_f = HASW && fin != nullptr;
 
   for (u64 tb = lo; tb < hi; tb += TILE) {
-    for (u32 i = tid; i < kWaves * 256; i += kBlock) (&s_wcnt[0][0])[i] = 0;
+    for (u32 i = tid; i < kWaves * NB; i += kBlock) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     K key[IT];
     i64 wt[HASW ? IT : 1];
@@ -347,8 +342,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
         wt[it] = valid ? win[idx] : 0;
         fl[it] = (valid && has_f) ? fin[idx] : (u8)0;
       }
-      u32 d = digit_of<K>(key[it], shift);
-      u64 m = match_digit(d, valid);
+      u32 d = digit_of<K, D>(key[it], shift);
+      u64 m = match_digit<D>(d, valid);
       int leader = (m ? (int)__builtin_ctzll(m) : -1);
       u32 old = 0;
       if (valid && (int)lane == leader) old = atomicAdd(&s_wcnt[wv][d], (u32)__popcll(m));
@@ -356,29 +351,29 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
       rk[it] = old + (u32)__popcll(m & lanemask_lt());
     }
     __syncthreads();
-    // digit totals of the tile and per-wave offsets
+    // digit totals of the tile and per-wave offsets (threads past the last
+    // bucket contribute nothing to the scan)
     u32 dsum = 0;
-    {
-      u32 d = tid;  // kBlock == 256 digits
+    if (tid < NB) {
       u32 run = 0;
 #pragma unroll
       for (int q = 0; q < kWaves; q++) {
-        u32 c = s_wcnt[q][d];
-        s_wcnt[q][d] = run;
+        u32 c = s_wcnt[q][tid];
+        s_wcnt[q][tid] = run;
         run += c;
       }
       dsum = run;
-      s_tot[d] = run;
+      s_tot[tid] = run;
     }
     u64 ttot;
     u64 st = block_excl_scan((u64)dsum, tmp, &ttot);
-    s_start[tid] = (u32)st;
+    if (tid < NB) s_start[tid] = (u32)st;
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IT; it++) {
       u64 idx = wb + (u64)it * 64 + lane;
       if (idx < hi) {
-        u32 d = digit_of<K>(key[it], shift);
+        u32 d = digit_of<K, D>(key[it], shift);
         u32 pos = s_start[d] + s_wcnt[wv][d] + rk[it];
         s_key[pos] = key[it];
         if constexpr (HASW) {
@@ -391,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const u32 tn = (u32)min((u64)TILE, hi - tb);
     for (u32 p = tid; p < tn; p += kBlock) {
       K k = s_key[p];
-      u32 d = digit_of<K>(k, shift);
+      u32 d = digit_of<K, D>(k, shift);
       u64 o = s_gbase[d] + (p - s_start[d]);
       kout[o] = k;
       if constexpr (HASW) {
@@ -400,9 +395,183 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
       }
     }
     __syncthreads();
-    s_gbase[tid] += s_tot[tid];
+    if (tid < NB) s_gbase[tid] += s_tot[tid];
     __syncthreads();
   }
+}
+
+// The candidate configurations the tuner times: digit bits x items per thread
+// (u128 keys at 16 items would not fit LDS with weights, so they use 4 / 8).
+struct RadixCand {
+  int digit, items;
+};
+constexpr RadixCand kRadixCands[] = {{8, 16}, {8, 8}, {7, 16}, {7, 8}, {6, 16}, {6, 8}};
+constexpr int kNCands = sizeof(kRadixCands) / sizeof(kRadixCands[0]);
+static_assert(kNCands <= RadixTune::kMaxCands, "tuner table width");
+
+template <class K>
+constexpr int items_for(int c) {
+  return sizeof(K) == 16 ? kRadixCands[c].items / 2 : kRadixCands[c].items;
+}
+
+// Algorithmic bytes: the histogram reads every key; the scatter reads and
+// writes every record (rec bytes).
+template <class K, bool HASW, int D, int IT>
+static void radix_pass(const Exec& x, const K* a, K* b, const i64* wa, i64* wb, const u8* fa, u8* fb, u64 n, u64 G,
+                       u64 chunk, int shift, u32* counts, u64* offs, double rec) {
+  constexpr u64 NB = 1ull << D;
+  {
+    Scope sc(x.timer, "radix_hist", KC_SORT, (double)n * sizeof(K), n);
+    k_radix_hist<K, D><<<(unsigned)G, kBlock, 0, x.stream>>>(a, n, chunk, shift, counts, (u32)G);
+    SK_HIP(hipGetLastError());
+  }
+  scan_u32(x, counts, offs, NB * G, nullptr);
+  Scope sc(x.timer, "radix_scatter", KC_SORT, 2.0 * rec * (double)n, n);
+  k_radix_scatter<K, HASW, D, IT><<<(unsigned)G, kBlock, 0, x.stream>>>(a, b, wa, wb, fa, fb, n, chunk, shift, offs,
+                                                                        (u32)G);
+  SK_HIP(hipGetLastError());
+}
+
+template <class K, bool HASW>
+static void radix_pass_c(int c, const Exec& x, const K* a, K* b, const i64* wa, i64* wb, const u8* fa, u8* fb, u64 n,
+                         u64 G, u64 chunk, int shift, u32* counts, u64* offs, double rec) {
+  switch (c) {
+#define SK_RADIX_CASE(I)                                                                                       \
+  case I:                                                                                                      \
+    radix_pass<K, HASW, kRadixCands[I].digit, items_for<K>(I)>(x, a, b, wa, wb, fa, fb, n, G, chunk, shift,  \
+                                                               counts, offs, rec);                             \
+    break;
+    SK_RADIX_CASE(0)
+    SK_RADIX_CASE(1)
+    SK_RADIX_CASE(2)
+    SK_RADIX_CASE(3)
+    SK_RADIX_CASE(4)
+    SK_RADIX_CASE(5)
+#undef SK_RADIX_CASE
+    default:
+      throw std::runtime_error("radix: configuration");
+  }
+}
+
+static int size_class(u64 n) {
+  int c = 0;
+  while (c < RadixTune::kClasses - 1 && (n >> (RadixTune::kMinLog2 + 2 * c + 1)) > 0) c++;
+  return c;
+}
+
+// Grid of a pass: chunks of whole tiles, at most `gcap` blocks.
+static void radix_grid(u64 n, u64 tile, u64 gcap, u64* G, u64* chunk) {
+  u64 g = (n + tile - 1) / tile;
+  if (g > gcap) g = gcap;
+  u64 ch = (n + g - 1) / g;
+  ch = (ch + tile - 1) / tile * tile;
+  *chunk = ch;
+  *G = (n + ch - 1) / ch;
+}
+
+template <class K>
+static void radix_run(const Exec& x, int c, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt, u64 n,
+                      int key_bits, int lo_bit, K** out_keys, i64** out_w, u8** out_inf) {
+  const int D = kRadixCands[c].digit;
+  const u64 tile = (u64)kBlock * items_for<K>(c);
+  const bool hasw = (w != nullptr);
+  const int passes = (key_bits + D - 1) / D;
+  const double rec = sizeof(K) + (hasw ? 8.0 : 0) + (inf ? 1.0 : 0);
+  u64 G, chunk;
+  radix_grid(n, tile, 2048, &G, &chunk);
+  DBuf<u32> counts(x.pool, ((u64)1 << D) * G);
+  DBuf<u64> offs(x.pool, ((u64)1 << D) * G);
+  K* a = keys;
+  K* b = keys_alt;
+  i64* wa = w;
+  i64* wb = w_alt;
+  u8* fa = inf;
+  u8* fb = inf_alt;
+  for (int p = 0; p < passes; p++) {
+    const int shift = lo_bit + D * p;
+    if (hasw) radix_pass_c<K, true>(c, x, a, b, wa, wb, fa, fb, n, G, chunk, shift, counts.get(), offs.get(), rec);
+    else radix_pass_c<K, false>(c, x, a, b, nullptr, nullptr, nullptr, nullptr, n, G, chunk, shift, counts.get(),
+                                offs.get(), rec);
+    std::swap(a, b);
+    std::swap(wa, wb);
+    std::swap(fa, fb);
+  }
+  *out_keys = a;
+  if (out_w) *out_w = wa;
+  if (out_inf) *out_inf = fa;
+}
+
+__global__ void k_tune_keys(u64* k, u64 n, u64 seed) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u64 z = seed + i * 0x9e3779b97f4a7c15ull;  // splitmix64
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  k[i] = z ^ (z >> 31);
+}
+
+// Times one pass of every candidate on random keys of the class's size
+// (median of three, HIP events on the context's stream) and records the
+// per-pass times; a sort then takes the candidate with the least
+// passes x time for its key width.
+template <class K>
+static void radix_tune_class(const Exec& x, RadixTune* t, bool hasw, int cls) {
+  const u64 n = 1ull << (RadixTune::kMinLog2 + 2 * cls);
+  const u64 words = n * sizeof(K) / 8;
+  DBuf<u64> k1(x.pool, words), k2(x.pool, words);
+  DBuf<i64> w1(x.pool, hasw ? n : 1), w2(x.pool, hasw ? n : 1);
+  k_tune_keys<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(k1.get(), words, 0x5eed + cls);
+  if (hasw) k_tune_keys<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>((u64*)w1.get(), n, 77);
+  SK_HIP(hipGetLastError());
+  hipEvent_t e0, e1;
+  SK_HIP(hipEventCreate(&e0));
+  SK_HIP(hipEventCreate(&e1));
+  Exec xq = x;
+  xq.timer = nullptr;
+  for (int c = 0; c < kNCands; c++) {
+    float ms[3];
+    for (int r = 0; r < 3; r++) {
+      K* ok;
+      i64* ow;
+      u8* of;
+      SK_HIP(hipEventRecord(e0, x.stream));
+      radix_run<K>(xq, c, (K*)k1.get(), (K*)k2.get(), hasw ? w1.get() : nullptr, hasw ? w2.get() : nullptr, nullptr,
+                   nullptr, n, kRadixCands[c].digit, 0, &ok, &ow, &of);
+      SK_HIP(hipEventRecord(e1, x.stream));
+      SK_HIP(hipEventSynchronize(e1));
+      SK_HIP(hipEventElapsedTime(&ms[r], e0, e1));
+    }
+    std::sort(ms, ms + 3);
+    t->pass_ms[sizeof(K) == 16][hasw][cls][c] = ms[1];
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t->tuned[sizeof(K) == 16][hasw][cls] = true;
+}
+
+template <class K>
+static int radix_choose(const Exec& x, u64 n, int key_bits, bool hasw) {
+  RadixTune* t = x.tune;
+  if (!t || n < (1ull << RadixTune::kMinLog2)) return t && t->forced >= 0 ? t->forced : 0;
+  if (t->forced >= 0) return t->forced;
+  const int cls = size_class(n);
+  const int kw = sizeof(K) == 16;
+  if (!t->tuned[kw][hasw][cls]) {
+    if (!t->lazy) return 0;
+    radix_tune_class<K>(x, t, hasw, cls);
+  }
+  int best = 0;
+  double bt = 1e30;
+  for (int c = 0; c < kNCands; c++) {
+    const int D = kRadixCands[c].digit;
+    const double tt = ((key_bits + D - 1) / D) * (double)t->pass_ms[kw][hasw][cls][c];
+    if (tt < bt) {
+      bt = tt;
+      best = c;
+    }
+  }
+  t->chosen[kw][hasw][cls] = best;
+  return best;
 }
 
 template <class K>
@@ -412,48 +581,24 @@ void radix_sort(const Exec& x, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf
   if (out_w) *out_w = w;
   if (out_inf) *out_inf = inf;
   if (n <= 1 || key_bits <= 0) return;
-  constexpr u64 TILE = (u64)kBlock * RadixCfg<K>::kItems;
-  const bool hasw = (w != nullptr);
-  int passes = (key_bits + 7) / 8;
-  const double rec = sizeof(K) + (hasw ? 8.0 : 0) + (inf ? 1.0 : 0);
-  u64 G = (n + TILE - 1) / TILE;
-  u64 maxG = 2048;
-  if (G > maxG) G = maxG;
-  u64 chunk = (n + G - 1) / G;
-  chunk = (chunk + TILE - 1) / TILE * TILE;
-  G = (n + chunk - 1) / chunk;
-  DBuf<u32> counts(x.pool, 256 * G);
-  DBuf<u64> offs(x.pool, 256 * G);
-  K* a = keys;
-  K* b = keys_alt;
-  i64* wa = w;
-  i64* wb = w_alt;
-  u8* fa = inf;
-  u8* fb = inf_alt;
-  for (int p = 0; p < passes; p++) {
-    int shift = lo_bit + 8 * p;
-    {
-      Scope sc(x.timer, "radix_hist", KC_SORT, (double)n * sizeof(K), n);
-      k_radix_hist<K><<<(unsigned)G, kBlock, 0, x.stream>>>(a, n, chunk, shift, counts.get(), (u32)G);
-    }
-    scan_u32(x, counts.get(), offs.get(), 256 * G, nullptr);
-    // algorithmic bytes of one scatter pass: read + write every record
-    Scope sc(x.timer, "radix_scatter", KC_SORT, 2.0 * rec * (double)n, n);
-    if (hasw)
-      k_radix_scatter<K, true><<<(unsigned)G, kBlock, 0, x.stream>>>(a, b, wa, wb, fa, fb, n, chunk, shift,
-                                                                     offs.get(), (u32)G);
-    else
-      k_radix_scatter<K, false><<<(unsigned)G, kBlock, 0, x.stream>>>(a, b, nullptr, nullptr, nullptr, nullptr,
-                                                                      n, chunk, shift, offs.get(), (u32)G);
-    SK_HIP(hipGetLastError());
-    std::swap(a, b);
-    std::swap(wa, wb);
-    std::swap(fa, fb);
-  }
-  *out_keys = a;
-  if (out_w) *out_w = wa;
-  if (out_inf) *out_inf = fa;
+  const int c = radix_choose<K>(x, n, key_bits, w != nullptr);
+  radix_run<K>(x, c, keys, keys_alt, w, w_alt, inf, inf_alt, n, key_bits, lo_bit, out_keys, out_w, out_inf);
 }
+
+void radix_autotune(const Exec& x, RadixTune* t) {
+  for (int cls = 0; cls < RadixTune::kClasses; cls++)
+    for (int hasw = 0; hasw < 2; hasw++) {
+      radix_tune_class<u64>(x, t, hasw != 0, cls);
+      radix_tune_class<u128>(x, t, hasw != 0, cls);
+    }
+}
+
+int radix_candidate_digit(int c) { return c >= 0 && c < kNCands ? kRadixCands[c].digit : 0; }
+int radix_candidate_items(int c, bool wide) {
+  if (c < 0 || c >= kNCands) return 0;
+  return wide ? items_for<u128>(c) : items_for<u64>(c);
+}
+int radix_candidates() { return kNCands; }
 
 // ============================================================ run collapse
 

@@ -178,6 +178,15 @@ SIGNATURES = [
     ("sk_batch_result_size", _I64, [_VP, _U32, C.c_char_p]),
     ("sk_batch_result_read_sym", C.c_int, [_VP, _U32, C.c_char_p, _VP, _VP, _VP]),
     ("sk_batch_execute_ms", C.c_double, [_VP]),
+    ("sk_sort_u64", C.c_int, [_VP, _VP, _U64, _VP]),
+    ("sk_sort_unique_u64", C.c_int, [_VP, _VP, _U64, _VP, C.POINTER(_U64)]),
+    ("sk_sort_unique_pairs", C.c_int, [_VP, _VP, _VP, _U64, _VP, _VP, C.POINTER(_U64)]),
+    ("sk_sort_min_pairs", C.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _VP, C.POINTER(_U64)]),
+    ("sk_radix_sort_device", C.c_int, [_VP, _VP, _VP, _VP, _VP, _U64, C.c_int, C.POINTER(C.c_int)]),
+    ("sk_ctx_set_autotune", C.c_int, [_VP, C.c_int]),
+    ("sk_ctx_force_radix", C.c_int, [_VP, C.c_int]),
+    ("sk_ctx_radix_choice", C.c_int, [_VP, C.c_int, C.c_int, _U64, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("sk_batch_kernel_count", _U32, [_VP]),
     ("sk_batch_kernel_record", C.c_int, [_VP, _U32, C.c_char_p, C.c_size_t, C.POINTER(_U32),
                                         C.POINTER(SkKernelStats)]),
@@ -376,6 +385,39 @@ class Context:
     @property
     def device_peak_bytes(self) -> int:
         return int(self.L.sk_ctx_device_peak(self.h))
+
+    # ----------------------------------------------------- radix autotuner
+    def set_autotune(self, mode: int):
+        """0: fixed 8-bit digits; 1 (default): time a size class on its first
+        sort; 2: time every class now."""
+        code = self.L.sk_ctx_set_autotune(self.h, int(mode))
+        if code:
+            _raise(code)
+
+    def force_radix(self, candidate: int):
+        """Pin every sort to one candidate configuration (-1: the tuner)."""
+        code = self.L.sk_ctx_force_radix(self.h, int(candidate))
+        if code:
+            _raise(code)
+
+    def radix_choice(self, n: int, key_bits: int, wide: bool = False, weighted: bool = False) -> dict:
+        d, it, est = C.c_int(), C.c_int(), C.c_double()
+        code = self.L.sk_ctx_radix_choice(self.h, 1 if wide else 0, 1 if weighted else 0, int(n), int(key_bits),
+                                          C.byref(d), C.byref(it), C.byref(est))
+        if code:
+            _raise(code)
+        return {"digit_bits": d.value, "items_per_thread": it.value, "est_ms": est.value}
+
+    def radix_sort_device(self, keys_ptr: int, alt_ptr: int, n: int, key_bits: int, w_ptr: int = 0,
+                          walt_ptr: int = 0) -> bool:
+        """Sort device-resident u64 keys (and i64 weights) on the context's
+        stream; True when the result is in the alternate buffers."""
+        flag = C.c_int()
+        code = self.L.sk_radix_sort_device(self.h, keys_ptr, alt_ptr, w_ptr or None, walt_ptr or None, int(n),
+                                           int(key_bits), C.byref(flag))
+        if code:
+            _raise(code)
+        return bool(flag.value)
 
     # ------------------------------------------------------------ staged run
     def prepare(self, prog) -> "Run":
@@ -688,6 +730,62 @@ def intern_terms(keys, ctx: Optional[Context] = None) -> List[int]:
     if code:
         _raise(code)
     return out[:len(k)].tolist()
+
+
+def sort_u64(keys, ctx: Optional[Context] = None) -> np.ndarray:
+    """gpu_sort_u64 (crates/strata-gpu/src/sort.rs:396): the keys in ascending order."""
+    ctx = ctx or default_context()
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+    out = np.empty(max(1, len(k)), dtype=np.uint64)
+    code = ctx.L.sk_sort_u64(ctx.h, k.ctypes.data, len(k), out.ctypes.data)
+    if code:
+        _raise(code)
+    return out[:len(k)]
+
+
+def sort_unique_u64(keys, ctx: Optional[Context] = None) -> np.ndarray:
+    """gpu_sort_unique_u64 (sort.rs:409): sorted distinct keys."""
+    ctx = ctx or default_context()
+    k = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+    out = np.empty(max(1, len(k)), dtype=np.uint64)
+    m = C.c_uint64()
+    code = ctx.L.sk_sort_unique_u64(ctx.h, k.ctypes.data, len(k), out.ctypes.data, C.byref(m))
+    if code:
+        _raise(code)
+    return out[:m.value]
+
+
+def sort_unique_pairs(pairs, ctx: Optional[Context] = None) -> List[Tuple[int, int]]:
+    """sort_unique_pairs (sort.rs:518)."""
+    ctx = ctx or default_context()
+    a, b = _pairs(pairs)
+    oa = np.empty(max(1, len(a)), np.uint32)
+    ob = np.empty(max(1, len(a)), np.uint32)
+    m = C.c_uint64()
+    code = ctx.L.sk_sort_unique_pairs(ctx.h, a.ctypes.data, b.ctypes.data, len(a), oa.ctypes.data, ob.ctypes.data,
+                                      C.byref(m))
+    if code:
+        _raise(code)
+    return list(zip(oa[:m.value].tolist(), ob[:m.value].tolist()))
+
+
+def sort_min_pairs(triples, ctx: Optional[Context] = None) -> List[Tuple[int, int, int]]:
+    """sort_min_pairs (sort.rs:565): per distinct (a, b) the minimum weight."""
+    ctx = ctx or default_context()
+    e = np.asarray(triples, dtype=np.int64).reshape(-1, 3) if len(triples) else np.zeros((0, 3), np.int64)
+    a = np.ascontiguousarray(e[:, 0], dtype=np.uint32)
+    b = np.ascontiguousarray(e[:, 1], dtype=np.uint32)
+    w = np.ascontiguousarray(e[:, 2], dtype=np.int64)
+    n = len(a)
+    oa, ob = np.empty(max(1, n), np.uint32), np.empty(max(1, n), np.uint32)
+    ow = np.empty(max(1, n), np.int64)
+    m = C.c_uint64()
+    code = ctx.L.sk_sort_min_pairs(ctx.h, a.ctypes.data, b.ctypes.data, w.ctypes.data, n, oa.ctypes.data,
+                                   ob.ctypes.data, ow.ctypes.data, C.byref(m))
+    if code:
+        _raise(code)
+    k = m.value
+    return list(zip(oa[:k].tolist(), ob[:k].tolist(), ow[:k].tolist()))
 
 
 def binary_plan_intermediate(edges) -> int:

@@ -1,0 +1,99 @@
+"""GPU: the sort primitives against host sorting, with the reference's own
+cases (crates/strata-gpu/src/sort.rs:608-692: sizes 0..4097 over the full
+64-bit range, duplicates-heavy unique, pair dedup, per-key minimum) and
+sizes past the autotuner's smallest class, under every radix candidate
+configuration (digit width x items per thread) and under the tuner."""
+from collections import OrderedDict
+
+import numpy as np
+import pytest
+
+import strata_k as sk
+
+pytestmark = pytest.mark.gpu
+
+
+class Lcg:
+    """The reference tests' generator (sort.rs:612-621)."""
+
+    def __init__(self, s):
+        self.s = s
+
+    def next(self):
+        self.s = (self.s * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return self.s >> 11
+
+
+SIZES = [0, 1, 2, 3, 5, 7, 8, 100, 1000, 4097]
+
+
+@pytest.fixture(params=[-1, 0, 1, 2, 3, 4, 5], ids=lambda c: f"cand{c}")
+def radix_ctx(ctx, request):
+    ctx.force_radix(request.param)
+    yield ctx
+    ctx.force_radix(-1)
+
+
+def test_sorts_like_host(radix_ctx):
+    rng = Lcg(0xA11CE)
+    for n in SIZES:
+        keys = np.array([rng.next() for _ in range(n)], dtype=np.uint64)
+        assert (sk.sort_u64(keys, radix_ctx) == np.sort(keys)).all(), n
+    r = np.random.default_rng(5)
+    for n in (70_000, 300_001):
+        keys = r.integers(0, 2**64, size=n, dtype=np.uint64)
+        assert (sk.sort_u64(keys, radix_ctx) == np.sort(keys)).all(), n
+        few = r.integers(0, 1 << 13, size=n, dtype=np.uint64)  # two passes of 7-bit digits
+        assert (sk.sort_u64(few, radix_ctx) == np.sort(few)).all(), n
+
+
+def test_unique_like_host(radix_ctx):
+    rng = Lcg(0xBEEF)
+    for n in [0, 1, 2, 4, 9, 255, 256, 257, 1000, 5000, 90_000]:
+        keys = np.array([rng.next() % 40 for _ in range(n)], dtype=np.uint64)
+        assert (sk.sort_unique_u64(keys, radix_ctx) == np.unique(keys)).all(), n
+
+
+def test_pairs_roundtrip(ctx):
+    rng = Lcg(0xF00D)
+    for _ in range(20):
+        m = rng.next() % 500
+        pairs = [(rng.next() % 30, rng.next() % 30) for _ in range(m)]
+        assert sk.sort_unique_pairs(pairs, ctx) == sorted(set(pairs))
+
+
+def test_min_by_key_like_host(radix_ctx):
+    rng = Lcg(0x5EED)
+    for _ in range(30):
+        m = rng.next() % 600
+        triples = [(rng.next() % 20, rng.next() % 20, rng.next() % 100 - 20) for _ in range(m)]
+        want = OrderedDict()
+        for s, d, w in sorted(triples):
+            want.setdefault((s, d), w)
+        assert sk.sort_min_pairs(triples, radix_ctx) == [(s, d, w) for (s, d), w in want.items()]
+    r = np.random.default_rng(9)
+    n = 200_000
+    a, b = r.integers(0, 300, n), r.integers(0, 300, n)
+    w = r.integers(-(2**62), 2**62, n)
+    got = sk.sort_min_pairs(np.stack([a, b, w], 1), radix_ctx)
+    order = np.lexsort((w, b, a))
+    a2, b2, w2 = a[order], b[order], w[order]
+    first = np.ones(n, bool)
+    first[1:] = (a2[1:] != a2[:-1]) | (b2[1:] != b2[:-1])
+    assert got == list(zip(a2[first].tolist(), b2[first].tolist(), w2[first].tolist()))
+
+
+def test_tuner_picks_a_candidate(ctx):
+    """Timing every class yields a configuration for every key width; a sort
+    of the configuration's own class then equals the host's."""
+    ctx.set_autotune(2)
+    try:
+        for wide in (False, True):
+            for weighted in (False, True):
+                c = ctx.radix_choice(1 << 20, 40, wide=wide, weighted=weighted)
+                assert c["digit_bits"] in (6, 7, 8) and c["items_per_thread"] in (4, 8, 16), c
+                assert c["est_ms"] > 0, c
+        keys = np.random.default_rng(3).integers(0, 1 << 40, size=1 << 20, dtype=np.uint64)
+        assert (sk.sort_u64(keys, ctx) == np.sort(keys)).all()
+    finally:
+        ctx.set_autotune(1)
