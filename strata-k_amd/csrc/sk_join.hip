@@ -332,7 +332,10 @@ __global__ void k_join_emit_rows(Table in, ProbeDesc p, OutDesc o, const K* __re
 // table reads, then all cell reads, then the applications — so that every
 // lane has kEmitItems independent memory requests in flight instead of one
 // dependent chain per output.
-template <class K, bool TROP>
+// PDOM: some head column is ranked within its program (batched runs); the
+// lookup is compiled only into that variant, which keeps the common one at
+// its register budget.
+template <class K, bool TROP, bool PDOM>
 __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
                                                        const i64* __restrict__ sw, const u8* __restrict__ sf,
                                                        const u64* __restrict__ lo, const u64* __restrict__ off,
@@ -424,9 +427,11 @@ __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, Ou
   u64 cell[kEmitItems];
 #pragma unroll
   for (int k = 0; k < kEmitItems; k++) cell[k] = 0;
-  u32 pidc[kEmitItems];  // column 0's codes (program ids in batched runs)
+  u32 pidc[PDOM ? kEmitItems : 1];  // column 0's codes (program ids in batched runs)
+  if constexpr (PDOM) {
 #pragma unroll
-  for (int k = 0; k < kEmitItems; k++) pidc[k] = 0;
+    for (int k = 0; k < kEmitItems; k++) pidc[k] = 0;
+  }
   for (int j = 0; j < o.ncols; j++) {
     u32 code[kEmitItems];
     const int kind = o.kind[j];
@@ -441,20 +446,26 @@ __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, Ou
 #pragma unroll
       for (int k = 0; k < kEmitItems; k++) code[k] = o.cnst[j];
     }
-    if (j == 0) {
-#pragma unroll
-      for (int k = 0; k < kEmitItems; k++) pidc[k] = code[k];
-    }
     const u32* db = o.cd.dom_bits[j];
     const int sh = o.cd.shift[j], eb = o.cd.ebits[j];
-    if (o.cd.pdom_off[j]) {
+    bool pdom_col = false;
+    if constexpr (PDOM) {
+      if (j == 0) {
 #pragma unroll
-      for (int k = 0; k < kEmitItems; k++) {
-        u64 rk = 0;
-        const bool in_dom = cell_rank(o.cd, j, code[k], pidc[k], &rk) && !(rk >> eb);
-        ok[k] = ok[k] && in_dom;
-        cell[k] |= rk << sh;
+        for (int k = 0; k < kEmitItems; k++) pidc[k] = code[k];
       }
+      if (o.cd.pdom_off[j]) {
+        pdom_col = true;
+#pragma unroll
+        for (int k = 0; k < kEmitItems; k++) {
+          u64 rk = 0;
+          const bool in_dom = cell_rank(o.cd, j, code[k], pidc[k], &rk) && !(rk >> eb);
+          ok[k] = ok[k] && in_dom;
+          cell[k] |= rk << sh;
+        }
+      }
+    }
+    if (pdom_col) {
     } else if (!db) {
 #pragma unroll
       for (int k = 0; k < kEmitItems; k++) {
@@ -542,7 +553,14 @@ static void emit_launch(const Exec& x, const Table& in, const ProbeDesc& p, cons
                         const i64* sw, const u8* sf, const u64* lo, const u32* cnt, const u64* off, u64 total) {
   if (FINAL && o.cd.on && p.nfilt == 0) {
     u64 nb = (total + kEmitTile - 1) / kEmitTile;
-    k_join_apply<K, TROP><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total, x.err);
+    bool pdom = false;
+    for (int j = 0; j < o.cd.ncols; j++) pdom = pdom || o.cd.pdom_off[j] != nullptr;
+    if (pdom)
+      k_join_apply<K, TROP, true><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total,
+                                                                         x.err);
+    else
+      k_join_apply<K, TROP, false><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total,
+                                                                          x.err);
   } else if (p.nfilt == 0) {
     u64 nb = (total + kEmitTile - 1) / kEmitTile;
     k_join_emit_lb<K, KO, FINAL, TROP><<<(unsigned)nb, kBlock, 0, x.stream>>>(in, p, o, skeys, sw, sf, lo, off, total,
