@@ -37,9 +37,14 @@ struct LoopState {
   u32 done;
   u32 stamp;
   u64 iters;
-  u32 any;  // the round's compaction found a touched cell
-  u32 pad;
+  u32 any;    // the round's compaction found a touched cell
+  u32 leave;  // Trop rounds: 1 the push join is cheaper next round, 2 a device
+              // status (overflow, widen) to handle, 3 the round cap exceeded
+  u64 n;      // Trop rounds: the last delta's size and its push cost
+  u64 pc;
+  DevErr st;  // the status that made the rounds leave
 };
+static_assert(sizeof(LoopState) == 64, "read back as eight words");
 
 struct CellDesc {
   int on = 0;
@@ -101,6 +106,9 @@ struct CellDesc {
   u8* pact = nullptr;
   int pact_shift = 0;
   const u32* pact_list = nullptr;
+  // Device-driven Trop rounds: the compaction kernels do nothing once the
+  // rounds are done or left.
+  const LoopState* gate = nullptr;
   u32* bad = nullptr;      // set when a value falls outside its column domain
 };
 
@@ -277,6 +285,17 @@ bool cells_into_fits(u64 ncells);
 // a graph.
 void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* ls, u8* active_prev,
                       u8* active_next, u64* piters, u32 nparts);
+// A device-driven Trop round's compaction (cd.gate set): the touched
+// compaction into the capacity buffers with the push cost and a snapshot of
+// the device status (as cells_compact_into, without the host read), then
+// the round's bookkeeping: a status sets leave = 2; otherwise iters += 1,
+// n and pc recorded, leave = 3 past cap_left rounds, done on an empty delta,
+// leave = 1 when pc * 4 < push_limit.  cnt / off / pcb hold
+// ncells / (kBlock * 32) entries, tot 8.
+template <class K>
+void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
+                           u32* cnt, u64* off, u64* pcb, u64* tot, const DevErr* status, LoopState* ls,
+                           u64 cap_left, u64 push_limit);
 template <class K>
 u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                        u64* push_cost, const DevErr* status, DevErr* status_out);

@@ -278,6 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_count(CellDesc cd, u64 w0, u64
                                                         u32* __restrict__ cnt, u64* __restrict__ pcb) {
   __shared__ u32 tmp[kBlock / 64];
   __shared__ u64 ptmp[kBlock / 64];
+  if (cd.gate && (cd.gate->done | cd.gate->leave)) return;  // uniform over the launch
   const u64 wl = (u64)blockIdx.x * kBlock + threadIdx.x;
   const u32 b = wl < nwords ? cells_word(cd, w0 + wl, touched_only != 0) : 0u;
   u32 tot;
@@ -322,6 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
                                                         i64* __restrict__ w, u8* __restrict__ inf) {
   __shared__ u32 tmp[kBlock / 64];
   __shared__ unsigned short sc[kCellsPerBlock];
+  if (cd.gate && (cd.gate->done | cd.gate->leave)) return;  // uniform over the launch
   const u64 wl = (u64)blockIdx.x * kBlock + threadIdx.x;
   const u64 wi = w0 + wl;
   u32 b = wl < nwords ? cells_word(cd, wi, touched_only != 0) : 0u;
@@ -503,6 +505,40 @@ void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* 
   SK_HIP(hipGetLastError());
 }
 
+__global__ void k_loop_book_trop(LoopState* ls, const u64* __restrict__ tot, u64 cap_left, u64 push_limit) {
+  if (threadIdx.x != 0 || ls->done || ls->leave) return;
+  const DevErr* st = (const DevErr*)(tot + 2);
+  if (st->code || st->widen) {
+    ls->st = *st;
+    ls->leave = 2;
+    return;
+  }
+  ls->iters += 1;
+  ls->n = tot[0];
+  ls->pc = tot[1];
+  if (ls->iters > cap_left) ls->leave = 3;
+  else if (tot[0] == 0) ls->done = 1;
+  else if (tot[1] * 4 < push_limit) ls->leave = 1;
+}
+
+template <class K>
+void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
+                           u32* cnt, u64* off, u64* pcb, u64* tot, const DevErr* status, LoopState* ls,
+                           u64 cap_left, u64 push_limit) {
+  static_assert(sizeof(DevErr) == 3 * sizeof(u64), "status record is three words");
+  const u64 nwords = (ncells + 31) / 32;
+  const u64 nblocks = (nwords + kBlock - 1) / kBlock;
+  if (!cells_into_fits(ncells) || !cd.gate || !cd.rowrec || !cd.pc_csr || cd.rlbits < 0 || cd.rlbits > 6)
+    throw std::runtime_error("cells_loop_round_trop: layout");
+  k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, cnt, pcb);
+  SK_HIP(hipGetLastError());
+  scan_u32_one(x, cnt, off, nblocks, tot, pcb, (const u64*)status, 3);
+  k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, off, bits, keys, w, inf);
+  SK_HIP(hipGetLastError());
+  k_loop_book_trop<<<1, 64, 0, x.stream>>>(ls, tot, cap_left, push_limit);
+  SK_HIP(hipGetLastError());
+}
+
 bool cells_into_fits(u64 ncells) {
   const u64 nwords = (ncells + 31) / 32;
   return (nwords + kBlock - 1) / kBlock <= kScanOneMax;
@@ -614,7 +650,9 @@ void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
                                 u64, u64*);                                                                \
   template u64 cells_compact_into<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u8*, u64*, const DevErr*,   \
                                      DevErr*);                                                                    \
-  template void cells_mark_touched<K>(const Exec&, const CellDesc&, const K*, u64, int, int);
+  template void cells_mark_touched<K>(const Exec&, const CellDesc&, const K*, u64, int, int);                 \
+  template void cells_loop_round_trop<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u8*, u32*, u64*, u64*, \
+                                         u64*, const DevErr*, LoopState*, u64, u64);
 SK_CELLS_INST(u64)
 SK_CELLS_INST(u128)
 

@@ -1074,7 +1074,23 @@ bool Engine::try_pull(const RulePlan& pl) {
     });
   }
   if (pull_mode_ == 1 && cand * 4 < E.full.n) return false;
-  Index& ixr = get_index(pr->E, false, {pr->eR, pr->eJ}, true, true);
+  const PullDesc pd = pull_desc(*pr, *dn, hr);
+  // algorithmic bytes: every index row of E (j, weight, its source's 16-byte
+  // row record) once, every candidate's delta cell (lane byte, weight) once
+  const double bytes =
+      (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 16.0) + (double)cand * (1.0 + (trop ? 8.0 : 0.0));
+  pull_apply(x, pd, dn->cd, trop, bytes, cand);
+  round_pulls_++;
+  return true;
+}
+
+// The launch descriptor of a pull rule over relation hr's current delta.
+PullDesc Engine::pull_desc(PullRule& pr, DenseRel& dn, int hr) {
+  Exec x = ctx_->exec();
+  DRel& H = R_[hr];
+  DRel& E = R_[pr.E];
+  const bool trop = H.sem == SK_TROP;
+  Index& ixr = get_index(pr.E, false, {pr.eR, pr.eJ}, true, true);
   const u64 nv = D_.nvals();
   // the chunk list is an index of E, kept with it across runs
   if (ixr.pull_version != ixr.csr_version) {
@@ -1086,18 +1102,135 @@ bool Engine::try_pull(const RulePlan& pl) {
   pd.chunks = ixr.pull.get();
   pd.adj = ixr.adj.get();
   pd.w = trop ? (ixr.direct ? E.full.w.get() : ixr.data.w.get()) : nullptr;
-  pd.rowrec = dn->rowrec.get();
+  pd.rowrec = dn.rowrec.get();
   pd.dw = trop ? H.delta.w.get() : nullptr;
-  pd.dlane = dn->dlane.get();
-  pd.lbits = dn->lbits;
-  pd.delta_first = pr->delta_first ? 1 : 0;
-  pd.nrows = dn->nrows;
-  // algorithmic bytes: every index row of E (j, weight, its source's 16-byte
-  // row record) once, every candidate's delta cell (lane byte, weight) once
-  const double bytes =
-      (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 16.0) + (double)cand * (1.0 + (trop ? 8.0 : 0.0));
-  pull_apply(x, pd, dn->cd, trop, bytes, cand);
-  return true;
+  pd.dlane = dn.dlane.get();
+  pd.lbits = dn.lbits;
+  pd.delta_first = pr.delta_first ? 1 : 0;
+  pd.nrows = dn.nrows;
+  return pd;
+}
+
+// The relation whose remaining Trop rounds may run device-driven, or -1:
+// one derived relation, dense with capacity compaction and a pull rule, the
+// stratum's only delta variant, which pulled in the round just run.
+int Engine::trop_graph_rel(u32 k) {
+  if (!graph_enabled_ || pull_mode_ == 0 || nparts_ || round_variants_ != 1 || round_pulls_ != 1) return -1;
+  int rel = -1;
+  for (size_t i = 0; i < R_.size(); i++)
+    if (in_idb_[i]) {
+      if (rel >= 0) return -1;
+      rel = (int)i;
+    }
+  if (rel < 0 || R_[rel].delta.n == 0 || R_[rel].sem != SK_TROP) return -1;
+  DenseRel* dn = dense_of(rel);
+  if (!dn || dn->pulls.empty() || !dn->into || dn->keep_full || !dn->cd.trop || !dn->cd.pc_csr) return -1;
+  int variants = 0;
+  for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+    const Rule& r = P_.rules[ri];
+    if (r.stratum != k || r.agg) continue;
+    for (size_t j = 0; j < r.body.size(); j++) {
+      const Atom& a = r.body[j];
+      if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
+      variants++;
+      if (dn->pulls[0].rule != (int)ri || dn->pulls[0].dpos != (int)j) return -1;
+    }
+  }
+  return variants == 1 ? rel : -1;
+}
+
+// Runs the stratum's next Trop rounds on the device: a hipGraph of
+// kGraphRounds rounds (the pull, the touched compaction into the capacity
+// buffers with its push cost and a status snapshot, the round's
+// bookkeeping, see cells_loop_round_trop) relaunched until the rounds are
+// done or leave.  Returns 0 converged, 1 the next round pushes (the delta,
+// its size and push cost are in place), 2 a device status to handle (*st),
+// 3 the round cap exceeded.  *iters counts the rounds run, as the host loop
+// counts them.
+int Engine::run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st) {
+  (void)k;
+  constexpr int kGraphRounds = 4;
+  DRel& H = R_[rel];
+  DenseRel& dn = *dense_of(rel);
+  PullRule& pr = dn.pulls[0];
+  Exec x = ctx_->exec();
+  PullDesc pd = pull_desc(pr, dn, rel);
+  const u64 nb = (((dn.ncells + 31) / 32) + kBlock - 1) / kBlock;
+  DBuf<LoopState> ls(x.pool, 1);
+  DBuf<u32> cnt(x.pool, nb);
+  DBuf<u64> off(x.pool, nb), pcb(x.pool, nb), tot(x.pool, 8);
+  LoopState h0{};
+  SK_HIP(hipMemcpyAsync(ls.get(), &h0, sizeof h0, hipMemcpyHostToDevice, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  CellDesc cdl = dn.cd;
+  cdl.gate = ls.get();
+  pd.gate = ls.get();
+  const u64 cap_left = cap > *iters ? cap - *iters : 0;
+  const u64 push_limit = pull_mode_ == 2 ? 0 : R_[pr.E].full.n;
+  const bool hf = has_inf_;
+  Exec xg = x;
+  xg.timer = nullptr;  // no event records inside the capture
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  SK_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int r = 0; r < kGraphRounds; r++) {
+      pull_apply(xg, pd, cdl, true, 0, 0);
+      with_key(H.wide, [&](auto* tag) {
+        using K = std::remove_pointer_t<decltype(tag)>;
+        cells_loop_round_trop<K>(xg, cdl, dn.ncells, H.bits, (K*)H.delta.k.ptr(), H.delta.w.get(),
+                                 hf ? H.delta.inf.get() : nullptr, cnt.get(), off.get(), pcb.get(), tot.get(),
+                                 ctx_->derr, ls.get(), cap_left, push_limit);
+      });
+    }
+  } catch (...) {
+    hipGraph_t bad = nullptr;
+    (void)hipStreamEndCapture(x.stream, &bad);
+    if (bad) (void)hipGraphDestroy(bad);
+    throw;
+  }
+  SK_HIP(hipStreamEndCapture(x.stream, &g));
+  hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    SK_HIP(ie);
+  }
+  LoopState hs{};
+  try {
+    for (;;) {
+      {
+        Scope sc(x.timer, "graph_rounds", KC_JOIN, 0.0, kGraphRounds);
+        SK_HIP(hipGraphLaunch(ge, x.stream));
+      }
+      read_u64s(x, (const u64*)ls.get(), (u64*)&hs, 8);
+      if (hs.done || hs.leave) break;
+    }
+  } catch (...) {
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    throw;
+  }
+  SK_HIP(hipGraphExecDestroy(ge));
+  SK_HIP(hipGraphDestroy(g));
+  *iters += hs.iters;
+  if (hs.leave == 2) {
+    *st = hs.st;
+    return 2;
+  }
+  if (hs.leave == 3) return 3;
+  // the last round's delta sits in the relation's capacity buffers
+  H.dversion++;
+  H.delta_order.clear();
+  if (hs.done || hs.n == 0) {
+    dn.spare = std::move(H.delta);
+    H.delta.release();
+    return 0;
+  }
+  H.delta.n = hs.n;
+  if (!dn.key_order) H.delta_order = dn.order;
+  pr.cost = hs.pc;
+  pr.cost_dv = H.dversion;
+  return 1;
 }
 
 // Word-parallel destination-side evaluation of a Bool delta variant (see
@@ -1214,7 +1347,8 @@ void Engine::run_graph_loop(u32 k, int rel, u64* iters, std::vector<u8>& active_
   DBuf<LoopState> ls(x.pool, 1);
   DBuf<u8> ap(x.pool, np ? np : 1), an(x.pool, np ? np : 1);
   DBuf<u64> pit(x.pool, np ? np : 1);
-  LoopState h0{0, dn.stamp, 0, 0, 0};
+  LoopState h0{};
+  h0.stamp = dn.stamp;
   SK_HIP(hipMemcpyAsync(ls.get(), &h0, sizeof h0, hipMemcpyHostToDevice, x.stream));
   if (np) {
     if (active_prev.size() != np) active_prev.assign(np, 0);
@@ -2655,7 +2789,7 @@ void Engine::run_stratum(u32 k) {
   bool bootstrap = true;
   for (;;) {
     std::map<int, std::vector<Chunk>> chunks;
-    round_variants_ = round_bitpulls_ = 0;
+    round_variants_ = round_bitpulls_ = round_pulls_ = 0;
     auto run_rules = [&]() {
       for (size_t ri = 0; ri < P_.rules.size(); ri++) {
         const Rule& r = P_.rules[ri];
@@ -2707,26 +2841,29 @@ void Engine::run_stratum(u32 k) {
       }
     }
     fold_req_ = false;
+    // restart the stratum from its initial state with wide cells: the
+    // relations it derives are dense and never merged into their full
+    // state, so that state is still the stratum's input
+    auto restart_wide = [&]() {
+      dense_.clear();
+      for (size_t i = 0; i < R_.size(); i++)
+        if (in_idb_[i]) {
+          R_[i].delta.release();
+          R_[i].delta_order.clear();
+          R_[i].dversion++;
+        }
+      memset_async(ctx_->exec(), &ctx_->derr->widen, 0, 4);
+      narrow_off_ = true;
+      plan_dense(k);
+      fold = fold_ok();
+      plans.clear();
+      iters = 0;
+      bootstrap = true;
+    };
     if (need_status && fold) {
       if (!fold_done_) folded_ = read_device_status();
       if (folded_.widen) {
-        // restart the stratum from its initial state with wide cells: the
-        // relations it derives are dense and never merged into their full
-        // state, so that state is still the stratum's input
-        dense_.clear();
-        for (size_t i = 0; i < R_.size(); i++)
-          if (in_idb_[i]) {
-            R_[i].delta.release();
-            R_[i].delta_order.clear();
-            R_[i].dversion++;
-          }
-        memset_async(ctx_->exec(), &ctx_->derr->widen, 0, 4);
-        narrow_off_ = true;
-        plan_dense(k);
-        fold = fold_ok();
-        plans.clear();
-        iters = 0;
-        bootstrap = true;
+        restart_wide();
         continue;
       }
       if (check_err) raise_device_error(folded_);
@@ -2749,6 +2886,21 @@ void Engine::run_stratum(u32 k) {
     if (grel >= 0) {
       run_graph_loop(k, grel, &iters, active_prev);
       break;
+    }
+    const int trel = is_trop ? trop_graph_rel(k) : -1;
+    if (trel >= 0) {
+      DevErr gst{};
+      const int rc = run_trop_graph_loop(k, trel, cap, &iters, &gst);
+      if (rc == 0) break;
+      if (rc == 3) throw mkerr(SK_ERR_NEGATIVE_WEIGHT_CYCLE, "", "negative-weight cycle: tropical stratum diverges");
+      if (rc == 2) {
+        if (gst.widen) {
+          restart_wide();
+          continue;
+        }
+        raise_device_error(gst);
+      }
+      // rc == 1: the next round pushes, on the host
     }
   }
   iters_[k] = iters;

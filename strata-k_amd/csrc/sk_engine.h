@@ -379,7 +379,8 @@ class Engine {
   bool dense_enabled_ = true;
   int pull_mode_ = 1;
   bool graph_enabled_ = true;
-  int round_variants_ = 0, round_bitpulls_ = 0;  // delta variants of the current round, and those pulled
+  int round_variants_ = 0, round_bitpulls_ = 0, round_pulls_ = 0;  // delta variants of the current round,
+                                                                    // and those pulled
   bool narrow_ok_ = true;   // every staged Trop weight is finite-or-inf in [0, 2^32-1)
 
   void build_dict();
@@ -402,6 +403,9 @@ class Engine {
   bool try_bitpull(const RulePlan& pl);
   void ensure_bitpull_graph(BitPullRule& bp, DenseRel& dn);
   int graph_loop_rel(u32 k);
+  PullDesc pull_desc(PullRule& pr, DenseRel& dn, int hr);
+  int trop_graph_rel(u32 k);
+  int run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st);
   void run_graph_loop(u32 k, int rel, u64* iters, std::vector<u8>& active_prev);
   void plan_bitpulls(u32 k, int i, DenseRel& dn);
   u64 finalize(int rel, std::vector<Chunk>& chunks);

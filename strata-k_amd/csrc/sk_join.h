@@ -96,6 +96,7 @@ struct PullDesc {
   int lbits = 0;                   // lanes per row = 2^lbits
   int delta_first = 1;             // ⊗ operand order: the delta's weight on the left
   u64 nrows = 0;                   // rows of the cell layout (j values at or above are absent)
+  const LoopState* gate = nullptr; // device-driven rounds: nothing to do once done or left
 };
 
 // Word-parallel pull of a Bool linear recursive rule over dense cells:

@@ -985,6 +985,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
   const u32 lane = threadIdx.x & 63;
   const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
+  if (pd.gate && (pd.gate->done | pd.gate->leave)) return;
   PullLds& L = sh[threadIdx.x >> 6];
   const PullChunk c = pd.chunks[ch];
   const u32 nr = c.nr_off & 0xFFu;
