@@ -116,6 +116,7 @@ def bench_cfg2(args):
     log(f"generated {w.params} in {time.time() - t:.1f}s")
     ctx = sk.Context(0)
     run = ctx.prepare(w.program)
+    run.set_graph(args.graph != 0)
     for p, (cols, wt, inf) in w.bulk.items():
         run.add_sym(p, cols, wt, inf)
     t = time.time()
@@ -409,6 +410,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="config 5: arrivals per second")
     ap.add_argument("--max-batch", type=int, default=16, help="config 5: queries in flight")
     ap.add_argument("--plain-queries", type=int, default=8, help="config 5: queries timed as plain runs")
+    ap.add_argument("--graph", type=int, default=1, help="device-driven rounds captured in a hipGraph (1/0)")
     ap.add_argument("--sort", action="store_true", help="time the device radix sort primitive")
     ap.add_argument("--sort-log2", type=int, default=26)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
