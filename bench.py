@@ -410,7 +410,10 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="config 5: arrivals per second")
     ap.add_argument("--max-batch", type=int, default=16, help="config 5: queries in flight")
     ap.add_argument("--plain-queries", type=int, default=8, help="config 5: queries timed as plain runs")
-    ap.add_argument("--graph", type=int, default=1, help="device-driven rounds captured in a hipGraph (1/0)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="config 2: device-driven rounds captured in a hipGraph (1/0).  Off by default so that "
+                         "every launch of the dominant kernel is timed by its own HIP events; on, the pull "
+                         "rounds inside graph launches are timed as one record per launch")
     ap.add_argument("--sort", action="store_true", help="time the device radix sort primitive")
     ap.add_argument("--sort-log2", type=int, default=26)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
