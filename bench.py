@@ -200,9 +200,10 @@ def bench_cfg3(args):
 
 def bench_cfg4(args):
     """Batched: 512 independent programs (256 Bool TC, 256 Trop cost) through
-    one batch, against 512 sequential plain runs of the same programs.  Both
-    sides are timed end to end (host EDB in, host results out); every batched
-    result is compared with its plain run."""
+    one batch, against 512 sequential plain runs of the same programs.  value
+    = the execute-phase speedup (every fixpoint over EDB already resident in
+    HBM, wall clock); both sides are also timed end to end (host EDB in, host
+    results out).  Every batched result is compared with its plain run."""
     import strata_k as sk
     from strata_k import workloads as wl
     t = time.time()
@@ -253,7 +254,10 @@ def bench_cfg4(args):
         exact = exact and same
     ctx.close()
     print(json.dumps({
-        "metric": "batched-vs-plain speedup", "value": round(plain_s / batch_s, 2), "unit": "x",
+        "metric": "batched-vs-plain speedup",
+        "value": round(plain_exec_ms / batch_exec_ms, 2) if batch_exec_ms else None, "unit": "x",
+        "value_is": "execute phase (fixpoints on resident EDB, wall clock): batched vs 512 plain runs",
+        "end_to_end_speedup": round(plain_s / batch_s, 2),
         "n_gpus": 1, "higher_is_better": True, "dtype": "int64", "data": "synthetic (seeded ER / RMAT)",
         "config": {"workload": f"{len(progs)} programs: {len(progs) // 2} Bool TC (n <= 2000, m <= 10000) + "
                                f"{len(progs) - len(progs) // 2} Trop cost (n in [1e3, 2e5], m in [4e3, 2e6], 1 source)",
@@ -261,7 +265,6 @@ def bench_cfg4(args):
         "plain_s": round(plain_s, 3), "batched_s": round(batch_s, 3),
         "plain_tuples_per_s": round(n_out / plain_s, 1), "batched_tuples_per_s": round(n_out / batch_s, 1),
         "plain_device_ms": round(plain_exec_ms, 1), "batched_device_ms": round(batch_exec_ms, 1),
-        "device_speedup": round(plain_exec_ms / batch_exec_ms, 2) if batch_exec_ms else None,
         "batched_phases": phases,
         "bit_exact_vs_plain": bool(exact)}), flush=True)
 

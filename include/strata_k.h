@@ -331,7 +331,9 @@ sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, i
 typedef struct sk_batch sk_batch;
 sk_status sk_batch_create(sk_ctx* ctx, sk_batch** out);
 void sk_batch_free(sk_batch* b);
-/* Add one program with all-Sym EDB facts (one entry per EDB predicate). */
+/* Add one program with all-Sym EDB facts (one entry per EDB predicate).  The
+ * fact buffers (columns and weights) are read at sk_batch_load, not copied
+ * here: they must stay valid and unchanged until sk_batch_load returns. */
 sk_status sk_batch_add(sk_batch* b, const sk_program* prog, uint32_t n_edb,
                        const char* const* edb_preds, const uint64_t* edb_n,
                        const uint32_t* const* const* edb_cols, const int64_t* const* edb_w,

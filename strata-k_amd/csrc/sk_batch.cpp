@@ -29,10 +29,11 @@ namespace {
 
 struct BProgram {
   Program prog;  // own copy
+  // the caller's fact buffers, borrowed until sk_batch_load returns
   std::vector<std::string> edb_preds;
   std::vector<u64> edb_n;
-  std::vector<std::vector<std::vector<u32>>> edb_cols;
-  std::vector<std::vector<i64>> edb_w;
+  std::vector<std::vector<const u32*>> edb_cols;
+  std::vector<const i64*> edb_w;
   int group = -1;
   int member = -1;  // index within the group
   EvalErr err;
@@ -120,6 +121,23 @@ void for_groups(sk_batch* b, F f) {
   for (auto& t : ts) t.join();
 }
 
+// Runs f(m) for m in [0, n) on up to 8 host threads.
+template <class F>
+void parallel_members(size_t n, F f) {
+  const size_t nt = std::min<size_t>(n, 8);
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> ts;
+  for (size_t t = 0; t < nt; t++)
+    ts.emplace_back([&] {
+      for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+    });
+  for (auto& t : ts) t.join();
+}
+
 // Group-wide failure of a load or execute step.
 template <class F>
 void guarded_group(sk_batch::Group& g, F f) {
@@ -169,13 +187,16 @@ sk_status sk_batch_add(sk_batch* b, const sk_program* prog, uint32_t n_edb, cons
     int r = bp.prog.rel_of(edb_preds[i]);
     if (r < 0) return SK_ERR_UNKNOWN_PRED;
     u32 ar = bp.prog.preds[r].arity;
+    if (edb_n[i] && !edb_cols[i]) return SK_ERR_INVALID_ARGUMENT;
     bp.edb_preds.push_back(edb_preds[i]);
     bp.edb_n.push_back(edb_n[i]);
-    std::vector<std::vector<u32>> cols(ar);
-    for (u32 j = 0; j < ar; j++) cols[j].assign(edb_cols[i][j], edb_cols[i][j] + edb_n[i]);
+    std::vector<const u32*> cols(ar);
+    for (u32 j = 0; j < ar; j++) {
+      cols[j] = edb_n[i] ? edb_cols[i][j] : nullptr;
+      if (edb_n[i] && !cols[j]) return SK_ERR_INVALID_ARGUMENT;
+    }
     bp.edb_cols.push_back(std::move(cols));
-    if (edb_w && edb_w[i]) bp.edb_w.emplace_back(edb_w[i], edb_w[i] + edb_n[i]);
-    else bp.edb_w.emplace_back();
+    bp.edb_w.push_back(edb_w ? edb_w[i] : nullptr);
   }
   if (out_index) *out_index = (uint32_t)b->progs.size();
   b->progs.push_back(std::move(bp));
@@ -209,46 +230,69 @@ sk_status sk_batch_load(sk_batch* b) {
       g.eng = std::make_unique<Engine>(g.gc.get(), g.fused);
       g.eng->validate();
       g.eng->set_partitions((u32)g.members.size());
-      // stage every member's facts with its program id as column 0
-      std::map<std::string, u64> rows;
-      u64 max_n = 0;
-      for (int m : g.members) {
-        BProgram& bp = b->progs[m];
-        for (size_t e = 0; e < bp.edb_preds.size(); e++) {
-          rows[p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name] += bp.edb_n[e];
-          max_n = std::max<u64>(max_n, bp.edb_n[e]);
-        }
-      }
-      for (auto& kv : rows) g.eng->reserve_staging(kv.first, kv.second);
-      std::vector<u32> pid(max_n);
-      for (int m = 0; m < (int)g.members.size(); m++) {
+      const size_t nm = g.members.size();
+      // per program: its universe (distinct Sym values of its EDB, a bitmap
+      // over its largest id) and its largest id, on up to 8 host threads
+      std::vector<u32> pmax(nm, 0);
+      parallel_members(nm, [&](size_t m) {
         BProgram& bp = b->progs[g.members[m]];
-        // per-program universe: distinct Sym values of its EDB (a bitmap
-        // over the program's largest id)
         u32 mx = 0;
-        for (auto& cols : bp.edb_cols)
-          for (auto& c : cols)
-            for (u32 v : c) mx = std::max(mx, v);
+        for (size_t e = 0; e < bp.edb_preds.size(); e++)
+          for (const u32* c : bp.edb_cols[e])
+            for (u64 i = 0; i < bp.edb_n[e]; i++) mx = std::max(mx, c[i]);
         std::vector<u64> seen(((u64)mx >> 6) + 1, 0);
-        for (auto& cols : bp.edb_cols)
-          for (auto& c : cols)
-            for (u32 v : c) seen[v >> 6] |= 1ull << (v & 63);
+        for (size_t e = 0; e < bp.edb_preds.size(); e++)
+          for (const u32* c : bp.edb_cols[e])
+            for (u64 i = 0; i < bp.edb_n[e]; i++) seen[c[i] >> 6] |= 1ull << (c[i] & 63);
         u64 distinct = 0;
         for (u64 wd : seen) distinct += (u64)__builtin_popcountll(wd);
         bp.universe = distinct;
-        g.eng->set_partition_universe((u32)m, bp.universe);
+        pmax[m] = mx;
+      });
+      u64 max_sym = nm ? nm - 1 : 0;  // program ids
+      for (size_t m = 0; m < nm; m++) {
+        g.eng->set_partition_universe((u32)m, b->progs[g.members[m]].universe);
+        max_sym = std::max<u64>(max_sym, pmax[m]);
+      }
+      g.eng->set_max_sym_hint(max_sym);
+      // stage every member's facts with its program id as column 0: sized
+      // once per relation, each program filling its own row range
+      std::map<std::string, u64> rows;
+      std::map<std::string, bool> weighted;
+      std::vector<std::vector<u64>> at(nm);
+      for (size_t m = 0; m < nm; m++) {
+        BProgram& bp = b->progs[g.members[m]];
         for (size_t e = 0; e < bp.edb_preds.size(); e++) {
           const std::string& pname = p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name;
-          u64 n = bp.edb_n[e];
-          std::fill(pid.begin(), pid.begin() + n, (u32)m);
+          at[m].push_back(rows[pname]);
+          rows[pname] += bp.edb_n[e];
+          weighted[pname] = bp.edb_w[e] != nullptr;
+        }
+      }
+      for (auto& kv : rows) g.eng->stage_sym_resize(kv.first, kv.second, weighted[kv.first]);
+      std::vector<char> narrow(nm, 1);
+      parallel_members(nm, [&](size_t m) {
+        BProgram& bp = b->progs[g.members[m]];
+        for (size_t e = 0; e < bp.edb_preds.size(); e++) {
+          const std::string& pname = p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name;
+          const u64 n = bp.edb_n[e];
+          if (!n) continue;
+          std::vector<u32> pid(n, (u32)m);
           std::vector<const u32*> cols;
           cols.push_back(pid.data());
-          for (auto& c : bp.edb_cols[e]) cols.push_back(c.data());
-          g.eng->stage_sym(pname, n, (u32)cols.size(), cols.data(), bp.edb_w[e].empty() ? nullptr : bp.edb_w[e].data(),
-                           nullptr);
+          for (const u32* c : bp.edb_cols[e]) cols.push_back(c);
+          bool nar = true;
+          g.eng->stage_sym_fill(pname, at[m][e], n, cols.data(), bp.edb_w[e], &nar);
+          if (!nar) narrow[m] = 0;
         }
-        std::vector<std::vector<std::vector<u32>>>().swap(bp.edb_cols);
-        std::vector<std::vector<i64>>().swap(bp.edb_w);
+      });
+      bool all_narrow = true;
+      for (char c : narrow) all_narrow = all_narrow && c;
+      if (!all_narrow) g.eng->mark_not_narrow();
+      for (int m : g.members) {
+        BProgram& bp = b->progs[m];
+        bp.edb_cols.clear();
+        bp.edb_w.clear();
       }
       g.eng->load();
     });

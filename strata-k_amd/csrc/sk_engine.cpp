@@ -346,14 +346,45 @@ void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* con
   R.st_rows += n;
 }
 
+void Engine::stage_sym_resize(const std::string& pred, u64 total, bool weighted) {
+  int r = P_.rel_of(pred);
+  if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
+  DRel& R = R_[r];
+  if (weighted != (R.sem == SK_TROP))
+    throw mkerr(SK_ERR_SEMIRING_MISMATCH, pred, "fact annotation differs from the predicate's semiring");
+  if (!R.st_kind.empty() || R.st_rows) throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "relation already staged");
+  R.st_sym.assign(R.arity, std::vector<u32>());
+  for (int j = 0; j < R.arity; j++) R.st_sym[j].resize(total);
+  if (weighted) {
+    R.st_w.resize(total);
+    R.st_inf.assign(total, 0);
+  }
+  R.st_weighted = weighted;
+  R.st_rows = total;
+  R.st_narrow = weighted ? 1 : -1;
+}
+
+void Engine::stage_sym_fill(const std::string& pred, u64 at, u64 n, const u32* const* cols, const i64* w,
+                            bool* narrow) {
+  DRel& R = R_[P_.rel_of(pred)];
+  for (int j = 0; j < R.arity; j++) std::memcpy(R.st_sym[j].data() + at, cols[j], n * 4);
+  bool nar = true;
+  if (w) {
+    std::memcpy(R.st_w.data() + at, w, n * 8);
+    for (u64 i = 0; i < n; i++) nar = nar && (u64)w[i] < 0xFFFFFFFFull;
+  }
+  if (narrow) *narrow = nar;
+}
+
 // =================================================================== load
 
 void Engine::build_dict() {
   u64 maxsym = 0;
   std::set<i64> ints;
   for (auto& R : R_) {
-    for (auto& c : R.st_sym)
-      for (u32 v : c) maxsym = std::max<u64>(maxsym, v);
+    if (max_sym_hint_ == ~0ull)
+      for (auto& c : R.st_sym)
+        for (u32 v : c) maxsym = std::max<u64>(maxsym, v);
     for (size_t i = 0; i < R.st_kind.size(); i++) {
       if (R.st_kind[i] == SK_VAL_SYM) {
         if (R.st_val[i] < 0 || R.st_val[i] > 0xffffffffll)
@@ -375,6 +406,7 @@ void Engine::build_dict() {
     for (auto& b : r.body) scan_terms(b);
   }
   maxsym = std::max<u64>(maxsym, attached_max_sym_);
+  if (max_sym_hint_ != ~0ull) maxsym = std::max<u64>(maxsym, max_sym_hint_);
   D_.sym_span = maxsym + 1;
   D_.ints.assign(ints.begin(), ints.end());
   if (D_.nvals() > 0xffffffffull) throw mkerr(SK_ERR_UNSUPPORTED_SHAPE, "", "more than 2^32 distinct values");
@@ -395,7 +427,8 @@ void Engine::upload_rel(int r) {
     return;
   }
   bool trop = R.sem == SK_TROP;
-  if (trop)
+  if (trop && R.st_narrow == 0) narrow_ok_ = false;
+  if (trop && R.st_narrow < 0)
     for (u64 i = 0; i < n && narrow_ok_; i++)
       if (!R.st_inf[i] && (u64)R.st_w[i] >= 0xFFFFFFFFull) narrow_ok_ = false;
   // encode to codes, one column at a time
@@ -461,6 +494,7 @@ void Engine::upload_rel(int r) {
   std::vector<i64>().swap(R.st_w);
   std::vector<u8>().swap(R.st_inf);
   R.st_rows = 0;
+  R.st_narrow = -1;
 }
 
 void Engine::compute_universe() {

@@ -150,6 +150,7 @@ struct DRel {
   bool st_weighted = false;
   u64 st_rows = 0;
   u64 st_reserve = 0;  // expected staged rows (capacity hint)
+  int st_narrow = -1;  // staged weights all in [0, 2^32 - 1): 1 / 0, or -1 when not known
 };
 
 // A rule `H(r, x) :- H(j, x), E(j, r)` (any body order, either column
@@ -337,6 +338,18 @@ class Engine {
     int r = P_.rel_of(pred);
     if (r >= 0) R_[r].st_reserve = rows;
   }
+  // Batched staging: size a relation's all-Sym stage for `total` rows, then
+  // fill disjoint row ranges (from several threads at once), recording
+  // whether every weight fits a narrow cell.  `max_sym` bounds every staged
+  // Sym id, so the dictionary needs no scan of the stage.
+  void stage_sym_resize(const std::string& pred, u64 total, bool weighted);
+  void stage_sym_fill(const std::string& pred, u64 at, u64 n, const u32* const* cols, const i64* w, bool* narrow);
+  void set_max_sym_hint(u64 max_sym) { max_sym_hint_ = max_sym; }
+  // Some staged weight does not fit a narrow cell.
+  void mark_not_narrow() {
+    for (auto& R : R_)
+      if (R.st_narrow == 1) R.st_narrow = 0;
+  }
   // Keep eligible relations as dense cells during their stratum (default on).
   void set_dense(bool on) { dense_enabled_ = on; }
   // Destination-side (pull) rounds: 0 never, 1 when the cost model prefers
@@ -359,6 +372,7 @@ class Engine {
   std::map<int, std::pair<u64, std::vector<u64>>> part_full_counts_;  // by relation: (version, counts)
   DBuf<u8> part_dead_dev_;
   u64 attached_max_sym_ = 0;
+  u64 max_sym_hint_ = ~0ull;
   u64 universe_override_ = ~0ull;
   std::vector<bool> attached_;
   void part_round_bookkeeping(u32 k, bool is_trop, std::vector<u8>& active_prev, bool count_round);

@@ -839,11 +839,12 @@ class Batch:
         if code:
             _raise(code)
         self.progs.append(prog)
-        self._keep = keep  # sk_batch_add copies the facts
+        self._keep.append(keep)  # the library reads the facts at sk_batch_load
         return idx.value
 
     def load(self):
         code = self.L.sk_batch_load(self.h)
+        self._keep = []
         if code:
             _raise(code)
 
