@@ -385,13 +385,30 @@ def bench_sort(args):
         res = alt if in_alt else keys
         ok = bool(torch.equal(res, torch.sort(src).values))
         t = float(np.median(ms))
+        # every candidate pinned in turn, the same keys (stderr)
+        per_cand = {}
+        for c in range(6):
+            ctx.force_radix(c)
+            cm = []
+            for _ in range(3):
+                keys.copy_(src)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                ctx.radix_sort_device(keys.data_ptr(), alt.data_ptr(), n, 44, w.data_ptr() if weighted else 0,
+                                      walt.data_ptr() if weighted else 0)
+                ctx.synchronize()
+                cm.append((time.perf_counter() - t1) * 1e3)
+            per_cand[c] = round(float(np.median(cm)), 3)
+        ctx.force_radix(-1)
+        log(f"weighted={weighted}: ms per sort by candidate {per_cand}")
         rec = 8 + (8 if weighted else 0)
         pass_bytes = n * 8 + 2 * rec * n
         gbs = pass_bytes * passes / (t * 1e-3) / 1e9
         out.append({"weighted": weighted, "n": n, "key_bits": 44, "digit_bits": choice["digit_bits"],
                     "items_per_thread": choice["items_per_thread"], "passes": passes, "ms_per_sort": round(t, 3),
                     "keys_per_s": round(n / (t * 1e-3), 1), "alg_gbps": round(gbs, 1),
-                    "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBPS, 4), "sorted_equals_torch": ok})
+                    "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBPS, 4), "sorted_equals_torch": ok,
+                    "ms_per_sort_by_candidate": per_cand})
         del keys, alt, w, walt, src
     ctx.close()
     print(json.dumps({"metric": "radix sort keys/s", "unit": "keys/s", "results": out}), flush=True)
