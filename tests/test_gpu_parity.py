@@ -156,6 +156,26 @@ def test_triangles_vectors(ctx):
     assert len(db.relation("tri")) == b * math.comb(h, 2) + math.comb(h, 3)
 
 
+def test_triangles_long_lists(ctx):
+    """Hubs whose oriented lists exceed the vertex kernel's hash table (1,500
+    leaves each: those rows take the per-row merge) beside a random graph
+    whose lists all fit it; the tri relation equals the oracle's and the
+    count the closed form plus brute force."""
+    h, b = 5, 1500
+    edges = [(i, j) for i in range(h) for j in range(i + 1, h)] + [(hub, h + k) for k in range(b) for hub in range(h)]
+    rnd = random.Random(17)
+    base = h + b
+    rand = set()
+    while len(rand) < 2500:
+        x, y = rnd.randrange(300), rnd.randrange(300)
+        if x != y:
+            rand.add((base + min(x, y), base + max(x, y)))
+    edges += sorted(rand)
+    db, _ = compare(ctx, triangle_program(), bulk=tri_bulk(edges), label="tri_long")
+    want = b * math.comb(h, 2) + math.comb(h, 3) + brute_triangles([(x - base, y - base) for x, y in rand], 300)
+    assert len(db.relation("tri")) == want
+
+
 def test_routing_workload(ctx):
     d = SymbolDict()
     edb = []
