@@ -15,17 +15,17 @@ import pytest
 
 import strata_k as sk
 from oracle import oracle as O
-from strata_k.core_ir import BOOL, CoreProgram, Sym, atom, pos, pred, rule, tc_program, v
+from strata_k.core_ir import BOOL, Agg, CoreProgram, Sym, atom, neg, pos, pred, rule, tc_program, v
 
 pytestmark = pytest.mark.gpu
 
 
 def rows(rel):
-    return [tuple(g.id for g in t) for t, _ in rel.rows()]
+    return [tuple(g.id if hasattr(g, "id") else ("int", g.value) for g in t) for t, _ in rel.rows()]
 
 
 def oracle_rows(rel):
-    return [tuple(x[1] for x in t) for t, _ in rel.rows()]
+    return [tuple(x[1] if x[0] == "sym" else ("int", x[1]) for x in t) for t, _ in rel.rows()]
 
 
 def random_edges(seed, n, m):
@@ -113,6 +113,21 @@ def test_second_rule_pushes(ctx):
     edb = [("edge", (Sym(a), Sym(b)), None) for a, b in random_edges(11, 120, 150)]
     edb += [("link", (Sym(a), Sym(b)), None) for a, b in random_edges(12, 120, 90)]
     check_modes(ctx, prog, edb, "two_rules")
+
+
+def test_later_strata_read_the_closure(ctx):
+    """The closure's stratum ends in device-driven rounds; later strata read
+    its full state: a count aggregate per node and a negation."""
+    prog = CoreProgram([pred("edge", 2, BOOL, 0), pred("node", 1, BOOL, 0), pred("path", 2, BOOL, 0),
+                        pred("reach_cnt", 2, BOOL, 1), pred("unreached", 2, BOOL, 1)],
+                       [rule(atom("path", v(0), v(1)), [pos("edge", v(0), v(1))], 2, 0),
+                        rule(atom("path", v(0), v(2)), [pos("edge", v(0), v(1)), pos("path", v(1), v(2))], 3, 0),
+                        rule(atom("reach_cnt", v(0), Agg("count", 1)), [pos("path", v(0), v(1))], 2, 1),
+                        rule(atom("unreached", v(0), v(1)), [pos("node", v(0)), pos("node", v(1)),
+                                                              neg("path", v(0), v(1))], 2, 1)], 2)
+    edb = [("edge", (Sym(a), Sym(b)), None) for a, b in random_edges(77, 90, 170)]
+    edb += [("node", (Sym(i),), None) for i in range(0, 90, 3)]
+    check_modes(ctx, prog, edb, "strata")
 
 
 def test_batched_tc_matches_plain(ctx):
