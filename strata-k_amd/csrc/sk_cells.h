@@ -290,7 +290,7 @@ void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* 
 // the device status (as cells_compact_into, without the host read), then
 // the round's bookkeeping: a status sets leave = 2; otherwise iters += 1,
 // n and pc recorded, leave = 3 past cap_left rounds, done on an empty delta,
-// leave = 1 when pc * 4 < push_limit.  cnt / off / pcb hold
+// leave = 1 when pc < push_limit.  cnt / off / pcb hold
 // ncells / (kBlock * 32) entries, tot 8.
 template <class K>
 void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,

@@ -518,7 +518,7 @@ __global__ void k_loop_book_trop(LoopState* ls, const u64* __restrict__ tot, u64
   ls->pc = tot[1];
   if (ls->iters > cap_left) ls->leave = 3;
   else if (tot[0] == 0) ls->done = 1;
-  else if (tot[1] * 4 < push_limit) ls->leave = 1;
+  else if (tot[1] < push_limit) ls->leave = 1;
 }
 
 template <class K>

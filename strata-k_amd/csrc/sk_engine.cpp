@@ -1081,8 +1081,10 @@ static int col_index(const std::vector<int>& slots, int slot) {
 }
 
 // Destination-side evaluation of a delta variant, when the rule has the
-// pull shape and this round's push join would produce at least a quarter
-// of |E| outputs.  Returns false to run the push join instead.
+// pull shape and this round's push join would produce at least |E| outputs
+// (a pull round reads all of E's index; measured on gfx950, a push join of
+// about |E| outputs costs as much).  Returns false to run the push join
+// instead.
 bool Engine::try_pull(const RulePlan& pl) {
   if (pl.dpos < 0 || pull_mode_ == 0) return false;
   const int hr = P_.rules[pl.rule].head.rel;
@@ -1107,7 +1109,7 @@ bool Engine::try_pull(const RulePlan& pl) {
       cand = push_cost<K>(x, (const K*)H.delta.k.ptr(), H.delta.n, 2, H.bits, pr->R, ixj.csr.get());
     });
   }
-  if (pull_mode_ == 1 && cand * 4 < E.full.n) return false;
+  if (pull_mode_ == 1 && cand < E.full.n) return false;
   const PullDesc pd = pull_desc(*pr, *dn, hr);
   // algorithmic bytes: every index row of E (j, weight, its source's 16-byte
   // row record) once, every candidate's delta cell (lane byte, weight) once

@@ -192,14 +192,15 @@ def reach_program() -> CoreProgram:
 def test_pull_rounds_all_row_widths(ctx, sem, nsrc):
     """Destination-side (pull) rounds at every row width: the lane column
     holds nsrc sources, so a cell row has 1, 4, 32 or 64 lanes.  Tuples,
-    weights and round counts equal the oracle's and the sorted path's, and
-    the pull kernel must have run in at least one round (Bool rows of whole
+    weights and round counts equal the oracle's and the sorted path's, and,
+    with pulls forced, the pull kernel must have run (Bool rows of whole
     words take the word-parallel pull)."""
     w = wl.config2(seed=5 + nsrc, scale=8, edge_factor=16, n_sources=nsrc)
     prog = cost_program() if sem == TROP else reach_program()
     bulk = w.bulk if sem == TROP else {p: (cols, None, None) for p, (cols, _, _) in w.bulk.items()}
     both(ctx, prog, bulk=bulk, label=f"pull-{sem}-{nsrc}")
     run = ctx.prepare(prog)
+    run.set_pull(2)  # pull whenever the shape allows, whatever the round's size
     for p, (cols, wt, inf) in bulk.items():
         run.add_sym(p, cols, wt, inf)
     run.load()
