@@ -176,6 +176,25 @@ def test_triangles_long_lists(ctx):
     assert len(db.relation("tri")) == want
 
 
+def test_routing_loaded_and_rendered(ctx):
+    """The routing workload loaded from its TSV (first-seen interning, Trop
+    weight column) and rendered canonically: the engine's full output text
+    equals the oracle's, and the queried lines are the reference's pinned
+    ones (crates/strata-cli/tests/workloads.rs:25-32)."""
+    from strata_k.io import SymbolDict as Dict, load_input, render_db
+    from test_io import _OracleDb, routing_program
+    d = Dict()
+    facts = load_input(__file__.replace("test_gpu_parity.py", "golden/routing_links.tsv"), "link", 2, TROP,
+                       ["node", "node"], d)
+    prog = routing_program()
+    db = ctx.run_semi_naive(prog, facts)
+    ref = O.run(prog, facts)
+    assert render_db(db, d) == render_db(_OracleDb(ref), d)
+    q = [("route", (Sym(d.get(a)), Sym(d.get(b)))) for a, b in (("n0_0", "n7_7"), ("n0_0", "n4_4"), ("n7_7", "n0_0"))]
+    assert render_db(db, d, filter=q).splitlines() == ["route(n0_0, n4_4) = 2", "route(n0_0, n7_7) = 5",
+                                                       "route(n7_7, n0_0) = 44"]
+
+
 def test_routing_workload(ctx):
     d = SymbolDict()
     edb = []
