@@ -414,6 +414,98 @@ def bench_sort(args):
     print(json.dumps({"metric": "radix sort keys/s", "unit": "keys/s", "results": out}), flush=True)
 
 
+def asp_program(rng, n_rules, n_atoms, max_body):
+    """A random ground program with the shape of the reference's large
+    simplify-filter case (crates/strata-gpu/tests/asp_diff.rs:24-58,72-77):
+    1/8 constraints, bodies of 0..max_body signed literals, ~75% of atoms
+    possible, a third of those certain (drawn with numpy, not its LCG)."""
+    head = rng.integers(0, n_atoms, n_rules, dtype=np.int32)
+    head[rng.integers(0, 8, n_rules) == 0] = -1
+    blen = rng.integers(0, max_body + 1, n_rules)
+    bs = np.zeros(n_rules + 1, np.uint32)
+    np.cumsum(blen, out=bs[1:])
+    n_lits = int(bs[-1])
+    lits = (rng.integers(0, n_atoms, n_lits, dtype=np.int32) + 1) * np.where(rng.integers(0, 2, n_lits) == 1, 1, -1)
+    poss = (rng.integers(0, 4, n_atoms) != 0).astype(np.uint8)
+    cert = (poss.astype(bool) & (rng.integers(0, 3, n_atoms) == 0)).astype(np.uint8)
+    return head, bs, lits.astype(np.int32), poss, cert
+
+
+def bench_asp(args):
+    """The ASP grounding simplify filter over a device-resident ground program
+    of 4M rules on 4096 atoms (bodies up to 6 literals); value = rules
+    filtered per second.  The dominant kernel's roofline uses its algorithmic
+    bytes (per rule 9: end offset read, alive flag and kept count written;
+    per literal 5: literal and atom state read)."""
+    import torch
+    import strata_k as sk
+    from oracle import oracle as O
+    n_rules, n_atoms, max_body = args.asp_rules, 4096, 6
+    rng = np.random.default_rng(args.seed)
+    head, bs, lits, poss, cert = asp_program(rng, n_rules, n_atoms, max_body)
+    n_lits = int(bs[-1])
+    ctx = sk.Context(0)
+    dev = "cuda:0"
+    t = {k: torch.from_numpy(a).to(dev) for k, a in
+         (("head", head), ("bs", bs.view(np.int32)), ("lits", lits), ("poss", poss), ("cert", cert))}
+    oh = torch.empty(n_rules, dtype=torch.int32, device=dev)
+    ob = torch.empty(n_rules + 1, dtype=torch.int32, device=dev)
+    ol = torch.empty(max(1, n_lits), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def call():
+        return ctx.asp_simplify_filter_device(t["head"].data_ptr(), t["bs"].data_ptr(), t["lits"].data_ptr(), n_rules,
+                                              n_lits, t["poss"].data_ptr(), t["cert"].data_ptr(), n_atoms,
+                                              oh.data_ptr(), ob.data_ptr(), ol.data_ptr())
+    for _ in range(args.warmup):
+        call()
+    ctx.synchronize()
+    ctx.kernel_reset()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nr, nl = call()
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / max(1, args.steps)
+    ctx.set_profiling(False)
+    recs = ctx.kernel_records()
+    top = max((r for r in recs if r["bytes"] > 0), key=lambda r: r["ms"], default=None)
+    # exactness on a sample of the same program: the first 200K rules
+    k = min(n_rules, 200_000)
+    sample = (head[:k], bs[:k + 1], lits[:int(bs[k])], poss, cert)
+    got = sk.asp_simplify_filter(*sample, ctx=ctx)
+    want = O.asp_simplify_filter(*sample)
+    exact = got[0].tolist() == want[0] and got[1].tolist() == want[1] and got[2].tolist() == want[2]
+    # PCIe-inclusive rate of the host-buffer entry point (stderr only)
+    t1 = time.perf_counter()
+    sk.asp_simplify_filter(head, bs, lits, poss, cert, ctx=ctx)
+    log(f"host-buffer entry point (copies in and out included): {n_rules / (time.perf_counter() - t1) / 1e6:.1f}M rules/s")
+    if args.kernels:
+        for r in sorted(recs, key=lambda r: -r["ms"]):
+            log(f"{r['name']:<24} launches {r['launches']:>4}  ms {r['ms']:.3f}  GB {r['bytes'] / 1e9:.3f}")
+    cpu = None
+    if not args.no_cpu:
+        m = min(n_rules, 4_000_000)
+        sample = (head[:m], bs[:m + 1], lits[:int(bs[m])], poss, cert)
+        t2 = time.perf_counter()
+        O.asp_simplify_filter(*sample)
+        cs = time.perf_counter() - t2
+        cpu = {"value": round(m / cs, 1), "unit": "rules/s", "cores": 1, "kind": "port",
+               "sample": f"the first {m} rules of the same program through the oracle's restatement of "
+                         f"simplify_filter_cpu: {cs:.1f} s",
+               "host_cpu": platform.processor() or platform.machine()}
+    line = {"metric": "ASP simplify-filter rules/s", "value": round(n_rules / (ms * 1e-3), 1), "unit": "rules/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i32",
+            "data": "synthetic", "exact_vs_oracle_sample": exact,
+            "config": {"workload": "ASP simplify filter, device-resident ground program", "rules": n_rules,
+                       "atoms": n_atoms, "max_body": max_body, "literals": n_lits, "kept_rules": nr,
+                       "kept_literals": nl},
+            "roofline": roofline_of(top, measured_traffic(top["name"]) if top else None), "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,6 +528,8 @@ def main():
                          "rounds inside graph launches are timed as one record per launch")
     ap.add_argument("--sort", action="store_true", help="time the device radix sort primitive")
     ap.add_argument("--sort-log2", type=int, default=26)
+    ap.add_argument("--asp", action="store_true", help="time the ASP grounding simplify filter")
+    ap.add_argument("--asp-rules", type=int, default=4_000_000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one known-size copy kernel (PMC counter calibration)")
     args = ap.parse_args()
@@ -447,6 +541,9 @@ def main():
         copy_probe(args.copy_probe)
     if args.sort:
         bench_sort(args)
+        return
+    if args.asp:
+        bench_asp(args)
         return
     {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3, 4: bench_cfg4, 5: bench_cfg5}[args.config](args)
 

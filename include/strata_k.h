@@ -241,6 +241,13 @@ uint32_t sk_run_kernel_count(const sk_run* run);
 sk_status sk_run_kernel_record(const sk_run* run, uint32_t i, char* name, size_t name_len, uint32_t* kernel_class,
                                sk_kernel_stats* out);
 void sk_set_profiling(sk_ctx* ctx, int enabled);
+/* Per-kernel records of context-level calls (the crate functions, sorts and
+ * the ASP filter) with profiling on, accumulated since the last
+ * sk_ctx_kernel_reset or sk_run_execute (no reference counterpart). */
+sk_status sk_ctx_kernel_reset(sk_ctx* ctx);
+uint32_t sk_ctx_kernel_count(sk_ctx* ctx);
+sk_status sk_ctx_kernel_record(const sk_ctx* ctx, uint32_t i, char* name, size_t name_len, uint32_t* kernel_class,
+                               sk_kernel_stats* out);
 /* Wall-clock milliseconds of the last sk_run_execute (host timer). */
 double sk_run_execute_ms(const sk_run* run);
 
@@ -280,6 +287,34 @@ sk_status sk_count_4cliques(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, u
 sk_status sk_intern_terms(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_t* out_ids);
 /* Σ_v d(v)^2 over the id-oriented simple graph (lib.rs:138-143). */
 uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint64_t n);
+
+/* ----------------------------------------------- ASP simplification */
+
+/* The grounding simplification filter (crates/strata-gpu/src/asp.rs:53-99,
+ * simplify_filter at asp.rs:124-139) over a ground program in CSR form: rule
+ * i has head head[i] (an atom id, or -1 for a constraint) and body
+ * body_lit[body_start[i] .. body_start[i+1]), literals signed +(atom+1) /
+ * -(atom+1).  A rule dies when a positive body atom is impossible (!poss)
+ * or a negative one certain (cert); in a kept rule, certain positive and
+ * impossible negative literals are dropped.  Outputs (caller buffers of
+ * n_rules heads, n_rules + 1 offsets, body_start[n_rules] literals) keep
+ * input order.  A literal naming no atom below n_atoms, or decreasing
+ * offsets, is SK_ERR_INVALID_ARGUMENT. */
+sk_status sk_asp_simplify_filter(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
+                                 const int32_t* body_lit, uint64_t n_rules, const uint8_t* poss,
+                                 const uint8_t* cert, uint64_t n_atoms, int32_t* out_head,
+                                 uint32_t* out_body_start, int32_t* out_body_lit, uint64_t* out_n_rules,
+                                 uint64_t* out_n_lits);
+/* The same filter over device-resident buffers (no reference counterpart:
+ * for callers whose ground program is already in HBM, and for
+ * benchmarking).  n_lits = body_start[n_rules]; outputs are device buffers
+ * of the sizes above; the two counts are returned to the host once the
+ * filter has finished. */
+sk_status sk_asp_simplify_filter_device(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
+                                        const int32_t* body_lit, uint64_t n_rules, uint64_t n_lits,
+                                        const uint8_t* poss, const uint8_t* cert, uint64_t n_atoms,
+                                        int32_t* out_head, uint32_t* out_body_start, int32_t* out_body_lit,
+                                        uint64_t* out_n_rules, uint64_t* out_n_lits);
 
 /* ------------------------------------------------------- sort primitives */
 

@@ -7,6 +7,10 @@ and bench.py's cpu_baseline leg use this module; the engine never does.
 
 Programs are duck-typed: any object with the attributes of the Core-IR model
 (predicates / rules / num_strata, CoreAtom.pred / args, Var.slot, ...) works.
+
+Also here: ``asp_simplify_filter``, a plain-Python restatement of the
+reference's CPU grounding-simplification filter (crates/strata-gpu/src/asp.rs:53-99),
+the checker of the device filter.
 """
 from __future__ import annotations
 
@@ -316,3 +320,36 @@ def tri_count_oriented(a, b, nnodes: int) -> int:
     a = np.ascontiguousarray(a, dtype=np.uint32)
     b = np.ascontiguousarray(b, dtype=np.uint32)
     return int(lib().orc_tri_count_oriented(a.ctypes.data, b.ctypes.data, len(a), nnodes))
+
+
+# ------------------------------------------------------ ASP simplification
+
+def asp_simplify_filter(head, body_start, body_lit, poss, cert):
+    """CPU restatement of simplify_filter_cpu (crates/strata-gpu/src/asp.rs:53-99):
+    per rule in order, a positive literal over an impossible atom or a negative
+    one over a certain atom kills the rule; certain positive and impossible
+    negative literals are dropped; kept literals stay in order."""
+    oh, ob, ol = [], [0], []
+    for i in range(len(head)):
+        kept, alive = [], True
+        for lit in body_lit[body_start[i]:body_start[i + 1]]:
+            lit = int(lit)
+            a, positive = abs(lit) - 1, lit > 0
+            if positive:
+                if not poss[a]:
+                    alive = False
+                    break
+                if not cert[a]:
+                    kept.append(lit)
+            else:
+                if cert[a]:
+                    alive = False
+                    break
+                if poss[a]:
+                    kept.append(lit)
+        if not alive:
+            continue
+        oh.append(int(head[i]))
+        ol.extend(kept)
+        ob.append(len(ol))
+    return oh, ob, ol

@@ -128,6 +128,41 @@ void sk_set_profiling(sk_ctx* ctx, int enabled) {
   ctx->c.timer.enabled = enabled != 0;
 }
 
+sk_status sk_ctx_kernel_reset(sk_ctx* ctx) {
+  if (!ctx) return SK_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->c.device);
+  EvalErr tmp;
+  sk_status s = guarded(&tmp, [&] { ctx->c.timer.collect(); });
+  ctx->c.timer.reset();
+  ctx->kernels.clear();
+  return s;
+}
+
+uint32_t sk_ctx_kernel_count(sk_ctx* ctx) {
+  if (!ctx) return 0;
+  (void)hipSetDevice(ctx->c.device);
+  EvalErr tmp;
+  guarded(&tmp, [&] { ctx->c.timer.collect(); });
+  ctx->kernels.clear();
+  for (auto& kv : ctx->c.timer.kernels) ctx->kernels.push_back(kv.second);
+  return (uint32_t)ctx->kernels.size();
+}
+
+sk_status sk_ctx_kernel_record(const sk_ctx* ctx, uint32_t i, char* name, size_t name_len, uint32_t* cls,
+                               sk_kernel_stats* out) {
+  if (!ctx || i >= ctx->kernels.size()) return SK_ERR_INVALID_ARGUMENT;
+  const KRecord& k = ctx->kernels[i];
+  if (name && name_len) std::snprintf(name, name_len, "%s", k.name.c_str());
+  if (cls) *cls = (uint32_t)k.cls;
+  if (out) {
+    out->launches = k.launches;
+    out->ms = k.ms;
+    out->bytes = k.bytes;
+    out->items = k.items;
+  }
+  return SK_OK;
+}
+
 sk_status sk_prepare(sk_ctx* ctx, const sk_program* prog, sk_run** out) {
   if (!ctx || !prog || !out) return SK_ERR_INVALID_ARGUMENT;
   *out = nullptr;

@@ -19,6 +19,9 @@ using namespace sk;
 
 namespace sk {
 u64 k4_count(const Exec& x, const u64* keys, u64 ne, const u64* csr, const u32* adj, int bits);
+bool asp_simplify(const Exec& x, const int32_t* head, const u32* bs, const int32_t* lits, u64 n_rules, u64 n_lits,
+                  const u8* poss, const u8* cert, u64 n_atoms, int32_t* out_head, u32* out_bs, int32_t* out_lits,
+                  u64* out_rules, u64* out_lits_n);
 void intern_lookup(const Exec& x, const u64* table, u64 nt, const u64* q, u64 nq, u32* ids);
 }  // namespace sk
 
@@ -378,6 +381,67 @@ sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, i
   *items = radix_candidate_items(best, wide != 0);
   if (est_ms) *est_ms = bt;
   return SK_OK;
+}
+
+sk_status sk_asp_simplify_filter(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
+                                 const int32_t* body_lit, uint64_t n_rules, const uint8_t* poss,
+                                 const uint8_t* cert, uint64_t n_atoms, int32_t* out_head,
+                                 uint32_t* out_body_start, int32_t* out_body_lit, uint64_t* out_n_rules,
+                                 uint64_t* out_n_lits) {
+  if (!ctx || !body_start || !out_body_start || !out_n_rules || !out_n_lits || (n_rules && (!head || !out_head)) ||
+      (n_atoms && (!poss || !cert)))
+    return SK_ERR_INVALID_ARGUMENT;
+  const uint64_t n_lits = body_start[n_rules];
+  if (n_lits && (!body_lit || !out_body_lit)) return SK_ERR_INVALID_ARGUMENT;
+  bool ok = true;
+  sk_status s = sort_guard(ctx, [&](Ctx& c) {
+    Exec x = c.exec();
+    Pool* p = c.pool.get();
+    DBuf<int32_t> dh(p, n_rules ? n_rules : 1), dl(p, n_lits ? n_lits : 1), oh(p, n_rules ? n_rules : 1),
+        ol(p, n_lits ? n_lits : 1);
+    DBuf<u32> db(p, n_rules + 1), ob(p, n_rules + 1);
+    DBuf<u8> dp(p, n_atoms ? n_atoms : 1), dc(p, n_atoms ? n_atoms : 1);
+    if (n_rules) SK_HIP(hipMemcpyAsync(dh.get(), head, n_rules * 4, hipMemcpyHostToDevice, c.stream));
+    SK_HIP(hipMemcpyAsync(db.get(), body_start, (n_rules + 1) * 4, hipMemcpyHostToDevice, c.stream));
+    if (n_lits) SK_HIP(hipMemcpyAsync(dl.get(), body_lit, n_lits * 4, hipMemcpyHostToDevice, c.stream));
+    if (n_atoms) {
+      SK_HIP(hipMemcpyAsync(dp.get(), poss, n_atoms, hipMemcpyHostToDevice, c.stream));
+      SK_HIP(hipMemcpyAsync(dc.get(), cert, n_atoms, hipMemcpyHostToDevice, c.stream));
+    }
+    u64 nr = 0, nl = 0;
+    ok = asp_simplify(x, dh.get(), db.get(), dl.get(), n_rules, n_lits, dp.get(), dc.get(), n_atoms, oh.get(),
+                      ob.get(), ol.get(), &nr, &nl);
+    if (!ok) return;
+    if (nr) SK_HIP(hipMemcpyAsync(out_head, oh.get(), nr * 4, hipMemcpyDeviceToHost, c.stream));
+    SK_HIP(hipMemcpyAsync(out_body_start, ob.get(), (nr + 1) * 4, hipMemcpyDeviceToHost, c.stream));
+    if (nl) SK_HIP(hipMemcpyAsync(out_body_lit, ol.get(), nl * 4, hipMemcpyDeviceToHost, c.stream));
+    SK_HIP(hipStreamSynchronize(c.stream));
+    *out_n_rules = nr;
+    *out_n_lits = nl;
+  });
+  if (s) return s;
+  return ok ? SK_OK : SK_ERR_INVALID_ARGUMENT;
+}
+
+sk_status sk_asp_simplify_filter_device(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
+                                        const int32_t* body_lit, uint64_t n_rules, uint64_t n_lits,
+                                        const uint8_t* poss, const uint8_t* cert, uint64_t n_atoms,
+                                        int32_t* out_head, uint32_t* out_body_start, int32_t* out_body_lit,
+                                        uint64_t* out_n_rules, uint64_t* out_n_lits) {
+  if (!ctx || !body_start || !out_body_start || !out_n_rules || !out_n_lits || (n_rules && (!head || !out_head)) ||
+      (n_atoms && (!poss || !cert)) || (n_lits && (!body_lit || !out_body_lit)))
+    return SK_ERR_INVALID_ARGUMENT;
+  bool ok = true;
+  sk_status s = sort_guard(ctx, [&](Ctx& c) {
+    Exec x = c.exec();
+    u64 nr = 0, nl = 0;
+    ok = asp_simplify(x, head, body_start, body_lit, n_rules, n_lits, poss, cert, n_atoms, out_head, out_body_start,
+                      out_body_lit, &nr, &nl);
+    *out_n_rules = nr;
+    *out_n_lits = nl;
+  });
+  if (s) return s;
+  return ok ? SK_OK : SK_ERR_INVALID_ARGUMENT;
 }
 
 uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint64_t n) {

@@ -436,4 +436,5 @@ class Engine {
 // The C ABI's context handle.
 struct sk_ctx {
   sk::Ctx c;
+  std::vector<sk::KRecord> kernels;  // snapshot taken by sk_ctx_kernel_count
 };

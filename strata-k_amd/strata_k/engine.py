@@ -156,6 +156,10 @@ SIGNATURES = [
     ("sk_run_kernel_record", C.c_int, [_VP, _U32, C.c_char_p, C.c_size_t, C.POINTER(_U32),
                                        C.POINTER(SkKernelStats)]),
     ("sk_set_profiling", None, [_VP, C.c_int]),
+    ("sk_ctx_kernel_reset", C.c_int, [_VP]),
+    ("sk_ctx_kernel_count", _U32, [_VP]),
+    ("sk_ctx_kernel_record", C.c_int, [_VP, _U32, C.c_char_p, C.c_size_t, C.POINTER(_U32),
+                                       C.POINTER(SkKernelStats)]),
     ("sk_run_execute_ms", C.c_double, [_VP]),
     ("sk_run_semi_naive", C.c_int, [_VP, C.POINTER(SkProgram), _U64, _VP, _VP, _VP, _VP, _VP, _VP,
                                     C.POINTER(_VP)]),
@@ -178,6 +182,10 @@ SIGNATURES = [
     ("sk_batch_result_size", _I64, [_VP, _U32, C.c_char_p]),
     ("sk_batch_result_read_sym", C.c_int, [_VP, _U32, C.c_char_p, _VP, _VP, _VP]),
     ("sk_batch_execute_ms", C.c_double, [_VP]),
+    ("sk_asp_simplify_filter", C.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _U64, _VP, _VP, _VP,
+                                         C.POINTER(_U64), C.POINTER(_U64)]),
+    ("sk_asp_simplify_filter_device", C.c_int, [_VP, _VP, _VP, _VP, _U64, _U64, _VP, _VP, _U64, _VP, _VP, _VP,
+                                                C.POINTER(_U64), C.POINTER(_U64)]),
     ("sk_sort_u64", C.c_int, [_VP, _VP, _U64, _VP]),
     ("sk_sort_unique_u64", C.c_int, [_VP, _VP, _U64, _VP, C.POINTER(_U64)]),
     ("sk_sort_unique_pairs", C.c_int, [_VP, _VP, _VP, _U64, _VP, _VP, C.POINTER(_U64)]),
@@ -418,6 +426,30 @@ class Context:
         if code:
             _raise(code)
         return bool(flag.value)
+
+    def asp_simplify_filter_device(self, head_ptr: int, bs_ptr: int, lits_ptr: int, n_rules: int, n_lits: int,
+                                   poss_ptr: int, cert_ptr: int, n_atoms: int, out_head_ptr: int, out_bs_ptr: int,
+                                   out_lits_ptr: int):
+        """The ASP simplify filter over device-resident buffers; returns the
+        kept rule and literal counts once the filter has finished."""
+        nr, nl = C.c_uint64(), C.c_uint64()
+        code = self.L.sk_asp_simplify_filter_device(self.h, head_ptr or None, bs_ptr, lits_ptr or None, int(n_rules),
+                                                    int(n_lits), poss_ptr or None, cert_ptr or None, int(n_atoms),
+                                                    out_head_ptr or None, out_bs_ptr, out_lits_ptr or None,
+                                                    C.byref(nr), C.byref(nl))
+        if code:
+            _raise(code)
+        return nr.value, nl.value
+
+    def kernel_reset(self):
+        """Clear the context-level kernel records (crate functions, sorts, the
+        ASP filter) collected while profiling is on."""
+        code = self.L.sk_ctx_kernel_reset(self.h)
+        if code:
+            _raise(code)
+
+    def kernel_records(self) -> List[dict]:
+        return _kernel_records(self.L, self.h, self.L.sk_ctx_kernel_count, self.L.sk_ctx_kernel_record)
 
     # ------------------------------------------------------------ staged run
     def prepare(self, prog) -> "Run":
@@ -786,6 +818,29 @@ def sort_min_pairs(triples, ctx: Optional[Context] = None) -> List[Tuple[int, in
         _raise(code)
     k = m.value
     return list(zip(oa[:k].tolist(), ob[:k].tolist(), ow[:k].tolist()))
+
+
+def asp_simplify_filter(head, body_start, body_lit, poss, cert, ctx: Optional[Context] = None):
+    """simplify_filter (crates/strata-gpu/src/asp.rs:124-139) over a ground
+    program in CSR form; returns (head, body_start, body_lit) as numpy arrays."""
+    ctx = ctx or default_context()
+    h = np.ascontiguousarray(head, dtype=np.int32)
+    bs = np.ascontiguousarray(body_start, dtype=np.uint32)
+    bl = np.ascontiguousarray(body_lit, dtype=np.int32)
+    p = np.ascontiguousarray(poss, dtype=np.uint8)
+    c = np.ascontiguousarray(cert, dtype=np.uint8)
+    if len(bs) != len(h) + 1 or len(p) != len(c):
+        raise ValueError("body_start needs n_rules + 1 offsets; poss and cert one entry per atom")
+    oh = np.empty(max(1, len(h)), np.int32)
+    ob = np.empty(len(h) + 1, np.uint32)
+    ol = np.empty(max(1, len(bl)), np.int32)
+    nr, nl = C.c_uint64(), C.c_uint64()
+    code = ctx.L.sk_asp_simplify_filter(ctx.h, h.ctypes.data, bs.ctypes.data, bl.ctypes.data, len(h), p.ctypes.data,
+                                        c.ctypes.data, len(p), oh.ctypes.data, ob.ctypes.data, ol.ctypes.data,
+                                        C.byref(nr), C.byref(nl))
+    if code:
+        _raise(code)
+    return oh[:nr.value].copy(), ob[:nr.value + 1].copy(), ol[:nl.value].copy()
 
 
 def binary_plan_intermediate(edges) -> int:

@@ -270,3 +270,23 @@ def test_indexed_equals_full_scan_fuzz():
         b = O.run(prog, edb, indexed=False)
         assert _same(a, b), seed
         assert a.iterations == b.iterations
+
+
+def test_asp_filter_restatement_truth_table():
+    """keep_literal (crates/strata-gpu/src/asp.rs:53-68): a positive literal
+    dies on an impossible atom and is dropped on a certain one; a negative
+    literal dies on a certain atom and is dropped on an impossible one;
+    otherwise kept.  Atoms: 0 impossible, 1 possible, 2 certain."""
+    poss, cert = [0, 1, 1], [0, 0, 1]
+    cases = [([1], None), ([2], [2]), ([3], []), ([-1], []), ([-2], [-2]), ([-3], None),
+             ([2, -2, 3, -1], [2, -2]), ([2, -3, 1], None), ([], [])]
+    head = list(range(len(cases)))
+    bs, bl = [0], []
+    for lits, _ in cases:
+        bl += lits
+        bs.append(len(bl))
+    oh, ob, ol = O.asp_simplify_filter(head, bs, bl, poss, cert)
+    want = [(h, k) for h, (_, k) in zip(head, cases) if k is not None]
+    assert oh == [h for h, _ in want]
+    assert [ol[ob[i]:ob[i + 1]] for i in range(len(oh))] == [k for _, k in want]
+    assert ob[-1] == len(ol)
