@@ -291,7 +291,7 @@ uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint6
 /* ----------------------------------------------- ASP simplification */
 
 /* The grounding simplification filter (crates/strata-gpu/src/asp.rs:53-99,
- * simplify_filter at asp.rs:124-139) over a ground program in CSR form: rule
+ * simplify_filter at asp.rs:121-135) over a ground program in CSR form: rule
  * i has head head[i] (an atom id, or -1 for a constraint) and body
  * body_lit[body_start[i] .. body_start[i+1]), literals signed +(atom+1) /
  * -(atom+1).  A rule dies when a positive body atom is impossible (!poss)

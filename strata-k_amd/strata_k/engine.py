@@ -821,7 +821,7 @@ def sort_min_pairs(triples, ctx: Optional[Context] = None) -> List[Tuple[int, in
 
 
 def asp_simplify_filter(head, body_start, body_lit, poss, cert, ctx: Optional[Context] = None):
-    """simplify_filter (crates/strata-gpu/src/asp.rs:124-139) over a ground
+    """simplify_filter (crates/strata-gpu/src/asp.rs:121-135) over a ground
     program in CSR form; returns (head, body_start, body_lit) as numpy arrays."""
     ctx = ctx or default_context()
     h = np.ascontiguousarray(head, dtype=np.int32)

@@ -57,6 +57,26 @@ def test_large_rule_set(ctx):
     assert 0 < len(got[0]) < 300_000
 
 
+@pytest.mark.parametrize("n_atoms", [16384, 16385, 100_000])
+def test_atom_table_sizes(ctx, n_atoms):
+    """Atom states staged in LDS (up to 16384 atoms) and gathered from the
+    global tables beyond."""
+    prog = random_program(n_atoms, 50_000, n_atoms, 6)
+    assert same(sk.asp_simplify_filter(*prog, ctx=ctx), O.asp_simplify_filter(*prog))
+
+
+def test_long_bodies(ctx):
+    """Bodies up to 40 literals (read back in several chunks), with mostly
+    possible atoms so that long rules survive."""
+    for seed in range(6):
+        head, bs, bl, poss, cert = random_program(seed + 17, 6000, 300, 40)
+        poss = [1 if i % 50 else 0 for i in range(300)]
+        cert = [1 if i % 7 == 0 and poss[i] else 0 for i in range(300)]
+        want = O.asp_simplify_filter(head, bs, bl, poss, cert)
+        assert max(want[1][i + 1] - want[1][i] for i in range(len(want[0]))) > 8
+        assert same(sk.asp_simplify_filter(head, bs, bl, poss, cert, ctx=ctx), want), seed
+
+
 def test_empty_and_bodiless(ctx):
     assert same(sk.asp_simplify_filter([], [0], [], [1], [0], ctx=ctx), ([], [0], []))
     assert same(sk.asp_simplify_filter([3, -1], [0, 0, 0], [], [1], [1], ctx=ctx), ([3, -1], [0, 0, 0], []))
@@ -95,4 +115,7 @@ def test_device_resident_entry_and_kernel_records(ctx):
     assert ol[:nl].cpu().numpy().tolist() == want[2].tolist()
     recs = {r["name"]: r for r in ctx.kernel_records()}
     assert recs["asp_count"]["launches"] == 2 and recs["asp_emit"]["launches"] == 2
-    assert recs["asp_count"]["bytes"] == 2 * (9.0 * len(head) + 5.0 * len(bl))
+    # per rule its end offset, per literal the literal, per block of rules two totals
+    rest = recs["asp_count"]["bytes"] - 2 * (4.0 * len(head) + 4.0 * len(bl))
+    assert rest > 0 and rest % 32 == 0
+    assert recs["asp_emit"]["bytes"] == 2 * (4.0 * len(head) + 4.0 * len(bl) + 12.0 * nr + 4.0 * nl)
