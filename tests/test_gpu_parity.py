@@ -239,6 +239,22 @@ def test_inf_weights(ctx):
     assert got[(0, 1)] is INF and got[(0, 2)] is INF and got[(0, 3)] == 100 and got[(1, 3)] == 5
 
 
+def test_json_program_and_high_ir_facts(ctx):
+    """A serialised Core-IR program and High-IR facts with an "inf" weight
+    (docs/ir-encoding.md rule 5) run on the device as on the oracle."""
+    import json
+    from strata_k import io as IO
+    doc = json.dumps(IO.core_program_to_json(tc_program(TROP)))
+    c = lambda x: {"kind": "const", "data": {"name": x}}  # noqa: E731
+    items = [{"kind": "fact", "data": {"atom": {"pred": "edge", "args": [c(a), c(b)]}, "weight": w}}
+             for a, b, w in (("s", "a", "inf"), ("a", "b", 4), ("s", "b", "inf"), ("b", "t", 1), ("s", "t", 100))]
+    d = IO.SymbolDict()
+    edb = IO.high_ir_facts({"ir_version": "0.1.0", "items": items}, d)
+    db, ref = compare(ctx, IO.core_program_from_json(doc), edb, label="json")
+    text = IO.render_db(db, d)
+    assert "path(s, a) = inf\n" in text and "path(s, t) = 100\n" in text and "path(a, t) = 5\n" in text
+
+
 def test_fuzz_programs(ctx):
     for seed in range(300):
         prog, edb = random_program(seed)

@@ -9,6 +9,8 @@ import pytest
 
 from oracle import oracle as O
 from strata_k.core_ir import BOOL, INF, TROP, CoreProgram, IntV, Sym, atom, pos, pred, rule, v
+from strata_k import io as IO
+from strata_k.core_ir import Agg, tc_program
 from strata_k.io import LoadError, SymbolDict, load_input, render_db, split_csv_line
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -127,3 +129,128 @@ def test_render_inf_and_ints():
     d.intern("a")
     d.intern("b")
     assert render_db(D(), d) == "p(a, -4) = inf\np(b, 3) = 12\n"
+
+
+# ---------------------------------------------------------------- IR JSON
+
+def test_weight_json_reference_cases():
+    """crates/strata-ir/src/trop.rs:199-209: 42 <-> 42, +inf <-> "inf",
+    "nan" refused; serde also refuses non-i64 numbers and booleans."""
+    import json
+    assert json.dumps(IO.weight_to_json(42)) == "42"
+    assert json.dumps(IO.weight_to_json(INF)) == '"inf"'
+    assert IO.weight_from_json(json.loads("42")) == 42
+    assert IO.weight_from_json(json.loads('"inf"')) is INF
+    for bad in ['"nan"', '"5"', "1.5", "true", str(2**63), "null"]:
+        with pytest.raises(LoadError):
+            IO.weight_from_json(json.loads(bad))
+    assert IO.weight_from_json(-(2**63)) == -(2**63)
+
+
+# The Core-IR program of crates/strata-ir/tests/roundtrip.rs:155-186, written
+# out in serde's encoding (snake_case fields, {"kind","data"} tags).
+CORE_TC_JSON = """{
+  "predicates": [
+    {"name": "edge", "arity": 2, "semiring": "bool", "stratum": 0},
+    {"name": "path", "arity": 2, "semiring": "bool", "stratum": 0}
+  ],
+  "rules": [
+    {"head": {"pred": "path", "args": [{"kind": "var", "data": {"slot": 0}}, {"kind": "var", "data": {"slot": 2}}]},
+     "body": [
+       {"kind": "pos", "data": {"pred": "edge", "args": [{"kind": "var", "data": {"slot": 0}},
+                                                         {"kind": "var", "data": {"slot": 1}}]}},
+       {"kind": "pos", "data": {"pred": "path", "args": [{"kind": "var", "data": {"slot": 1}},
+                                                         {"kind": "var", "data": {"slot": 2}}]}}],
+     "stratum": 0, "var_count": 3}
+  ],
+  "num_strata": 1
+}"""
+
+
+def test_core_program_json_roundtrip():
+    import json
+    prog = IO.core_program_from_json(CORE_TC_JSON)
+    assert prog.num_strata == 1 and len(prog.rules_in_stratum(0)) == 1
+    assert prog.predicates[1].semiring == BOOL
+    assert prog.rules[0].head.args[1].slot == 2 and prog.rules[0].var_count == 3
+    assert IO.core_program_to_json(prog) == json.loads(CORE_TC_JSON)
+    # every shape this engine runs survives the round trip
+    for p in (tc_program(BOOL), tc_program(TROP), IO.core_program_from_json(CORE_TC_JSON)):
+        assert IO.core_program_from_json(json.dumps(IO.core_program_to_json(p))) == p
+    agg = {"kind": "agg", "data": {"op": "min", "slot": 1}}
+    assert IO._core_term(agg, "t") == Agg("min", 1) and IO._term_json(Agg("min", 1)) == agg
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda d: d["predicates"][0].update(semiring="prob"), "semiring"),
+    (lambda d: d["rules"][0]["head"]["args"][0].update(kind="compound"), "compound"),
+    (lambda d: d["rules"][0]["body"][0].update(kind="maybe"), "literal kind"),
+    (lambda d: d["rules"][0].update(extra=1), "unknown field"),
+    (lambda d: d.pop("num_strata"), "missing field"),
+    (lambda d: d["rules"][0]["head"]["args"][0]["data"].update(slot=-1), "u32"),
+])
+def test_core_program_json_errors(mutate, msg):
+    import json
+    d = json.loads(CORE_TC_JSON)
+    mutate(d)
+    with pytest.raises(LoadError, match=msg):
+        IO.core_program_from_json(d)
+
+
+def _const(name):
+    return {"kind": "const", "data": {"name": name}}
+
+
+def _fact(p, args, **extra):
+    return {"kind": "fact", "data": dict({"atom": {"pred": p, "args": args}}, **extra)}
+
+
+def test_high_ir_facts_sssp_with_inf():
+    """The sssp() document of crates/strata-ir/tests/roundtrip.rs:67-110: a
+    finite and a +inf Trop fact; constants interned first seen."""
+    doc = {"ir_version": "0.1.0", "items": [
+        {"kind": "domain", "data": {"name": "node"}},
+        _fact("edge", [_const("a"), _const("b")], weight=5),
+        _fact("edge", [_const("b"), _const("c")], weight="inf"),
+        _fact("hop", [_const("c"), {"kind": "int", "data": {"value": -3}}])]}
+    d = SymbolDict()
+    facts = IO.high_ir_facts(doc, d)
+    assert facts == [("edge", (Sym(0), Sym(1)), 5), ("edge", (Sym(1), Sym(2)), INF), ("hop", (Sym(2), IntV(-3)), None)]
+    assert [d.resolve(i) for i in range(3)] == ["a", "b", "c"]
+
+
+@pytest.mark.parametrize("version,ok", [("0.1.0", True), ("0.1.9", True), ("0.2.0", False), ("1.1.0", False)])
+def test_high_ir_version_gate(version, ok):
+    doc = {"ir_version": version, "items": [_fact("e", [_const("a")])]}
+    if ok:
+        assert IO.high_ir_facts(doc, SymbolDict()) == [("e", (Sym(0),), None)]
+    else:
+        with pytest.raises(LoadError, match="incompatible"):
+            IO.high_ir_facts(doc, SymbolDict())
+    for bad in ("0.1", "0.1.0.0", "a.b.c"):
+        with pytest.raises(LoadError, match="bad ir_version"):
+            IO.high_ir_facts({"ir_version": bad, "items": []}, SymbolDict())
+
+
+def test_high_ir_facts_refusals():
+    for fact in (_fact("e", [_const("a")], prob=0.5), _fact("e", [{"kind": "var", "data": {"name": "X"}}]),
+                 _fact("e", [_const("a")], weight="nan")):
+        with pytest.raises(LoadError):
+            IO.high_ir_facts({"ir_version": "0.1.0", "items": [fact]}, SymbolDict())
+
+
+def test_json_program_and_facts_through_the_oracle():
+    """A Core-IR document (the program above plus its base rule) and High-IR
+    facts evaluate to the reference's interp.rs:84-114 answer: a->b->c->d
+    gives 6 path tuples."""
+    import json
+    d = json.loads(CORE_TC_JSON)
+    var = lambda i: {"kind": "var", "data": {"slot": i}}  # noqa: E731
+    d["rules"].insert(0, {"head": {"pred": "path", "args": [var(0), var(1)]},
+                          "body": [{"kind": "pos", "data": {"pred": "edge", "args": [var(0), var(1)]}}],
+                          "stratum": 0, "var_count": 2})
+    prog = IO.core_program_from_json(json.dumps(d))
+    doc = {"ir_version": "0.1.0",
+           "items": [_fact("edge", [_const(x), _const(y)]) for x, y in ("ab", "bc", "cd")]}
+    r = O.run(prog, IO.high_ir_facts(doc, SymbolDict()))
+    assert r.ok and len(r.relations["path"]) == 6
