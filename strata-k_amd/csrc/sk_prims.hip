@@ -251,7 +251,7 @@ void scan_u8(const Exec& x, const u8* in, u64* out, u64 n, u64* t) { scan_impl<u
 //
 // One LSD pass over a D-bit digit at `shift` (D = 6, 7 or 8; 2^D <= kBlock
 // buckets):
-//   k_radix_hist     per-block digit histogram (wave ballot match + LDS)
+//   k_radix_hist     per-block digit histogram (per-wave LDS counters)
 //   scan             exclusive scan of counts in digit-major order
 //   k_radix_scatter  per tile of kBlock * IT records: stable in-wave ranking
 //                    by ballot match, tile-local counting sort in LDS, then
@@ -287,12 +287,24 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
   __syncthreads();
   u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
   u32 w = wave_id();
-  for (u64 base = lo; base < hi; base += kBlock) {
-    u64 i = base + threadIdx.x;
-    bool valid = i < hi;
-    u32 d = valid ? digit_of<K, D>(keys[i], shift) : 0;
-    u64 m = match_digit<D>(d, valid);
-    if (valid && (int)lane_id() == (m ? (int)__builtin_ctzll(m) : -1)) atomicAdd(&h[w][d], (u32)__popcll(m));
+  // kHistBatch keys per thread and step, loaded at clamped addresses so that
+  // all of the loads are in flight together; positions past the chunk are
+  // masked out of the count
+  constexpr int kHistBatch = 8;
+  for (u64 base = lo; base < hi; base += (u64)kBlock * kHistBatch) {
+    K kv[kHistBatch];
+#pragma unroll
+    for (int q = 0; q < kHistBatch; q++) {
+      const u64 i = base + (u64)q * kBlock + threadIdx.x;
+      kv[q] = keys[i < hi ? i : hi - 1];
+    }
+    // one LDS atomic per key: cheaper than matching equal digits across the
+    // wave first (D ballots per key)
+#pragma unroll
+    for (int q = 0; q < kHistBatch; q++) {
+      const bool valid = base + (u64)q * kBlock + threadIdx.x < hi;
+      if (valid) atomicAdd(&h[w][digit_of<K, D>(kv[q], shift)], 1u);
+    }
   }
   __syncthreads();
   for (u32 d = threadIdx.x; d < NB; d += kBlock) {
@@ -333,15 +345,26 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     u8 fl[HASW ? IT : 1];
     u32 rk[IT];
     const u64 wb = tb + (u64)wv * 64 * IT;
+    // every key load of the tile first, at clamped addresses, so that they
+    // are all in flight together (ranking between loads would serialise them
+    // behind its LDS atomics); positions past the chunk are never used.  The
+    // weights follow the ranking, in flight while the tile's offsets are
+    // scanned.
+    const bool full = wb + (u64)IT * 64 <= hi;  // the wave's whole stretch is in the chunk
+    if (full) {
+#pragma unroll
+      for (int it = 0; it < IT; it++) key[it] = kin[wb + (u64)it * 64 + lane];
+    } else {
+#pragma unroll
+      for (int it = 0; it < IT; it++) {
+        const u64 idx = wb + (u64)it * 64 + lane;
+        key[it] = kin[idx < hi ? idx : hi - 1];
+      }
+    }
 #pragma unroll
     for (int it = 0; it < IT; it++) {
       u64 idx = wb + (u64)it * 64 + lane;
       bool valid = idx < hi;
-      key[it] = valid ? kin[idx] : (K)0;
-      if constexpr (HASW) {
-        wt[it] = valid ? win[idx] : 0;
-        fl[it] = (valid && has_f) ? fin[idx] : (u8)0;
-      }
       u32 d = digit_of<K, D>(key[it], shift);
       u64 m = match_digit<D>(d, valid);
       int leader = (m ? (int)__builtin_ctzll(m) : -1);
@@ -349,6 +372,23 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
       if (valid && (int)lane == leader) old = atomicAdd(&s_wcnt[wv][d], (u32)__popcll(m));
       old = __shfl(old, leader < 0 ? 0 : leader, 64);
       rk[it] = old + (u32)__popcll(m & lanemask_lt());
+    }
+    if constexpr (HASW) {
+      if (full) {
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+          wt[it] = win[wb + (u64)it * 64 + lane];
+          fl[it] = has_f ? fin[wb + (u64)it * 64 + lane] : (u8)0;
+        }
+      } else {
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+          const u64 idx = wb + (u64)it * 64 + lane;
+          const u64 ic = idx < hi ? idx : hi - 1;
+          wt[it] = win[ic];
+          fl[it] = has_f ? fin[ic] : (u8)0;
+        }
+      }
     }
     __syncthreads();
     // digit totals of the tile and per-wave offsets (threads past the last
