@@ -956,15 +956,18 @@ double tri_list_len(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, co
 // what changes is that each head cell is written once per chunk instead of
 // once per candidate.
 
+// Pairs per lane and step of the pull kernel.
+constexpr int kPullPairs = 2;
+
 // Per wave of the pull kernel: one batch of 64 index rows.
 struct PullLds {
-  u64 m[64];                  // delta lanes of each index row's source row
-  u64 base[64];               // delta-list position of that row's first delta cell
+  u64 base[64];               // delta-list position of each index row's source row's first delta cell
   i64 w[64];                  // edge weight
-  u32 incl[64];               // inclusive prefix of popcount(m)
+  u32 excl[64];               // exclusive prefix of the index rows' pair counts
   u32 slot[64];               // which of the chunk's destination rows the index row feeds
   i64 acc[kPullRows * 64];    // running minimum per (destination row, lane value)
   u8 have[kPullRows * 64];    // (destination row, lane value) has a candidate
+  u8 start[64 * kPullPairs];  // 1 + the index row whose pairs start at this position of the step
 };
 
 // Rows of at most 64 lanes (lbits <= 6).  A wave takes one chunk (up to
@@ -972,12 +975,15 @@ struct PullLds {
 // l's source, weight and the source's delta record, a wave scan counts the
 // (edge, delta lane) pairs, and the pairs are dealt round-robin to lanes, so
 // every lane does work whatever the masks' density.  A lane finds its pair's
-// edge by a branch-free search over the scan; the pair's delta cell sits at
-// base + rank in the delta list, where it reads the cell's weight and lane
-// value (pairs of one edge read consecutive positions).  Candidates fold into the wave's LDS minimum
-// per (destination row, lane value), applied to the cells once per chunk.
-// The destination cells are peeked at the start, overlapping the loads.
-constexpr int kPullPairs = 2;
+// edge from the step's edge starts: every edge whose first pair falls in the
+// step marks that position in LDS, a ballot of the marks gives the start
+// mask, and the edge of pair p is the one marked at the highest start at or
+// below p (or, with none in the step, the edge the step began in).  The
+// pair's delta cell sits at base + rank in the delta list, where it reads
+// the cell's weight and lane value (pairs of one edge read consecutive
+// positions).  Candidates fold into the wave's LDS minimum per (destination
+// row, lane value), applied to the cells once per chunk.  The destination
+// cells are peeked at the start, overlapping the loads.
 
 template <bool TROP>
 __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevErr* err) {
@@ -1024,29 +1030,45 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
     const u32 cnt = (u32)__popcll(m);
     const u32 incl = (u32)wave_incl_scan_j(cnt);
     const u32 total = __shfl(incl, 63);
-    L.m[lane] = m;
+    const u32 excl = incl - cnt;
     L.base[lane] = base;
     L.w[lane] = we;
-    L.incl[lane] = incl;
+    L.excl[lane] = excl;
     L.slot[lane] = slot;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // kPullPairs pairs per lane per step: their searches and delta-weight
-    // loads are independent, so they overlap instead of queueing
+#pragma unroll
+    for (int u = 0; u < kPullPairs; u++) L.start[u * 64 + lane] = 0;
+    const u64 le = ~0ull >> (63 - lane);  // lanes at or below this one
+    u32 carry = 0;  // the edge holding the step's first pair (pair 0 starts an edge)
+    // kPullPairs pairs per lane per step: their delta-weight loads are
+    // independent, so they overlap instead of queueing
     for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairs) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (cnt && excl >= p0 && excl < p0 + 64 * kPullPairs) L.start[excl - p0] = (u8)(lane + 1);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      u32 rs[kPullPairs];
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) rs[u] = L.start[u * 64 + lane];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) L.start[u * 64 + lane] = 0;
       u32 jj[kPullPairs], kk[kPullPairs], xx[kPullPairs];
       i64 vv[kPullPairs];
 #pragma unroll
       for (int u = 0; u < kPullPairs; u++) {
-        // the edge holding pair pidx: the number of edges whose inclusive
-        // pair count is <= pidx (branch-free search over 64 entries)
+        // the edge holding pair pidx: marked at the highest start at or
+        // below this lane, else the edge the window began in
+        const u64 sm = __ballot(rs[u] != 0);
+        const u64 mine = sm & le;
+        const int h = mine ? 63 - __builtin_clzll(mine) : 0;
+        const u32 from = (u32)__shfl((int)rs[u], h, 64);
+        const u32 j = mine ? from - 1 : carry;
+        if (sm) carry = (u32)__builtin_amdgcn_readlane((int)rs[u], 63 - __builtin_clzll(sm)) - 1;
         const u32 pidx = p0 + (u32)u * 64 + lane;
-        u32 j = 0;
-#pragma unroll
-        for (u32 st = 32; st; st >>= 1)
-          if (L.incl[j + st - 1] <= pidx) j += st;
         jj[u] = j;
-        kk[u] = pidx - (L.incl[j] - (u32)__popcll(L.m[j]));
+        kk[u] = pidx - L.excl[j];
       }
 #pragma unroll
       for (int u = 0; u < kPullPairs; u++) {
