@@ -387,7 +387,7 @@ def bench_sort(args):
         t = float(np.median(ms))
         # every candidate pinned in turn, the same keys (stderr)
         per_cand = {}
-        for c in range(6):
+        for c in range(8):
             ctx.force_radix(c)
             cm = []
             for _ in range(3):

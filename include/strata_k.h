@@ -348,9 +348,9 @@ sk_status sk_radix_sort_device(sk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, 
  * mode 0: a fixed default (8-bit digits); 1 (the default): a class is timed
  * on its first sort; 2: time every class now. */
 sk_status sk_ctx_set_autotune(sk_ctx* ctx, int mode);
-/* Pin every sort of the context to candidate c (0 .. 5: digit bits 8, 8, 7,
- * 7, 6, 6 with 16 / 8 items per thread, half for 128-bit keys), or -1 to
- * return to the tuner.  A testing knob. */
+/* Pin every sort of the context to candidate c (0 .. 7: digit bits 8, 8, 7,
+ * 7, 6, 6 with 16 / 8 items per thread, then 8 and 6 with 32; half the
+ * items for 128-bit keys), or -1 to return to the tuner.  A testing knob. */
 sk_status sk_ctx_force_radix(sk_ctx* ctx, int candidate);
 /* The configuration a sort of n records with key_bits significant bits would
  * take now; *est_ms the tuner's estimate (-1 when the class is untimed). */

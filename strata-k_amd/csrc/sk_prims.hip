@@ -441,11 +441,11 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 }
 
 // The candidate configurations the tuner times: digit bits x items per thread
-// (u128 keys at 16 items would not fit LDS with weights, so they use 4 / 8).
+// (u128 keys take half the items, so that a tile with weights fits LDS).
 struct RadixCand {
   int digit, items;
 };
-constexpr RadixCand kRadixCands[] = {{8, 16}, {8, 8}, {7, 16}, {7, 8}, {6, 16}, {6, 8}};
+constexpr RadixCand kRadixCands[] = {{8, 16}, {8, 8}, {7, 16}, {7, 8}, {6, 16}, {6, 8}, {8, 32}, {6, 32}};
 constexpr int kNCands = sizeof(kRadixCands) / sizeof(kRadixCands[0]);
 static_assert(kNCands <= RadixTune::kMaxCands, "tuner table width");
 
@@ -487,6 +487,8 @@ static void radix_pass_c(int c, const Exec& x, const K* a, K* b, const i64* wa, 
     SK_RADIX_CASE(3)
     SK_RADIX_CASE(4)
     SK_RADIX_CASE(5)
+    SK_RADIX_CASE(6)
+    SK_RADIX_CASE(7)
 #undef SK_RADIX_CASE
     default:
       throw std::runtime_error("radix: configuration");

@@ -27,7 +27,7 @@ class Lcg:
 SIZES = [0, 1, 2, 3, 5, 7, 8, 100, 1000, 4097]
 
 
-@pytest.fixture(params=[-1, 0, 1, 2, 3, 4, 5], ids=lambda c: f"cand{c}")
+@pytest.fixture(params=[-1, 0, 1, 2, 3, 4, 5, 6, 7], ids=lambda c: f"cand{c}")
 def radix_ctx(ctx, request):
     ctx.force_radix(request.param)
     yield ctx
