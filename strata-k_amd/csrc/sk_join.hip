@@ -869,10 +869,15 @@ __device__ __forceinline__ void intersect_each(const u32* __restrict__ A, u64 na
   }
 }
 
+// A driver row whose two lists hold more than kTriHeavy values together is
+// left to a wave of its own (k_tri_count_heavy, k_tri_emit_heavy): one lane
+// per row would hold its whole wave for the length of the longest merge.
+constexpr u64 kTriHeavy = 512;
+
 template <class K>
 __global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
                             const u32* __restrict__ adj2, const u64* __restrict__ csr3, const u32* __restrict__ adj3,
-                            u32* __restrict__ cnt) {
+                            u32* __restrict__ cnt, u32* __restrict__ heavy, unsigned long long* __restrict__ hstat) {
   u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nd) return;
   K k = d_keys[e];
@@ -880,9 +885,117 @@ __global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, con
   u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
   u64 a0 = csr2[v], a1 = csr2[v + 1];
   u64 b0 = csr3[u], b1 = csr3[u + 1];
+  if ((a1 - a0) + (b1 - b0) > kTriHeavy) {
+    // hstat: {heavy rows, their list values}
+    heavy[atomicAdd(&hstat[0], 1ull)] = (u32)e;
+    atomicAdd(&hstat[1], (unsigned long long)((a1 - a0) + (b1 - b0)));
+    cnt[e] = 0;  // written by the heavy pass
+    return;
+  }
   u32 c = 0;
   intersect_each(adj2 + a0, a1 - a0, adj3 + b0, b1 - b0, [&](u32) { c++; });
   cnt[e] = c;
+}
+
+// The lists of a heavy row, the shorter first.
+template <class K>
+__device__ __forceinline__ void tri_heavy_lists(const K* __restrict__ d_keys, u64 e, const TriDesc& t,
+                                                const u64* __restrict__ csr2, const u32* __restrict__ adj2,
+                                                const u64* __restrict__ csr3, const u32* __restrict__ adj3, u32* u,
+                                                u32* v, const u32** A, u64* na, const u32** B, u64* nb) {
+  const K k = d_keys[e];
+  *u = col_of<K>(k, 2, t.bits, t.u_col);
+  *v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
+  const u64 a0 = csr2[*v], a1 = csr2[*v + 1], b0 = csr3[*u], b1 = csr3[*u + 1];
+  *A = adj2 + a0;
+  *na = a1 - a0;
+  *B = adj3 + b0;
+  *nb = b1 - b0;
+  if (*na > *nb) {
+    const u32* tp = *A;
+    *A = *B;
+    *B = tp;
+    const u64 tn = *na;
+    *na = *nb;
+    *nb = tn;
+  }
+}
+
+// x in the sorted, duplicate-free list B[0, nb)
+__device__ __forceinline__ bool tri_member(const u32* __restrict__ B, u64 nb, u32 x) {
+  u64 lo = 0, hi = nb;
+  while (lo < hi) {
+    const u64 mid = (lo + hi) >> 1;
+    if (B[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nb && B[lo] == x;
+}
+
+// One wave per heavy row: lane l tests values l, l + 64, ... of the shorter
+// list against the longer one by binary search (the lists are sets).
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_tri_count_heavy(const K* __restrict__ d_keys, TriDesc t,
+                                                            const u64* __restrict__ csr2,
+                                                            const u32* __restrict__ adj2,
+                                                            const u64* __restrict__ csr3,
+                                                            const u32* __restrict__ adj3,
+                                                            const u32* __restrict__ heavy, u64 nh,
+                                                            u32* __restrict__ cnt) {
+  const u64 w = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (w >= nh) return;  // whole waves
+  const u32 lane = threadIdx.x & 63;
+  const u64 e = heavy[w];
+  u32 u, v;
+  const u32 *A, *B;
+  u64 na, nb;
+  tri_heavy_lists<K>(d_keys, e, t, csr2, adj2, csr3, adj3, &u, &v, &A, &na, &B, &nb);
+  u32 c = 0;
+  for (u64 i = lane; i < na; i += 64) c += tri_member(B, nb, A[i]) ? 1u : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if (lane == 0) cnt[e] = c;
+}
+
+// One wave per heavy row: 64 values of the shorter list per step, tested as
+// in the count; a ballot orders the hits, so each step's outputs are written
+// together, ascending.
+template <class K, class KO>
+__global__ __launch_bounds__(kBlock) void k_tri_emit_heavy(const K* __restrict__ d_keys, TriDesc t,
+                                                           const u64* __restrict__ csr2, const u32* __restrict__ adj2,
+                                                           const u64* __restrict__ csr3, const u32* __restrict__ adj3,
+                                                           const u32* __restrict__ heavy, u64 nh,
+                                                           const u64* __restrict__ off, OutDesc o) {
+  const u64 w = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (w >= nh) return;  // whole waves
+  const u32 lane = threadIdx.x & 63;
+  const u64 e = heavy[w];
+  u32 u, v;
+  const u32 *A, *B;
+  u64 na, nb;
+  tri_heavy_lists<K>(d_keys, e, t, csr2, adj2, csr3, adj3, &u, &v, &A, &na, &B, &nb);
+  u64 oi = off[e];
+  const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (u64 base = 0; base < na; base += 64) {
+    const u64 i = base + lane;
+    u32 x = 0;
+    bool hit = false;
+    if (i < na) {
+      x = A[i];
+      hit = tri_member(B, nb, x);
+    }
+    const u64 m = __ballot(hit);
+    if (hit) {
+      u32 vals[3] = {u, v, x};  // binding slots: 0 = u, 1 = v, 2 = w
+      KO key = 0;
+      for (int c = 0; c < o.ncols; c++) {
+        u32 val = o.kind[c] == 0 ? vals[o.idx[c]] : o.cnst[c];
+        key = (key << o.bits) | (KO)val;
+      }
+      ((KO*)o.keys)[oi + (u64)__popcll(m & below)] = key;
+    }
+    oi += (u64)__popcll(m);
+  }
 }
 
 template <class K, class KO>
@@ -898,6 +1011,7 @@ __global__ void k_tri_emit(const K* __restrict__ d_keys, u64 nd, TriDesc t, cons
   u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
   u64 a = csr2[v], a1 = csr2[v + 1];
   u64 b = csr3[u], b1 = csr3[u + 1];
+  if ((a1 - a) + (b1 - b) > kTriHeavy) return;  // k_tri_emit_heavy's
   u64 oi = off[e];
   const u64 oend = oi + need;
   // a plain merge here (the adaptive search made this pass slower, 384 ->
@@ -930,10 +1044,28 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
     if (!count_only) halloc(hctx, o, 0);
     return 0;
   }
+  if (nd > 0xffffffffull) throw BudgetError{nd};  // heavy rows are listed by u32 index
   double lb = sum_list_len ? *sum_list_len : 0;
+  DBuf<u32> heavy(x.pool, nd);
+  DBuf<u64> hstat(x.pool, 2);
+  memset_async(x, hstat.get(), 0, 16);
   {
-    Scope sc(x.timer, "leapfrog_count", KC_LEAPFROG, (double)nd * sizeof(K) + lb * 4.0, nd);
-    k_tri_count<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, cnt.get());
+    // bytes credited once the heavy rows' share of the lists is known
+    Scope sc(x.timer, "leapfrog_count", KC_LEAPFROG, 0.0, nd);
+    k_tri_count<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, cnt.get(),
+                                                      heavy.get(), (unsigned long long*)hstat.get());
+    SK_HIP(hipGetLastError());
+  }
+  u64 hs[2];
+  read_u64s(x, hstat.get(), hs, 2);
+  const u64 nh = hs[0];
+  const double heavy_vals = (double)hs[1], light_vals = std::max(0.0, lb - heavy_vals);
+  const unsigned hgrid = (unsigned)((nh + kBlock / 64 - 1) / (kBlock / 64));
+  if (x.timer) x.timer->credit("leapfrog_count", KC_LEAPFROG, (double)nd * sizeof(K) + light_vals * 4.0, 0);
+  if (nh) {
+    Scope sc(x.timer, "leapfrog_count_heavy", KC_LEAPFROG, (double)nh * 4.0 + heavy_vals * 4.0, nh);
+    k_tri_count_heavy<K><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
+                                                         cnt.get());
     SK_HIP(hipGetLastError());
   }
   scan_u32(x, cnt.get(), off.get(), nd, tot.get());
@@ -941,15 +1073,27 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
   if (count_only) return total;
   halloc(hctx, o, total);
   if (total == 0) return 0;
-  Scope sc(x.timer, "leapfrog_emit", KC_LEAPFROG,
-           (double)nd * sizeof(K) + lb * 4.0 + (double)total * (k128_out ? 16.0 : 8.0), nd);
-  if (k128_out)
-    k_tri_emit<K, u128><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, off.get(),
-                                                             cnt.get(), *o);
-  else
-    k_tri_emit<K, u64><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, off.get(),
-                                                            cnt.get(), *o);
-  SK_HIP(hipGetLastError());
+  {
+    Scope sc(x.timer, "leapfrog_emit", KC_LEAPFROG,
+             (double)nd * sizeof(K) + light_vals * 4.0 + (double)total * (k128_out ? 16.0 : 8.0), nd);
+    if (k128_out)
+      k_tri_emit<K, u128><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, off.get(),
+                                                               cnt.get(), *o);
+    else
+      k_tri_emit<K, u64><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, off.get(),
+                                                              cnt.get(), *o);
+    SK_HIP(hipGetLastError());
+  }
+  if (nh) {
+    Scope sc(x.timer, "leapfrog_emit_heavy", KC_LEAPFROG, (double)nh * 4.0 + heavy_vals * 4.0, nh);
+    if (k128_out)
+      k_tri_emit_heavy<K, u128><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
+                                                                 off.get(), *o);
+    else
+      k_tri_emit_heavy<K, u64><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
+                                                                off.get(), *o);
+    SK_HIP(hipGetLastError());
+  }
   return total;
 }
 
