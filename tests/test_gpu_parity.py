@@ -157,10 +157,11 @@ def test_triangles_vectors(ctx):
 
 
 def test_triangles_long_lists(ctx):
-    """Hubs whose oriented lists exceed the vertex kernel's hash table (1,500
-    leaves each: those rows take the per-row merge) beside a random graph
-    whose lists all fit it; the tri relation equals the oracle's and the
-    count the closed form plus brute force."""
+    """Hubs of 1,500 leaves each (their rows' two lists hold more than 512
+    values: one wave per row, binary-search membership, ballot-ordered
+    writes) beside a random graph whose rows take the one-lane merge; the
+    tri relation equals the oracle's and the count the closed form plus
+    brute force."""
     h, b = 5, 1500
     edges = [(i, j) for i in range(h) for j in range(i + 1, h)] + [(hub, h + k) for k in range(b) for hub in range(h)]
     rnd = random.Random(17)
