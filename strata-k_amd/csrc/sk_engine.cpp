@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cstring>
 #include <set>
+#include <thread>
 #include <unordered_set>
 
 namespace sk {
@@ -123,9 +124,53 @@ void Ctx::init(int dev, u64 budget) {
   SK_HIP(hipStreamSynchronize(stream));
 }
 
+void Ctx::d2h(void* dst, const void* src, u64 bytes) {
+  constexpr u64 kDirect = 8ull << 20;  // below this, one enqueued copy
+  if (bytes < kDirect) {
+    if (bytes) SK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    return;
+  }
+  for (int b = 0; b < 2; b++)
+    if (!stage[b]) {
+      SK_HIP(hipHostMalloc((void**)&stage[b], kStageBytes, hipHostMallocDefault));
+      SK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
+    }
+  const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const u64 nchunks = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](u64 k) {
+    const u64 off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    SK_HIP(hipMemcpyAsync(stage[k & 1], (const u8*)src + off, len, hipMemcpyDeviceToHost, stream));
+    SK_HIP(hipEventRecord(stage_ev[k & 1], stream));
+  };
+  issue(0);
+  for (u64 k = 0; k < nchunks; k++) {
+    // chunk k - 1's buffer was drained in the previous step, so chunk k + 1
+    // may fill it while chunk k is drained
+    if (k + 1 < nchunks) issue(k + 1);
+    SK_HIP(hipEventSynchronize(stage_ev[k & 1]));
+    const u64 off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    const u8* from = stage[k & 1];
+    u8* to = (u8*)dst + off;
+    const u64 part = (len + nthreads - 1) / nthreads;
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < nthreads; t++) {
+      const u64 a = std::min(len, t * part), e = std::min(len, a + part);
+      if (a < e) ts.emplace_back([=] { std::memcpy(to + a, from + a, e - a); });
+    }
+    std::memcpy(to, from, std::min(len, part));
+    for (auto& th : ts) th.join();
+  }
+}
+
 void Ctx::release() {
   (void)hipSetDevice(device);
   if (stream) (void)hipStreamSynchronize(stream);
+  for (int b = 0; b < 2; b++) {
+    if (stage[b]) (void)hipHostFree(stage[b]);
+    if (stage_ev[b]) (void)hipEventDestroy(stage_ev[b]);
+    stage[b] = nullptr;
+    stage_ev[b] = nullptr;
+  }
   timer.reset();
   pool.reset();
   if (derr) (void)hipFree(derr);
@@ -3092,11 +3137,11 @@ void Engine::partition_read_sym(const std::string& pred, u32 p, u32* const* cols
     unpack_cols<K>(x, (const K*)R.full.k.ptr() + lo, n, R.arity, R.bits, cp.data());
   });
   for (int j = 1; j < R.arity; j++)
-    if (cols[j - 1]) SK_HIP(hipMemcpyAsync(cols[j - 1], cp[j], n * 4, hipMemcpyDeviceToHost, x.stream));
+    if (cols[j - 1]) ctx_->d2h(cols[j - 1], cp[j], n * 4);
   bool trop = R.sem == SK_TROP;
-  if (w && trop) SK_HIP(hipMemcpyAsync(w, R.full.w.get() + lo, n * 8, hipMemcpyDeviceToHost, x.stream));
+  if (w && trop) ctx_->d2h(w, R.full.w.get() + lo, n * 8);
   if (inf) {
-    if (trop && has_inf_) SK_HIP(hipMemcpyAsync(inf, R.full.inf.get() + lo, n, hipMemcpyDeviceToHost, x.stream));
+    if (trop && has_inf_) ctx_->d2h(inf, R.full.inf.get() + lo, n);
     else std::memset(inf, 0, n);
   }
   SK_HIP(hipStreamSynchronize(x.stream));
@@ -3210,7 +3255,7 @@ void Engine::read_sym(const std::string& pred, u32* const* cols, i64* w, u8* inf
       unpack_cols<K>(x, (const K*)R.full.k.ptr(), n, R.arity, bits, cp.data());
     });
     for (int j = 0; j < R.arity; j++)
-      if (cols[j]) SK_HIP(hipMemcpyAsync(cols[j], cp[j], n * 4, hipMemcpyDeviceToHost, x.stream));
+      if (cols[j]) ctx_->d2h(cols[j], cp[j], n * 4);
     SK_HIP(hipStreamSynchronize(x.stream));
     if (!D_.ints.empty())
       for (int j = 0; j < R.arity; j++)
@@ -3220,11 +3265,11 @@ void Engine::read_sym(const std::string& pred, u32* const* cols, i64* w, u8* inf
   }
   bool trop = R.sem == SK_TROP;
   if (w) {
-    if (trop) SK_HIP(hipMemcpyAsync(w, R.full.w.get(), n * 8, hipMemcpyDeviceToHost, x.stream));
+    if (trop) ctx_->d2h(w, R.full.w.get(), n * 8);
     else std::memset(w, 0, n * 8);
   }
   if (inf) {
-    if (trop && has_inf_) SK_HIP(hipMemcpyAsync(inf, R.full.inf.get(), n, hipMemcpyDeviceToHost, x.stream));
+    if (trop && has_inf_) ctx_->d2h(inf, R.full.inf.get(), n);
     else std::memset(inf, 0, n);
   }
   SK_HIP(hipStreamSynchronize(x.stream));

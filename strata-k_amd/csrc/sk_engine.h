@@ -282,9 +282,19 @@ struct Ctx {
   hipEvent_t read_ev = nullptr;
   EvalErr last;
   RadixTune tune;
+  // Two pinned staging buffers for large device-to-host result copies,
+  // allocated on first use (d2h).
+  static constexpr u64 kStageBytes = 64ull << 20;
+  u8* stage[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   Exec exec() { return Exec{pool.get(), stream, &timer, derr, host_words, read_ev, &tune}; }
   void init(int dev, u64 budget);  // throws DeviceError
   void release();
+  // Device-to-host copy into caller memory.  Small copies are enqueued on
+  // the stream (the caller synchronises); large ones go through the pinned
+  // staging buffers in chunks, each drained into `dst` by several host
+  // threads while the next chunk is in flight, and have finished on return.
+  void d2h(void* dst, const void* src, u64 bytes);
 };
 
 struct Chunk {
