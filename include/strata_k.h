@@ -165,6 +165,22 @@ const char* sk_build_info(void);
  * (with a message) for a rule the planner does not lower; there is no CPU
  * fallback. */
 sk_status sk_prepare(sk_ctx* ctx, const sk_program* prog, sk_run** out);
+
+/* A serialised CoreProgram: the JSON strata-ir's serde derives write for it
+ * (crates/strata-ir/src/core/mod.rs:26-94; docs/ir-encoding.md: snake_case
+ * fields, unknown fields ignored, {"kind","data"} tags for literals and
+ * terms), decoded into an sk_program owned by *out (sk_program_json_get,
+ * valid until sk_program_json_free).  Needs no device.  A malformed document
+ * is SK_ERR_INVALID_ARGUMENT with the reason, naming the offending field's
+ * path, in err (err_len bytes). */
+typedef struct sk_program_json sk_program_json;
+sk_status sk_program_from_json(const char* json, size_t len, sk_program_json** out, char* err, size_t err_len);
+const sk_program* sk_program_json_get(const sk_program_json* p);
+void sk_program_json_free(sk_program_json* p);
+/* sk_prepare from a serialised CoreProgram (the form a Rust caller gets from
+ * serde_json::to_string(&CoreProgram)); a decoding error is
+ * SK_ERR_INVALID_ARGUMENT with its reason in sk_ctx_error's message. */
+sk_status sk_prepare_json(sk_ctx* ctx, const char* json, size_t len, sk_run** out);
 void sk_run_free(sk_run* run);
 
 /* Seed EDB facts, ⊕-combined on insert (seminaive.rs:30-36).  Rows are

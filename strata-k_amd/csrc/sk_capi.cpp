@@ -181,6 +181,23 @@ sk_status sk_prepare(sk_ctx* ctx, const sk_program* prog, sk_run** out) {
   return SK_OK;
 }
 
+sk_status sk_prepare_json(sk_ctx* ctx, const char* json, size_t len, sk_run** out) {
+  if (!ctx || !json || !out) return SK_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  sk_program_json* pj = nullptr;
+  char err[256];
+  sk_status s = sk_program_from_json(json, len, &pj, err, sizeof err);
+  if (s != SK_OK) {
+    ctx->c.last = EvalErr{};
+    ctx->c.last.code = s;
+    ctx->c.last.msg = err;
+    return s;
+  }
+  s = sk_prepare(ctx, sk_program_json_get(pj), out);  // the engine copies what it keeps
+  sk_program_json_free(pj);
+  return s;
+}
+
 void sk_run_free(sk_run* run) {
   if (!run) return;
   (void)hipSetDevice(run->ctx->c.device);

@@ -184,6 +184,10 @@ SIGNATURES = [
     ("sk_batch_execute_ms", C.c_double, [_VP]),
     ("sk_asp_simplify_filter", C.c_int, [_VP, _VP, _VP, _VP, _U64, _VP, _VP, _U64, _VP, _VP, _VP,
                                          C.POINTER(_U64), C.POINTER(_U64)]),
+    ("sk_program_from_json", C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(_VP), C.c_char_p, C.c_size_t]),
+    ("sk_program_json_get", C.POINTER(SkProgram), [_VP]),
+    ("sk_program_json_free", None, [_VP]),
+    ("sk_prepare_json", C.c_int, [_VP, C.c_char_p, C.c_size_t, C.POINTER(_VP)]),
     ("sk_asp_simplify_filter_device", C.c_int, [_VP, _VP, _VP, _VP, _U64, _U64, _VP, _VP, _U64, _VP, _VP, _VP,
                                                 C.POINTER(_U64), C.POINTER(_U64)]),
     ("sk_sort_u64", C.c_int, [_VP, _VP, _U64, _VP]),
@@ -455,6 +459,10 @@ class Context:
     def prepare(self, prog) -> "Run":
         return Run(self, prog)
 
+    def prepare_json(self, json_text) -> "Run":
+        """Prepare a serialised CoreProgram (sk_prepare_json)."""
+        return Run(self, program_from_json(json_text), json_text=json_text)
+
     def run_semi_naive(self, prog, edb: Iterable = (), bulk: Optional[dict] = None,
                        want: Optional[Sequence[str]] = None, dense: bool = True, pull: int = 1,
                        graph: bool = True) -> Db:
@@ -482,15 +490,22 @@ class Context:
 
 
 class Run:
-    """A prepared program (sk_run): add facts, load, execute, read results."""
+    """A prepared program (sk_run): add facts, load, execute, read results.
+    ``json_text``: prepare from a serialised CoreProgram instead
+    (sk_prepare_json); ``prog`` is then its decoded form."""
 
-    def __init__(self, ctx: Context, prog):
+    def __init__(self, ctx: Context, prog, json_text: Optional[str] = None):
         self.ctx = ctx
         self.L = ctx.L
         self.prog = prog
-        self.flat = FlatProgram(prog)
         h = C.c_void_p()
-        code = self.L.sk_prepare(ctx.h, C.byref(self.flat.c), C.byref(h))
+        if json_text is not None:
+            self.flat = None
+            raw = json_text.encode() if isinstance(json_text, str) else bytes(json_text)
+            code = self.L.sk_prepare_json(ctx.h, raw, len(raw), C.byref(h))
+        else:
+            self.flat = FlatProgram(prog)
+            code = self.L.sk_prepare(ctx.h, C.byref(self.flat.c), C.byref(h))
         if code:
             info = SkErrorInfo()
             self.L.sk_ctx_error(ctx.h, C.byref(info))
@@ -841,6 +856,49 @@ def asp_simplify_filter(head, body_start, body_lit, poss, cert, ctx: Optional[Co
     if code:
         _raise(code)
     return oh[:nr.value].copy(), ob[:nr.value + 1].copy(), ol[:nl.value].copy()
+
+
+def program_from_json(json_text):
+    """A serialised CoreProgram decoded by the library's decoder
+    (sk_program_from_json; no device), returned as a core_ir.CoreProgram."""
+    from . import core_ir as ir
+    L = load_library()
+    raw = json_text.encode() if isinstance(json_text, str) else bytes(json_text)
+    h = C.c_void_p()
+    err = C.create_string_buffer(256)
+    code = L.sk_program_from_json(raw, len(raw), C.byref(h), err, 256)
+    if code:
+        _raise(code, {"message": err.value.decode(errors="replace")})
+    try:
+        p = L.sk_program_json_get(h).contents
+        ops = ir.AGG_OPS
+
+        def term(t):
+            if t.kind == 0:
+                return ir.Var(t.a)
+            if t.kind == 1:
+                return ir.Const(t.a)
+            if t.kind == 2:
+                return ir.Int(t.b)
+            if t.kind == 3:
+                return ir.Agg(ops[t.a], t.b)
+            _raise(103, {"message": "compound terms (@terms) are outside this engine"})
+
+        def atom(i):
+            a = p.atoms[i]
+            return ir.CoreAtom(a.pred.decode(), [term(p.terms[a.first_term + k]) for k in range(a.n_terms)])
+
+        preds = [ir.CorePred(p.preds[i].name.decode(), p.preds[i].arity, ir.SEMIRINGS[p.preds[i].semiring],
+                             p.preds[i].stratum) for i in range(p.n_preds)]
+        rules = []
+        for i in range(p.n_rules):
+            r = p.rules[i]
+            body = [(ir.Neg if p.atoms[r.first_lit + k].negated else ir.Pos)(atom(r.first_lit + k))
+                    for k in range(r.n_lits)]
+            rules.append(ir.CoreRule(atom(r.head), body, r.stratum, r.var_count))
+        return ir.CoreProgram(preds, rules, p.num_strata)
+    finally:
+        L.sk_program_json_free(h)
 
 
 def binary_plan_intermediate(edges) -> int:

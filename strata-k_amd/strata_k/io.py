@@ -278,7 +278,7 @@ def weight_to_json(w):
 
 
 def _tagged(v, where):
-    if not isinstance(v, dict) or set(v) != {"kind", "data"}:
+    if not isinstance(v, dict) or "kind" not in v or "data" not in v:
         raise LoadError(f"{where}: expected a {{\"kind\", \"data\"}} object")
     return v["kind"], v["data"]
 
@@ -290,11 +290,10 @@ def _u32(v, where):
 
 
 def _fields(d, names, optional=(), where=""):
+    """Required ``names`` present; other fields are ignored, as serde does
+    for a struct without ``deny_unknown_fields``."""
     if not isinstance(d, dict):
         raise LoadError(f"{where}: expected an object")
-    extra = set(d) - set(names) - set(optional)
-    if extra:
-        raise LoadError(f"{where}: unknown field(s) {sorted(extra)}")
     missing = [n for n in names if n not in d]
     if missing:
         raise LoadError(f"{where}: missing field(s) {missing}")

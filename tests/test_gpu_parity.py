@@ -256,6 +256,31 @@ def test_json_program_and_high_ir_facts(ctx):
     assert "path(s, a) = inf\n" in text and "path(s, t) = 100\n" in text and "path(a, t) = 5\n" in text
 
 
+def test_prepare_json(ctx):
+    """sk_prepare_json: a serialised program runs as its struct form does;
+    decoding errors and compound terms are refused with their reasons."""
+    import json
+    from strata_k import io as IO
+    edb = [("edge", (Sym(a), Sym(b)), (a * 7 + b) % 13 + 1) for a in range(40) for b in range(40) if (a * b) % 11 == 3]
+    edb += [("src", (Sym(0),), 0), ("src", (Sym(5),), 0)]
+    prog = cost_program()
+    text = json.dumps(IO.core_program_to_json(prog))
+    ref = O.run(prog, edb)
+    for r in (ctx.prepare_json(text), ctx.prepare(prog)):
+        r.add_facts(edb)
+        r.load()
+        r.execute()
+        assert engine_rows(r.relation("cost")) == ref.relations["cost"].rows()
+        assert r.iterations() == ref.iterations
+        r.free()
+    with pytest.raises(sk.EngineError, match="missing field `num_strata`"):
+        ctx.prepare_json('{"predicates": [], "rules": []}')
+    d = IO.core_program_to_json(prog)
+    d["rules"][0]["head"]["args"][0] = {"kind": "compound", "data": {"functor": 3, "args": []}}
+    with pytest.raises(sk.EngineError):
+        ctx.prepare_json(json.dumps(d))
+
+
 def test_fuzz_programs(ctx):
     for seed in range(300):
         prog, edb = random_program(seed)

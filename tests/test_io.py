@@ -185,7 +185,6 @@ def test_core_program_json_roundtrip():
     (lambda d: d["predicates"][0].update(semiring="prob"), "semiring"),
     (lambda d: d["rules"][0]["head"]["args"][0].update(kind="compound"), "compound"),
     (lambda d: d["rules"][0]["body"][0].update(kind="maybe"), "literal kind"),
-    (lambda d: d["rules"][0].update(extra=1), "unknown field"),
     (lambda d: d.pop("num_strata"), "missing field"),
     (lambda d: d["rules"][0]["head"]["args"][0]["data"].update(slot=-1), "u32"),
 ])
@@ -254,3 +253,63 @@ def test_json_program_and_facts_through_the_oracle():
            "items": [_fact("edge", [_const(x), _const(y)]) for x, y in ("ab", "bc", "cd")]}
     r = O.run(prog, IO.high_ir_facts(doc, SymbolDict()))
     assert r.ok and len(r.relations["path"]) == 6
+
+
+# ------------------------------------------- the library's JSON decoder (C)
+
+def test_native_json_decoder_matches_python():
+    """sk_program_from_json (no device) decodes the same programs as the
+    Python decoder, including every shape this engine runs and aggregates."""
+    import json
+    import strata_k as sk
+    from strata_k.core_ir import CoreProgram, cost_program, neg, triangle_program
+    agg = CoreProgram([pred("e", 2, BOOL), pred("n", 1, BOOL), pred("deg", 2, BOOL, 1), pred("iso", 1, BOOL, 1)],
+                      [rule(atom("deg", v(0), Agg("count", 1)), [pos("e", v(0), v(1))], 2, 1),
+                       rule(atom("iso", v(0)), [pos("n", v(0)), neg("e", v(0), v(0))], 1, 1)], 2)
+    for p in (IO.core_program_from_json(CORE_TC_JSON), tc_program(BOOL), tc_program(TROP), cost_program(),
+              triangle_program(), agg):
+        text = json.dumps(IO.core_program_to_json(p))
+        assert sk.program_from_json(text) == IO.core_program_from_json(text) == p
+    assert sk.program_from_json(CORE_TC_JSON) == IO.core_program_from_json(CORE_TC_JSON)
+
+
+def test_native_json_decoder_serde_rules():
+    """Unknown fields ignored and neg_weight_cycle_check defaulted, as serde
+    does; string escapes (a surrogate pair included) decoded."""
+    import json
+    import strata_k as sk
+    d = json.loads(CORE_TC_JSON)
+    d["comment"] = {"nested": [1, 2, {"x": None}]}
+    d["rules"][0]["neg_weight_cycle_check"] = True
+    d["predicates"][0]["name"] = "edgé\U0001F600"
+    d["rules"][0]["body"][0]["data"]["pred"] = "edgé\U0001F600"
+    text = json.dumps(d)  # ASCII with \\u escapes
+    assert "\\ud83d\\ude00" in text
+    p = sk.program_from_json(text)
+    assert p.predicates[0].name == "edgé\U0001F600" == p.rules[0].body[0].atom.pred
+    q = IO.core_program_from_json(text)
+    # the C layout has no place for the checker's flag: the engine checks the
+    # universe + 2 round cap of every Trop stratum anyway
+    assert q.rules[0].neg_weight_cycle_check and not p.rules[0].neg_weight_cycle_check
+    q.rules[0].neg_weight_cycle_check = False
+    assert p == q
+
+
+@pytest.mark.parametrize("text,msg", [
+    ('{"predicates": [], "rules": [], "num_strata": 1,}', "expected a field name"),
+    ('{"predicates": [], "rules": []}', "missing field `num_strata`"),
+    ('{"predicates": [], "rules": [], "num_strata": 1.5}', "fraction or exponent"),
+    ('{"predicates": [], "rules": [], "num_strata": -1}', "num_strata: expected a u32"),
+    ('{"predicates": [{"name": "e", "arity": 2, "semiring": "prob", "stratum": 0}], "rules": [], '
+     '"num_strata": 1}', "predicates[0].semiring"),
+    ('{"predicates": [], "rules": [{"head": {"pred": "p", "args": [{"kind": "var", "data": {"slot": 0}}]}, '
+     '"body": [{"kind": "maybe", "data": {"pred": "e", "args": []}}], "stratum": 0, "var_count": 1}], '
+     '"num_strata": 1}', "rules[0].body[0]: unknown literal kind"),
+    ('{"predicates": [], "rules": [], "num_strata": 1} x', "trailing characters"),
+    ('{"predicates": "', "unterminated string"),
+    ("[" * 300 + "]" * 300, "nesting too deep"),
+])
+def test_native_json_decoder_errors(text, msg):
+    import strata_k as sk
+    with pytest.raises(sk.EngineError, match=msg.replace("[", r"\[").replace("]", r"\]").replace(".", r"\.")):
+        sk.program_from_json(text)
