@@ -414,6 +414,66 @@ def bench_sort(args):
     print(json.dumps({"metric": "radix sort keys/s", "unit": "keys/s", "results": out}), flush=True)
 
 
+# The reference's published chain-union figures (crates/strata-gpu/README.md:56-58,
+# RTX-4090-class card, CUDA 11.8; examples/bench.rs:22-90 times the whole host
+# call): (semiring, edges, chain length, seconds).
+CHAIN_UNION_REF = [("bool", 10_000_000, 5, 1.08), ("trop", 10_000_000, 5, 2.07),
+                   ("bool", 100_000_000, 2, 3.2), ("trop", 100_000_000, 2, 7.9),
+                   ("bool", 100_000_000, 5, 9.1)]
+
+
+def bench_chain_union(args):
+    """The reference's own benchmark workload: K disjoint chains of C edges
+    (chain k on nodes k(C+1) .. k(C+1)+C), closure K·C(C+1)/2, through the
+    crate functions transitive_closure_bool / shortest_paths_trop (unit
+    weights: distance = d - s), timed as the reference times them: the whole
+    host call, host edges in and host result out (H2D, fixpoint, D2H).
+    Checked: the closure size, and for Trop that every distance is d - s."""
+    import ctypes as C
+    import strata_k as sk
+    from strata_k.engine import _take
+    ctx = sk.Context(0)
+    L = ctx.L
+    out = []
+    for sem, m, c, ref_s in CHAIN_UNION_REF:
+        k = m // c
+        base = np.arange(k, dtype=np.int64) * (c + 1)
+        src = (base[:, None] + np.arange(c)[None, :]).reshape(-1).astype(np.uint32)
+        dst = src + np.uint32(1)
+        w = np.ones(len(src), dtype=np.int64)
+        expected = k * c * (c + 1) // 2
+        times = []
+        for rep in range(2):  # the first call also pays one-time set-up (autotuner classes, pool growth)
+            pa, pb, pw, n = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_uint64()
+            t0 = time.perf_counter()
+            if sem == "bool":
+                code = L.sk_transitive_closure_bool(ctx.h, src.ctypes.data, dst.ctypes.data, len(src), C.byref(pa),
+                                                    C.byref(pb), C.byref(n))
+            else:
+                code = L.sk_shortest_paths_trop(ctx.h, src.ctypes.data, dst.ctypes.data, w.ctypes.data, len(src),
+                                                C.byref(pa), C.byref(pb), C.byref(pw), C.byref(n))
+            times.append(time.perf_counter() - t0)
+            if code:
+                raise RuntimeError(f"chain union {sem} {m}: status {code}")
+            a = _take(L, pa, n.value, np.uint32)
+            b = _take(L, pb, n.value, np.uint32)
+            ok = n.value == expected
+            if sem == "trop":
+                ww = _take(L, pw, n.value, np.int64)
+                ok = ok and bool(np.array_equal(ww, b.astype(np.int64) - a.astype(np.int64)))
+            del a, b
+        t = min(times)
+        log(f"chain union {sem} {m} edges, C={c}: {n.value} pairs, calls {['%.3f' % x for x in times]} s")
+        out.append({"semiring": sem, "edges": m, "chain_len": c, "closure": expected, "exact": bool(ok),
+                    "seconds": round(t, 4), "closure_tuples_per_s": round(expected / t, 1),
+                    "first_call_s": round(times[0], 4), "reference_seconds": ref_s,
+                    "reference_tuples_per_s": round(expected / ref_s, 1), "speedup_vs_reference": round(ref_s / t, 2)})
+    ctx.close()
+    print(json.dumps({"metric": "chain-union closure tuples/s (the reference's bench workload, whole host call)",
+                      "unit": "tuples/s", "reference_hardware": "RTX-4090-class, CUDA 11.8 (published)",
+                      "results": out}), flush=True)
+
+
 def asp_program(rng, n_rules, n_atoms, max_body):
     """A random ground program with the shape of the reference's large
     simplify-filter case (crates/strata-gpu/tests/asp_diff.rs:24-58,72-77):
@@ -528,6 +588,8 @@ def main():
                          "rounds inside graph launches are timed as one record per launch")
     ap.add_argument("--sort", action="store_true", help="time the device radix sort primitive")
     ap.add_argument("--sort-log2", type=int, default=26)
+    ap.add_argument("--chain-union", action="store_true",
+                    help="the reference's chain-union benchmark through the crate functions")
     ap.add_argument("--asp", action="store_true", help="time the ASP grounding simplify filter")
     ap.add_argument("--asp-rules", type=int, default=4_000_000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -544,6 +606,9 @@ def main():
         return
     if args.asp:
         bench_asp(args)
+        return
+    if args.chain_union:
+        bench_chain_union(args)
         return
     {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3, 4: bench_cfg4, 5: bench_cfg5}[args.config](args)
 

@@ -56,11 +56,10 @@ struct TcProg {
   }
 };
 
+// An sk_free-able result buffer of n entries (at least one byte).
 template <class T>
-T* dup_out(const std::vector<T>& v) {
-  T* p = (T*)std::malloc(std::max<size_t>(1, v.size() * sizeof(T)));
-  if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
-  return p;
+T* out_alloc(int64_t n) {
+  return (T*)std::malloc(std::max<size_t>(1, (size_t)std::max<int64_t>(n, 0) * sizeof(T)));
 }
 
 }  // namespace
@@ -77,14 +76,21 @@ sk_status sk_transitive_closure_bool(sk_ctx* ctx, const uint32_t* src, const uin
   const uint32_t* cols[2] = {src, dst};
   if (!(s = sk_run_add_facts_sym(run, "edge", n, 2, cols, nullptr, nullptr)) && !(s = sk_run_load(run)) &&
       !(s = sk_run_execute(run))) {
-    int64_t m = sk_result_size(run, "path");
-    std::vector<uint32_t> a(m), b(m);
-    uint32_t* oc[2] = {a.data(), b.data()};
-    s = sk_result_read_sym(run, "path", oc, nullptr, nullptr);
+    // the result is read straight into the caller's buffers (no zeroed
+    // staging vectors, no second copy)
+    const int64_t m = sk_result_size(run, "path");
+    uint32_t* a = out_alloc<uint32_t>(m);
+    uint32_t* b = out_alloc<uint32_t>(m);
+    uint32_t* oc[2] = {a, b};
+    if (!a || !b) s = SK_ERR_OUT_OF_BUDGET;
+    else s = sk_result_read_sym(run, "path", oc, nullptr, nullptr);
     if (!s) {
-      *out_src = dup_out(a);
-      *out_dst = dup_out(b);
+      *out_src = a;
+      *out_dst = b;
       *out_n = (uint64_t)m;
+    } else {
+      std::free(a);
+      std::free(b);
     }
   }
   sk_run_free(run);
@@ -102,16 +108,22 @@ sk_status sk_shortest_paths_trop(sk_ctx* ctx, const uint32_t* src, const uint32_
   int64_t dummy = 0;
   if (!(s = sk_run_add_facts_sym(run, "edge", n, 2, cols, n ? w : &dummy, nullptr)) && !(s = sk_run_load(run)) &&
       !(s = sk_run_execute(run))) {
-    int64_t m = sk_result_size(run, "path");
-    std::vector<uint32_t> a(m), b(m);
-    std::vector<int64_t> ww(m);
-    uint32_t* oc[2] = {a.data(), b.data()};
-    s = sk_result_read_sym(run, "path", oc, ww.data(), nullptr);
+    const int64_t m = sk_result_size(run, "path");
+    uint32_t* a = out_alloc<uint32_t>(m);
+    uint32_t* b = out_alloc<uint32_t>(m);
+    int64_t* ww = out_alloc<int64_t>(m);
+    uint32_t* oc[2] = {a, b};
+    if (!a || !b || !ww) s = SK_ERR_OUT_OF_BUDGET;
+    else s = sk_result_read_sym(run, "path", oc, ww, nullptr);
     if (!s) {
-      *out_src = dup_out(a);
-      *out_dst = dup_out(b);
-      *out_w = dup_out(ww);
+      *out_src = a;
+      *out_dst = b;
+      *out_w = ww;
       *out_n = (uint64_t)m;
+    } else {
+      std::free(a);
+      std::free(b);
+      std::free(ww);
     }
   }
   sk_run_free(run);
