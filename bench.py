@@ -474,6 +474,40 @@ def bench_chain_union(args):
                       "results": out}), flush=True)
 
 
+def bench_tri_hubs(args):
+    """The reference's skewed triangle example (crates/strata-gpu/examples/
+    tri.rs, defaults: 16 hubs, 100,000 bridges, 500,000 random neighbours per
+    hub below 10 M ids: 9.6 M edges, 12,000,560 triangles) through
+    count_triangles, timed as the example times it (the host call); the
+    reference publishes ~0.54 s (crates/strata-gpu/README.md:73-77)."""
+    import ctypes as C
+    import strata_k as sk
+    h, b, d, n = 16, 100_000, 500_000, 10_000_000
+    ctx = sk.Context(0)
+    L = ctx.L
+    cap = h * (h - 1) // 2 + h * b + h * d
+    src = np.empty(cap, np.uint32)
+    dst = np.empty(cap, np.uint32)
+    m = int(L.sk_wl_tri_hubs(h, b, d, n, src.ctypes.data, dst.ctypes.data))
+    expected = b * (h * (h - 1) // 2) + h * (h - 1) * (h - 2) // 6
+    times, got = [], None
+    for _ in range(3):  # the first call also pays one-time set-up
+        cnt = C.c_uint64()
+        t0 = time.perf_counter()
+        code = L.sk_count_triangles(ctx.h, src.ctypes.data, dst.ctypes.data, m, C.byref(cnt))
+        times.append(time.perf_counter() - t0)
+        if code:
+            raise RuntimeError(f"count_triangles: status {code}")
+        got = cnt.value
+    t = min(times)
+    ctx.close()
+    print(json.dumps({"metric": "triangle count, the reference's skewed hub graph (whole host call)", "unit": "s",
+                      "edges": m, "triangles": got, "exact": got == expected, "seconds": round(t, 4),
+                      "calls_s": [round(x, 4) for x in times], "reference_seconds": 0.54,
+                      "speedup_vs_reference": round(0.54 / t, 2),
+                      "reference_hardware": "RTX-4090-class, CUDA 11.8 (published)"}), flush=True)
+
+
 def asp_program(rng, n_rules, n_atoms, max_body):
     """A random ground program with the shape of the reference's large
     simplify-filter case (crates/strata-gpu/tests/asp_diff.rs:24-58,72-77):
@@ -590,6 +624,8 @@ def main():
     ap.add_argument("--sort-log2", type=int, default=26)
     ap.add_argument("--chain-union", action="store_true",
                     help="the reference's chain-union benchmark through the crate functions")
+    ap.add_argument("--tri-hubs", action="store_true",
+                    help="the reference's skewed triangle example through count_triangles")
     ap.add_argument("--asp", action="store_true", help="time the ASP grounding simplify filter")
     ap.add_argument("--asp-rules", type=int, default=4_000_000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -609,6 +645,9 @@ def main():
         return
     if args.chain_union:
         bench_chain_union(args)
+        return
+    if args.tri_hubs:
+        bench_tri_hubs(args)
         return
     {1: bench_cfg1, 2: bench_cfg2, 3: bench_cfg3, 4: bench_cfg4, 5: bench_cfg5}[args.config](args)
 

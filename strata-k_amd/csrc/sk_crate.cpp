@@ -148,11 +148,16 @@ static std::vector<uint64_t> orient_by_id(const uint32_t* a, const uint32_t* b, 
 sk_status sk_count_triangles(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* out) {
   if (!ctx || !out || (n && (!a || !b))) return SK_ERR_INVALID_ARGUMENT;
   // tri(X,Y,Z) :- e(X,Y), e(Y,Z), e(X,Z).  ntri(count<X>) :- tri(X,_Y,_Z).
-  std::vector<uint64_t> k = orient_by_id(a, b, n);
-  std::vector<uint32_t> u(k.size()), v(k.size());
-  for (size_t i = 0; i < k.size(); i++) {
-    u[i] = (uint32_t)(k[i] >> 32);
-    v[i] = (uint32_t)k[i];
+  // Orient u < v and drop self-loops here (wcoj.rs:99-131); the duplicates
+  // the orientation creates are removed by the load's device sort.
+  std::vector<uint32_t> u, v;
+  u.reserve(n);
+  v.reserve(n);
+  for (uint64_t i = 0; i < n; i++) {
+    const uint32_t x = a[i], y = b[i];
+    if (x == y) continue;
+    u.push_back(x < y ? x : y);
+    v.push_back(x < y ? y : x);
   }
   sk_pred preds[2] = {{"e", 2, SK_BOOL, 0, 0}, {"tri", 3, SK_BOOL, 0, 0}};
   auto V = [](uint32_t s) { return sk_term{SK_T_VAR, s, 0}; };

@@ -309,6 +309,39 @@ uint64_t sk_wl_log_uniform(uint64_t seed, uint64_t i, uint64_t lo, uint64_t hi) 
   return r;
 }
 
+// The reference's skewed triangle workload (crates/strata-gpu/examples/tri.rs:
+// 27-56): a clique on h hubs, b bridge nodes joined to every hub, and for
+// each hub d random neighbours drawn from its own block of the remaining ids
+// below n by the example's LCG (state * 6364136223846793005 +
+// 1442695040888963407, value (state >> 33) % block, from
+// 0x123456789abcdef0).  src/dst hold h(h-1)/2 + h*b + h*min(d, block)
+// entries; returns the count written.
+uint64_t sk_wl_tri_hubs(uint32_t h, uint32_t b, uint32_t d, uint32_t n, uint32_t* src, uint32_t* dst) {
+  uint64_t m = 0;
+  for (uint32_t i = 0; i < h; i++)
+    for (uint32_t j = i + 1; j < h; j++) {
+      src[m] = i;
+      dst[m++] = j;
+    }
+  for (uint32_t k = 0; k < b; k++)
+    for (uint32_t hub = 0; hub < h; hub++) {
+      src[m] = hub;
+      dst[m++] = h + k;
+    }
+  uint64_t state = 0x123456789abcdef0ull;
+  const uint32_t lo = h + b;
+  const uint32_t block = std::max<uint32_t>(h ? (n - lo) / h : (n - lo), 1u);
+  for (uint32_t hub = 0; hub < h; hub++) {
+    const uint32_t base = lo + hub * block;
+    for (uint32_t q = 0; q < std::min(d, block); q++) {
+      state = state * 6364136223846793005ull + 1442695040888963407ull;
+      src[m] = hub;
+      dst[m++] = base + (uint32_t)((state >> 33) % block);
+    }
+  }
+  return m;
+}
+
 // Uniform double in [0, 1) from stream (6, i) (Poisson arrival gaps).
 double sk_wl_uniform01(uint64_t seed, uint64_t i) {
   return (double)(draw(seed, 6, i) >> 11) * (1.0 / 9007199254740992.0);

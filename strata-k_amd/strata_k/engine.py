@@ -211,6 +211,7 @@ SIGNATURES = [
     ("sk_server_release", C.c_int, [_VP, _U64]),
     # workload generators (host code in the same library)
     ("sk_wl_uniform_graph", _U64, [_U64, _U32, _U64, _VP, _VP]),
+    ("sk_wl_tri_hubs", _U64, [_U32, _U32, _U32, _U32, _VP, _VP]),
     ("sk_wl_rmat", _U64, [_U64, _U32, _U64, C.c_double, C.c_double, C.c_double, _U32, _VP, _VP]),
     ("sk_wl_normalize", _U64, [_VP, _VP, _U64, C.c_int, C.c_int]),
     ("sk_wl_weights", None, [_U64, _U64, _I64, _I64, _VP]),
