@@ -124,9 +124,68 @@ void Ctx::init(int dev, u64 budget) {
   SK_HIP(hipStreamSynchronize(stream));
 }
 
+namespace {
+
+constexpr u64 kStageDirect = 8ull << 20;  // copies below this are one enqueued hipMemcpyAsync
+
+// memcpy of len bytes split over up to 8 host threads
+void par_copy(u8* to, const u8* from, u64 len) {
+  const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const u64 part = (len + nthreads - 1) / nthreads;
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < nthreads; t++) {
+    const u64 a = std::min(len, t * part), e = std::min(len, a + part);
+    if (a < e) ts.emplace_back([=] { std::memcpy(to + a, from + a, e - a); });
+  }
+  std::memcpy(to, from, std::min(len, part));
+  for (auto& th : ts) th.join();
+}
+
+// largest of n u32 values, over up to 8 host threads
+u32 par_max(const u32* v, u64 n) {
+  const unsigned nthreads = n < (1u << 20) ? 1u : std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const u64 part = (n + nthreads - 1) / nthreads;
+  std::vector<u32> best(nthreads, 0);
+  auto scan = [&](unsigned t) {
+    const u64 a = std::min(n, t * part), e = std::min(n, a + part);
+    u32 m = 0;
+    for (u64 i = a; i < e; i++) m = std::max(m, v[i]);
+    best[t] = m;
+  };
+  std::vector<std::thread> ts;
+  for (unsigned t = 1; t < nthreads; t++) ts.emplace_back(scan, t);
+  scan(0);
+  for (auto& th : ts) th.join();
+  return *std::max_element(best.begin(), best.end());
+}
+
+}  // namespace
+
+void Ctx::h2d(void* dst, const void* src, u64 bytes) {
+  if (bytes < kStageDirect) {
+    if (bytes) SK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    return;
+  }
+  for (int b = 0; b < 2; b++)
+    if (!stage[b]) {
+      SK_HIP(hipHostMalloc((void**)&stage[b], kStageBytes, hipHostMallocDefault));
+      SK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
+    }
+  // a buffer is refilled once the copy that read it has finished (events
+  // recorded by earlier calls on this stream count too)
+  const u64 nchunks = (bytes + kStageBytes - 1) / kStageBytes;
+  for (u64 k = 0; k < nchunks; k++) {
+    const int b = (int)(k & 1);
+    SK_HIP(hipEventSynchronize(stage_ev[b]));
+    const u64 off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    par_copy(stage[b], (const u8*)src + off, len);
+    SK_HIP(hipMemcpyAsync((u8*)dst + off, stage[b], len, hipMemcpyHostToDevice, stream));
+    SK_HIP(hipEventRecord(stage_ev[b], stream));
+  }
+}
+
 void Ctx::d2h(void* dst, const void* src, u64 bytes) {
-  constexpr u64 kDirect = 8ull << 20;  // below this, one enqueued copy
-  if (bytes < kDirect) {
+  if (bytes < kStageDirect) {
     if (bytes) SK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
     return;
   }
@@ -135,7 +194,6 @@ void Ctx::d2h(void* dst, const void* src, u64 bytes) {
       SK_HIP(hipHostMalloc((void**)&stage[b], kStageBytes, hipHostMallocDefault));
       SK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
     }
-  const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
   const u64 nchunks = (bytes + kStageBytes - 1) / kStageBytes;
   auto issue = [&](u64 k) {
     const u64 off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
@@ -149,16 +207,7 @@ void Ctx::d2h(void* dst, const void* src, u64 bytes) {
     if (k + 1 < nchunks) issue(k + 1);
     SK_HIP(hipEventSynchronize(stage_ev[k & 1]));
     const u64 off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
-    const u8* from = stage[k & 1];
-    u8* to = (u8*)dst + off;
-    const u64 part = (len + nthreads - 1) / nthreads;
-    std::vector<std::thread> ts;
-    for (unsigned t = 1; t < nthreads; t++) {
-      const u64 a = std::min(len, t * part), e = std::min(len, a + part);
-      if (a < e) ts.emplace_back([=] { std::memcpy(to + a, from + a, e - a); });
-    }
-    std::memcpy(to, from, std::min(len, part));
-    for (auto& th : ts) th.join();
+    par_copy((u8*)dst + off, stage[k & 1], len);
   }
 }
 
@@ -428,8 +477,7 @@ void Engine::build_dict() {
   std::set<i64> ints;
   for (auto& R : R_) {
     if (max_sym_hint_ == ~0ull)
-      for (auto& c : R.st_sym)
-        for (u32 v : c) maxsym = std::max<u64>(maxsym, v);
+      for (auto& c : R.st_sym) maxsym = std::max<u64>(maxsym, par_max(c.data(), c.size()));
     for (size_t i = 0; i < R.st_kind.size(); i++) {
       if (R.st_kind[i] == SK_VAL_SYM) {
         if (R.st_val[i] < 0 || R.st_val[i] > 0xffffffffll)
@@ -483,8 +531,7 @@ void Engine::upload_rel(int r) {
   for (int j = 0; j < R.arity; j++) {
     dcols[j].reset(x.pool, n);
     cols.c[j] = dcols[j].get();
-    if (nsym)
-      SK_HIP(hipMemcpyAsync(dcols[j].get(), R.st_sym[j].data(), nsym * 4, hipMemcpyHostToDevice, x.stream));
+    if (nsym) ctx_->h2d(dcols[j].get(), R.st_sym[j].data(), nsym * 4);
     u64 ng = n - nsym;
     if (ng) {
       std::vector<u32> enc(ng);
@@ -504,11 +551,11 @@ void Engine::upload_rel(int r) {
   if (trop) {
     w1.reset(x.pool, n);
     w2.reset(x.pool, n);
-    SK_HIP(hipMemcpyAsync(w1.get(), R.st_w.data(), n * 8, hipMemcpyHostToDevice, x.stream));
+    ctx_->h2d(w1.get(), R.st_w.data(), n * 8);
     if (has_inf_) {
       f1.reset(x.pool, n);
       f2.reset(x.pool, n);
-      SK_HIP(hipMemcpyAsync(f1.get(), R.st_inf.data(), n, hipMemcpyHostToDevice, x.stream));
+      ctx_->h2d(f1.get(), R.st_inf.data(), n);
     }
   }
   with_key(R.wide, [&](auto* tag) {

@@ -295,6 +295,10 @@ struct Ctx {
   // staging buffers in chunks, each drained into `dst` by several host
   // threads while the next chunk is in flight, and have finished on return.
   void d2h(void* dst, const void* src, u64 bytes);
+  // Host-to-device copy from caller memory, the mirror of d2h: large copies
+  // are packed into the pinned buffers by several host threads while the
+  // previous chunk is in flight; `src` may be reused on return.
+  void h2d(void* dst, const void* src, u64 bytes);
 };
 
 struct Chunk {
