@@ -128,8 +128,12 @@ namespace {
 
 constexpr u64 kStageDirect = 8ull << 20;  // copies below this are one enqueued hipMemcpyAsync
 
-// memcpy of len bytes split over up to 8 host threads
+// memcpy of len bytes split over up to 8 host threads (small copies: one)
 void par_copy(u8* to, const u8* from, u64 len) {
+  if (len < (4ull << 20)) {
+    if (len) std::memcpy(to, from, len);
+    return;
+  }
   const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
   const u64 part = (len + nthreads - 1) / nthreads;
   std::vector<std::thread> ts;
@@ -418,7 +422,9 @@ void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* con
     R.st_sym.resize(arity);
     for (u32 j = 0; j < arity; j++) {
       if (R.st_reserve > R.st_sym[j].capacity()) R.st_sym[j].reserve(R.st_reserve);
-      R.st_sym[j].insert(R.st_sym[j].end(), cols[j], cols[j] + n);
+      const u64 at = R.st_sym[j].size();
+      R.st_sym[j].resize(at + n);  // uninitialised: filled by the copy
+      par_copy((u8*)(R.st_sym[j].data() + at), (const u8*)cols[j], n * 4);
     }
   }
   if (weighted) {
@@ -447,7 +453,7 @@ void Engine::stage_sym_resize(const std::string& pred, u64 total, bool weighted)
   if (weighted != (R.sem == SK_TROP))
     throw mkerr(SK_ERR_SEMIRING_MISMATCH, pred, "fact annotation differs from the predicate's semiring");
   if (!R.st_kind.empty() || R.st_rows) throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "relation already staged");
-  R.st_sym.assign(R.arity, std::vector<u32>());
+  R.st_sym.assign(R.arity, SymCol());
   for (int j = 0; j < R.arity; j++) R.st_sym[j].resize(total);
   if (weighted) {
     R.st_w.resize(total);
@@ -582,7 +588,7 @@ void Engine::upload_rel(int r) {
   // release staging
   std::vector<u32>().swap(R.st_kind);
   std::vector<i64>().swap(R.st_val);
-  std::vector<std::vector<u32>>().swap(R.st_sym);
+  std::vector<SymCol>().swap(R.st_sym);
   std::vector<i64>().swap(R.st_w);
   std::vector<u8>().swap(R.st_inf);
   R.st_rows = 0;

@@ -130,6 +130,30 @@ struct Index {
   u64 pull_n = 0, pull_version = ~0ull;
 };
 
+
+// Allocator whose value-initialisation is a no-op: staging columns are
+// resized and then filled by parallel copies, so zeroing them first would
+// be a wasted pass over the memory.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U>&) {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+using SymCol = std::vector<u32, NoInitAlloc<u32>>;
+
 struct DRel {
   int arity = 0;
   u32 sem = SK_BOOL;
@@ -144,7 +168,7 @@ struct DRel {
   // staged EDB facts (host, before sk_run_load)
   std::vector<u32> st_kind;  // row-major kinds (empty when all Sym)
   std::vector<i64> st_val;   // row-major values
-  std::vector<std::vector<u32>> st_sym;  // all-Sym column stage
+  std::vector<SymCol> st_sym;  // all-Sym column stage
   std::vector<i64> st_w;
   std::vector<u8> st_inf;
   bool st_weighted = false;
