@@ -314,8 +314,11 @@ uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint6
  * or a negative one certain (cert); in a kept rule, certain positive and
  * impossible negative literals are dropped.  Outputs (caller buffers of
  * n_rules heads, n_rules + 1 offsets, body_start[n_rules] literals) keep
- * input order.  A literal naming no atom below n_atoms, or decreasing
- * offsets, is SK_ERR_INVALID_ARGUMENT. */
+ * input order.  Every certain atom must be possible (cert[a] implies
+ * poss[a]): the reference's CPU filter (asp.rs:53-66) and its device filter
+ * disagree on an atom certain but not possible, so such an atom, a literal
+ * naming no atom below n_atoms, or decreasing offsets, is
+ * SK_ERR_INVALID_ARGUMENT. */
 sk_status sk_asp_simplify_filter(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
                                  const int32_t* body_lit, uint64_t n_rules, const uint8_t* poss,
                                  const uint8_t* cert, uint64_t n_atoms, int32_t* out_head,

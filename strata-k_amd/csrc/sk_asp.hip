@@ -125,9 +125,18 @@ __device__ __forceinline__ u64 asp_block_incl(u64 v, u64* total) {
 
 }  // namespace
 
-__global__ void k_asp_state(const u8* __restrict__ poss, const u8* __restrict__ cert, u64 n, u8* __restrict__ st) {
+// Atom states: 0 impossible, 1 possible, 2 certain.  Certain atoms must also
+// be possible (the two reference paths, asp.rs:53-66 and its device filter,
+// disagree otherwise); an atom certain but not possible raises the malformed
+// flag.
+__global__ void k_asp_state(const u8* __restrict__ poss, const u8* __restrict__ cert, u64 n, u8* __restrict__ st,
+                            u64* __restrict__ flag) {
   const u64 a = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < n) st[a] = cert[a] ? 2 : (poss[a] ? 1 : 0);
+  if (a < n) {
+    const bool c = cert[a] != 0, p = poss[a] != 0;
+    st[a] = c ? 2 : (p ? 1 : 0);
+    if (c && !p) atomicOr((unsigned long long*)flag, 1ull);
+  }
 }
 
 __global__ __launch_bounds__(kAspT) void k_asp_count(const u32* __restrict__ bs, const int32_t* __restrict__ lits,
@@ -291,13 +300,13 @@ bool asp_simplify(const Exec& x, const int32_t* head, const u32* bs, const int32
   DBuf<u64> ba(x.pool, nblk ? nblk : 1), bk(x.pool, nblk ? nblk : 1), tot(x.pool, 3);
   DBuf<u8> st(x.pool, n_atoms ? n_atoms : 1);
   memset_async(x, tot.get(), 0, 24);
+  if (n_atoms) {
+    Scope sc(x.timer, "asp_state", KC_MISC, (double)n_atoms * 3.0, n_atoms);
+    k_asp_state<<<(unsigned)((n_atoms + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(poss, cert, n_atoms,
+                                                                                    st.get(), tot.get() + 2);
+    SK_HIP(hipGetLastError());
+  }
   if (n_rules) {
-    if (n_atoms) {
-      Scope sc(x.timer, "asp_state", KC_MISC, (double)n_atoms * 3.0, n_atoms);
-      k_asp_state<<<(unsigned)((n_atoms + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(poss, cert, n_atoms,
-                                                                                      st.get());
-      SK_HIP(hipGetLastError());
-    }
     {
       // per rule its end offset, per literal the literal (the atom states
       // stay in cache), per block two totals written

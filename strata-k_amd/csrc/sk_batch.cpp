@@ -188,6 +188,10 @@ sk_status sk_batch_add(sk_batch* b, const sk_program* prog, uint32_t n_edb, cons
     if (r < 0) return SK_ERR_UNKNOWN_PRED;
     u32 ar = bp.prog.preds[r].arity;
     if (edb_n[i] && !edb_cols[i]) return SK_ERR_INVALID_ARGUMENT;
+    // weights exactly for Trop predicates, as a plain run's staging requires
+    // (Engine::stage_sym): one group shares one staged layout per relation
+    const bool weighted = edb_w && edb_w[i];
+    if (weighted != (bp.prog.preds[r].sem == SK_TROP)) return SK_ERR_SEMIRING_MISMATCH;
     bp.edb_preds.push_back(edb_preds[i]);
     bp.edb_n.push_back(edb_n[i]);
     std::vector<const u32*> cols(ar);
@@ -399,11 +403,16 @@ sk_status sk_batch_result_read_sym(sk_batch* b, uint32_t index, const char* pred
   if (bp.err.code) return (sk_status)bp.err.code;
   int r = bp.prog.rel_of(pred);
   if (r < 0) return SK_ERR_UNKNOWN_PRED;
+  (void)hipSetDevice(b->ctx->c.device);
   try {
     g.eng->partition_read_sym(g.fused.preds[r].name, (u32)bp.member, cols, w, inf);
   } catch (const EvalErr& e) {
     return (sk_status)e.code;
   } catch (const DeviceError&) {
+    return SK_ERR_DEVICE;
+  } catch (const BudgetError&) {
+    return SK_ERR_OUT_OF_BUDGET;
+  } catch (const std::exception&) {
     return SK_ERR_DEVICE;
   }
   return SK_OK;

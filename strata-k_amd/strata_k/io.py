@@ -27,6 +27,7 @@ scope and are refused.
 """
 import json
 import os
+import re
 from typing import Dict, List, Optional, Sequence, Tuple
 
 from .core_ir import (AGG_OPS, BOOL, INF, SEMIRINGS, TROP, Agg, Const, CoreAtom, CorePred, CoreProgram, CoreRule,
@@ -139,6 +140,10 @@ def _json_rows(path: str):
         for v in row:
             if isinstance(v, bool) or not isinstance(v, (str, int, float)):
                 raise LoadError(f"{path}:{i + 1}: unsupported JSON value {json.dumps(v)} (strings and numbers only)")
+            if isinstance(v, int) and not -(2**63) <= v < 2**63:
+                # serde's as_i64 fails outside i64 and the number is read as a
+                # float cell (input.rs:316-321)
+                v = float(v)
             cells.append(v)
         rows.append((i + 1, cells))
     return rows
@@ -157,10 +162,14 @@ def read_rows(path: str):
     raise LoadError(f"{path}: unsupported input format `{ext}` (tsv, csv, or json)")
 
 
+_INT_TEXT = re.compile(r"[+-]?[0-9]+")
+
+
 def _parse_int(s: str) -> Optional[int]:
+    """`s.trim().parse::<i64>()`: an optional sign and ASCII digits only,
+    within i64."""
     s = s.strip()
-    body = s[1:] if s[:1] in "+-" else s
-    if not body.isdigit():
+    if not _INT_TEXT.fullmatch(s):
         return None
     v = int(s)
     return v if -(2**63) <= v < 2**63 else None

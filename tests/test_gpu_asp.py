@@ -88,6 +88,19 @@ def test_malformed_literals_refused(ctx, lits):
         sk.asp_simplify_filter([1], [0, len(lits)], lits, [1, 1], [0, 0], ctx=ctx)
 
 
+@pytest.mark.parametrize("n_rules", [1, 0])
+def test_certain_atom_must_be_possible(ctx, n_rules):
+    """An atom certain but not possible is refused, with or without rules:
+    the reference's CPU filter (asp.rs:53-66) and its device filter decide
+    such an atom differently."""
+    head, bs, bl = ([1], [0, 1], [2]) if n_rules else ([], [0], [])
+    with pytest.raises(sk.EngineError):
+        sk.asp_simplify_filter(head, bs, bl, [1, 0], [0, 1], ctx=ctx)
+    # the same atoms with cert within poss are accepted
+    assert same(sk.asp_simplify_filter(head, bs, bl, [1, 1], [0, 1], ctx=ctx),
+                O.asp_simplify_filter(head, bs, bl, [1, 1], [0, 1]))
+
+
 def test_device_resident_entry_and_kernel_records(ctx):
     """The device-buffer entry point equals the host-buffer one; the context's
     kernel records name the count and emit passes."""

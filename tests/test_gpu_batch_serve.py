@@ -106,6 +106,27 @@ def test_batch_isolates_errors(ctx):
     b.free()
 
 
+def test_batch_add_refuses_semiring_mismatch(ctx):
+    """Weights go with Trop predicates only, as a plain run's staging
+    requires: weights on a Bool predicate, or none on a Trop one, are
+    SEMIRING_MISMATCH at add time, and the batch stays usable."""
+    b = sk.Batch(ctx)
+    e = [np.array([0, 1], np.uint32), np.array([1, 2], np.uint32)]
+    with pytest.raises(sk.EngineError) as ei:
+        b.add(tc_program(), {"edge": (e, np.array([1, 1], np.int64))})
+    assert ei.value.code == 50
+    with pytest.raises(sk.EngineError) as ei:
+        b.add(cost_program(), {"edge": (e, None), "src": ([np.array([0], np.uint32)], np.zeros(1, np.int64))})
+    assert ei.value.code == 50
+    i = b.add(tc_program(), {"edge": (e, None)})
+    b.load()
+    b.execute()
+    assert b.status(i)[0] == 0
+    cols, _ = b.relation_sym(i, "path")
+    assert list(zip(cols[0].tolist(), cols[1].tolist())) == [(0, 1), (0, 2), (1, 2)]
+    b.free()
+
+
 def _server_graph():
     s, d = wl.rmat_graph(51, 12, 60_000)
     w = wl.weights(51, len(s))

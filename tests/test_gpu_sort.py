@@ -91,7 +91,12 @@ def test_tuner_picks_a_candidate(ctx):
         for wide in (False, True):
             for weighted in (False, True):
                 c = ctx.radix_choice(1 << 20, 40, wide=wide, weighted=weighted)
-                assert c["digit_bits"] in (6, 7, 8) and c["items_per_thread"] in (4, 8, 16), c
+                # the tuner's eight candidates (digit bits, items per thread for
+                # u64 keys; half the items for u128 keys)
+                cands = [(8, 16), (8, 8), (7, 16), (7, 8), (6, 16), (6, 8), (8, 32), (6, 32)]
+                if wide:
+                    cands = [(d, i // 2) for d, i in cands]
+                assert (c["digit_bits"], c["items_per_thread"]) in cands, c
                 assert c["est_ms"] > 0, c
         keys = np.random.default_rng(3).integers(0, 1 << 40, size=1 << 20, dtype=np.uint64)
         assert (sk.sort_u64(keys, ctx) == np.sort(keys)).all()

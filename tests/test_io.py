@@ -63,6 +63,34 @@ def test_load_errors_name_the_line(tmp_path, name, text, msg):
         load_input(str(tmp_path / name), "p", 2, BOOL, ["node", "node"], SymbolDict())
 
 
+@pytest.mark.parametrize("cell", ["\u00b2", "\u0663", "1\u0663", "+", "-", "", "9223372036854775808", "1_000"])
+def test_int_cells_are_ascii_i64(tmp_path, cell):
+    """`s.trim().parse::<i64>()` (input.rs) takes an optional sign and ASCII
+    digits within i64 only: superscripts, other scripts' digits, a bare sign
+    and out-of-range values are load errors that name the line."""
+    (tmp_path / "n.tsv").write_text(f"a\t{cell}\n", encoding="utf-8")
+    with pytest.raises(LoadError, match="n.tsv:1:"):
+        load_input(str(tmp_path / "n.tsv"), "n", 2, BOOL, ["node", "int"], SymbolDict())
+    (tmp_path / "w.tsv").write_text(f"a\tb\t{cell}\n", encoding="utf-8")
+    with pytest.raises(LoadError, match="w.tsv:1:"):
+        load_input(str(tmp_path / "w.tsv"), "w", 2, TROP, ["node", "node"], SymbolDict())
+
+
+def test_json_integers_outside_i64_are_not_integers(tmp_path):
+    """serde_json's as_i64 fails outside i64, so such a number is a float
+    cell (input.rs:316-321): refused in an int column and as a weight."""
+    for big in (2**63, 2**64, -(2**63) - 1):
+        (tmp_path / "j.json").write_text(f'[["a", {big}]]')
+        with pytest.raises(LoadError, match="j.json:1:"):
+            load_input(str(tmp_path / "j.json"), "j", 2, BOOL, ["node", "int"], SymbolDict())
+        (tmp_path / "k.json").write_text(f'[["a", "b", {big}]]')
+        with pytest.raises(LoadError, match="k.json:1:"):
+            load_input(str(tmp_path / "k.json"), "k", 2, TROP, ["node", "node"], SymbolDict())
+    (tmp_path / "e.json").write_text(f'[["a", {2**63 - 1}], ["b", {-(2**63)}]]')
+    facts = load_input(str(tmp_path / "e.json"), "e", 2, BOOL, ["node", "int"], SymbolDict())
+    assert [f[1][1].value for f in facts] == [2**63 - 1, -(2**63)]
+
+
 def test_trop_needs_integer_weight(tmp_path):
     (tmp_path / "w.tsv").write_text("x\ty\tcheap\n")
     with pytest.raises(LoadError, match="w.tsv:1: `w` needs a trailing integer weight"):
