@@ -122,6 +122,9 @@ def lib():
         L.orc_rel_arity.argtypes = [C.c_void_p, C.c_char_p]
         L.orc_rel_copy.restype = C.c_int32
         L.orc_rel_copy.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p]
+        L.orc_cost_dijkstra.restype = C.c_int32
+        L.orc_cost_dijkstra.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
         L.orc_tri_count_oriented.restype = C.c_uint64
         L.orc_tri_count_oriented.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64]
         _lib = L
@@ -313,6 +316,43 @@ def run(prog, edb=(), bulk: Optional[dict] = None, semi: bool = True, indexed: b
         return res
     finally:
         L.orc_db_free(db)
+
+
+def cost_dijkstra(nnodes: int, src, dst, w, sources):
+    """Multi-source cheapest cost by Dijkstra over (weight, hops) labels
+    (orc_cost_dijkstra): returns (W, H, max_hops) with W[s, z] the least
+    weight of a walk of one or more edges from sources[s] to z (INT64_MAX when
+    none), H[s, z] the fewest edges among such walks, and max_hops the
+    stratum's semi-naive round count."""
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.int64)
+    sources = np.ascontiguousarray(sources, dtype=np.uint32)
+    ns = len(sources)
+    W = np.empty((ns, nnodes), dtype=np.int64)
+    H = np.empty((ns, nnodes), dtype=np.uint32)
+    mh = C.c_uint64()
+    rc = lib().orc_cost_dijkstra(nnodes, len(src), src.ctypes.data, dst.ctypes.data, w.ctypes.data, ns,
+                                 sources.ctypes.data, W.ctypes.data, H.ctypes.data, C.byref(mh))
+    if rc:
+        raise ValueError(f"orc_cost_dijkstra: {STATUS.get(rc, rc)}")
+    return W, H, int(mh.value)
+
+
+def cost_rows(W, sources):
+    """The (S, Z, weight) rows of cost in key order from cost_dijkstra's W."""
+    sources = np.asarray(sources, dtype=np.int64)
+    order = np.argsort(sources, kind="stable")
+    ss, zz, ww = [], [], []
+    for i in order:
+        z = np.nonzero(W[i] != np.iinfo(np.int64).max)[0]
+        ss.append(np.full(len(z), sources[i], np.int64))
+        zz.append(z.astype(np.int64))
+        ww.append(W[i, z])
+    if not ss:
+        e = np.zeros(0, np.int64)
+        return e, e, e
+    return np.concatenate(ss), np.concatenate(zz), np.concatenate(ww)
 
 
 def tri_count_oriented(a, b, nnodes: int) -> int:

@@ -1276,3 +1276,166 @@ uint64_t orc_tri_count_oriented(const uint32_t* a, const uint32_t* b, uint64_t n
     free(off);
     return total;
 }
+
+/* ------------------------------------------------------------------------
+ * Multi-source cheapest cost by Dijkstra with a hop tie-break: an algorithm
+ * independent of the semi-naive restatement above, used to check
+ *   cost(S,Y) :- src(S), edge(S,Y).   cost(S,Z) :- cost(S,Y), edge(Y,Z).
+ * (src facts of weight 0) at sizes the restatement cannot finish.
+ *
+ * cost(S,Z) is the least weight of a walk of one or more edges from S to Z,
+ * and hops(S,Z) the fewest edges among the walks of that weight.  Labels are
+ * ordered lexicographically by (weight, hops); with non-negative weights
+ * every edge raises a label, so the first label settled for a node is its
+ * least.  The heap starts from S's out-edges (one hop), not from S itself,
+ * so cost(S,S) exists only through a cycle, as in the rules.
+ *
+ * The semi-naive round count of this stratum follows from the hops
+ * (seminaive.rs:94-104): after the bootstrap the relation holds the best
+ * walks of at most one edge, after delta round k those of at most k+1, and
+ * round k's delta is empty exactly when no label improves any more, i.e. when
+ * k reaches the largest hops(S,Z) over every source and reachable Z (0 when
+ * the bootstrap derives nothing).
+ * ------------------------------------------------------------------------ */
+
+typedef struct {
+    int64_t w;
+    uint32_t h;
+    uint32_t v;
+} dj_item;
+
+static int dj_less(const dj_item* a, const dj_item* b) {
+    return a->w < b->w || (a->w == b->w && a->h < b->h);
+}
+
+typedef struct {
+    dj_item* a;
+    uint64_t cap;
+    int oom;
+} dj_heap;
+
+static void dj_push(dj_heap* hp, uint64_t* n, dj_item x) {
+    if (*n == hp->cap) {
+        uint64_t cap = hp->cap ? 2 * hp->cap : 1024;
+        dj_item* a = (dj_item*)realloc(hp->a, cap * sizeof(dj_item));
+        if (!a) {
+            hp->oom = 1;
+            return;
+        }
+        hp->a = a;
+        hp->cap = cap;
+    }
+    dj_item* heap = hp->a;
+    uint64_t i = (*n)++;
+    while (i > 0) {
+        uint64_t p = (i - 1) / 2;
+        if (!dj_less(&x, &heap[p])) break;
+        heap[i] = heap[p];
+        i = p;
+    }
+    heap[i] = x;
+}
+
+static dj_item dj_pop(dj_item* heap, uint64_t* n) {
+    dj_item top = heap[0];
+    dj_item x = heap[--(*n)];
+    uint64_t i = 0;
+    for (;;) {
+        uint64_t c = 2 * i + 1;
+        if (c >= *n) break;
+        if (c + 1 < *n && dj_less(&heap[c + 1], &heap[c])) c++;
+        if (!dj_less(&heap[c], &x)) break;
+        heap[i] = heap[c];
+        i = c;
+    }
+    if (*n) heap[i] = x;
+    return top;
+}
+
+int32_t orc_cost_dijkstra(uint64_t nnodes, uint64_t nedges, const uint32_t* src, const uint32_t* dst,
+                          const int64_t* w, uint32_t nsources, const uint32_t* sources, int64_t* out_w,
+                          uint32_t* out_h, uint64_t* out_max_hops) {
+    for (uint64_t i = 0; i < nedges; i++)
+        if (src[i] >= nnodes || dst[i] >= nnodes || w[i] < 0) return ORC_ERR_BAD_INPUT;
+    for (uint32_t s = 0; s < nsources; s++)
+        if (sources[s] >= nnodes) return ORC_ERR_BAD_INPUT;
+    uint64_t* off = (uint64_t*)calloc(nnodes + 1, sizeof(uint64_t));
+    uint32_t* adj = (uint32_t*)malloc((nedges ? nedges : 1) * sizeof(uint32_t));
+    int64_t* aw = (int64_t*)malloc((nedges ? nedges : 1) * sizeof(int64_t));
+    uint64_t* fill = (uint64_t*)malloc((nnodes + 1) * sizeof(uint64_t));
+    if (!off || !adj || !aw || !fill) {
+        free(off); free(adj); free(aw); free(fill);
+        return ORC_ERR_BAD_INPUT;
+    }
+    for (uint64_t i = 0; i < nedges; i++) off[src[i] + 1]++;
+    for (uint64_t v = 0; v < nnodes; v++) off[v + 1] += off[v];
+    memcpy(fill, off, (nnodes + 1) * sizeof(uint64_t));
+    for (uint64_t i = 0; i < nedges; i++) {
+        uint64_t p = fill[src[i]]++;
+        adj[p] = dst[i];
+        aw[p] = w[i];
+    }
+    free(fill);
+    uint64_t maxh = 0;
+    int32_t rc = ORC_OK;
+#pragma omp parallel reduction(max : maxh)
+    {
+        dj_heap hp = {NULL, 0, 0};
+        uint8_t* done = (uint8_t*)malloc(nnodes ? nnodes : 1);
+#pragma omp for schedule(dynamic, 1)
+        for (uint32_t s = 0; s < nsources; s++) {
+            int64_t* dw = out_w + (uint64_t)s * nnodes;
+            uint32_t* dh = out_h + (uint64_t)s * nnodes;
+            if (!done || hp.oom) {
+#pragma omp atomic write
+                rc = ORC_ERR_BAD_INPUT;
+                continue;
+            }
+            for (uint64_t v = 0; v < nnodes; v++) {
+                dw[v] = INT64_MAX;
+                dh[v] = UINT32_MAX;
+            }
+            memset(done, 0, nnodes);
+            uint64_t hn = 0;
+            const uint32_t s0 = sources[s];
+            for (uint64_t p = off[s0]; p < off[s0 + 1]; p++) {
+                dj_item it = {aw[p], 1, adj[p]};
+                dj_item cur = {dw[it.v], dh[it.v], it.v};
+                if (dj_less(&it, &cur)) {
+                    dw[it.v] = it.w;
+                    dh[it.v] = 1;
+                    dj_push(&hp, &hn, it);
+                }
+            }
+            while (hn && !hp.oom) {
+                dj_item x = dj_pop(hp.a, &hn);
+                if (done[x.v] || x.w != dw[x.v] || x.h != dh[x.v]) continue;
+                done[x.v] = 1;
+                if (x.h > maxh) maxh = x.h;
+                for (uint64_t p = off[x.v]; p < off[x.v + 1]; p++) {
+                    const uint32_t y = adj[p];
+                    if (done[y]) continue;
+                    if (x.w > INT64_MAX - aw[p]) continue; /* cannot beat a finite label */
+                    dj_item it = {x.w + aw[p], x.h + 1, y};
+                    dj_item cur = {dw[y], dh[y], y};
+                    if (dj_less(&it, &cur)) {
+                        dw[y] = it.w;
+                        dh[y] = it.h;
+                        dj_push(&hp, &hn, it);
+                    }
+                }
+            }
+            if (hp.oom) {
+#pragma omp atomic write
+                rc = ORC_ERR_BAD_INPUT;
+            }
+        }
+        free(hp.a);
+        free(done);
+    }
+    free(off);
+    free(adj);
+    free(aw);
+    *out_max_hops = maxh;
+    return rc;
+}

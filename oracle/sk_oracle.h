@@ -170,6 +170,19 @@ int32_t orc_rel_copy(orc_db* db, const char* pred, orc_val* rows, orc_ann* anns)
  * pairs a[i] -> b[i], ids < nnodes): crates/strata-gpu/src/wcoj.rs:27-43. */
 uint64_t orc_tri_count_oriented(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t nnodes);
 
+/* Multi-source cheapest cost, an independent check of
+ *   cost(S,Y) :- src(S), edge(S,Y).  cost(S,Z) :- cost(S,Y), edge(Y,Z).
+ * by Dijkstra over (weight, hops) labels (weights must be non-negative).
+ * Edges are (src[i], dst[i], w[i]), ids < nnodes; duplicates allowed (the
+ * least weight wins).  For source s, out_w[s*nnodes + z] is the least weight
+ * of a walk of >= 1 edges to z (INT64_MAX when none) and out_h the fewest
+ * edges among such walks (UINT32_MAX when none).  *out_max_hops is the
+ * largest hop count over every source: the stratum's semi-naive round count
+ * (seminaive.rs:94-104).  OpenMP over sources. */
+int32_t orc_cost_dijkstra(uint64_t nnodes, uint64_t nedges, const uint32_t* src, const uint32_t* dst,
+                          const int64_t* w, uint32_t nsources, const uint32_t* sources, int64_t* out_w,
+                          uint32_t* out_h, uint64_t* out_max_hops);
+
 #ifdef __cplusplus
 }
 #endif

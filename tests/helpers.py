@@ -263,3 +263,103 @@ def oracle_rows_as_engine(orel):
         tt = tuple(Sym(x) if k == "sym" else IntV(x) for k, x in t)
         out.append((tt, a))
     return out
+
+
+# ------------------------------------------------------- error programs
+
+class ErrorCase:
+    """A program, its EDB and the outcome the reference gives it: a status
+    (the oracle's names), the error's slot or predicate where it has one, and
+    for "ok" the expected rows of some relations (Int values as ints, Sym
+    values as ("s", id))."""
+
+    def __init__(self, label, prog, edb, status, slot=None, pred=None, want=None):
+        self.label, self.prog, self.edb, self.status = label, prog, edb, status
+        self.slot, self.pred, self.want = slot, pred, want
+
+
+def _error_cases():
+    P = lambda *specs: [pred(*s) for s in specs]  # noqa: E731
+    q_a = [("q", (Sym(3),), None)]
+    cases = []
+    # EDB fact naming no declared predicate (seminaive.rs:30-34)
+    prog = CoreProgram(P(("q", 1, BOOL)), [], 1)
+    cases += [ErrorCase("unknown_fact", prog, [("nope", (Sym(0),), None)], "unknown_pred", pred="nope"),
+              ErrorCase("known_fact", prog, [("q", (Sym(0),), None)], "ok")]
+    # p(X) :- q(X), [not] nope(X): the relation lookup (seminaive.rs:254-257)
+    # and the negated literal's stratum lookup (naive.rs:545-547) happen when
+    # solve reaches the literal; with q empty it never does
+    for negated in (False, True):
+        lit = neg("nope", v(0)) if negated else pos("nope", v(0))
+        prog = CoreProgram(P(("q", 1, BOOL, 0), ("p", 1, BOOL, 1)),
+                           [rule(atom("p", v(0)), [pos("q", v(0)), lit], 1, 1)], 2)
+        tag = "neg" if negated else "pos"
+        cases += [ErrorCase(f"unknown_body_{tag}", prog, q_a, "unknown_pred", pred="nope"),
+                  ErrorCase(f"unknown_body_{tag}_unreached", prog, [], "ok", want={"p": []})]
+    # the head relation is looked up before any body is solved
+    # (seminaive.rs:128-131, naive.rs:193-195), whatever the data
+    prog = CoreProgram(P(("q", 1, BOOL)), [rule(atom("nope", v(0)), [pos("q", v(0))], 1)], 1)
+    cases.append(ErrorCase("unknown_head", prog, [], "unknown_pred", pred="nope"))
+    # p(X, Y) :- q(X): slot 1 never bound; build_head fails on the first
+    # solution (naive.rs:485-490)
+    prog = CoreProgram(P(("q", 1, BOOL), ("p", 2, BOOL)), [rule(atom("p", v(0), v(1)), [pos("q", v(0))], 2)], 1)
+    cases += [ErrorCase("unsafe_head", prog, q_a, "unsafe_head_var", slot=1),
+              ErrorCase("unsafe_head_unreached", prog, [], "ok", want={"p": []})]
+    # aggregand slot unbound (naive.rs:360); group-key slot unbound
+    # (head_arg_value, naive.rs:513-522)
+    for tag, head in (("aggregand", atom("p", v(0), Agg("sum", 1))), ("group_key", atom("p", v(1), Agg("sum", 0)))):
+        prog = CoreProgram(P(("q", 1, BOOL, 0), ("p", 2, BOOL, 1)), [rule(head, [pos("q", v(0))], 2, 1)], 2)
+        cases += [ErrorCase(f"unsafe_{tag}", prog, [("q", (IntV(4),), None)], "unsafe_head_var", slot=1),
+                  ErrorCase(f"unsafe_{tag}_unreached", prog, [], "ok", want={"p": []})]
+    # p(X) :- q(X), not r(X, Y): Y unbound when the negation is reached
+    # (ground_atom with negated = true, naive.rs:485-488)
+    prog = CoreProgram(P(("q", 1, BOOL, 0), ("r", 2, BOOL, 0), ("p", 1, BOOL, 1)),
+                       [rule(atom("p", v(0)), [pos("q", v(0)), neg("r", v(0), v(1))], 2, 1)], 2)
+    cases += [ErrorCase("unsafe_neg", prog, [("q", (Sym(0),), None)], "unsafe_neg_var", slot=1),
+              ErrorCase("unsafe_neg_unreached", prog, [("r", (Sym(0), Sym(0)), None)], "ok", want={"p": []})]
+    # p(X) :- q(X), not p(X): the stratum check sits in negation_holds
+    # (naive.rs:548-553), reached only through a solution of q
+    prog = CoreProgram(P(("q", 1, BOOL), ("p", 1, BOOL)), [rule(atom("p", v(0)), [pos("q", v(0)), neg("p", v(0))], 1)],
+                       1)
+    cases += [ErrorCase("neg_same_stratum", prog, q_a, "negation_not_stratified", pred="p"),
+              ErrorCase("neg_same_stratum_unreached", prog, [], "ok", want={"p": []})]
+    # c(count<X>) :- d(X) with d in the rule's own stratum: the body strata
+    # are checked before any solving (naive.rs:307-318), whatever the data
+    prog = CoreProgram(P(("d", 1, BOOL, 0), ("c", 1, BOOL, 0)), [rule(atom("c", Agg("count", 0)), [pos("d", v(0))], 1)],
+                       1)
+    cases.append(ErrorCase("aggregate_in_recursion", prog, [], "aggregate_in_recursion", pred="d"))
+    prog = CoreProgram(P(("d", 1, BOOL, 0), ("c", 1, BOOL, 1)),
+                       [rule(atom("c", Agg("count", 0)), [pos("d", v(0))], 1, 1)], 2)
+    cases.append(ErrorCase("aggregate_lower_stratum", prog, [("d", (Sym(0),), None), ("d", (Sym(1),), None)], "ok",
+                           want={"c": [(2,)]}))
+    # two aggregate terms in one head (naive.rs:320-330), checked after the
+    # body strata: a recursive malformed rule reports AggregateInRecursion
+    prog = CoreProgram(P(("q", 2, BOOL, 0), ("m", 2, BOOL, 1)),
+                       [rule(atom("m", Agg("count", 0), Agg("sum", 1)), [pos("q", v(0), v(1))], 2, 1)], 2)
+    cases.append(ErrorCase("malformed_head", prog, [], "malformed_aggregate_head"))
+    prog = CoreProgram(P(("q", 2, BOOL, 0), ("m", 2, BOOL, 0)),
+                       [rule(atom("m", Agg("count", 0), Agg("sum", 1)), [pos("q", v(0), v(1))], 2, 0)], 1)
+    cases.append(ErrorCase("malformed_head_recursive", prog, [], "aggregate_in_recursion", pred="q"))
+    # sum/min/max over a Sym value fail (naive.rs:384-403); count does not
+    # look at the values; no group, no fold, no error
+    mixed = [("q", (Sym(0), IntV(5)), None), ("q", (Sym(0), Sym(7)), None)]
+    ints = [("q", (Sym(0), IntV(5)), None), ("q", (Sym(1), IntV(-2)), None)]
+    for op in ("sum", "min", "max", "count"):
+        prog = CoreProgram(P(("q", 2, BOOL, 0), ("s", 2, BOOL, 1)),
+                           [rule(atom("s", v(0), Agg(op, 1)), [pos("q", v(0), v(1))], 2, 1)], 2)
+        if op == "count":
+            cases.append(ErrorCase("count_over_sym", prog, mixed, "ok", want={"s": [(("s", 0), 2)]}))
+            continue
+        cases += [ErrorCase(f"{op}_over_sym", prog, mixed, "non_integer_aggregand"),
+                  ErrorCase(f"{op}_no_group", prog, [], "ok", want={"s": []}),
+                  ErrorCase(f"{op}_over_ints", prog, ints, "ok", want={"s": [(("s", 0), 5), (("s", 1), -2)]})]
+    # prob-or is refused inside fold_aggregate (naive.rs:401): only once a
+    # group exists
+    prog = CoreProgram(P(("q", 2, BOOL, 0), ("s", 2, BOOL, 1)),
+                       [rule(atom("s", v(0), Agg("prob_or", 1)), [pos("q", v(0), v(1))], 2, 1)], 2)
+    cases += [ErrorCase("prob_or", prog, [("q", (Sym(0), IntV(1)), None)], "unsupported"),
+              ErrorCase("prob_or_no_group", prog, [], "ok", want={"s": []})]
+    return cases
+
+
+ERROR_CASES = _error_cases()

@@ -373,3 +373,31 @@ def test_intern_terms_sort_cases(ctx):
             got = sk.intern_terms(keys, ctx)
             _, inv = np.unique(keys, return_inverse=True)
             assert got == inv.tolist(), n
+
+
+# ----------------------------------------------- pinned error programs
+
+from helpers import ERROR_CASES  # noqa: E402
+
+
+@pytest.mark.parametrize("case", ERROR_CASES, ids=[c.label for c in ERROR_CASES])
+def test_error_cases(ctx, case):
+    """The hand-written programs whose outcome is read off the reference's
+    raise points (pinned in tests/test_oracle_pins.py): the engine gives the
+    same status, slot or predicate, or the same rows, on both paths."""
+    for dense in (True, False):
+        try:
+            db = ctx.run_semi_naive(case.prog, case.edb, dense=dense)
+        except sk.EngineError as e:
+            assert case.status != "ok", f"engine {e.kind}: {e.info}"
+            assert e.kind == ERR[case.status], (e.kind, case.status)
+            if case.slot is not None:
+                assert e.info["slot"] == case.slot
+            if case.pred is not None:
+                assert e.info["pred"] == case.pred
+            continue
+        assert case.status == "ok", f"engine ok, reference {case.status}"
+        for name, rows in (case.want or {}).items():
+            got = [tuple(g.value if isinstance(g, IntV) else ("s", g.id) for g in t)
+                   for t, _ in db.relation(name).rows()]
+            assert got == rows, name
