@@ -296,9 +296,17 @@ sk_status sk_shortest_paths_trop(sk_ctx* ctx, const uint32_t* src, const uint32_
  * self-loops allowed in the input; each triangle counted once) (lib.rs:94-101). */
 sk_status sk_count_triangles(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n,
                              uint64_t* out);
-/* 4-cliques of the undirected graph (lib.rs:109-114). */
+/* 4-cliques of the undirected graph (lib.rs:109-114, count_4cliques at
+ * crates/strata-gpu/src/wcoj.rs:162-184; duplicates, both directions and
+ * self-loops allowed in the input; each 4-clique counted once).  The edge
+ * list is simplified, oriented by (degree, id) and indexed on the device. */
 sk_status sk_count_4cliques(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n,
                             uint64_t* out);
+/* Testing knob of the 4-clique counter (no reference counterpart): send
+ * every edge to the wave-per-edge path, and cap that path's LDS table of
+ * common out-neighbours at lds_cap entries (1 .. 1024; 0 = 1024) so larger
+ * sets take the global-memory membership test. */
+sk_status sk_ctx_set_clique_test(sk_ctx* ctx, int force_heavy, uint32_t lds_cap);
 /* Each key's index in the sorted distinct key table (lib.rs:122-130). */
 sk_status sk_intern_terms(sk_ctx* ctx, const uint64_t* keys, uint64_t n, uint32_t* out_ids);
 /* Σ_v d(v)^2 over the id-oriented simple graph (lib.rs:138-143). */

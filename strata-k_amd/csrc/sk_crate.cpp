@@ -18,7 +18,7 @@
 using namespace sk;
 
 namespace sk {
-u64 k4_count(const Exec& x, const u64* keys, u64 ne, const u64* csr, const u32* adj, int bits);
+u64 clique4_count(const Exec& x, const u32* a, const u32* b, u64 n, bool force_heavy, u32 tcap);
 bool asp_simplify(const Exec& x, const int32_t* head, const u32* bs, const int32_t* lits, u64 n_rules, u64 n_lits,
                   const u8* poss, const u8* cert, u64 n_atoms, int32_t* out_head, u32* out_bs, int32_t* out_lits,
                   u64* out_rules, u64* out_lits_n);
@@ -130,21 +130,6 @@ sk_status sk_shortest_paths_trop(sk_ctx* ctx, const uint32_t* src, const uint32_
   return s;
 }
 
-// Orient u<v, drop self-loops, dedup (wcoj.rs:99-131).
-static std::vector<uint64_t> orient_by_id(const uint32_t* a, const uint32_t* b, uint64_t n) {
-  std::vector<uint64_t> k;
-  k.reserve(n);
-  for (uint64_t i = 0; i < n; i++) {
-    uint32_t x = a[i], y = b[i];
-    if (x == y) continue;
-    if (y < x) std::swap(x, y);
-    k.push_back(((uint64_t)x << 32) | y);
-  }
-  std::sort(k.begin(), k.end());
-  k.erase(std::unique(k.begin(), k.end()), k.end());
-  return k;
-}
-
 sk_status sk_count_triangles(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* out) {
   if (!ctx || !out || (n && (!a || !b))) return SK_ERR_INVALID_ARGUMENT;
   // tri(X,Y,Z) :- e(X,Y), e(Y,Z), e(X,Z).  ntri(count<X>) :- tri(X,_Y,_Z).
@@ -178,35 +163,32 @@ sk_status sk_count_triangles(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, 
 
 sk_status sk_count_4cliques(sk_ctx* ctx, const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* out) {
   if (!ctx || !out || (n && (!a || !b))) return SK_ERR_INVALID_ARGUMENT;
-  std::vector<uint64_t> k = orient_by_id(a, b, n);
   *out = 0;
-  if (k.empty()) return SK_OK;
-  EvalErr e;
+  if (n == 0) return SK_OK;
   try {
     Ctx& c = ctx->c;
     (void)hipSetDevice(c.device);
     Exec x = c.exec();
-    uint64_t ne = k.size();
-    uint32_t maxv = 0;
-    for (auto kk : k) maxv = std::max(maxv, (uint32_t)kk);
-    uint64_t nv = (uint64_t)maxv + 1;
-    int bits = 1;
-    while ((1ull << bits) < nv) bits++;
-    std::vector<uint64_t> packed(ne);
-    for (uint64_t i = 0; i < ne; i++) packed[i] = ((k[i] >> 32) << bits) | (k[i] & 0xffffffffull);
-    DBuf<u64> dk(c.pool.get(), ne);
-    DBuf<u64> csr(c.pool.get(), nv + 1);
-    DBuf<u32> adj(c.pool.get(), ne), first(c.pool.get(), ne);
-    SK_HIP(hipMemcpyAsync(dk.get(), packed.data(), ne * 8, hipMemcpyHostToDevice, c.stream));
-    build_csr<u64>(x, dk.get(), ne, bits, nv, csr.get());
-    u32* cp[2] = {first.get(), adj.get()};
-    unpack_cols<u64>(x, dk.get(), ne, 2, bits, cp);
-    *out = k4_count(x, dk.get(), ne, csr.get(), adj.get(), bits);
-  } catch (const DeviceError& d) {
+    // the raw pairs go to the device as they are; simplification,
+    // orientation and the CSR are built there
+    DBuf<u32> da(c.pool.get(), n), db(c.pool.get(), n);
+    c.h2d(da.get(), a, n * 4);
+    c.h2d(db.get(), b, n * 4);
+    *out = clique4_count(x, da.get(), db.get(), n, c.k4_force_heavy, c.k4_tcap);
+  } catch (const DeviceError&) {
     return SK_ERR_DEVICE;
   } catch (const BudgetError&) {
     return SK_ERR_OUT_OF_BUDGET;
+  } catch (const std::bad_alloc&) {
+    return SK_ERR_OUT_OF_BUDGET;
   }
+  return SK_OK;
+}
+
+sk_status sk_ctx_set_clique_test(sk_ctx* ctx, int force_heavy, uint32_t lds_cap) {
+  if (!ctx || lds_cap > 1024) return SK_ERR_INVALID_ARGUMENT;
+  ctx->c.k4_force_heavy = force_heavy != 0;
+  ctx->c.k4_tcap = lds_cap ? lds_cap : 1024;
   return SK_OK;
 }
 
@@ -462,7 +444,14 @@ sk_status sk_asp_simplify_filter_device(sk_ctx* ctx, const int32_t* head, const 
 }
 
 uint64_t sk_binary_plan_intermediate(const uint32_t* a, const uint32_t* b, uint64_t n) {
-  std::vector<uint64_t> k = orient_by_id(a, b, n);
+  // the id-oriented simple graph: (min, max) pairs, self-loops dropped,
+  // duplicates removed; then the sum over sources of out-degree squared
+  std::vector<uint64_t> k;
+  k.reserve(n);
+  for (uint64_t i = 0; i < n; i++)
+    if (a[i] != b[i]) k.push_back(((uint64_t)std::min(a[i], b[i]) << 32) | std::max(a[i], b[i]));
+  std::sort(k.begin(), k.end());
+  k.erase(std::unique(k.begin(), k.end()), k.end());
   uint64_t s = 0, run = 0;
   for (size_t i = 0; i < k.size(); i++) {
     if (i > 0 && (k[i] >> 32) != (k[i - 1] >> 32)) {

@@ -306,6 +306,10 @@ struct Ctx {
   hipEvent_t read_ev = nullptr;
   EvalErr last;
   RadixTune tune;
+  // 4-clique counter test knobs: every edge to the wave path, and the wave
+  // path's LDS table size (1 .. 1024).
+  bool k4_force_heavy = false;
+  u32 k4_tcap = 1024;
   // Two pinned staging buffers for large device-to-host result copies,
   // allocated on first use (d2h).
   static constexpr u64 kStageBytes = 64ull << 20;

@@ -1664,51 +1664,6 @@ template void remap_keys<u64, u128>(const Exec&, const u64*, u64, int, int, int,
 template void remap_keys<u128, u64>(const Exec&, const u128*, u64, int, int, int, const u32*, u64*);
 template void remap_keys<u128, u128>(const Exec&, const u128*, u64, int, int, int, const u32*, u128*);
 
-// ============================================================ 4-cliques
-//
-// For each id-oriented edge (u, v): every w1 in adj(u) ∩ adj(v) closes a
-// triangle; the 4-cliques through it are the w2 in adj(u) ∩ adj(v) ∩ adj(w1)
-// beyond w1 (crates/strata-gpu/src/wcoj.rs:49-77 counts the same set).
-__global__ void k_k4(const u64* __restrict__ keys, u64 ne, const u64* __restrict__ csr, const u32* __restrict__ adj,
-                     int bits, unsigned long long* __restrict__ total) {
-  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  u64 c = 0;
-  if (e < ne) {
-    u64 k = keys[e];
-    u32 u = (u32)(k >> bits), v = (u32)(k & ((1ull << bits) - 1));
-    u64 a = csr[u], ae = csr[u + 1], b = csr[v], be = csr[v + 1];
-    while (a < ae && b < be) {
-      u32 x = adj[a], y = adj[b];
-      if (x < y) { a++; continue; }
-      if (x > y) { b++; continue; }
-      u64 p = a + 1, q = b + 1, r = csr[x], re = csr[x + 1];
-      while (p < ae && q < be && r < re) {
-        u32 xp = adj[p], xq = adj[q], xr = adj[r];
-        u32 mx = max(xp, max(xq, xr));
-        if (xp < mx) p++;
-        else if (xq < mx) q++;
-        else if (xr < mx) r++;
-        else { c++; p++; q++; r++; }
-      }
-      a++;
-      b++;
-    }
-  }
-  c = wave_incl_scan_j(c);
-  if (lane_id_j() == 63 && c) atomicAdd(total, (unsigned long long)c);
-}
-
-u64 k4_count(const Exec& x, const u64* keys, u64 ne, const u64* csr, const u32* adj, int bits) {
-  DBuf<u64> acc(x.pool, 1);
-  memset_async(x, acc.get(), 0, 8);
-  if (ne) {
-    Scope sc(x.timer, "k4_count", KC_LEAPFROG, (double)ne * 8.0, ne);
-    k_k4<<<gridj(ne), kBlock, 0, x.stream>>>(keys, ne, csr, adj, bits, (unsigned long long*)acc.get());
-    SK_HIP(hipGetLastError());
-  }
-  return read_u64(x, acc.get());
-}
-
 // Interned id = position of each key in the sorted distinct table.
 __global__ void k_intern(const u64* __restrict__ t, u64 nt, const u64* __restrict__ q, u64 nq, u32* __restrict__ ids) {
   u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;

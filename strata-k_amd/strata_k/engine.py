@@ -170,6 +170,7 @@ SIGNATURES = [
                                          C.POINTER(_VP), C.POINTER(_U64)]),
     ("sk_count_triangles", C.c_int, [_VP, _VP, _VP, _U64, C.POINTER(_U64)]),
     ("sk_count_4cliques", C.c_int, [_VP, _VP, _VP, _U64, C.POINTER(_U64)]),
+    ("sk_ctx_set_clique_test", C.c_int, [_VP, C.c_int, _U32]),
     ("sk_intern_terms", C.c_int, [_VP, _VP, _U64, _VP]),
     ("sk_binary_plan_intermediate", _U64, [_VP, _VP, _U64]),
     ("sk_batch_create", C.c_int, [_VP, C.POINTER(_VP)]),
@@ -404,6 +405,13 @@ class Context:
         """0: fixed 8-bit digits; 1 (default): time a size class on its first
         sort; 2: time every class now."""
         code = self.L.sk_ctx_set_autotune(self.h, int(mode))
+        if code:
+            _raise(code)
+
+    def set_clique_test(self, force_heavy: bool = False, lds_cap: int = 0):
+        """4-clique counter testing knob: every edge to the wave path, and
+        that path's LDS table capped at lds_cap entries (0: 1024)."""
+        code = self.L.sk_ctx_set_clique_test(self.h, 1 if force_heavy else 0, int(lds_cap))
         if code:
             _raise(code)
 

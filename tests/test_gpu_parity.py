@@ -364,6 +364,59 @@ def test_crate_functions(ctx):
         assert sk.count_4cliques(edges, ctx) == brute_4cliques(edges, n), trial
 
 
+def _k4_sets(edges):
+    """Independent 4-clique count with Python sets: for each id-ordered edge
+    (a, b), the common neighbours above b, and the edges among them."""
+    nb = {}
+    for x, y in edges:
+        if x != y:
+            nb.setdefault(x, set()).add(y)
+            nb.setdefault(y, set()).add(x)
+    total = 0
+    for a in nb:
+        for b in nb[a]:
+            if b <= a:
+                continue
+            t = {c for c in nb[a] & nb[b] if c > b}
+            total += sum(1 for c in t for d in nb[c] & t if d > c)
+    return total
+
+
+@pytest.mark.parametrize("mode", [(False, 0), (True, 0), (True, 3)], ids=["auto", "all_waves", "waves_lds_cap_3"])
+def test_4cliques_every_path(ctx, mode):
+    """count_4cliques (wcoj.rs:162-184) on the reference's fixtures and
+    brute-force graphs (wcoj_diff.rs:109-171), complete graphs, hub graphs
+    (closed form C(h,4) + b*C(h,3)) and a 6,000-edge random graph against a
+    set-based count, with the lane path, the wave path, and the wave path
+    with a 3-entry LDS table (common out-neighbour sets above 3 take the
+    global-memory membership test)."""
+    ctx.set_clique_test(*mode)
+    try:
+        for case in G["wcoj_diff"]["fourcliques_fixed_and_complete"]["cases"]:
+            assert sk.count_4cliques(case["edges"], ctx) == case["k4"]
+        for n in (4, 5, 10, 20, 40, 70):
+            assert sk.count_4cliques([(a, b) for a in range(n) for b in range(a + 1, n)], ctx) == math.comb(n, 4), n
+        rng = random.Random(0x4c1)
+        for trial in range(60):
+            n = rng.randint(6, 23)
+            edges = [(rng.randrange(n), rng.randrange(n)) for _ in range(rng.randint(0, 4 * n))]
+            assert sk.count_4cliques(edges, ctx) == brute_4cliques(edges, n), trial
+        for h, b in ((5, 30), (9, 200)):
+            edges = [(i, j) for i in range(h) for j in range(i + 1, h)]
+            edges += [(hub, h + k) for k in range(b) for hub in range(h)]
+            edges += [(h + k, hub) for k in range(0, b, 3) for hub in range(h)]  # both directions
+            edges += [(h + k, h + k) for k in range(0, b, 7)]  # self-loops
+            assert sk.count_4cliques(edges, ctx) == math.comb(h, 4) + b * math.comb(h, 3), (h, b)
+        r = np.random.default_rng(5)
+        edges = list(zip(r.integers(0, 300, 6000).tolist(), r.integers(0, 300, 6000).tolist()))
+        edges += [(i, j) for i in range(280, 300) for j in range(i + 1, 300)]  # a planted K20
+        assert sk.count_4cliques(edges, ctx) == _k4_sets(edges)
+        assert sk.count_4cliques([], ctx) == 0
+        assert sk.count_4cliques([(3, 3), (4, 4)], ctx) == 0
+    finally:
+        ctx.set_clique_test(False, 0)
+
+
 def test_intern_terms_sort_cases(ctx):
     # crates/strata-gpu/src/sort.rs:623-664 sizes, full 64-bit keys and heavy duplicates
     rng = np.random.default_rng(0xa11ce)
