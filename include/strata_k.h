@@ -343,6 +343,22 @@ sk_status sk_asp_simplify_filter_device(sk_ctx* ctx, const int32_t* head, const 
                                         int32_t* out_head, uint32_t* out_body_start, int32_t* out_body_lit,
                                         uint64_t* out_n_rules, uint64_t* out_n_lits);
 
+/* The same filter with no device allocation (no counterpart in the
+ * reference: for callers that cannot allocate device memory, such as code
+ * under HIP graph capture or a serving loop that must not call hipMalloc).
+ * Its scratch (block totals, counts, the atom state table) is a static
+ * device array of about 6 MiB per device, so at most 2^27 rules and 2^22
+ * atoms; larger inputs are SK_ERR_INVALID_ARGUMENT with a message in
+ * sk_ctx_error.  Every call on one device shares that array: calls that
+ * overlap on one device (two contexts or host threads) race.  Arguments,
+ * outputs and results are those of sk_asp_simplify_filter_device. */
+sk_status sk_asp_simplify_filter_device_noalloc(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
+                                                const int32_t* body_lit, uint64_t n_rules, uint64_t n_lits,
+                                                const uint8_t* poss, const uint8_t* cert, uint64_t n_atoms,
+                                                int32_t* out_head, uint32_t* out_body_start,
+                                                int32_t* out_body_lit, uint64_t* out_n_rules,
+                                                uint64_t* out_n_lits);
+
 /* ------------------------------------------------------- sort primitives */
 
 /* The reference crate's device sort helpers over host slices:

@@ -26,6 +26,9 @@
 // certain).  A thread walks its body in order and stops where the rule dies,
 // as in the reference.  (Tried and slower: staging the state table in LDS
 // per block, and loading literals four at a time at clamped addresses.)
+#include <map>
+#include <mutex>
+
 #include "sk_dev.h"
 
 namespace sk {
@@ -289,21 +292,91 @@ __global__ __launch_bounds__(kAspT) void k_asp_emit(const int32_t* __restrict__ 
     for (u64 t = threadIdx.x; t < n_kept; t += kAspT) out_lits[lbase + t] = s_lit[t];
 }
 
+// Clears the two counts and the flag; with no rules, also writes the
+// output's single offset (0).
+__global__ void k_asp_clear(u64* __restrict__ tot, u32* __restrict__ out_bs0) {
+  if (threadIdx.x < 3) tot[threadIdx.x] = 0;
+  if (threadIdx.x == 0 && out_bs0) *out_bs0 = 0;
+}
+
+// ------------------------------------------------ allocation-free variant
+//
+// The filter's scratch for callers that cannot allocate device memory: a
+// file-scope device array holding the two counts and the malformed flag,
+// the per-block totals of up to kAspMaxRules rules, and the atom state table
+// of up to kAspMaxAtoms atoms.  Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): each device's
+// address is resolved once with hipGetSymbolAddress and kept per device.
+// Every call on one device shares that device's copy; calls that overlap on
+// one device (two streams, two host threads) race on it.
+
+struct AspScratch {
+  u64* ba;
+  u64* bk;
+  u64* tot;
+  u8* st;
+};
+
+constexpr u64 kAspMaxBlocks = kAspMaxRules / kAspT;
+constexpr u64 kAspScratchBytes = 256 + 2 * kAspMaxBlocks * sizeof(u64) + kAspMaxAtoms;
+static_assert(kAspMaxRules % kAspT == 0, "whole blocks");
+
+__device__ __align__(256) unsigned char scratch[kAspScratchBytes];
+
+static AspScratch asp_scratch_parts(u8* base) {
+  u64* ba = reinterpret_cast<u64*>(base + 256);
+  return AspScratch{ba, ba + kAspMaxBlocks, reinterpret_cast<u64*>(base), base + 256 + 2 * kAspMaxBlocks * 8};
+}
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, u8*> g_scratch_ptr;  // device ordinal -> that device's copy of `scratch`
+}  // namespace
+
+u8* asp_scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) return it->second;
+  int cur = -1;
+  SK_HIP(hipGetDevice(&cur));
+  if (cur != device) throw DeviceError{hipErrorInvalidDevice, "asp_scratch_address: device not current"};
+  void* p = nullptr;
+  SK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<u8*>(p);
+  return static_cast<u8*>(p);
+}
+
 // Device-resident inputs and outputs (caller-sized: n_rules heads, n_rules + 1
 // offsets, n_lits literals).  Returns false when a literal or an offset is
 // malformed (outputs then undefined).
 bool asp_simplify(const Exec& x, const int32_t* head, const u32* bs, const int32_t* lits, u64 n_rules, u64 n_lits,
                   const u8* poss, const u8* cert, u64 n_atoms, int32_t* out_head, u32* out_bs, int32_t* out_lits,
-                  u64* out_rules, u64* out_lits_n) {
+                  u64* out_rules, u64* out_lits_n, u8* scratch_ptr) {
   const u64 nblk = (n_rules + kAspT - 1) / kAspT;
   if (nblk > 0x7fffffffull) throw BudgetError{n_rules};
-  DBuf<u64> ba(x.pool, nblk ? nblk : 1), bk(x.pool, nblk ? nblk : 1), tot(x.pool, 3);
-  DBuf<u8> st(x.pool, n_atoms ? n_atoms : 1);
-  memset_async(x, tot.get(), 0, 24);
+  // block totals, the two counts and the malformed flag, the atom states:
+  // from the pool, or carved from the caller-resolved static scratch
+  DBuf<u64> ba_b, bk_b, tot_b;
+  DBuf<u8> st_b;
+  AspScratch sc_ = scratch_ptr ? asp_scratch_parts(scratch_ptr) : AspScratch{};
+  if (!scratch_ptr) {
+    ba_b.reset(x.pool, nblk ? nblk : 1);
+    bk_b.reset(x.pool, nblk ? nblk : 1);
+    tot_b.reset(x.pool, 3);
+    st_b.reset(x.pool, n_atoms ? n_atoms : 1);
+    sc_ = AspScratch{ba_b.get(), bk_b.get(), tot_b.get(), st_b.get()};
+  }
+  u64* const ba = sc_.ba;
+  u64* const bk = sc_.bk;
+  u64* const tot = sc_.tot;
+  u8* const st = sc_.st;
+  // a fill kernel, not a memset, so the sequence can be captured in a graph
+  k_asp_clear<<<1, 64, 0, x.stream>>>(tot, n_rules ? nullptr : out_bs);
+  SK_HIP(hipGetLastError());
   if (n_atoms) {
     Scope sc(x.timer, "asp_state", KC_MISC, (double)n_atoms * 3.0, n_atoms);
-    k_asp_state<<<(unsigned)((n_atoms + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(poss, cert, n_atoms,
-                                                                                    st.get(), tot.get() + 2);
+    k_asp_state<<<(unsigned)((n_atoms + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(poss, cert, n_atoms, st,
+                                                                                    tot + 2);
     SK_HIP(hipGetLastError());
   }
   if (n_rules) {
@@ -312,29 +385,26 @@ bool asp_simplify(const Exec& x, const int32_t* head, const u32* bs, const int32
       // stay in cache), per block two totals written
       Scope sc(x.timer, "asp_count", KC_JOIN, (double)n_rules * 4.0 + (double)n_lits * 4.0 + (double)nblk * 16.0,
                n_rules);
-      k_asp_count<<<(unsigned)nblk, kAspT, 0, x.stream>>>(bs, lits, n_rules, n_lits, st.get(), n_atoms, ba.get(),
-                                                          bk.get(), tot.get() + 2);
+      k_asp_count<<<(unsigned)nblk, kAspT, 0, x.stream>>>(bs, lits, n_rules, n_lits, st, n_atoms, ba,
+                                                          bk, tot + 2);
       SK_HIP(hipGetLastError());
     }
     {
       Scope sc(x.timer, "asp_scan", KC_MISC, (double)nblk * 32.0, nblk);
-      k_asp_scan<<<1, 1024, 0, x.stream>>>(ba.get(), bk.get(), nblk, tot.get(), out_bs);
+      k_asp_scan<<<1, 1024, 0, x.stream>>>(ba, bk, nblk, tot, out_bs);
       SK_HIP(hipGetLastError());
     }
     {
       // bytes credited once the totals are known
       Scope sc(x.timer, "asp_emit", KC_JOIN, 0.0, n_rules);
-      k_asp_emit<<<(unsigned)nblk, kAspT, 0, x.stream>>>(head, bs, lits, n_rules, n_lits, st.get(), n_atoms,
-                                                         ba.get(), bk.get(), tot.get() + 2, out_head, out_bs,
+      k_asp_emit<<<(unsigned)nblk, kAspT, 0, x.stream>>>(head, bs, lits, n_rules, n_lits, st, n_atoms,
+                                                         ba, bk, tot + 2, out_head, out_bs,
                                                          out_lits);
       SK_HIP(hipGetLastError());
     }
-  } else {
-    const u32 zero = 0;
-    SK_HIP(hipMemcpyAsync(out_bs, &zero, 4, hipMemcpyHostToDevice, x.stream));
   }
   u64 hv[3] = {0, 0, 0};
-  read_u64s(x, tot.get(), hv, 3);
+  read_u64s(x, tot, hv, 3);
   if (hv[2]) return false;
   // emit: per rule its end offset, per literal the literal, per kept rule
   // its head read and head + offset written, per kept literal one write

@@ -21,7 +21,7 @@ namespace sk {
 u64 clique4_count(const Exec& x, const u32* a, const u32* b, u64 n, bool force_heavy, u32 tcap);
 bool asp_simplify(const Exec& x, const int32_t* head, const u32* bs, const int32_t* lits, u64 n_rules, u64 n_lits,
                   const u8* poss, const u8* cert, u64 n_atoms, int32_t* out_head, u32* out_bs, int32_t* out_lits,
-                  u64* out_rules, u64* out_lits_n);
+                  u64* out_rules, u64* out_lits_n, u8* scratch = nullptr);
 void intern_lookup(const Exec& x, const u64* table, u64 nt, const u64* q, u64 nq, u32* ids);
 }  // namespace sk
 
@@ -440,6 +440,42 @@ sk_status sk_asp_simplify_filter_device(sk_ctx* ctx, const int32_t* head, const 
     *out_n_lits = nl;
   });
   if (s) return s;
+  return ok ? SK_OK : SK_ERR_INVALID_ARGUMENT;
+}
+
+sk_status sk_asp_simplify_filter_device_noalloc(sk_ctx* ctx, const int32_t* head, const uint32_t* body_start,
+                                                const int32_t* body_lit, uint64_t n_rules, uint64_t n_lits,
+                                                const uint8_t* poss, const uint8_t* cert, uint64_t n_atoms,
+                                                int32_t* out_head, uint32_t* out_body_start, int32_t* out_body_lit,
+                                                uint64_t* out_n_rules, uint64_t* out_n_lits) {
+  if (!ctx || !body_start || !out_body_start || !out_n_rules || !out_n_lits || (n_rules && (!head || !out_head)) ||
+      (n_atoms && (!poss || !cert)) || (n_lits && (!body_lit || !out_body_lit)))
+    return SK_ERR_INVALID_ARGUMENT;
+  Ctx& c = ctx->c;
+  if (n_rules > kAspMaxRules || n_atoms > kAspMaxAtoms) {
+    c.last = EvalErr{};
+    c.last.code = SK_ERR_INVALID_ARGUMENT;
+    c.last.msg = "allocation-free ASP filter: at most 2^27 rules and 2^22 atoms (its static device scratch)";
+    return SK_ERR_INVALID_ARGUMENT;
+  }
+  bool ok = true;
+  try {
+    (void)hipSetDevice(c.device);
+    if (!c.asp_scratch) return SK_ERR_DEVICE;
+    // no pool (nothing may be allocated) and no timer (no events created)
+    Exec x{nullptr, c.stream, nullptr, c.derr, c.host_words, c.read_ev, nullptr};
+    u64 nr = 0, nl = 0;
+    ok = asp_simplify(x, head, body_start, body_lit, n_rules, n_lits, poss, cert, n_atoms, out_head, out_body_start,
+                      out_body_lit, &nr, &nl, c.asp_scratch);
+    *out_n_rules = nr;
+    *out_n_lits = nl;
+  } catch (const DeviceError&) {
+    return SK_ERR_DEVICE;
+  } catch (const BudgetError&) {
+    return SK_ERR_OUT_OF_BUDGET;
+  } catch (const std::exception&) {
+    return SK_ERR_DEVICE;
+  }
   return ok ? SK_OK : SK_ERR_INVALID_ARGUMENT;
 }
 

@@ -281,6 +281,13 @@ void repack(const Exec& x, const K* keys, u64 n, int arity, int bits, const int*
 template <class K>
 void member(const Exec& x, const K* sorted, u64 ns, const K* q, u64 nq, u8* flags);
 
+// Limits of the ASP filter's allocation-free variant (its static scratch),
+// and that scratch's address on `device`, which must be current (resolved
+// once per device with hipGetSymbolAddress, then cached).
+constexpr u64 kAspMaxRules = 1ull << 27;
+constexpr u64 kAspMaxAtoms = 1ull << 22;
+u8* asp_scratch_address(int device);
+
 void memset_async(const Exec& x, void* p, int v, u64 bytes);
 void fill_u8(const Exec& x, u8* p, u8 v, u64 n);
 

@@ -121,6 +121,7 @@ void Ctx::init(int dev, u64 budget) {
   SK_HIP(hipMemsetAsync(derr, 0, sizeof(DevErr), stream));
   SK_HIP(hipHostMalloc((void**)&host_words, 64, hipHostMallocDefault));
   SK_HIP(hipEventCreateWithFlags(&read_ev, hipEventDisableTiming));
+  asp_scratch = asp_scratch_address(dev);
   SK_HIP(hipStreamSynchronize(stream));
 }
 

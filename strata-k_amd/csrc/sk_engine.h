@@ -310,6 +310,8 @@ struct Ctx {
   // path's LDS table size (1 .. 1024).
   bool k4_force_heavy = false;
   u32 k4_tcap = 1024;
+  // this device's copy of the ASP filter's static scratch (resolved at init)
+  u8* asp_scratch = nullptr;
   // Two pinned staging buffers for large device-to-host result copies,
   // allocated on first use (d2h).
   static constexpr u64 kStageBytes = 64ull << 20;

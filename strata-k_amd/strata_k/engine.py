@@ -191,6 +191,8 @@ SIGNATURES = [
     ("sk_prepare_json", C.c_int, [_VP, C.c_char_p, C.c_size_t, C.POINTER(_VP)]),
     ("sk_asp_simplify_filter_device", C.c_int, [_VP, _VP, _VP, _VP, _U64, _U64, _VP, _VP, _U64, _VP, _VP, _VP,
                                                 C.POINTER(_U64), C.POINTER(_U64)]),
+    ("sk_asp_simplify_filter_device_noalloc", C.c_int, [_VP, _VP, _VP, _VP, _U64, _U64, _VP, _VP, _U64, _VP, _VP,
+                                                        _VP, C.POINTER(_U64), C.POINTER(_U64)]),
     ("sk_sort_u64", C.c_int, [_VP, _VP, _U64, _VP]),
     ("sk_sort_unique_u64", C.c_int, [_VP, _VP, _U64, _VP, C.POINTER(_U64)]),
     ("sk_sort_unique_pairs", C.c_int, [_VP, _VP, _VP, _U64, _VP, _VP, C.POINTER(_U64)]),
@@ -442,14 +444,15 @@ class Context:
 
     def asp_simplify_filter_device(self, head_ptr: int, bs_ptr: int, lits_ptr: int, n_rules: int, n_lits: int,
                                    poss_ptr: int, cert_ptr: int, n_atoms: int, out_head_ptr: int, out_bs_ptr: int,
-                                   out_lits_ptr: int):
+                                   out_lits_ptr: int, noalloc: bool = False):
         """The ASP simplify filter over device-resident buffers; returns the
-        kept rule and literal counts once the filter has finished."""
+        kept rule and literal counts once the filter has finished.  noalloc
+        takes the entry point that allocates no device memory."""
         nr, nl = C.c_uint64(), C.c_uint64()
-        code = self.L.sk_asp_simplify_filter_device(self.h, head_ptr or None, bs_ptr, lits_ptr or None, int(n_rules),
-                                                    int(n_lits), poss_ptr or None, cert_ptr or None, int(n_atoms),
-                                                    out_head_ptr or None, out_bs_ptr, out_lits_ptr or None,
-                                                    C.byref(nr), C.byref(nl))
+        fn = self.L.sk_asp_simplify_filter_device_noalloc if noalloc else self.L.sk_asp_simplify_filter_device
+        code = fn(self.h, head_ptr or None, bs_ptr, lits_ptr or None, int(n_rules), int(n_lits), poss_ptr or None,
+                  cert_ptr or None, int(n_atoms), out_head_ptr or None, out_bs_ptr, out_lits_ptr or None,
+                  C.byref(nr), C.byref(nl))
         if code:
             _raise(code)
         return nr.value, nl.value
