@@ -9,6 +9,19 @@ A step is one complete semi-naive fixpoint over the EDB already resident in
 HBM (relations reset to the loaded state before each step).  value = derived
 tuples (|cost|) per second.
 
+The same JSON line also carries, as extra keys:
+  end_to_end   config 2 timed as the reference times its benchmark (host EDB
+               in, host result out), and with the result left in HBM;
+  batched      config 4: 512 programs in one batch against 512 plain runs
+               (execute-phase and end-to-end speedups, bit-exact check);
+  served       config 5: Poisson queries on a resident RMAT-24 graph (p50/p99,
+               queries/s, batched vs one at a time, answer contents compared
+               with plain runs);
+  cpu_baseline the oracle (C restatement of strata-eval) on host threads, one
+               source per run; cpu_scan_config1 the faithful scan-mode
+               interpreter's labelled extrapolation to config 1.
+--no-extras leaves out batched and served, --no-cpu the CPU baselines.
+
 Also: --config 1|3|4|5 for the other workloads (TC, triangles, batched,
 served) used by the documentation; they print their own JSON line.
 """
@@ -56,19 +69,78 @@ def measured_traffic(kernel):
         return None
 
 
-def cpu_baseline_cfg2(wl_full, k_sources=1):
-    """The oracle (single-threaded C restatement of strata-eval semi-naive)
-    on the same graph restricted to the first k sources: cost(S,.) depends on
-    S only, so this is an exact sub-workload."""
+def host_info():
+    """CPU model and the cores this process may use (the CPU baselines' host)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_baseline_cfg2(wl_full, threads):
+    """The oracle (the single-threaded C restatement of strata-eval's indexed
+    semi-naive evaluation) on the same graph, one run per source, `threads`
+    runs at a time on host threads: cost(S,.) depends on S only, so each run
+    is an exact sub-workload and the runs together are the multi-threaded CPU
+    engine's evaluation of that many sources."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     from strata_k import workloads as wl
-    ws = wl.restrict_sources(wl_full, k_sources)
-    r = O.run(ws.program, bulk=ws.bulk, want=["cost"])
-    n = len(r.relations["cost"])
-    return {"value": round(n / r.seconds, 1), "unit": "tuples/s", "cores": 1, "kind": "port",
-            "sample": f"same RMAT-20 graph, first {k_sources} of 64 sources: {n} cost tuples, "
-                      f"{r.iterations[0]} rounds, {r.seconds:.1f} s of semi-naive evaluation (EDB load excluded)",
-            "host_cpu": platform.processor() or platform.machine()}
+    src = wl_full.bulk["src"][0][0][:threads]
+
+    def one(s0):
+        bulk = dict(wl_full.bulk)
+        bulk["src"] = ([np.array([s0], np.uint32)], np.zeros(1, np.int64), None)
+        r = O.run(wl_full.program, bulk=bulk, want=["cost"])
+        return len(r.relations["cost"]), r.iterations[0], r.seconds
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, src.tolist()))
+    wall = time.perf_counter() - t0
+    n = sum(r[0] for r in res)
+    single = res[0][0] / res[0][2]
+    info = host_info()
+    return {"value": round(n / wall, 1), "unit": "tuples/s", "cores": threads, "kind": "port",
+            "sample": f"same RMAT-20 graph, {len(src)} of 64 sources, one oracle run (indexed semi-naive "
+                      f"restatement of strata-eval) per source on {threads} host threads: {n} cost tuples in "
+                      f"{wall:.1f} s wall, EDB conversion included",
+            "single_thread_tuples_per_s": round(single, 1), **info}
+
+
+def cpu_scan_extrapolation_cfg1():
+    """The oracle's scan mode (the reference's nested-loop join order, one
+    core) on config 1's program over a 400-node / 2,000-edge sample, where its
+    unify-step count is checked against |E| * (|path| + rounds + 2); full
+    config 1's count by the same formula (from the indexed run's |path| and
+    rounds) divided by the sample's steps/s is the labelled extrapolation of
+    the faithful interpreter on config 1."""
+    from oracle import oracle as O
+    from strata_k import workloads as wl
+    ws = wl.config1(seed=1, n_nodes=400, n_edges=2000)
+    r = O.run(ws.program, bulk=ws.bulk, want=["path"], indexed=False)
+    m = len(ws.bulk["edge"][0][0])
+    formula_ok = r.steps == m * (len(r.relations["path"]) + r.iterations[0] + 2)
+    rate = r.steps / r.seconds
+    wf = wl.config1()
+    ri = O.run(wf.program, bulk=wf.bulk, want=["path"])
+    mf = len(wf.bulk["edge"][0][0])
+    steps_full = mf * (len(ri.relations["path"]) + ri.iterations[0] + 2)
+    return {"kind": "port, extrapolated", "cores": 1, "steps_per_s": round(rate, 1),
+            "sample_steps_match_formula": bool(formula_ok), "config1_unify_steps": steps_full,
+            "config1_seconds_extrapolated": round(steps_full / rate, 1),
+            "config1_tuples_per_s_extrapolated": round(len(ri.relations["path"]) / (steps_full / rate), 1),
+            "config1_indexed_seconds": round(ri.seconds, 2)}
 
 
 def run_steps(ctx, run, steps, warmup):
@@ -151,12 +223,71 @@ def bench_cfg2(args):
         log(json.dumps(sorted(recs, key=lambda r: -r["ms"]), indent=1))
     run.free()
     ctx.close()
+    line["end_to_end"] = cfg2_end_to_end(w, n_cost)
+    if not args.no_extras:
+        line["batched"] = cfg4_summary(args)
+        line["served"] = cfg5_summary(args)
     if not args.no_cpu:
-        log("timing the CPU baseline (oracle, 1 source) ...")
-        line["cpu_baseline"] = cpu_baseline_cfg2(w, 1)
+        threads = max(1, min(16, host_info()["usable_cpus"]))
+        log(f"timing the CPU baseline (oracle, {threads} sources on {threads} threads) ...")
+        line["cpu_baseline"] = cpu_baseline_cfg2(w, threads)
+        log("timing the scan-mode extrapolation of config 1 ...")
+        line["cpu_scan_config1"] = cpu_scan_extrapolation_cfg1()
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+
+
+def cfg2_end_to_end(w, n_cost):
+    """Config 2 as the reference times its benchmark (examples/bench.rs:53-63):
+    host EDB in, fixpoint, host result out, on a fresh context; also the same
+    without the result read (the answer left in HBM)."""
+    import strata_k as sk
+    ctx = sk.Context(0)
+    t0 = time.perf_counter()
+    run = ctx.prepare(w.program)
+    for p, (cols, wt, inf) in w.bulk.items():
+        run.add_sym(p, cols, wt, inf)
+    run.load()
+    run.execute()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    (sa, za), cw, _ = run.relation_sym("cost")
+    t2 = time.perf_counter()
+    assert len(sa) == n_cost
+    run.free()
+    ctx.close()
+    return {"seconds": round(t2 - t0, 4), "tuples_per_s": round(n_cost / (t2 - t0), 1),
+            "seconds_result_in_hbm": round(t1 - t0, 4), "tuples_per_s_result_in_hbm": round(n_cost / (t1 - t0), 1),
+            "includes": "prepare, H2D of the host EDB, device sort, fixpoint, D2H of the result"}
+
+
+def cfg4_summary(args):
+    """Config 4 (512 batched programs against 512 plain runs) as extra keys."""
+    a = argparse.Namespace(**vars(args))
+    a.programs, a.seed = 512, 4
+    log("config 4: 512 programs batched and plain ...")
+    r = bench_cfg4(a, emit=False)
+    keep = ("value", "value_is", "end_to_end_speedup", "plain_s", "batched_s", "plain_device_ms",
+            "batched_device_ms", "batched_phases", "bit_exact_vs_plain")
+    out = {"workload": r["config"]["workload"], "derived_tuples": r["config"]["derived_tuples"]}
+    out.update({("execute_speedup" if k == "value" else k): r[k] for k in keep})
+    return out
+
+
+def cfg5_summary(args):
+    """Config 5 (served queries on a resident RMAT-24 graph) as extra keys."""
+    a = argparse.Namespace(**vars(args))
+    a.scale, a.queries, a.max_sources, a.rate, a.max_batch, a.plain_queries = 24, 24, 16, 20.0, 16, 4
+    log("config 5: served queries on RMAT scale 24 ...")
+    r = bench_cfg5(a, emit=False)
+    out = {"workload": r["config"]["workload"]}
+    for k in ("p50_ms", "p99_ms", "served_qps", "capacity_qps_batched", "capacity_qps_one_at_a_time",
+              "batched_vs_one_at_a_time", "plain_run_per_query_qps", "served_vs_plain_runs",
+              "same_rows_and_rounds_alone_batched_and_plain", "answers_compared_with_plain_runs",
+              "answer_contents_equal"):
+        out[k] = r[k]
+    return out
 
 
 def bench_cfg1(args):
@@ -198,7 +329,7 @@ def bench_cfg3(args):
                       "roofline": roofline_of(top)}), flush=True)
 
 
-def bench_cfg4(args):
+def bench_cfg4(args, emit=True):
     """Batched: 512 independent programs (256 Bool TC, 256 Trop cost) through
     one batch, against 512 sequential plain runs of the same programs.  value
     = the execute-phase speedup (every fixpoint over EDB already resident in
@@ -253,7 +384,7 @@ def bench_cfg4(args):
             same = same and (pw == bw).all()
         exact = exact and same
     ctx.close()
-    print(json.dumps({
+    out = {
         "metric": "batched-vs-plain speedup",
         "value": round(plain_exec_ms / batch_exec_ms, 2) if batch_exec_ms else None, "unit": "x",
         "value_is": "execute phase (fixpoints on resident EDB, wall clock): batched vs 512 plain runs",
@@ -266,14 +397,19 @@ def bench_cfg4(args):
         "plain_tuples_per_s": round(n_out / plain_s, 1), "batched_tuples_per_s": round(n_out / batch_s, 1),
         "plain_device_ms": round(plain_exec_ms, 1), "batched_device_ms": round(batch_exec_ms, 1),
         "batched_phases": phases,
-        "bit_exact_vs_plain": bool(exact)}), flush=True)
+        "bit_exact_vs_plain": bool(exact)}
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def bench_cfg5(args):
+def bench_cfg5(args, emit=True):
     """Served: resident RMAT graph, Poisson query stream (arrival times from
     the workload generator), continuous batching; p50/p99 latency and
     queries/s.  Capacity: the same queries offered at once, batched versus
-    one query at a time on the same resident graph."""
+    one query at a time on the same resident graph.  The answers of the
+    first `check_queries` queries are compared with the multi-source Dijkstra
+    checker (rows, weights and round counts)."""
     import strata_k as sk
     from strata_k import workloads as wl
     t = time.time()
@@ -283,11 +419,13 @@ def bench_cfg5(args):
     qs = wl.config5_queries(seed=5, n_queries=args.queries, n_nodes=n, max_sources=args.max_sources,
                             n_targets=64, filter_frac=0.1, rate_per_s=args.rate)
     ctx = sk.Context(0)
+    n_check = min(len(qs), args.plain_queries, getattr(args, "check_queries", 4))
+    checked = None
 
-    def serve(max_batch, open_loop):
+    def serve(max_batch, open_loop, keep=()):
         srv = sk.Server(ctx, s, d, w, max_batch=max_batch)
         pending = list(range(len(qs)))
-        tickets, done = {}, {}
+        tickets, done, answers = {}, {}, {}
         t0 = time.perf_counter()
         while len(done) < len(qs):
             now = time.perf_counter() - t0
@@ -301,18 +439,21 @@ def bench_cfg5(args):
                 r = srv.poll(tk)
                 if r is not None:
                     done[i] = r
-                    srv.release(tk)
+                    if i in keep:
+                        answers[i] = srv.take(tk, r["rows"])
+                    else:
+                        srv.release(tk)
             if active == 0 and waiting == 0 and pending and open_loop:
                 time.sleep(max(0.0, min(0.002, qs[pending[0]]["arrival_s"] - (time.perf_counter() - t0))))
         total = time.perf_counter() - t0
         srv.free()
-        return total, done
+        return total, done, answers
 
-    t_open, res = serve(args.max_batch, True)
+    t_open, res, answers = serve(args.max_batch, True, keep=set(range(n_check)))
     lat = np.array([res[i]["latency_ms"] for i in range(len(qs))])
     rows = sum(res[i]["rows"] for i in range(len(qs)))
-    t_cap, _ = serve(args.max_batch, False)
-    t_seq, res1 = serve(1, False)
+    t_cap, _, _ = serve(args.max_batch, False)
+    t_seq, res1, _ = serve(1, False)
     exact = all(res1[i]["rows"] == res[i]["rows"] and res1[i]["iterations"] == res[i]["iterations"]
                 for i in range(len(qs)))
     # one plain run per query (EDB loaded per run), on the first queries
@@ -324,12 +465,20 @@ def bench_cfg5(args):
         bulk = {"edge": ([s, d], w, None), "src": ([q["sources"]], np.zeros(len(q["sources"]), np.int64), None)}
         if filt:
             bulk["tgt"] = ([q["targets"]], np.zeros(len(q["targets"]), np.int64), None)
-        db = ctx.run_semi_naive(wl.query_program(filt), bulk=bulk, want=["ans" if filt else "cost"])
-        exact = exact and len(db.relation("ans" if filt else "cost")) == res[i]["rows"] \
-            and db.iterations[0] == res[i]["iterations"]
+        out_rel = "ans" if filt else "cost"
+        db = ctx.run_semi_naive(wl.query_program(filt), bulk=bulk, want=[out_rel])
+        exact = exact and len(db.relation(out_rel)) == res[i]["rows"] and db.iterations[0] == res[i]["iterations"]
+        if i in answers:
+            # the served answer's rows and weights against the plain run's
+            # (the engine's dense cells and pull rounds, not the server's)
+            rel = db.relation(out_rel)
+            ps, pd_, pw = rel.vals[:, 0], rel.vals[:, 1], rel.w
+            gs, gd, gw = answers[i]
+            same = len(gs) == len(ps) and bool((gs == ps).all() and (gd == pd_).all() and (gw == pw).all())
+            checked = same if checked is None else (checked and same)
     t_plain = (time.perf_counter() - t0) / max(1, n_plain)
     ctx.close()
-    print(json.dumps({
+    out = {
         "metric": "p99 query ms", "value": round(float(np.percentile(lat, 99)), 2), "unit": "ms",
         "n_gpus": 1, "higher_is_better": False, "dtype": "int64", "data": "synthetic (seeded RMAT, Poisson arrivals)",
         "config": {"workload": f"served cheapest cost, RMAT scale {scale} ({n} nodes, {len(s)} edges), "
@@ -341,7 +490,12 @@ def bench_cfg5(args):
         "batched_vs_one_at_a_time": round(t_seq / t_cap, 2),
         "plain_run_per_query_qps": round(1.0 / t_plain, 3), "plain_queries_timed": n_plain,
         "served_vs_plain_runs": round((len(qs) / t_cap) * t_plain, 2),
-        "same_rows_and_rounds_alone_batched_and_plain": bool(exact)}), flush=True)
+        "same_rows_and_rounds_alone_batched_and_plain": bool(exact),
+        "answers_compared_with_plain_runs": n_check if checked is not None else 0,
+        "answer_contents_equal": checked}
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_sort(args):
@@ -609,6 +763,8 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="config 2: leave out the batched (config 4) and served (config 5) measurements")
     ap.add_argument("--kernels", action="store_true")
     ap.add_argument("--programs", type=int, default=512, help="config 4: number of programs")
     ap.add_argument("--queries", type=int, default=64, help="config 5: number of queries")

@@ -254,3 +254,92 @@ def test_server_failures_stay_per_query(ctx):
     oa, ob, ow, oit = _oracle_answer(src, dst, w, q)
     assert (a == oa).all() and (b == ob).all() and (ww == ow).all() and res["iterations"] == oit
     srv.free()
+
+
+# ------------------------------------------------ full-shape configurations
+
+NONE64 = np.iinfo(np.int64).max
+
+
+def test_config4_full_shape(ctx):
+    """The batched configuration at its recorded shape: 512 programs (256
+    Bool TC with n <= 2000, m <= 10,000; 256 Trop cost, n in [1e3, 2e5], m in
+    [4e3, 2e6], one source), in one batch.  Every program equals its own plain
+    run (rows, weights, round counts); every cost program equals the
+    multi-source Dijkstra checker (weights and round count = largest fewest
+    hops); every TC program's closure and round count equal unit-weight
+    Dijkstra from every node (reachability, longest shortest path)."""
+    progs = wl.batch_programs(seed=4, count=512, cost_sources=1, tc_node_cap=2000, tc_edge_cap=10_000)
+    b = sk.Batch(ctx)
+    for p in progs:
+        b.add(p.program, p.bulk)
+    b.load()
+    b.execute()
+    n_rows = 0
+    for i, p in enumerate(progs):
+        assert b.status(i)[0] == 0, i
+        cols, w = b.relation_sym(i, p.output)
+        its = b.iterations(i)
+        n_rows += len(cols[0])
+        run = ctx.prepare(p.program)
+        for pred, (pc, pw) in p.bulk.items():
+            run.add_sym(pred, pc, pw, None)
+        run.load()
+        run.execute()
+        (pcols, pw, _), pits = run.relation_sym(p.output), run.iterations()
+        run.free()
+        assert its == pits, i
+        assert all((x == y).all() for x, y in zip(cols, pcols)), i
+        (es, ed), ew = p.bulk["edge"][0], p.bulk["edge"][1]
+        n = int(max(es.max(), ed.max())) + 1 if len(es) else 1
+        if p.output == "cost":
+            assert (w == pw).all(), i
+            srcs = p.bulk["src"][0][0]
+            W, _, mh = O.cost_dijkstra(n, es, ed, ew, srcs)
+            S, Z, WW = O.cost_rows(W, srcs)
+            assert (cols[0] == S).all() and (cols[1] == Z).all() and (w == WW).all(), i
+            assert its == [mh], i
+        else:
+            W, _, mh = O.cost_dijkstra(n, es, ed, np.ones(len(es), np.int64), np.arange(n))
+            S, Z, _ = O.cost_rows(W, np.arange(n))
+            assert (cols[0] == S).all() and (cols[1] == Z).all(), i
+            assert its == [mh], i
+    b.free()
+    print("config4 full shape:", len(progs), "programs,", n_rows, "derived tuples, all exact")
+
+
+def test_config5_rmat22_answers(ctx):
+    """The served configuration on an RMAT scale-22 graph (4,194,304 nodes,
+    65M edges) with queries of up to 6 sources, a third filtered to 64
+    targets, all submitted at once and run together: every answer's rows and
+    weights equal the multi-source Dijkstra checker's (restricted to the
+    targets when filtered), and its round count the largest fewest-hops count
+    over its sources."""
+    s, d, w, n = wl.config5_graph(seed=5, scale=22)
+    qs = wl.config5_queries(seed=9, n_queries=10, n_nodes=n, max_sources=6, n_targets=64, filter_frac=0.3,
+                            rate_per_s=1e9)
+    srv = sk.Server(ctx, s, d, w, max_batch=16)
+    tickets = [srv.submit(q["sources"], q["targets"]) for q in qs]
+    done = {}
+    for _ in range(100_000):
+        srv.step()
+        for i, tk in enumerate(tickets):
+            if i not in done:
+                r = srv.poll(tk)
+                if r is not None:
+                    done[i] = (r, srv.take(tk, r["rows"]))
+        if len(done) == len(qs):
+            break
+    srv.free()
+    assert len(done) == len(qs)
+    for i, q in enumerate(qs):
+        r, (gs, gd, gw) = done[i]
+        W, _, mh = O.cost_dijkstra(n, s, d, w, q["sources"])
+        S, Z, WW = O.cost_rows(W, q["sources"])
+        if len(q["targets"]):
+            keep = np.isin(Z, q["targets"])
+            S, Z, WW = S[keep], Z[keep], WW[keep]
+        assert len(gs) == len(S), i
+        assert (gs == S).all() and (gd == Z).all() and (gw == WW).all(), i
+        assert r["iterations"] == mh, i
+    print("config5 rmat22:", sum(r["rows"] for r, _ in done.values()), "answer rows, all exact")
