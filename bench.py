@@ -189,6 +189,7 @@ def bench_cfg2(args):
     ctx = sk.Context(0)
     run = ctx.prepare(w.program)
     run.set_graph(args.graph != 0)
+    run.set_pack(args.pack != 0)
     for p, (cols, wt, inf) in w.bulk.items():
         run.add_sym(p, cols, wt, inf)
     t = time.time()
@@ -776,6 +777,8 @@ def main():
                     help="config 2: device-driven rounds captured in a hipGraph (1/0).  Off by default so that "
                          "every launch of the dominant kernel is timed by its own HIP events; on, the pull "
                          "rounds inside graph launches are timed as one record per launch")
+    ap.add_argument("--pack", type=int, default=1,
+                    help="config 2: packed pull deltas (1, the default) or the unpacked weights and lanes (0)")
     ap.add_argument("--sort", action="store_true", help="time the device radix sort primitive")
     ap.add_argument("--sort-log2", type=int, default=26)
     ap.add_argument("--chain-union", action="store_true",

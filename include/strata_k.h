@@ -221,6 +221,12 @@ sk_status sk_run_set_pull(sk_run* run, int mode);
  * set, with no host read inside a launch.  Results and round counts are
  * identical either way.  No reference counterpart. */
 sk_status sk_run_set_graph(sk_run* run, int enabled);
+/* Packed pull deltas (default on): when every staged Trop weight is below
+ * 2^26 and the cells are narrow, a pull round reads its delta as one 32-bit
+ * word per entry ((weight << 6) | lane) and 8-byte row records; a derived
+ * weight at or above 2^26 widens the cells.  Results and round counts are
+ * identical either way.  No reference counterpart. */
+sk_status sk_run_set_pack(sk_run* run, int enabled);
 void sk_run_error(const sk_run* run, sk_error_info* out);
 
 /* Results: rows in BTreeMap order (sorted by GroundVal tuple). */

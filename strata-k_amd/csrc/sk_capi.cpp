@@ -264,6 +264,12 @@ sk_status sk_run_set_graph(sk_run* run, int enabled) {
   return SK_OK;
 }
 
+sk_status sk_run_set_pack(sk_run* run, int enabled) {
+  if (!run) return SK_ERR_INVALID_ARGUMENT;
+  run->eng->set_pack(enabled != 0);
+  return SK_OK;
+}
+
 void sk_run_error(const sk_run* run, sk_error_info* out) {
   if (run) set_info(run->err, out);
 }

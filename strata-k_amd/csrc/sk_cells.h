@@ -28,6 +28,8 @@ namespace sk {
 
 constexpr u64 kCellTop = ~0ull;
 constexpr u32 kCellTop32 = ~0u;
+// Narrow weights below this fit a packed delta word beside a 6-bit lane.
+constexpr u64 kPackLim = 1ull << 26;
 
 // State of a device-driven run of delta rounds (a hipGraph of rounds that
 // the host relaunches until `done`): each round's kernels do nothing once
@@ -74,6 +76,9 @@ struct CellDesc {
   int narrow = 0;
   u32* cell32 = nullptr;
   u32* widen = nullptr;
+  // Narrow derivations of weight >= narrow_lim set *widen (kCellTop32, or
+  // 2^26 when the delta is also written packed, below).
+  u32 narrow_lim = kCellTop32;
   u32* present = nullptr;  // Bool presence bits
   u32* maxf = nullptr;     // Trop: present with finite INT64_MAX
   u32* infb = nullptr;     // Trop: present with +inf
@@ -87,6 +92,12 @@ struct CellDesc {
   // With rowrec: that compaction also writes each listed cell's lane
   // (cell mod 2^rlbits), in list order.
   u8* dlane = nullptr;
+  // Narrow cells below 2^26 (narrow_lim): with rowrec, that compaction also
+  // writes each listed cell as one word, (weight << 6) | lane, and a second
+  // row record of one word per row, rowrec8[row] = position | count << 40,
+  // so a pull reads one array per delta run and 8 bytes per row.
+  u32* dpack = nullptr;
+  u64* rowrec8 = nullptr;
   // When set with rowrec, the counting pass of that compaction also sums,
   // over the touched cells, pc_csr[row+1] - pc_csr[row] (row < pc_nv): the
   // output count of the push join of the new delta against that CSR.
@@ -176,7 +187,7 @@ __device__ __forceinline__ void cell_apply_t(const CellDesc& cd, u64 c, i64 w, u
   }
   if (cd.narrow) {
     if (!inf) {
-      if ((u64)w >= (u64)kCellTop32) {  // negative, or too large for a narrow cell
+      if ((u64)w >= (u64)cd.narrow_lim) {  // negative, or too large for a narrow cell
         atomicOr(cd.widen, 1u);
         return;
       }

@@ -351,6 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
           v.x = (u64)b | ((u64)hi << 32);
           v.y = g;
           *(ulonglong2*)(cd.rowrec + 2 * r) = v;
+          if (cd.rowrec8) cd.rowrec8[r] = g | ((u64)(__popc(b) + __popc(hi)) << 40);
         }
       } else {
         const int lb = cd.rlbits, per = 32 >> lb;
@@ -363,6 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
           v.x = (b >> sh) & m;
           v.y = g + (u32)__popc(b & ((1u << sh) - 1u));
           *(ulonglong2*)(cd.rowrec + 2 * r) = v;
+          if (cd.rowrec8) cd.rowrec8[r] = v.y | ((u64)__popc((u32)v.x) << 40);
         }
       }
     }
@@ -398,6 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
       else f = 1;
       w[base + j] = wv;
       if (inf) inf[base + j] = f;
+      if (touched_only && cd.dpack) cd.dpack[base + j] = ((u32)wv << 6) | (u32)(c & ((1u << cd.rlbits) - 1u));
     }
   }
 }

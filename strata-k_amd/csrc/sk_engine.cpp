@@ -528,9 +528,15 @@ void Engine::upload_rel(int r) {
   }
   bool trop = R.sem == SK_TROP;
   if (trop && R.st_narrow == 0) narrow_ok_ = false;
+  // batched staging records only whether its weights are narrow
+  if (trop && R.st_narrow > 0) pack_ok_ = false;
   if (trop && R.st_narrow < 0)
     for (u64 i = 0; i < n && narrow_ok_; i++)
-      if (!R.st_inf[i] && (u64)R.st_w[i] >= 0xFFFFFFFFull) narrow_ok_ = false;
+      if (!R.st_inf[i]) {
+        const u64 v = (u64)R.st_w[i];
+        if (v >= 0xFFFFFFFFull) narrow_ok_ = false;
+        if (v >= kPackLim) pack_ok_ = false;
+      }
   // encode to codes, one column at a time
   std::vector<DBuf<u32>> dcols(R.arity);
   Cols cols{};
@@ -623,6 +629,7 @@ void Engine::compute_universe() {
 void Engine::load() {
   if (pending.code) throw pending;
   narrow_ok_ = true;
+  pack_ok_ = true;
   build_dict();
   for (size_t r = 0; r < R_.size(); r++) upload_rel((int)r);
   // distinct Sym values after the EDB load (crates/strata-eval/src/naive.rs:143-155)
@@ -1240,6 +1247,8 @@ PullDesc Engine::pull_desc(PullRule& pr, DenseRel& dn, int hr) {
   pd.rowrec = dn.rowrec.get();
   pd.dw = trop ? H.delta.w.get() : nullptr;
   pd.dlane = dn.dlane.get();
+  pd.dpack = dn.cd.dpack;
+  pd.rowrec8 = dn.cd.rowrec8;
   pd.lbits = dn.lbits;
   pd.delta_first = pr.delta_first ? 1 : 0;
   pd.nrows = dn.nrows;
@@ -2709,6 +2718,16 @@ void Engine::plan_dense(u32 k) {
         cd.dlane = dn->dlane.get();
         cd.rnrows = dn->nrows;
         cd.rlbits = dn->lbits;
+        // narrow cells whose weights stay below 2^26 also get the packed
+        // delta (one word per entry) and 8-byte row records; a derivation at
+        // or above 2^26 widens the cells as any narrow overflow does
+        if (trop && cd.narrow && pack_ok_ && pack_enabled_) {
+          dn->dpack.reset(x.pool, std::max<u64>(dn->ncells, 1));
+          dn->rowrec8.reset(x.pool, dn->nrows ? dn->nrows : 1);
+          cd.dpack = dn->dpack.get();
+          cd.rowrec8 = dn->rowrec8.get();
+          cd.narrow_lim = (u32)kPackLim;
+        }
       }
     }
     if (!trop && R.arity >= 2) plan_bitpulls(k, (int)i, *dn);
@@ -2807,6 +2826,13 @@ void Engine::widen_dense() {
     d->cd.narrow = 0;
     d->cd.cell32 = nullptr;
     d->cd.cell = d->cell.get();
+    // wide cells pull from the unpacked delta (weights and lanes), which
+    // every compaction writes as well
+    d->cd.dpack = nullptr;
+    d->cd.rowrec8 = nullptr;
+    d->cd.narrow_lim = kCellTop32;
+    d->dpack.release();
+    d->rowrec8.release();
   }
   memset_async(x, &ctx_->derr->widen, 0, 4);
 }

@@ -225,6 +225,8 @@ struct DenseRel {
   std::vector<PullRule> pulls;
   DBuf<u64> rowrec;  // per row: {delta lanes, delta-list position}
   DBuf<u8> dlane;    // per delta entry: its cell's lane
+  DBuf<u32> dpack;   // narrow cells below 2^26: per delta entry (weight << 6) | lane
+  DBuf<u64> rowrec8; // with dpack: per row, delta-list position | count << 40
   // the delta is compacted into capacity buffers (ncells records) with no
   // host read between passes; `spare` keeps them while the delta is empty
   bool into = false;
@@ -402,6 +404,7 @@ class Engine {
   // Device-driven rounds captured in a hipGraph for strata whose every delta
   // variant takes the word-parallel pull (default on).
   void set_graph(bool on) { graph_enabled_ = on; }
+  void set_pack(bool on) { pack_enabled_ = on; }
 
  private:
   std::vector<RelData> snap_;  // post-load state of relations written by rules
@@ -437,9 +440,11 @@ class Engine {
   bool dense_enabled_ = true;
   int pull_mode_ = 1;
   bool graph_enabled_ = true;
+  bool pack_enabled_ = true;
   int round_variants_ = 0, round_bitpulls_ = 0, round_pulls_ = 0;  // delta variants of the current round,
                                                                     // and those pulled
   bool narrow_ok_ = true;   // every staged Trop weight is finite-or-inf in [0, 2^32-1)
+  bool pack_ok_ = true;     // ... and below 2^26 (pull deltas may be packed; see CellDesc::dpack)
 
   void build_dict();
   void upload_rel(int r);

@@ -93,6 +93,8 @@ struct PullDesc {
                                    //           delta cell in the delta list}
   const i64* dw = nullptr;         // delta weights, in cell order
   const u8* dlane = nullptr;       // delta cells' lanes, in cell order
+  const u32* dpack = nullptr;      // packed narrow delta ((weight << 6) | lane), in cell order, or null
+  const u64* rowrec8 = nullptr;    // with dpack: per row, position | count << 40
   int lbits = 0;                   // lanes per row = 2^lbits
   int delta_first = 1;             // ⊗ operand order: the delta's weight on the left
   u64 nrows = 0;                   // rows of the cell layout (j values at or above are absent)

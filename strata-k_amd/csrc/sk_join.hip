@@ -1298,12 +1298,143 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
   }
 }
 
+// Per wave of the packed pull kernel (narrow Trop cells below 2^26).
+struct PullLdsP {
+  u64 base[64];               // delta-list position of the index row's source row's run
+  u32 w[64];                  // edge weight
+  u32 excl[64];               // exclusive prefix of the index rows' pair counts
+  u32 slot[64];               // which of the chunk's destination rows the index row feeds
+  u32 acc[kPullRows * 64];    // running minimum per (destination row, lane value); kCellTop32 = none
+  u8 start[64 * kPullPairs];  // 1 + the index row whose pairs start at this position of the step
+};
+
+// The pull round of k_pull<true> over a packed delta (CellDesc::dpack,
+// rowrec8): a source row's delta run is one array of words
+// (weight << 6) | lane at the record's position, its length the record's
+// count, so a pair costs one 4-byte load and the row record 8 bytes.  Edge
+// and delta weights are below 2^32, so their sum cannot overflow; a sum at or
+// above narrow_lim raises the widen flag and is dropped, as a narrow cell
+// drops it.  Pairs are dealt to lanes and folded into a u32 LDS minimum per
+// (destination row, lane value) as in k_pull.
+__global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd) {
+  __shared__ PullLdsP sh[kBlock / 64];
+  const u32 lane = threadIdx.x & 63;
+  const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
+  if (pd.gate && (pd.gate->done | pd.gate->leave)) return;
+  PullLdsP& L = sh[threadIdx.x >> 6];
+  const PullChunk c = pd.chunks[ch];
+  const u32 nr = c.nr_off & 0xFFu;
+  const u64 lo = c.lo, hi = c.lo + c.n;
+  const int lb = pd.lbits;
+  const u32 nl = 1u << lb;
+  const u32 ncell = nr << lb;  // <= kPullRows * 64
+  const u64 lim = cd.narrow_lim;
+  for (u32 i = lane; i < kPullRows * 64; i += 64) L.acc[i] = kCellTop32;
+  u64 seen[kPullRows];
+#pragma unroll
+  for (u32 u = 0; u < kPullRows; u++) {
+    const u32 i = lane + 64 * u;
+    seen[u] = 0;
+    if (i < ncell) seen[u] = cell_peek(cd, ((u64)c.row[i >> lb] << lb) + (i & (nl - 1)));
+  }
+  bool wide = false;
+  for (u64 e0 = lo; e0 < hi; e0 += 64) {
+    const u64 e = e0 + lane;
+    u32 cnt = 0, we = 0, slot = 0;
+    u64 base = 0;
+    if (e < hi) {
+      const u32 q = pd.adj[e];
+      // an edge weight outside [0, lim) (a lower stratum's relation is not
+      // bounded by the staged weights) makes every candidate it forms reach
+      // lim, i.e. widen, where the wide cells redo the round exactly
+      const i64 wraw = pd.w[e];
+      we = (wraw < 0 || (u64)wraw >= lim) ? (u32)lim : (u32)wraw;
+      if (q < pd.nrows) {
+        const u64 rec = pd.rowrec8[q];
+        base = rec & ((1ull << 40) - 1);
+        cnt = (u32)(rec >> 40);
+      }
+      const u32 el = (u32)(e - lo);
+      for (u32 s = 1; s < nr; s++) slot += el >= ((c.nr_off >> (8 * s)) & 0xFFu) ? 1u : 0u;
+    }
+    const u32 incl = (u32)wave_incl_scan_j(cnt);
+    const u32 total = __shfl(incl, 63);
+    const u32 excl = incl - cnt;
+    L.base[lane] = base;
+    L.w[lane] = we;
+    L.excl[lane] = excl;
+    L.slot[lane] = slot;
+#pragma unroll
+    for (int u = 0; u < kPullPairs; u++) L.start[u * 64 + lane] = 0;
+    const u64 le = ~0ull >> (63 - lane);  // lanes at or below this one
+    u32 carry = 0;  // the edge holding the step's first pair (pair 0 starts an edge)
+    for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairs) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (cnt && excl >= p0 && excl < p0 + 64 * kPullPairs) L.start[excl - p0] = (u8)(lane + 1);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      u32 rs[kPullPairs];
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) rs[u] = L.start[u * 64 + lane];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) L.start[u * 64 + lane] = 0;
+      u32 jj[kPullPairs], kk[kPullPairs], vv[kPullPairs];
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) {
+        const u64 sm = __ballot(rs[u] != 0);
+        const u64 mine = sm & le;
+        const int h = mine ? 63 - __builtin_clzll(mine) : 0;
+        const u32 from = (u32)__shfl((int)rs[u], h, 64);
+        const u32 j = mine ? from - 1 : carry;
+        if (sm) carry = (u32)__builtin_amdgcn_readlane((int)rs[u], 63 - __builtin_clzll(sm)) - 1;
+        const u32 pidx = p0 + (u32)u * 64 + lane;
+        jj[u] = j;
+        kk[u] = pidx - L.excl[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) {
+        vv[u] = kCellTop32;
+        if (p0 + (u32)u * 64 + lane < total) vv[u] = pd.dpack[L.base[jj[u]] + kk[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kPullPairs; u++) {
+        if (p0 + (u32)u * 64 + lane >= total) continue;
+        const u32 j = jj[u];
+        const u64 cand = (u64)(vv[u] >> 6) + (u64)L.w[j];
+        if (cand >= lim) {
+          wide = true;
+          continue;
+        }
+        atomicMin(&L.acc[L.slot[j] * 64 + (vv[u] & 63u)], (u32)cand);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+#pragma unroll
+  for (u32 u = 0; u < kPullRows; u++) {
+    const u32 i = lane + 64 * u;
+    if (i >= ncell) continue;
+    const u32 sl = i >> lb, xl = i & (nl - 1);
+    const u32 v = L.acc[sl * 64 + xl];
+    if (v == kCellTop32) continue;
+    const u64 cl = ((u64)c.row[sl] << lb) + xl;
+    cell_apply_t<true>(cd, cl, (i64)v, 0, seen[u]);
+  }
+  if (__ballot(wide) && lane == 0) atomicOr(cd.widen, 1u);
+}
+
 void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop, double bytes, u64 items) {
   if (!p.nchunks) return;
   Scope sc(x.timer, "join_pull", KC_JOIN, bytes, items);
   const u64 waves_per_block = kBlock / 64;
   const unsigned nb = (unsigned)((p.nchunks + waves_per_block - 1) / waves_per_block);
-  if (trop) k_pull<true><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
+  if (trop && p.dpack) k_pull_packed<<<nb, kBlock, 0, x.stream>>>(p, cd);
+  else if (trop) k_pull<true><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
   else k_pull<false><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
   SK_HIP(hipGetLastError());
 }

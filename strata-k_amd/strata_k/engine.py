@@ -144,6 +144,7 @@ SIGNATURES = [
     ("sk_run_set_dense", C.c_int, [_VP, C.c_int]),
     ("sk_run_set_pull", C.c_int, [_VP, C.c_int]),
     ("sk_run_set_graph", C.c_int, [_VP, C.c_int]),
+    ("sk_run_set_pack", C.c_int, [_VP, C.c_int]),
     ("sk_run_error", None, [_VP, C.POINTER(SkErrorInfo)]),
     ("sk_result_size", _I64, [_VP, C.c_char_p]),
     ("sk_result_arity", _I64, [_VP, C.c_char_p]),
@@ -477,20 +478,22 @@ class Context:
 
     def run_semi_naive(self, prog, edb: Iterable = (), bulk: Optional[dict] = None,
                        want: Optional[Sequence[str]] = None, dense: bool = True, pull: int = 1,
-                       graph: bool = True) -> Db:
+                       graph: bool = True, pack: bool = True) -> Db:
         """``run_semi_naive(prog, edb)`` (crates/strata-eval/src/seminaive.rs:29).
 
         ``edb``: iterable of ``(pred, tuple-of-Sym/IntV, ann)``; ``bulk``: pred ->
         ``(columns, weights, inf)`` numpy arrays of Sym ids for large inputs.
         ``dense=False`` keeps every relation on the sorted-list path; ``pull``
         selects destination-side rounds (see ``Run.set_pull``), ``graph``
-        device-driven rounds (see ``Run.set_graph``).
+        device-driven rounds (see ``Run.set_graph``), ``pack`` packed pull
+        deltas (see ``Run.set_pack``).
         """
         run = self.prepare(prog)
         try:
             run.set_dense(dense)
             run.set_pull(pull)
             run.set_graph(graph)
+            run.set_pack(pack)
             run.add_facts(edb)
             for p, (cols, w, inf) in (bulk or {}).items():
                 run.add_sym(p, cols, w, inf)
@@ -619,6 +622,11 @@ class Run:
         """Device-driven rounds captured in a hipGraph once every delta
         variant takes the word-parallel pull (True, the default)."""
         self._check(self.L.sk_run_set_graph(self.h, 1 if on else 0))
+
+    def set_pack(self, on: bool):
+        """Packed pull deltas when every Trop weight stays below 2^26 (True,
+        the default); results are identical either way."""
+        self._check(self.L.sk_run_set_pack(self.h, 1 if on else 0))
 
     @property
     def execute_ms(self) -> float:

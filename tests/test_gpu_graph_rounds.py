@@ -17,7 +17,9 @@ from strata_k.core_ir import INF, Sym, cost_program
 pytestmark = pytest.mark.gpu
 
 ERR = {"overflow": "Overflow", "negative_weight_cycle": "NegativeWeightCycle"}
-MODES = [(1, True), (1, False), (2, True), (2, False), (0, True)]
+# (pull mode, graph rounds, packed pull deltas)
+MODES = [(1, True, True), (1, False, True), (2, True, True), (2, False, True), (0, True, True), (2, True, False),
+         (1, False, False)]
 
 
 def rows(rel):
@@ -28,20 +30,20 @@ def oracle_rows(rel):
     return [(tuple(x[1] for x in t), a) for t, a in rel.rows()]
 
 
-def check(ctx, edb=(), bulk=None, label=""):
-    prog = cost_program()
+def check(ctx, edb=(), bulk=None, label="", prog=None, out="cost"):
+    prog = prog or cost_program()
     ref = O.run(prog, edb, bulk=bulk)
-    for pull, graph in MODES:
-        lab = f"{label}[pull={pull},graph={graph}]"
+    for pull, graph, pack in MODES:
+        lab = f"{label}[pull={pull},graph={graph},pack={pack}]"
         try:
-            db = ctx.run_semi_naive(prog, edb, bulk=bulk, pull=pull, graph=graph)
+            db = ctx.run_semi_naive(prog, edb, bulk=bulk, pull=pull, graph=graph, pack=pack)
         except sk.EngineError as e:
             assert not ref.ok, (lab, e.kind)
             assert e.kind == ERR[ref.status], (lab, e.kind, ref.status)
             continue
         assert ref.ok, (lab, ref.status)
         assert db.iterations == ref.iterations, (lab, db.iterations, ref.iterations)
-        assert rows(db.relation("cost")) == oracle_rows(ref.relations["cost"]), lab
+        assert rows(db.relation(out)) == oracle_rows(ref.relations[out]), lab
     return ref
 
 
@@ -89,3 +91,34 @@ def test_negative_cycle_in_device_rounds(ctx):
     edb += [("src", (Sym(0),), 0), ("src", (Sym(4),), 0)]
     ref = check(ctx, edb, label="negcycle")
     assert ref.status == "negative_weight_cycle"
+
+
+def test_packed_delta_widens_past_2_26(ctx):
+    """Staged weights below 2^26 start on packed pull deltas; sums crossing
+    2^26 after two hops widen the cells mid-fixpoint (the round is redone on
+    wide cells), with results equal to the oracle's."""
+    rng = random.Random(26)
+    n = 400
+    edb = [("edge", (Sym(i), Sym((i * 7 + 3) % n)), (1 << 25) + rng.randrange(1000)) for i in range(n)]
+    edb += [("edge", (Sym(rng.randrange(n)), Sym(rng.randrange(n))), rng.randint(1, 50)) for _ in range(6 * n)]
+    edb += [("src", (Sym(s),), 0) for s in range(0, n, 9)]
+    check(ctx, edb, label="pack-widen")
+
+
+def test_packed_pull_over_lower_stratum_weights(ctx):
+    """The pull's fixed relation is derived by a lower stratum (two-hop sums
+    of staged weights below 2^26), so its own weights reach 2^26 and beyond:
+    the packed pull must not truncate them (they widen the cells)."""
+    from strata_k.core_ir import TROP, CoreProgram, atom, pos, pred, rule, v
+    prog = CoreProgram([pred("e", 2, TROP, 0), pred("big", 2, TROP, 0), pred("src", 1, TROP, 1),
+                        pred("cost", 2, TROP, 1)],
+                       [rule(atom("big", v(0), v(2)), [pos("e", v(0), v(1)), pos("e", v(1), v(2))], 3, 0),
+                        rule(atom("cost", v(0), v(1)), [pos("src", v(0)), pos("big", v(0), v(1))], 2, 1),
+                        rule(atom("cost", v(0), v(2)), [pos("cost", v(0), v(1)), pos("big", v(1), v(2))], 3, 1)], 2)
+    rng = random.Random(27)
+    n = 300
+    for big in (False, True):
+        hi = (1 << 25) + 5 if big else 1000
+        edb = [("e", (Sym(rng.randrange(n)), Sym(rng.randrange(n))), rng.randint(1, hi)) for _ in range(5 * n)]
+        edb += [("src", (Sym(s),), 0) for s in range(0, n, 11)]
+        check(ctx, edb, label=f"lower-stratum-{big}", prog=prog)
