@@ -1191,6 +1191,14 @@ static int col_index(const std::vector<int>& slots, int slot) {
 // (a pull round reads all of E's index; measured on gfx950, a push join of
 // about |E| outputs costs as much).  Returns false to run the push join
 // instead.
+// A round pulls when the push join would emit at least this many outputs
+// (|E| scaled by SK_PULL_RATIO_PCT percent): there a pull round, which reads
+// all of E's index, and the push join cost about the same.
+#ifndef SK_PULL_RATIO_PCT
+#define SK_PULL_RATIO_PCT 100
+#endif
+static u64 pull_threshold(u64 edges) { return edges / 100 * SK_PULL_RATIO_PCT + edges % 100 * SK_PULL_RATIO_PCT / 100; }
+
 bool Engine::try_pull(const RulePlan& pl) {
   if (pl.dpos < 0 || pull_mode_ == 0) return false;
   const int hr = P_.rules[pl.rule].head.rel;
@@ -1215,7 +1223,7 @@ bool Engine::try_pull(const RulePlan& pl) {
       cand = push_cost<K>(x, (const K*)H.delta.k.ptr(), H.delta.n, 2, H.bits, pr->R, ixj.csr.get());
     });
   }
-  if (pull_mode_ == 1 && cand < E.full.n) return false;
+  if (pull_mode_ == 1 && cand < pull_threshold(E.full.n)) return false;
   const PullDesc pd = pull_desc(*pr, *dn, hr);
   // algorithmic bytes: every index row of E (j, weight, its source's 16-byte
   // row record) once, every candidate's delta cell (lane byte, weight) once
@@ -1310,7 +1318,7 @@ int Engine::run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st)
   cdl.gate = ls.get();
   pd.gate = ls.get();
   const u64 cap_left = cap > *iters ? cap - *iters : 0;
-  const u64 push_limit = pull_mode_ == 2 ? 0 : R_[pr.E].full.n;
+  const u64 push_limit = pull_mode_ == 2 ? 0 : pull_threshold(R_[pr.E].full.n);
   const bool hf = has_inf_;
   Exec xg = x;
   xg.timer = nullptr;  // no event records inside the capture
