@@ -327,6 +327,8 @@ def bench_cfg3(args):
         log(json.dumps(sorted(recs, key=lambda r: -r["ms"]), indent=1))
     print(json.dumps({"config": "wcoj_triangles_rmat", "params": w.params, "tri": n, "ntri": str(ntri),
                       "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
+                      "kernel_ms_per_step": {r["name"]: round(r["ms"] / args.steps, 2)
+                                             for r in sorted(recs, key=lambda r: -r["ms"])[:8]},
                       "roofline": roofline_of(top)}), flush=True)
 
 

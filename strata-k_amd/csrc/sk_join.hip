@@ -877,7 +877,8 @@ constexpr u64 kTriHeavy = 512;
 template <class K>
 __global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
                             const u32* __restrict__ adj2, const u64* __restrict__ csr3, const u32* __restrict__ adj3,
-                            u32* __restrict__ cnt, u32* __restrict__ heavy, unsigned long long* __restrict__ hstat) {
+                            u32* __restrict__ cnt, u32* __restrict__ heavy, u32* __restrict__ hshort,
+                            unsigned long long* __restrict__ hstat) {
   u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nd) return;
   K k = d_keys[e];
@@ -887,7 +888,9 @@ __global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, con
   u64 b0 = csr3[u], b1 = csr3[u + 1];
   if ((a1 - a0) + (b1 - b0) > kTriHeavy) {
     // hstat: {heavy rows, their list values}
-    heavy[atomicAdd(&hstat[0], 1ull)] = (u32)e;
+    const u64 h = atomicAdd(&hstat[0], 1ull);
+    heavy[h] = (u32)e;
+    hshort[h] = (u32)min(a1 - a0, b1 - b0);
     atomicAdd(&hstat[1], (unsigned long long)((a1 - a0) + (b1 - b0)));
     cnt[e] = 0;  // written by the heavy pass
     return;
@@ -902,7 +905,8 @@ template <class K>
 __device__ __forceinline__ void tri_heavy_lists(const K* __restrict__ d_keys, u64 e, const TriDesc& t,
                                                 const u64* __restrict__ csr2, const u32* __restrict__ adj2,
                                                 const u64* __restrict__ csr3, const u32* __restrict__ adj3, u32* u,
-                                                u32* v, const u32** A, u64* na, const u32** B, u64* nb) {
+                                                u32* v, const u32** A, u64* na, const u32** B, u64* nb,
+                                                bool* a_in3 = nullptr) {
   const K k = d_keys[e];
   *u = col_of<K>(k, 2, t.bits, t.u_col);
   *v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
@@ -911,6 +915,7 @@ __device__ __forceinline__ void tri_heavy_lists(const K* __restrict__ d_keys, u6
   *na = a1 - a0;
   *B = adj3 + b0;
   *nb = b1 - b0;
+  if (a_in3) *a_in3 = *na > *nb;
   if (*na > *nb) {
     const u32* tp = *A;
     *A = *B;
@@ -932,8 +937,34 @@ __device__ __forceinline__ bool tri_member(const u32* __restrict__ B, u64 nb, u3
   return lo < nb && B[lo] == x;
 }
 
+// Heavy rows keep, per 64 values of their shorter list, a word of hit bits
+// (hmask, at the row's word offset hwoff[h]): the count pass finds the
+// common values once, and the emit pass only expands the bits.
+#ifndef SK_TRI_LDS
+#define SK_TRI_LDS 1024
+#endif
+constexpr u64 kTriLds = SK_TRI_LDS;  // longer lists up to this length are searched in LDS
+
+__global__ void k_tri_heavy_words(const u32* __restrict__ hshort, u64 nh, u32* __restrict__ nw) {
+  const u64 h = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < nh) nw[h] = (u32)((hshort[h] + 63) / 64);
+}
+
+// x in the sorted, duplicate-free list B[0, nb) held in LDS
+__device__ __forceinline__ bool tri_member_lds(const u32* B, u32 nb, u32 x) {
+  u32 lo = 0, hi = nb;
+  while (lo < hi) {
+    const u32 mid = (lo + hi) >> 1;
+    if (B[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < nb && B[lo] == x;
+}
+
 // One wave per heavy row: lane l tests values l, l + 64, ... of the shorter
-// list against the longer one by binary search (the lists are sets).
+// list against the longer one by binary search (the lists are sets), in LDS
+// when the longer list fits kTriLds values (loaded once, coalesced), else in
+// global memory.  Each step's ballot is the row's next hit word.
 template <class K>
 __global__ __launch_bounds__(kBlock) void k_tri_count_heavy(const K* __restrict__ d_keys, TriDesc t,
                                                             const u64* __restrict__ csr2,
@@ -941,60 +972,101 @@ __global__ __launch_bounds__(kBlock) void k_tri_count_heavy(const K* __restrict_
                                                             const u64* __restrict__ csr3,
                                                             const u32* __restrict__ adj3,
                                                             const u32* __restrict__ heavy, u64 nh,
-                                                            u32* __restrict__ cnt) {
+                                                            const u64* __restrict__ hwoff, u64* __restrict__ hmask,
+                                                            u64* __restrict__ hrec, u32* __restrict__ cnt) {
+  __shared__ u32 sB[kBlock / 64][kTriLds];
   const u64 w = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (w >= nh) return;  // whole waves
+  if (w >= nh) return;  // whole waves; no block barrier below
   const u32 lane = threadIdx.x & 63;
+  u32* L = sB[threadIdx.x >> 6];
   const u64 e = heavy[w];
   u32 u, v;
   const u32 *A, *B;
   u64 na, nb;
-  tri_heavy_lists<K>(d_keys, e, t, csr2, adj2, csr3, adj3, &u, &v, &A, &na, &B, &nb);
+  bool in3 = false;
+  tri_heavy_lists<K>(d_keys, e, t, csr2, adj2, csr3, adj3, &u, &v, &A, &na, &B, &nb, &in3);
+  const bool lds = nb <= kTriLds;
+  if (lds) {
+    for (u64 i = lane; i < nb; i += 64) L[i] = B[i];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  u64* words = hmask + hwoff[w];
   u32 c = 0;
-  for (u64 i = lane; i < na; i += 64) c += tri_member(B, nb, A[i]) ? 1u : 0u;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if (lane == 0) cnt[e] = c;
-}
-
-// One wave per heavy row: 64 values of the shorter list per step, tested as
-// in the count; a ballot orders the hits, so each step's outputs are written
-// together, ascending.
-template <class K, class KO>
-__global__ __launch_bounds__(kBlock) void k_tri_emit_heavy(const K* __restrict__ d_keys, TriDesc t,
-                                                           const u64* __restrict__ csr2, const u32* __restrict__ adj2,
-                                                           const u64* __restrict__ csr3, const u32* __restrict__ adj3,
-                                                           const u32* __restrict__ heavy, u64 nh,
-                                                           const u64* __restrict__ off, OutDesc o) {
-  const u64 w = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  if (w >= nh) return;  // whole waves
-  const u32 lane = threadIdx.x & 63;
-  const u64 e = heavy[w];
-  u32 u, v;
-  const u32 *A, *B;
-  u64 na, nb;
-  tri_heavy_lists<K>(d_keys, e, t, csr2, adj2, csr3, adj3, &u, &v, &A, &na, &B, &nb);
-  u64 oi = off[e];
-  const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (u64 base = 0; base < na; base += 64) {
     const u64 i = base + lane;
-    u32 x = 0;
     bool hit = false;
     if (i < na) {
-      x = A[i];
-      hit = tri_member(B, nb, x);
+      const u32 x = A[i];
+      hit = lds ? tri_member_lds(L, (u32)nb, x) : tri_member(B, nb, x);
     }
     const u64 m = __ballot(hit);
-    if (hit) {
-      u32 vals[3] = {u, v, x};  // binding slots: 0 = u, 1 = v, 2 = w
-      KO key = 0;
-      for (int c = 0; c < o.ncols; c++) {
-        u32 val = o.kind[c] == 0 ? vals[o.idx[c]] : o.cnst[c];
-        key = (key << o.bits) | (KO)val;
+    if (lane == 0) words[base / 64] = m;
+    c += (u32)__popcll(m);
+  }
+  if (lane == 0) {
+    cnt[e] = c;
+    // the row for the emit pass: where its shorter list starts (bit 63: in
+    // adj3, else adj2) and its binding (u, v)
+    hrec[2 * w] = ((u64)in3 << 63) | (u64)(A - (in3 ? adj3 : adj2));
+    hrec[2 * w + 1] = ((u64)u << 32) | v;
+  }
+}
+
+// After the scan: each heavy row's output offset, in heavy-list order.
+__global__ void k_tri_heavy_off(const u32* __restrict__ heavy, u64 nh, const u64* __restrict__ off,
+                                u64* __restrict__ hoff) {
+  const u64 h = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h < nh) hoff[h] = off[heavy[h]];
+}
+
+// One wave per 64 heavy rows: lane l loads row l's record, word offset,
+// output offset and shorter-list length in one step; then, row by row, the
+// row's hit words one at a time: a set bit l of word c is the common value
+// A[64 c + l], written at its rank among the row's hits (ascending, as A is).
+template <class KO>
+__global__ __launch_bounds__(kBlock) void k_tri_emit_heavy(const u32* __restrict__ adj2, const u32* __restrict__ adj3,
+                                                           u64 nh, const u64* __restrict__ hrec,
+                                                           const u32* __restrict__ hshort,
+                                                           const u64* __restrict__ hwoff,
+                                                           const u64* __restrict__ hoff,
+                                                           const u64* __restrict__ hmask, OutDesc o) {
+  const u64 g = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const u64 h0 = g * 64;
+  if (h0 >= nh) return;  // whole waves
+  const u32 lane = threadIdx.x & 63;
+  const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  u64 ra = 0, ruv = 0, rw = 0, ro = 0, rn = 0;
+  if (h0 + lane < nh) {
+    const u64 h = h0 + lane;
+    ra = hrec[2 * h];
+    ruv = hrec[2 * h + 1];
+    rw = hwoff[h];
+    ro = hoff[h];
+    rn = hshort[h];
+  }
+  const u32 nrows = (u32)min((u64)64, nh - h0);
+  for (u32 r = 0; r < nrows; r++) {
+    const u64 a = __shfl(ra, (int)r, 64), uv = __shfl(ruv, (int)r, 64);
+    const u64 woff = __shfl(rw, (int)r, 64), na = __shfl(rn, (int)r, 64);
+    u64 oi = __shfl(ro, (int)r, 64);
+    const u32* A = ((a >> 63) ? adj3 : adj2) + (a & ~(1ull << 63));
+    const u32 u = (u32)(uv >> 32), v = (u32)uv;
+    const u64 nw = (na + 63) / 64;
+    for (u64 cw = 0; cw < nw; cw++) {
+      const u64 m = hmask[woff + cw];
+      if ((m >> lane) & 1ull) {
+        const u32 x = A[cw * 64 + lane];
+        KO key = 0;
+        for (int c = 0; c < o.ncols; c++) {
+          // binding slots: 0 = u, 1 = v, 2 = w
+          const u32 val = o.kind[c] != 0 ? o.cnst[c] : o.idx[c] == 0 ? u : o.idx[c] == 1 ? v : x;
+          key = (key << o.bits) | (KO)val;
+        }
+        ((KO*)o.keys)[oi + (u64)__popcll(m & below)] = key;
       }
-      ((KO*)o.keys)[oi + (u64)__popcll(m & below)] = key;
+      oi += (u64)__popcll(m);
     }
-    oi += (u64)__popcll(m);
   }
 }
 
@@ -1020,10 +1092,10 @@ __global__ void k_tri_emit(const K* __restrict__ d_keys, u64 nd, TriDesc t, cons
   while (a < a1 && b < b1 && oi < oend) {
     u32 x = adj2[a], y = adj3[b];
     if (x == y) {
-      u32 vals[3] = {u, v, x};  // binding slots: 0 = u, 1 = v, 2 = w
       KO key = 0;
       for (int c = 0; c < o.ncols; c++) {
-        u32 val = o.kind[c] == 0 ? vals[o.idx[c]] : o.cnst[c];
+        // binding slots: 0 = u, 1 = v, 2 = w
+        const u32 val = o.kind[c] != 0 ? o.cnst[c] : o.idx[c] == 0 ? u : o.idx[c] == 1 ? v : x;
         key = (key << o.bits) | (KO)val;
       }
       ((KO*)o.keys)[oi++] = key;
@@ -1046,14 +1118,14 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
   }
   if (nd > 0xffffffffull) throw BudgetError{nd};  // heavy rows are listed by u32 index
   double lb = sum_list_len ? *sum_list_len : 0;
-  DBuf<u32> heavy(x.pool, nd);
+  DBuf<u32> heavy(x.pool, nd), hshort(x.pool, nd);
   DBuf<u64> hstat(x.pool, 2);
   memset_async(x, hstat.get(), 0, 16);
   {
     // bytes credited once the heavy rows' share of the lists is known
     Scope sc(x.timer, "leapfrog_count", KC_LEAPFROG, 0.0, nd);
     k_tri_count<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, cnt.get(),
-                                                      heavy.get(), (unsigned long long*)hstat.get());
+                                                      heavy.get(), hshort.get(), (unsigned long long*)hstat.get());
     SK_HIP(hipGetLastError());
   }
   u64 hs[2];
@@ -1062,10 +1134,18 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
   const double heavy_vals = (double)hs[1], light_vals = std::max(0.0, lb - heavy_vals);
   const unsigned hgrid = (unsigned)((nh + kBlock / 64 - 1) / (kBlock / 64));
   if (x.timer) x.timer->credit("leapfrog_count", KC_LEAPFROG, (double)nd * sizeof(K) + light_vals * 4.0, 0);
+  // the heavy rows' hit words: ceil(shorter list / 64) per row
+  DBuf<u32> hnw(x.pool, nh ? nh : 1);
+  DBuf<u64> hwoff(x.pool, nh ? nh : 1), hwtot(x.pool, 1), hrec(x.pool, 2 * (nh ? nh : 1)), hoff(x.pool, nh ? nh : 1);
+  DBuf<u64> hmask;
   if (nh) {
+    k_tri_heavy_words<<<gridj(nh), kBlock, 0, x.stream>>>(hshort.get(), nh, hnw.get());
+    SK_HIP(hipGetLastError());
+    scan_u32(x, hnw.get(), hwoff.get(), nh, hwtot.get());
+    hmask.reset(x.pool, std::max<u64>(read_u64(x, hwtot.get()), 1));
     Scope sc(x.timer, "leapfrog_count_heavy", KC_LEAPFROG, (double)nh * 4.0 + heavy_vals * 4.0, nh);
     k_tri_count_heavy<K><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
-                                                         cnt.get());
+                                                         hwoff.get(), hmask.get(), hrec.get(), cnt.get());
     SK_HIP(hipGetLastError());
   }
   scan_u32(x, cnt.get(), off.get(), nd, tot.get());
@@ -1085,13 +1165,18 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
     SK_HIP(hipGetLastError());
   }
   if (nh) {
-    Scope sc(x.timer, "leapfrog_emit_heavy", KC_LEAPFROG, (double)nh * 4.0 + heavy_vals * 4.0, nh);
+    k_tri_heavy_off<<<gridj(nh), kBlock, 0, x.stream>>>(heavy.get(), nh, off.get(), hoff.get());
+    SK_HIP(hipGetLastError());
+    // per heavy row its record (16), word and output offsets (16), list
+    // length (4); per hit its shorter-list value (4) and the output written
+    Scope sc(x.timer, "leapfrog_emit_heavy", KC_LEAPFROG, 0.0, nh);
+    const unsigned egrid = (unsigned)((nh + 64 * (kBlock / 64) - 1) / (64 * (kBlock / 64)));
     if (k128_out)
-      k_tri_emit_heavy<K, u128><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
-                                                                 off.get(), *o);
+      k_tri_emit_heavy<u128><<<egrid, kBlock, 0, x.stream>>>(adj2, adj3, nh, hrec.get(), hshort.get(), hwoff.get(),
+                                                             hoff.get(), hmask.get(), *o);
     else
-      k_tri_emit_heavy<K, u64><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
-                                                                off.get(), *o);
+      k_tri_emit_heavy<u64><<<egrid, kBlock, 0, x.stream>>>(adj2, adj3, nh, hrec.get(), hshort.get(), hwoff.get(),
+                                                            hoff.get(), hmask.get(), *o);
     SK_HIP(hipGetLastError());
   }
   return total;

@@ -156,13 +156,15 @@ def test_triangles_vectors(ctx):
     assert len(db.relation("tri")) == b * math.comb(h, 2) + math.comb(h, 3)
 
 
-def test_triangles_long_lists(ctx):
-    """Hubs of 1,500 leaves each (their rows' two lists hold more than 512
-    values: one wave per row, binary-search membership, ballot-ordered
-    writes) beside a random graph whose rows take the one-lane merge; the
-    tri relation equals the oracle's and the count the closed form plus
-    brute force."""
-    h, b = 5, 1500
+@pytest.mark.parametrize("b", [1500, 2500])
+def test_triangles_long_lists(ctx, b):
+    """Hubs of b leaves each (their rows' two lists hold more than 512
+    values: one wave per row, binary-search membership into the longer list
+    held in LDS up to 2,048 values (b = 1500) or searched in global memory
+    (b = 2500), hit words expanded by the emit pass) beside a random graph
+    whose rows take the one-lane merge; the tri relation equals the oracle's
+    and the count the closed form plus brute force."""
+    h = 5
     edges = [(i, j) for i in range(h) for j in range(i + 1, h)] + [(hub, h + k) for k in range(b) for hub in range(h)]
     rnd = random.Random(17)
     base = h + b
