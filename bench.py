@@ -505,8 +505,9 @@ def bench_sort(args):
     """Device-resident LSD radix sort (the engine's sort primitive) of 2^26
     packed two-column keys (44 significant bits: 22-bit ids), keys alone and
     with i64 weights, under the context's autotuner.  Algorithmic bytes of a
-    pass: the histogram reads every key, the scatter reads and writes every
-    record."""
+    sort of n records of r bytes with k-byte keys in P passes:
+    n*k + P*2*n*r (one histogram read of the keys, a read and a write of
+    every record per pass), whichever candidate runs."""
     import torch
     import strata_k as sk
     n = 1 << args.sort_log2
@@ -544,7 +545,7 @@ def bench_sort(args):
         t = float(np.median(ms))
         # every candidate pinned in turn, the same keys (stderr)
         per_cand = {}
-        for c in range(8):
+        for c in range(8):  # every tuner candidate (sk_ctx_force_radix)
             ctx.force_radix(c)
             cm = []
             for _ in range(3):
@@ -559,8 +560,8 @@ def bench_sort(args):
         ctx.force_radix(-1)
         log(f"weighted={weighted}: ms per sort by candidate {per_cand}")
         rec = 8 + (8 if weighted else 0)
-        pass_bytes = n * 8 + 2 * rec * n
-        gbs = pass_bytes * passes / (t * 1e-3) / 1e9
+        alg_bytes = n * 8 + passes * 2 * rec * n
+        gbs = alg_bytes / (t * 1e-3) / 1e9
         out.append({"weighted": weighted, "n": n, "key_bits": 44, "digit_bits": choice["digit_bits"],
                     "items_per_thread": choice["items_per_thread"], "passes": passes, "ms_per_sort": round(t, 3),
                     "keys_per_s": round(n / (t * 1e-3), 1), "alg_gbps": round(gbs, 1),

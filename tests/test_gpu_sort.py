@@ -47,6 +47,53 @@ def test_sorts_like_host(radix_ctx):
         assert (sk.sort_u64(few, radix_ctx) == np.sort(few)).all(), n
 
 
+def test_many_tiles_and_weights(radix_ctx):
+    """5M records (over a thousand tiles and thousands of blocks' digit
+    counts in the offset scan) with duplicate-heavy keys and weights: the
+    per-key minimum equals the host's."""
+    r = np.random.default_rng(55)
+    n = 5_000_000
+    a = r.integers(0, 1 << 20, size=n, dtype=np.uint32)
+    b = r.integers(0, 64, size=n, dtype=np.uint32)
+    w = r.integers(-10**9, 10**9, size=n, dtype=np.int64)
+    import ctypes as C
+    oa, ob, ow = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.int64)
+    m = C.c_uint64()
+    code = radix_ctx.L.sk_sort_min_pairs(radix_ctx.h, a.ctypes.data, b.ctypes.data, w.ctypes.data, n,
+                                         oa.ctypes.data, ob.ctypes.data, ow.ctypes.data, C.byref(m))
+    assert code == 0
+    k = a.astype(np.uint64) << np.uint64(32) | b.astype(np.uint64)
+    order = np.lexsort((w, k))
+    ks, ws = k[order], w[order]
+    first = np.r_[True, ks[1:] != ks[:-1]]
+    u = m.value
+    assert u == int(first.sum())
+    assert (oa[:u].astype(np.uint64) << np.uint64(32) | ob[:u].astype(np.uint64) == ks[first]).all()
+    assert (ow[:u] == ws[first]).all()
+
+
+def test_wide_keys_every_candidate(radix_ctx):
+    """A Trop relation of three 30-bit columns (90-bit, u128 keys) with
+    duplicates, normalised on load (device sort, per-key minimum), equals a
+    host lexsort."""
+    from strata_k.core_ir import TROP, CoreProgram, pred
+    r = np.random.default_rng(66)
+    n = 400_000
+    cols = [r.integers(0, 1 << 30, size=n, dtype=np.uint32) for _ in range(2)]
+    cols.append(r.integers(0, 50, size=n, dtype=np.uint32))
+    cols[0][: n // 2] = cols[0][n // 2:]  # duplicates across halves
+    cols[1][: n // 2] = cols[1][n // 2:]
+    cols[2][: n // 2] = cols[2][n // 2:]
+    w = r.integers(0, 1 << 40, size=n, dtype=np.int64)
+    db = radix_ctx.run_semi_naive(CoreProgram([pred("p", 3, TROP)], [], 1), bulk={"p": (cols, w, None)})
+    rel = db.relation("p")
+    order = np.lexsort((w, cols[2], cols[1], cols[0]))
+    c0, c1, c2, ws = cols[0][order], cols[1][order], cols[2][order], w[order]
+    first = np.r_[True, (c0[1:] != c0[:-1]) | (c1[1:] != c1[:-1]) | (c2[1:] != c2[:-1])]
+    assert (rel.vals[:, 0] == c0[first]).all() and (rel.vals[:, 1] == c1[first]).all()
+    assert (rel.vals[:, 2] == c2[first]).all() and (rel.w == ws[first]).all()
+
+
 def test_unique_like_host(radix_ctx):
     rng = Lcg(0xBEEF)
     for n in [0, 1, 2, 4, 9, 255, 256, 257, 1000, 5000, 90_000]:
@@ -91,8 +138,8 @@ def test_tuner_picks_a_candidate(ctx):
         for wide in (False, True):
             for weighted in (False, True):
                 c = ctx.radix_choice(1 << 20, 40, wide=wide, weighted=weighted)
-                # the tuner's eight candidates (digit bits, items per thread for
-                # u64 keys; half the items for u128 keys)
+                # the tuner's candidates (digit bits, items per thread for u64
+                # keys; half the items for u128 keys)
                 cands = [(8, 16), (8, 8), (7, 16), (7, 8), (6, 16), (6, 8), (8, 32), (6, 32)]
                 if wide:
                     cands = [(d, i // 2) for d, i in cands]
