@@ -444,8 +444,12 @@ sk_status sk_batch_kernel_record(const sk_batch* b, uint32_t i, char* name, size
  *   cost(S,Y) :- src(S), edge(S,Y).  cost(S,Z) :- cost(S,Y), edge(Y,Z).
  * optionally filtered to a target set
  *   ans(S,Z) :- cost(S,Z), tgt(Z).
- * Queries are admitted while their delta/full buffers fit the HBM budget and
- * run together on the device (continuous batching). */
+ * Every source runs as an instance in a lane of a 64-lane group of dense
+ * cells; the number of groups comes from query_budget_bytes (0: half the
+ * context's budget).  Lanes go to sources as they free up, so a query of any
+ * number of sources runs, in pieces when it has more sources than free
+ * lanes, and all instances advance together one round per step (continuous
+ * batching).  Answers are the same alone, in any mix and in any pieces. */
 typedef struct sk_server sk_server;
 sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst,
                            const int64_t* w, uint64_t n_edges, uint64_t query_budget_bytes,
@@ -464,6 +468,23 @@ sk_status sk_server_take(sk_server* s, uint64_t ticket, uint32_t* src, uint32_t*
                          int64_t* w);
 /* Release a finished answer without copying. */
 sk_status sk_server_release(sk_server* s, uint64_t ticket);
+/* Streaming, one source at a time in source order, while the query runs:
+ * SK_OK with the next source's position in the sorted source set, its id and
+ * its row count once its answer is ready (SK_ERR_NOT_READY before); after
+ * the last source, *index = the number of sources and *n_rows = 0.
+ * sk_server_take_source copies that source's rows (Z, w), sorted by Z, and
+ * releases them (with dst and w null: releases them without a copy).
+ * sk_server_take then returns only the sources not yet taken. */
+sk_status sk_server_poll_source(sk_server* s, uint64_t ticket, uint32_t* index, uint32_t* source,
+                                uint64_t* n_rows);
+sk_status sk_server_take_source(sk_server* s, uint64_t ticket, uint32_t* dst, int64_t* w);
+/* A query whose ready answers not yet taken hold more than `bytes` of device
+ * memory gets no new lanes until they are taken (0, the default: no limit). */
+sk_status sk_server_set_hold(sk_server* s, uint64_t bytes);
+/* Counters: out[0] steps, [1] pull rounds (one per group), [2] push joins,
+ * [3] lanes, [4] 1 while the cells are narrow, [5] groups, [6] distinct
+ * edges; the first n are written. */
+sk_status sk_server_stats(const sk_server* s, uint64_t* out, uint32_t n);
 
 #ifdef __cplusplus
 }

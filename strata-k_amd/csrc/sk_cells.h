@@ -310,6 +310,11 @@ void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bi
 template <class K>
 u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                        u64* push_cost, const DevErr* status, DevErr* status_out);
+// Touched compaction (from cell 0) writing row records (cd.rowrec8 and/or
+// cd.rowrec, rows of 2^rlbits <= 64 cells) and, per listed cell, the packed
+// word (*dpack; narrow cells) or the weight and the lane (*w, *lane), no keys.
+// Outputs are (re)allocated to the returned count.
+u64 cells_compact_rows(const Exec& x, CellDesc cd, u64 ncells, DBuf<u32>* dpack, DBuf<i64>* w, DBuf<u8>* lane);
 // Mark the cells of n packed keys touched (the keys' cells are in the layout).
 template <class K>
 void cells_mark_touched(const Exec& x, const CellDesc& cd, const K* keys, u64 n, int arity, int bits);

@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
   }
   u32 tot;
   u32 o = cells_block_scan((u32)__popc(b), tmp, &tot);
-  if (touched_only && cd.rowrec) {
+  if (touched_only && (cd.rowrec || cd.rowrec8)) {
     // row records (w0 == 0, so word wi sits in lane wi % 64 of its wave)
     const u64 g = off[blockIdx.x] + o;
     const u32 hi = __shfl_down(b, 1, 64);
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
           ulonglong2 v;
           v.x = (u64)b | ((u64)hi << 32);
           v.y = g;
-          *(ulonglong2*)(cd.rowrec + 2 * r) = v;
+          if (cd.rowrec) *(ulonglong2*)(cd.rowrec + 2 * r) = v;
           if (cd.rowrec8) cd.rowrec8[r] = g | ((u64)(__popc(b) + __popc(hi)) << 40);
         }
       } else {
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
           ulonglong2 v;
           v.x = (b >> sh) & m;
           v.y = g + (u32)__popc(b & ((1u << sh) - 1u));
-          *(ulonglong2*)(cd.rowrec + 2 * r) = v;
+          if (cd.rowrec) *(ulonglong2*)(cd.rowrec + 2 * r) = v;
           if (cd.rowrec8) cd.rowrec8[r] = v.y | ((u64)__popc((u32)v.x) << 40);
         }
       }
@@ -379,17 +379,19 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
   const u64 cbase = w0 * 32 + (u64)blockIdx.x * kCellsPerBlock;
   for (u32 j = threadIdx.x; j < tot; j += kBlock) {
     const u64 c = cbase + sc[j];
-    K key = 0;
-    u32 pid = 0;
-    for (int q = 0; q < cd.ncols; q++) {
-      u64 r = (c >> cd.shift[q]) & ((1ull << cd.ebits[q]) - 1);
-      u32 code = cd.pdom_off[q]  ? cd.pdom_codes[q][cd.pdom_off[q][pid] + r]
-                 : cd.dom_list[q] ? cd.dom_list[q][r]
-                                  : (u32)r;
-      if (q == 0) pid = code;
-      key = (key << bits) | (K)code;
+    if (keys) {
+      K key = 0;
+      u32 pid = 0;
+      for (int q = 0; q < cd.ncols; q++) {
+        u64 r = (c >> cd.shift[q]) & ((1ull << cd.ebits[q]) - 1);
+        u32 code = cd.pdom_off[q]  ? cd.pdom_codes[q][cd.pdom_off[q][pid] + r]
+                   : cd.dom_list[q] ? cd.dom_list[q][r]
+                                    : (u32)r;
+        if (q == 0) pid = code;
+        key = (key << bits) | (K)code;
+      }
+      keys[base + j] = key;
     }
-    keys[base + j] = key;
     if (touched_only && cd.dlane) cd.dlane[base + j] = (u8)(c & ((1u << cd.rlbits) - 1u));
     if (cd.trop) {
       const u64 e = cell_peek(cd, c);
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
       if (!cell_code_absent(cd, e)) wv = cell_code_value(cd, e);
       else if (!cd.narrow && (cd.maxf[c >> 5] & (1u << (c & 31)))) wv = LLONG_MAX;
       else f = 1;
-      w[base + j] = wv;
+      if (w) w[base + j] = wv;
       if (inf) inf[base + j] = f;
       if (touched_only && cd.dpack) cd.dpack[base + j] = ((u32)wv << 6) | (u32)(c & ((1u << cd.rlbits) - 1u));
     }
@@ -582,6 +584,56 @@ u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, 
   if (status) std::memcpy((void*)status_out, &hv[2], sizeof(DevErr));
   const u64 n = hv[0];
   if (x.timer) x.timer->credit("cells_write", KC_DEDUP, (sizeof(K) + (cd.trop ? 16.0 + (inf ? 1.0 : 0) : 0)) * n, 0);
+  return n;
+}
+
+// Touched compaction (from cell 0) of a layout whose rows are 2^rlbits <= 64
+// cells, writing the row records (cd.rowrec8, and cd.rowrec when set) and
+// per listed cell only what is asked: the packed word ((weight << 6) | lane,
+// narrow cells below narrow_lim) into *dpack, or the weight and the lane into
+// *w and *lane.  The outputs are (re)allocated to the count, which is
+// returned; no keys are written.
+u64 cells_compact_rows(const Exec& x, CellDesc cd, u64 ncells, DBuf<u32>* dpack, DBuf<i64>* w, DBuf<u8>* lane) {
+  if (!cd.trop || (!cd.rowrec && !cd.rowrec8) || cd.rlbits < 0 || cd.rlbits > 6 || cd.dbits)
+    throw std::runtime_error("cells_compact_rows: Trop cells with row records of at most 64 cells");
+  if (dpack && !cd.narrow) throw std::runtime_error("cells_compact_rows: packed words need narrow cells");
+  const u64 nwords = (ncells + 31) / 32;
+  const u64 nblocks = (nwords + kBlock - 1) / kBlock;
+  if (nblocks == 0) return 0;
+  DBuf<u32> cnt(x.pool, nblocks);
+  DBuf<u64> off(x.pool, nblocks);
+  DBuf<u64> tot(x.pool, 2);
+  const double scan_bytes = 4.0 * nwords;
+  {
+    Scope sc(x.timer, "cells_count", KC_DEDUP, scan_bytes, ncells);
+    k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, cnt.get(), nullptr);
+    SK_HIP(hipGetLastError());
+  }
+  u64 n = 0;
+  if (nblocks <= kScanOneMax) {
+    scan_u32_one(x, cnt.get(), off.get(), nblocks, tot.get(), nullptr);
+    n = read_u64(x, tot.get());
+  } else {
+    scan_u32(x, cnt.get(), off.get(), nblocks, tot.get());
+    n = read_u64(x, tot.get());
+  }
+  cd.dpack = nullptr;
+  cd.dlane = nullptr;
+  if (dpack) {
+    dpack->reset(x.pool, n ? n : 1);
+    cd.dpack = dpack->get();
+  }
+  if (w) w->reset(x.pool, n ? n : 1);
+  if (lane) {
+    lane->reset(x.pool, n ? n : 1);
+    cd.dlane = lane->get();
+  }
+  // the row records are rewritten even when nothing was touched
+  const double rec = (dpack ? 4.0 : 0) + (w ? 8.0 : 0) + (lane ? 1.0 : 0);
+  Scope sc(x.timer, "cells_write", KC_DEDUP, scan_bytes + rec * n + 8.0 * (double)cd.rnrows, ncells);
+  k_cells_write<u64><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, off.get(), 0, nullptr,
+                                                                 w ? w->get() : nullptr, nullptr);
+  SK_HIP(hipGetLastError());
   return n;
 }
 

@@ -213,6 +213,10 @@ SIGNATURES = [
     ("sk_server_poll", C.c_int, [_VP, _U64, C.POINTER(_U64), C.POINTER(_U64), C.POINTER(C.c_double)]),
     ("sk_server_take", C.c_int, [_VP, _U64, _VP, _VP, _VP]),
     ("sk_server_release", C.c_int, [_VP, _U64]),
+    ("sk_server_poll_source", C.c_int, [_VP, _U64, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U64)]),
+    ("sk_server_take_source", C.c_int, [_VP, _U64, _VP, _VP]),
+    ("sk_server_set_hold", C.c_int, [_VP, _U64]),
+    ("sk_server_stats", C.c_int, [_VP, _VP, _U32]),
     # workload generators (host code in the same library)
     ("sk_wl_uniform_graph", _U64, [_U64, _U32, _U64, _VP, _VP]),
     ("sk_wl_tri_hubs", _U64, [_U32, _U32, _U32, _U32, _VP, _VP]),
@@ -1088,6 +1092,46 @@ class Server:
 
     def release(self, ticket: int):
         self.L.sk_server_release(self.h, ticket)
+
+    def poll_source(self, ticket: int):
+        """Streaming: None while the next source's answer runs; else
+        dict(index, source, rows) (index == number of sources once every
+        source was taken)."""
+        i, src, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        code = self.L.sk_server_poll_source(self.h, ticket, C.byref(i), C.byref(src), C.byref(n))
+        if code == 105:
+            return None
+        if code:
+            _raise(code)
+        return dict(index=i.value, source=src.value, rows=n.value)
+
+    def take_source(self, ticket: int, n: int, discard: bool = False):
+        """The next source's rows (Z, w), sorted by Z, then released on the
+        device (with discard, released without a copy)."""
+        if discard:
+            code = self.L.sk_server_take_source(self.h, ticket, None, None)
+            if code:
+                _raise(code)
+            return None
+        d = np.empty(max(1, n), dtype=np.uint32)
+        w = np.empty(max(1, n), dtype=np.int64)
+        code = self.L.sk_server_take_source(self.h, ticket, d.ctypes.data, w.ctypes.data)
+        if code:
+            _raise(code)
+        return d[:n], w[:n]
+
+    def set_hold(self, nbytes: int):
+        code = self.L.sk_server_set_hold(self.h, nbytes)
+        if code:
+            _raise(code)
+
+    def stats(self) -> dict:
+        out = np.zeros(7, dtype=np.uint64)
+        code = self.L.sk_server_stats(self.h, out.ctypes.data, 7)
+        if code:
+            _raise(code)
+        keys = ("steps", "pull_rounds", "push_joins", "lanes", "narrow", "groups", "edges")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def free(self):
         if self.h and self.ctx.h:

@@ -253,6 +253,173 @@ def test_server_failures_stay_per_query(ctx):
     q = dict(sources=np.array([0, 2], np.uint32), targets=np.empty(0, np.uint32))
     oa, ob, ow, oit = _oracle_answer(src, dst, w, q)
     assert (a == oa).all() and (b == ob).all() and (ww == ow).all() and res["iterations"] == oit
+    # the failed instances' lanes are reset and handed out again
+    q = dict(sources=np.array([0, 1, 2, 3, 21], np.uint32), targets=np.empty(0, np.uint32))
+    again = srv.submit(q["sources"])
+    for _ in range(100):
+        a, wt = srv.step()
+        if a == 0 and wt == 0:
+            break
+    res = srv.poll(again)
+    a, b, ww = srv.take(again, res["rows"])
+    oa, ob, ow, oit = _oracle_answer(src, dst, w, q)
+    assert (a == oa).all() and (b == ob).all() and (ww == ow).all() and res["iterations"] == oit
+    srv.free()
+
+
+def _group_bytes(s, d):
+    """Bytes the server charges per 64-lane group for this edge list: wide
+    cells, touched and maxf bitmaps, row records and the packed-delta
+    reserve (2^zbits rows of 64 cells, zbits = max(id bits, 6))."""
+    n = int(max(s.max(), d.max())) + 1
+    b = 1
+    while (1 << b) < n:
+        b += 1
+    z = max(b, 6)
+    gc = 1 << (z + 6)
+    return gc * 8 + gc // 4 + (1 << z) * 8 + gc * 4
+
+
+def _dijkstra_answer(s, d, w, q):
+    n = int(max(s.max(), d.max())) + 1
+    W, _, mh = O.cost_dijkstra(n, s, d, w, q["sources"])
+    S, Z, WW = O.cost_rows(W, q["sources"])
+    if len(q["targets"]):
+        keep = np.isin(Z, q["targets"])
+        S, Z, WW = S[keep], Z[keep], WW[keep]
+    return S, Z, WW, mh
+
+
+def _serve_all(srv, tickets, max_steps=100_000):
+    done = {}
+    for _ in range(max_steps):
+        srv.step()
+        for t in tickets:
+            if t not in done:
+                r = srv.poll(t)
+                if r is not None:
+                    done[t] = (r, srv.take(t, r["rows"]))
+        if len(done) == len(tickets):
+            return done
+    raise AssertionError("queries did not finish")
+
+
+def test_server_sources_beyond_lanes(ctx):
+    """A server of one 64-lane group: queries of 150 and 300 sources (the
+    second filtered to 64 targets) run in pieces as lanes free up, beside
+    small queries arriving with them; every answer equals the Dijkstra
+    checker's and every round count its largest fewest-hops count."""
+    s, d, w, n = _server_graph()
+    nn = int(max(s.max(), d.max())) + 1
+    srv = sk.Server(ctx, s, d, w, budget_bytes=_group_bytes(s, d) + 4096, max_batch=4)
+    assert srv.stats()["lanes"] == 64
+    rng = np.random.default_rng(77)
+    qs = [dict(sources=np.sort(rng.choice(nn, 150, replace=False)).astype(np.uint32),
+               targets=np.empty(0, np.uint32)),
+          dict(sources=np.sort(rng.choice(nn, 300, replace=False)).astype(np.uint32),
+               targets=np.sort(rng.choice(nn, 64, replace=False)).astype(np.uint32))]
+    qs += wl.config5_queries(seed=78, n_queries=5, n_nodes=nn, max_sources=9, n_targets=16, filter_frac=0.4)
+    tickets = [srv.submit(q["sources"], q["targets"]) for q in qs]
+    done = _serve_all(srv, tickets)
+    for q, t in zip(qs, tickets):
+        r, (gs, gd, gw) = done[t]
+        S, Z, WW, mh = _dijkstra_answer(s, d, w, q)
+        assert len(gs) == len(S)
+        assert (gs == S).all() and (gd == Z).all() and (gw == WW).all()
+        assert r["iterations"] == mh
+    st = srv.stats()
+    srv.free()
+    print("sources beyond lanes:", st)
+
+
+def test_server_streams_sources(ctx):
+    """Answers taken source by source while the query runs (the server
+    withholds lanes while an untaken answer is held): in source order, each
+    equal to the checker's rows for that source."""
+    s, d, w, n = _server_graph()
+    nn = int(max(s.max(), d.max())) + 1
+    srv = sk.Server(ctx, s, d, w, budget_bytes=_group_bytes(s, d) + 4096, max_batch=2)
+    srv.set_hold(1)
+    rng = np.random.default_rng(79)
+    q = dict(sources=np.sort(rng.choice(nn, 200, replace=False)).astype(np.uint32), targets=np.empty(0, np.uint32))
+    t = srv.submit(q["sources"])
+    S, Z, WW, mh = _dijkstra_answer(s, d, w, q)
+    got = 0
+    for _ in range(100_000):
+        srv.step()
+        fin = False
+        while True:
+            p = srv.poll_source(t)
+            if p is None:
+                break
+            if p["index"] == len(q["sources"]):
+                fin = True
+                break
+            assert p["index"] == got and p["source"] == q["sources"][got]
+            z, ww = srv.take_source(t, p["rows"])
+            keep = S == p["source"]
+            assert (z == Z[keep]).all() and (ww == WW[keep]).all(), got
+            got += 1
+        if fin:
+            break
+    assert got == len(q["sources"])
+    assert srv.poll(t)["iterations"] == mh
+    srv.free()
+
+
+def test_server_pull_rounds(ctx):
+    """64 sources fill one group; rounds whose push join would emit at least
+    |E| rows pull all 64 lanes in one pass over the in-edges.  Answers equal
+    the checker's and pull rounds ran."""
+    s, d, w, n = _server_graph()
+    nn = int(max(s.max(), d.max())) + 1
+    srv = sk.Server(ctx, s, d, w, max_batch=4)
+    rng = np.random.default_rng(80)
+    qs = [dict(sources=np.sort(rng.choice(nn, 64, replace=False)).astype(np.uint32), targets=np.empty(0, np.uint32)),
+          dict(sources=np.sort(rng.choice(nn, 40, replace=False)).astype(np.uint32),
+               targets=np.sort(rng.choice(nn, 64, replace=False)).astype(np.uint32))]
+    tickets = [srv.submit(q["sources"], q["targets"]) for q in qs]
+    done = _serve_all(srv, tickets)
+    for q, t in zip(qs, tickets):
+        r, (gs, gd, gw) = done[t]
+        S, Z, WW, mh = _dijkstra_answer(s, d, w, q)
+        assert (gs == S).all() and (gd == Z).all() and (gw == WW).all()
+        assert r["iterations"] == mh
+    st = srv.stats()
+    srv.free()
+    assert st["narrow"] == 1 and st["pull_rounds"] > 0, st
+
+
+def test_server_widens_past_2_26(ctx):
+    """Edge weights below 2^26 start narrow cells with pull rounds; path sums
+    pass 2^26 mid-run, the cells widen in place and the round is applied
+    again: answers (and those of a query after the widening) equal the
+    checker's."""
+    rng = np.random.default_rng(81)
+    n = 3000
+    cs = np.arange(n - 1, dtype=np.uint32)
+    cw = (1 << 24) + rng.integers(0, 1000, n - 1)  # a chain whose sums pass 2^26 after 4 hops
+    # sparse shortcuts of weight >= 2^23: far nodes stay many hops away, so
+    # the sums of the chain's later rounds reach 2^26
+    xs = rng.integers(0, n, 3_000).astype(np.uint32)
+    xd = rng.integers(0, n, 3_000).astype(np.uint32)
+    xw = rng.integers(1 << 23, 1 << 24, 3_000)
+    s = np.concatenate([cs, xs]).astype(np.uint32)
+    d = np.concatenate([cs + 1, xd]).astype(np.uint32)
+    w = np.concatenate([cw, xw]).astype(np.int64)
+    srv = sk.Server(ctx, s, d, w, max_batch=4)
+    assert srv.stats()["narrow"] == 1
+    qs = [dict(sources=np.sort(rng.choice(n, 64, replace=False)).astype(np.uint32), targets=np.empty(0, np.uint32)),
+          dict(sources=np.array([0, 7], np.uint32), targets=np.empty(0, np.uint32))]
+    t0 = srv.submit(qs[0]["sources"])
+    d0 = _serve_all(srv, [t0])
+    assert srv.stats()["narrow"] == 0
+    t1 = srv.submit(qs[1]["sources"])
+    d1 = _serve_all(srv, [t1])
+    for q, (r, (gs, gd, gw)) in zip(qs, [d0[t0], d1[t1]]):
+        S, Z, WW, mh = _dijkstra_answer(s, d, w, q)
+        assert (gs == S).all() and (gd == Z).all() and (gw == WW).all()
+        assert r["iterations"] == mh
     srv.free()
 
 
