@@ -469,6 +469,7 @@ def bench_cfg5(args, emit=True):
                 log(f"  served {len(done)}/{len(qs)} queries, {time.perf_counter() - t0:.0f}s")
         total = time.perf_counter() - t0
         stats[max_batch if not open_loop else "open"] = srv.stats()
+        log(f"  server counters: {srv.stats()}")
         srv.free()
         return total, done, answers
 
@@ -526,7 +527,7 @@ def bench_cfg5(args, emit=True):
         "plain_run_sources_per_s": round(plain_src / t_plain, 1) if n_plain else None,
         "plain_queries_timed": n_plain,
         "served_vs_plain_runs": round((n_src / t_cap) / (plain_src / t_plain), 2) if n_plain else None,
-        "server_rounds": {"steps": st["steps"], "pull_rounds": st["pull_rounds"], "push_joins": st["push_joins"]},
+        "server_rounds": {k: st[k] for k in ("steps", "pull_rounds", "push_joins", "pull_pairs", "push_pairs")},
         "same_rows_and_rounds_alone_batched_and_plain": bool(exact),
         "answers_compared_with_plain_runs": len(answers) if checked is not None else 0,
         "answer_contents_equal": checked,

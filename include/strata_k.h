@@ -483,7 +483,8 @@ sk_status sk_server_take_source(sk_server* s, uint64_t ticket, uint32_t* dst, in
 sk_status sk_server_set_hold(sk_server* s, uint64_t bytes);
 /* Counters: out[0] steps, [1] pull rounds (one per group), [2] push joins,
  * [3] lanes, [4] 1 while the cells are narrow, [5] groups, [6] distinct
- * edges; the first n are written. */
+ * edges, [7] (edge, delta cell) pairs of the delta rounds that pulled, [8]
+ * of those that pushed; the first n are written. */
 sk_status sk_server_stats(const sk_server* s, uint64_t* out, uint32_t n);
 
 #ifdef __cplusplus

@@ -103,6 +103,16 @@ struct CellDesc {
   // output count of the push join of the new delta against that CSR.
   const u64* pc_csr = nullptr;
   u64 pc_nv = 0;
+  // Groups of 2^grp_shift rows of 64 cells (rlbits == 6, groups of at least
+  // 256 rows): with grp_mask set, a touched compaction from cell 0 also ORs
+  // each group's touched lanes into grp_mask[g], adds its push cost (per
+  // touched cell, the degree in pc_csr of its row's index within the group)
+  // to grp_cost[g] (both arrays zeroed by the caller), and writes the list
+  // position of the group's first cell to grp_start[g].
+  u64* grp_mask = nullptr;
+  u64* grp_cost = nullptr;
+  u64* grp_start = nullptr;
+  int grp_shift = 0;
   // Bool relations with a word-parallel pull rule: a touched compaction
   // (from cell 0) also copies every touched word into `dbits` (the delta as
   // a bitmap) and stamps each row of 2^(5 + dwshift) cells that holds a

@@ -369,6 +369,43 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
       }
     }
   }
+  if (touched_only && cd.grp_mask) {
+    // per-group lanes and push cost of this block's rows (one group)
+    __shared__ u64 gm[kBlock / 64], gp[kBlock / 64];
+    const u32 hi = __shfl_down(b, 1, 64);
+    u64 m = 0, pc = 0;
+    if (wl < nwords && !(wi & 1)) {
+      const u64 r = wi >> 1;
+      if (r < cd.rnrows) {
+        m = (u64)b | ((u64)hi << 32);
+        const u64 z = r & ((1ull << cd.grp_shift) - 1);
+        if (m && z < cd.pc_nv) pc = (u64)__popcll(m) * (cd.pc_csr[z + 1] - cd.pc_csr[z]);
+        if (!z) cd.grp_start[r >> cd.grp_shift] = off[blockIdx.x] + o;
+      }
+    }
+#pragma unroll
+    for (int q = 32; q > 0; q >>= 1) {
+      m |= __shfl_xor(m, q, 64);
+      pc += __shfl_xor(pc, q, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      gm[threadIdx.x >> 6] = m;
+      gp[threadIdx.x >> 6] = pc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u64 bm = 0, bp = 0;
+#pragma unroll
+      for (int q = 0; q < kBlock / 64; q++) {
+        bm |= gm[q];
+        bp += gp[q];
+      }
+      const u64 g = ((u64)blockIdx.x * kBlock / 2) >> cd.grp_shift;
+      if (bp) atomicAdd((unsigned long long*)&cd.grp_cost[g], (unsigned long long)bp);
+      if (bm && (bm & ~__hip_atomic_load(&cd.grp_mask[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+        atomicOr((unsigned long long*)&cd.grp_mask[g], (unsigned long long)bm);
+    }
+  }
   while (b) {
     int bit = __ffs(b) - 1;
     b &= b - 1;
@@ -596,6 +633,8 @@ u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, 
 u64 cells_compact_rows(const Exec& x, CellDesc cd, u64 ncells, DBuf<u32>* dpack, DBuf<i64>* w, DBuf<u8>* lane) {
   if (!cd.trop || (!cd.rowrec && !cd.rowrec8) || cd.rlbits < 0 || cd.rlbits > 6 || cd.dbits)
     throw std::runtime_error("cells_compact_rows: Trop cells with row records of at most 64 cells");
+  if (cd.grp_mask && (cd.rlbits != 6 || cd.grp_shift < 8 || !cd.pc_csr || !cd.grp_cost || !cd.grp_start))
+    throw std::runtime_error("cells_compact_rows: group statistics need rows of 64 cells, groups of >= 256 rows");
   if (dpack && !cd.narrow) throw std::runtime_error("cells_compact_rows: packed words need narrow cells");
   const u64 nwords = (ncells + 31) / 32;
   const u64 nblocks = (nwords + kBlock - 1) / kBlock;

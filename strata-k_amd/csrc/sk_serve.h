@@ -13,15 +13,6 @@
 
 namespace sk {
 
-// Per-step statistics of the compacted delta of rows [0, nrows) (groups of
-// 2^zbits rows): lane_flag[g * 64 + lane] = 1 for every instance holding a
-// delta cell (the array is not cleared here), gcost[g] += Σ over the group's
-// delta cells of the out-degree of their node in ecsr (the push join's
-// output count), gstart[g] = the delta position of the group's first
-// entry.  The lanes come from dpack (narrow) or dlane (wide).
-void srv_stats(const Exec& x, const u64* rowrec8, u64 nrows, int zbits, const u32* dpack, const u8* dlane,
-               const u64* ecsr, u64 n_nodes, u8* lane_flag, u64* gcost, u64* gstart);
-
 // Push table of group g's delta entries at positions [p0, p1) of the delta
 // list: columns (group, lane, node, instance = g * 64 + lane) and the
 // weight, entry p at row p - p0.  Weights come from dpack (narrow) or dw

@@ -40,33 +40,6 @@ __device__ __forceinline__ void srv_fill(const CellDesc& cd, u64 c, u64 fill, bo
   if (clear_maxf) atomicAnd(&cd.maxf[c >> 5], ~(1u << (c & 31)));
 }
 
-// One thread per row; the rows of a wave share their group (2^zbits is a
-// multiple of 64), so the push cost is summed per wave.
-__global__ __launch_bounds__(kBlock) void k_srv_stats(const u64* __restrict__ rowrec8, u64 nrows, int zbits,
-                                                      const u32* __restrict__ dpack, const u8* __restrict__ dlane,
-                                                      const u64* __restrict__ ecsr, u64 n_nodes,
-                                                      u8* __restrict__ lane_flag, u64* __restrict__ gcost,
-                                                      u64* __restrict__ gstart) {
-  const u64 r = (u64)blockIdx.x * kBlock + threadIdx.x;
-  const u64 g = r >> zbits;
-  u64 pc = 0;
-  if (r < nrows) {
-    const u64 rec = rowrec8[r];
-    if (!(r & ((1ull << zbits) - 1))) gstart[g] = rec & kPosMask;
-    const u32 cnt = (u32)(rec >> 40);
-    if (cnt) {
-      const u64 pos = rec & kPosMask;
-      const u64 z = r & ((1ull << zbits) - 1);
-      if (z < n_nodes) pc = (u64)cnt * (ecsr[z + 1] - ecsr[z]);
-      u8* lf = lane_flag + g * 64;
-      for (u32 k = 0; k < cnt; k++) lf[dpack ? (dpack[pos + k] & 63u) : dlane[pos + k]] = 1;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o, 64);
-  if ((threadIdx.x & 63) == 0 && pc) atomicAdd((unsigned long long*)&gcost[g], (unsigned long long)pc);
-}
-
 __global__ __launch_bounds__(kBlock) void k_srv_table(const u64* __restrict__ rowrec8, int zbits, u32 g,
                                                       const u32* __restrict__ dpack, const i64* __restrict__ dw,
                                                       const u8* __restrict__ dlane, u64 p0, u64 p1,
@@ -189,16 +162,6 @@ __global__ __launch_bounds__(kBlock) void k_srv_lane_write(CellDesc cd, u64 n_no
 }
 
 }  // namespace
-
-void srv_stats(const Exec& x, const u64* rowrec8, u64 nrows, int zbits, const u32* dpack, const u8* dlane,
-               const u64* ecsr, u64 n_nodes, u8* lane_flag, u64* gcost, u64* gstart) {
-  if (!nrows) return;
-  if (zbits < 6) throw std::runtime_error("srv_stats: groups of at least 64 rows");
-  Scope sc(x.timer, "srv_stats", KC_MISC, 8.0 * nrows, nrows);
-  k_srv_stats<<<grid_of(nrows), kBlock, 0, x.stream>>>(rowrec8, nrows, zbits, dpack, dlane, ecsr, n_nodes, lane_flag,
-                                                       gcost, gstart);
-  SK_HIP(hipGetLastError());
-}
 
 void srv_table(const Exec& x, const u64* rowrec8, int zbits, u32 g, const u32* dpack, const i64* dw,
                const u8* dlane, u64 p0, u64 p1, u32* tg, u32* tl, u32* ty, u32* ti, i64* tw) {

@@ -133,7 +133,7 @@ struct sk_server {
   u64 next_ticket = 1;
   u64 hold = 0;  // lanes are withheld from a query holding more untaken part bytes
   // lane groups
-  int zbits = 6;  // rows per group = 2^zbits >= n_nodes
+  int zbits = 8;  // rows per group = 2^zbits >= max(n_nodes, 256)
   int gbits = 1;
   u32 ngroups = 0;
   u64 gcells = 0;   // cells per group = 2^(zbits + 6)
@@ -154,6 +154,7 @@ struct sk_server {
   std::vector<u64> gcost, gstart;
   // counters
   u64 steps = 0, pulls = 0, pushes = 0;
+  u64 pull_cand = 0, push_out = 0;  // delta rounds' (edge, delta cell) pairs by pull and by push
 };
 
 namespace {
@@ -326,7 +327,14 @@ void widen_cells(sk_server* s, const Exec& x) {
   memset_async(x, s->widen.get(), 0, 4);
 }
 
-u64 pull_limit(u64 edges) { return edges; }
+// A group pulls when its push join would emit at least this many rows:
+// measured at RMAT scale 24 (263M edges), a pull costs ~5 ms plus ~3.7 ps
+// per (edge, delta cell) pair and a push ~60 ps per output row, even at
+// about a third of |E|.
+#ifndef SK_SRV_PULL_PCT
+#define SK_SRV_PULL_PCT 30
+#endif
+u64 pull_limit(u64 edges) { return edges / 100 * SK_SRV_PULL_PCT + edges % 100 * SK_SRV_PULL_PCT / 100; }
 
 void fail(Query& q, int code, i64 lhs = 0, i64 rhs = 0) {
   if (q.state != 1) return;
@@ -447,7 +455,7 @@ sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst
     }
     // lane groups from the byte budget (default: half the pool's budget);
     // a group's cells are sized for wide codes, narrow cells use half
-    s->zbits = std::max(s->bits, 6);
+    s->zbits = std::max(s->bits, 8);  // blocks of 256 rows stay within a group
     s->gcells = 1ull << (s->zbits + 6);
     const u64 rows = 1ull << s->zbits;
     const u64 per_group = s->gcells * 8 + s->gcells / 4 + rows * 8 + s->gcells * 4;
@@ -614,8 +622,13 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
       std::vector<u32> pushg;
       for (u32 g = 0; g < s->dhi; g++) {
         if (!s->gcost[g]) continue;  // no delta, or no out-edges from it
-        if (allow_pull && s->narrow && s->dpacked && s->gcost[g] >= pull_limit(s->n_edges)) pull_group(s, x, g);
-        else pushg.push_back(g);
+        if (allow_pull && s->narrow && s->dpacked && s->gcost[g] >= pull_limit(s->n_edges)) {
+          pull_group(s, x, g);
+          s->pull_cand += s->gcost[g];
+        } else {
+          pushg.push_back(g);
+          s->push_out += s->gcost[g];
+        }
       }
       push_groups(s, x, pushg);
     };
@@ -635,14 +648,29 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
     for (u32 i = 0; i < L; i++)
       if (s->lanes[i].q >= 0 || s->lanes[i].poisoned) hi = i / 64 + 1;
     const u64 rows = 1ull << s->zbits;
+    std::vector<u64> hmask;
     {
       DBuf<u32> np;
       DBuf<i64> nw;
       DBuf<u8> nl;
       u64 n = 0;
+      // with it, per group: the lanes holding a delta cell, the push cost
+      // (delta cells times their node's out-degree) and the first position
+      DBuf<u64> lm, gc, gs;
       if (hi) {
+        lm.reset(x.pool, hi);
+        gc.reset(x.pool, hi);
+        gs.reset(x.pool, hi);
+        memset_async(x, lm.get(), 0, (u64)hi * 8);
+        memset_async(x, gc.get(), 0, (u64)hi * 8);
         CellDesc cc = s->cd;
         cc.rnrows = (u64)hi * rows;
+        cc.pc_csr = s->ecsr.get();
+        cc.pc_nv = s->n_nodes;
+        cc.grp_shift = s->zbits;
+        cc.grp_mask = lm.get();
+        cc.grp_cost = gc.get();
+        cc.grp_start = gs.get();
         n = s->narrow ? cells_compact_rows(x, cc, (u64)hi * s->gcells, &np, nullptr, nullptr)
                       : cells_compact_rows(x, cc, (u64)hi * s->gcells, nullptr, &nw, &nl);
       }
@@ -652,27 +680,19 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
       s->dpacked = s->narrow;
       s->dn = n;
       s->dhi = hi;
+      std::fill(s->gcost.begin(), s->gcost.end(), 0);
+      std::fill(s->gstart.begin(), s->gstart.end(), 0);
+      hmask.assign(hi, 0);
+      if (hi) {
+        SK_HIP(hipMemcpyAsync(hmask.data(), lm.get(), (u64)hi * 8, hipMemcpyDeviceToHost, x.stream));
+        SK_HIP(hipMemcpyAsync(s->gcost.data(), gc.get(), (u64)hi * 8, hipMemcpyDeviceToHost, x.stream));
+        SK_HIP(hipMemcpyAsync(s->gstart.data(), gs.get(), (u64)hi * 8, hipMemcpyDeviceToHost, x.stream));
+        SK_HIP(hipStreamSynchronize(x.stream));
+      }
     }
-    std::vector<u8> hflag((u64)hi * 64, 0);
-    std::fill(s->gcost.begin(), s->gcost.end(), 0);
-    std::fill(s->gstart.begin(), s->gstart.end(), 0);
     std::vector<DevErr> herr;
     u32 bad = 0;
     {
-      DBuf<u8> flag;
-      DBuf<u64> gc, gs;
-      if (hi && s->dn) {
-        flag.reset(x.pool, (u64)hi * 64);
-        gc.reset(x.pool, hi);
-        gs.reset(x.pool, hi);
-        memset_async(x, flag.get(), 0, (u64)hi * 64);
-        memset_async(x, gc.get(), 0, (u64)hi * 8);
-        srv_stats(x, s->rowrec8.get(), (u64)hi * rows, s->zbits, s->dpacked ? s->dpack.get() : nullptr,
-                  s->dpacked ? nullptr : s->dlane.get(), s->ecsr.get(), s->n_nodes, flag.get(), gc.get(), gs.get());
-        SK_HIP(hipMemcpyAsync(hflag.data(), flag.get(), (u64)hi * 64, hipMemcpyDeviceToHost, x.stream));
-        SK_HIP(hipMemcpyAsync(s->gcost.data(), gc.get(), (u64)hi * 8, hipMemcpyDeviceToHost, x.stream));
-        SK_HIP(hipMemcpyAsync(s->gstart.data(), gs.get(), (u64)hi * 8, hipMemcpyDeviceToHost, x.stream));
-      }
       if (!s->narrow && hi) {
         herr.resize((u64)hi * 64);
         SK_HIP(hipMemcpyAsync(herr.data(), s->perr.get(), (u64)hi * 64 * sizeof(DevErr), hipMemcpyDeviceToHost,
@@ -705,7 +725,7 @@ sk_status sk_server_step(sk_server* s, uint32_t* n_active, uint32_t* n_waiting) 
       Lane& Ln = s->lanes[i];
       const u32 g = i / 64;
       const u64 bit = 1ull << (i % 64);
-      const bool d = hflag[i] != 0;
+      const bool d = (hmask[g] >> (i % 64)) & 1;
       if (Ln.q < 0) {
         if (Ln.poisoned && !d) clear[g] |= bit;  // its last delta died out
         continue;
@@ -924,9 +944,9 @@ sk_status sk_server_set_hold(sk_server* s, uint64_t bytes) {
 
 sk_status sk_server_stats(const sk_server* s, uint64_t* out, uint32_t n) {
   if (!s || (n && !out)) return SK_ERR_INVALID_ARGUMENT;
-  const uint64_t v[7] = {s->steps, s->pulls, s->pushes, (u64)s->ngroups * 64, s->narrow ? 1ull : 0ull,
-                         s->ngroups, s->n_edges};
-  for (uint32_t i = 0; i < n && i < 7; i++) out[i] = v[i];
+  const uint64_t v[9] = {s->steps,    s->pulls,      s->pushes,   (u64)s->ngroups * 64, s->narrow ? 1ull : 0ull,
+                         s->ngroups, s->n_edges, s->pull_cand, s->push_out};
+  for (uint32_t i = 0; i < n && i < 9; i++) out[i] = v[i];
   return SK_OK;
 }
 

@@ -1126,11 +1126,12 @@ class Server:
             _raise(code)
 
     def stats(self) -> dict:
-        out = np.zeros(7, dtype=np.uint64)
-        code = self.L.sk_server_stats(self.h, out.ctypes.data, 7)
+        out = np.zeros(9, dtype=np.uint64)
+        code = self.L.sk_server_stats(self.h, out.ctypes.data, 9)
         if code:
             _raise(code)
-        keys = ("steps", "pull_rounds", "push_joins", "lanes", "narrow", "groups", "edges")
+        keys = ("steps", "pull_rounds", "push_joins", "lanes", "narrow", "groups", "edges", "pull_pairs",
+                "push_pairs")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def free(self):

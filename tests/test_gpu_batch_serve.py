@@ -270,12 +270,12 @@ def test_server_failures_stay_per_query(ctx):
 def _group_bytes(s, d):
     """Bytes the server charges per 64-lane group for this edge list: wide
     cells, touched and maxf bitmaps, row records and the packed-delta
-    reserve (2^zbits rows of 64 cells, zbits = max(id bits, 6))."""
+    reserve (2^zbits rows of 64 cells, zbits = max(id bits, 8))."""
     n = int(max(s.max(), d.max())) + 1
     b = 1
     while (1 << b) < n:
         b += 1
-    z = max(b, 6)
+    z = max(b, 8)
     gc = 1 << (z + 6)
     return gc * 8 + gc // 4 + (1 << z) * 8 + gc * 4
 
