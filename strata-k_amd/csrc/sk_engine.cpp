@@ -1985,31 +1985,23 @@ u64 Engine::finalize(int rel, std::vector<Chunk>& chunks) {
       cf = hf ? uf.get() : nullptr;
     }
     if (R.full.n == 0) {
-      // full is empty: every candidate is fresh and the union is the candidate set
+      // full is empty: every candidate is fresh and the union is the
+      // candidate set.  Full takes the candidate buffers and the delta is a
+      // view of them: nothing writes a delta in place, and the next round
+      // reads it before any merge replaces full (both on this stream).
       RelData nf;
       nf.n = cn;
-      nf.k.ensure(x.pool, cn, R.wide);
-      SK_HIP(hipMemcpyAsync(nf.k.ptr(), ck, cn * sizeof(K), hipMemcpyDeviceToDevice, x.stream));
-      if (trop) {
-        nf.w.reset(x.pool, cn);
-        SK_HIP(hipMemcpyAsync(nf.w.get(), cw, cn * 8, hipMemcpyDeviceToDevice, x.stream));
-        if (hf) {
-          nf.inf.reset(x.pool, cn);
-          SK_HIP(hipMemcpyAsync(nf.inf.get(), cf, cn, hipMemcpyDeviceToDevice, x.stream));
-        }
+      if (!cand.sorted_unique) {
+        nf.k = std::move(uk);
+        nf.w = std::move(uw);
+        nf.inf = std::move(uf);
+      } else {
+        nf.k = std::move(cand.k);
+        nf.w = std::move(cand.w);
+        nf.inf = std::move(cand.inf);
       }
       R.full = std::move(nf);
-      // the delta reuses the candidate buffers
-      R.delta.n = cn;
-      if (!cand.sorted_unique) {
-        R.delta.k = std::move(uk);
-        R.delta.w = std::move(uw);
-        R.delta.inf = std::move(uf);
-      } else {
-        R.delta.k = std::move(cand.k);
-        R.delta.w = std::move(cand.w);
-        R.delta.inf = std::move(cand.inf);
-      }
+      R.delta = view_of(R.full);
       nfresh = cn;
       return;
     }

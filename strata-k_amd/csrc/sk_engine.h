@@ -116,6 +116,23 @@ struct RelData {
   }
 };
 
+// A view of another relation state's buffers: it never frees them (its
+// buffers have no pool), so it must not outlive them or be written.
+inline RelData view_of(const RelData& s) {
+  RelData v;
+  v.n = s.n;
+  v.k.wide = s.k.wide;
+  v.k.a.p = s.k.a.p;
+  v.k.a.n = s.k.a.n;
+  v.k.b.p = s.k.b.p;
+  v.k.b.n = s.k.b.n;
+  v.w.p = s.w.p;
+  v.w.n = s.w.n;
+  v.inf.p = s.inf.p;
+  v.inf.n = s.inf.n;
+  return v;
+}
+
 // A permuted (and CSR-indexed) copy of a relation state.
 struct Index {
   u64 version = ~0ull;

@@ -872,7 +872,12 @@ __device__ __forceinline__ void intersect_each(const u32* __restrict__ A, u64 na
 // A driver row whose two lists hold more than kTriHeavy values together is
 // left to a wave of its own (k_tri_count_heavy, k_tri_emit_heavy): one lane
 // per row would hold its whole wave for the length of the longest merge.
-constexpr u64 kTriHeavy = 512;
+// (RMAT scale 22, ms per listing: 256 -> 151.1, 384 -> 151.2, 512 -> 154.0,
+// 768 -> 197.1, 1024 -> 263.9.)
+#ifndef SK_TRI_HEAVY
+#define SK_TRI_HEAVY 256
+#endif
+constexpr u64 kTriHeavy = SK_TRI_HEAVY;
 
 template <class K>
 __global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
