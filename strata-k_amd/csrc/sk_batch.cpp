@@ -273,23 +273,28 @@ sk_status sk_batch_load(sk_batch* b) {
           weighted[pname] = bp.edb_w[e] != nullptr;
         }
       }
-      for (auto& kv : rows) g.eng->stage_sym_resize(kv.first, kv.second, weighted[kv.first]);
+      // weights that do not fit a narrow cell (host threads), then every
+      // relation straight to device columns, program id first
       std::vector<char> narrow(nm, 1);
       parallel_members(nm, [&](size_t m) {
         BProgram& bp = b->progs[g.members[m]];
+        for (size_t e = 0; e < bp.edb_preds.size() && narrow[m]; e++)
+          if (const i64* w = bp.edb_w[e])
+            for (u64 i = 0; i < bp.edb_n[e]; i++)
+              if ((u64)w[i] >= 0xFFFFFFFFull) {
+                narrow[m] = 0;
+                break;
+              }
+      });
+      std::map<std::string, std::vector<Engine::DevSeg>> segs;
+      for (size_t m = 0; m < nm; m++) {
+        BProgram& bp = b->progs[g.members[m]];
         for (size_t e = 0; e < bp.edb_preds.size(); e++) {
           const std::string& pname = p0.preds[bp.prog.rel_of(bp.edb_preds[e])].name;
-          const u64 n = bp.edb_n[e];
-          if (!n) continue;
-          std::vector<u32> pid(n, (u32)m);
-          std::vector<const u32*> cols;
-          cols.push_back(pid.data());
-          for (const u32* c : bp.edb_cols[e]) cols.push_back(c);
-          bool nar = true;
-          g.eng->stage_sym_fill(pname, at[m][e], n, cols.data(), bp.edb_w[e], &nar);
-          if (!nar) narrow[m] = 0;
+          segs[pname].push_back(Engine::DevSeg{(u32)m, at[m][e], bp.edb_n[e], bp.edb_cols[e].data(), bp.edb_w[e]});
         }
-      });
+      }
+      for (auto& kv : segs) g.eng->stage_dev(kv.first, rows[kv.first], weighted[kv.first], kv.second);
       bool all_narrow = true;
       for (char c : narrow) all_narrow = all_narrow && c;
       if (!all_narrow) g.eng->mark_not_narrow();

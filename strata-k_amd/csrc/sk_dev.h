@@ -290,6 +290,7 @@ u8* asp_scratch_address(int device);
 
 void memset_async(const Exec& x, void* p, int v, u64 bytes);
 void fill_u8(const Exec& x, u8* p, u8 v, u64 n);
+void fill_u32(const Exec& x, u32* p, u32 v, u64 n);
 
 u64 read_u64(const Exec& x, const u64* d);
 // n <= 8 device words to the host with one copy and one synchronisation.

@@ -189,6 +189,48 @@ void Ctx::h2d(void* dst, const void* src, u64 bytes) {
   }
 }
 
+void Ctx::h2d_gather(void* dst, const std::vector<HostSeg>& segs) {
+  std::vector<u64> pre(segs.size() + 1, 0);
+  for (size_t i = 0; i < segs.size(); i++) pre[i + 1] = pre[i] + segs[i].bytes;
+  const u64 total = pre.back();
+  if (total < kStageDirect) {
+    for (size_t i = 0; i < segs.size(); i++)
+      if (segs[i].bytes)
+        SK_HIP(hipMemcpyAsync((u8*)dst + pre[i], segs[i].p, segs[i].bytes, hipMemcpyHostToDevice, stream));
+    return;
+  }
+  for (int b = 0; b < 2; b++)
+    if (!stage[b]) {
+      SK_HIP(hipHostMalloc((void**)&stage[b], kStageBytes, hipHostMallocDefault));
+      SK_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
+    }
+  // bytes [a, e) of the pieces laid end to end, into `to`
+  auto gather = [&](u8* to, const u64 a, const u64 e) {
+    for (size_t i = std::upper_bound(pre.begin(), pre.end(), a) - pre.begin() - 1; i < segs.size() && pre[i] < e;
+         i++) {
+      const u64 s0 = std::max(a, pre[i]), s1 = std::min(e, pre[i + 1]);
+      if (s0 < s1) std::memcpy(to + (s0 - a), (const u8*)segs[i].p + (s0 - pre[i]), s1 - s0);
+    }
+  };
+  const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const u64 nchunks = (total + kStageBytes - 1) / kStageBytes;
+  for (u64 k = 0; k < nchunks; k++) {
+    const int b = (int)(k & 1);
+    SK_HIP(hipEventSynchronize(stage_ev[b]));
+    const u64 off = k * kStageBytes, len = std::min(kStageBytes, total - off);
+    const u64 part = (len + nthreads - 1) / nthreads;
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < nthreads; t++) {
+      const u64 a = std::min(len, t * part), e = std::min(len, a + part);
+      if (a < e) ts.emplace_back([&, a, e] { gather(stage[b] + a, off + a, off + e); });
+    }
+    gather(stage[b], off, off + std::min(len, part));
+    for (auto& th : ts) th.join();
+    SK_HIP(hipMemcpyAsync((u8*)dst + off, stage[b], len, hipMemcpyHostToDevice, stream));
+    SK_HIP(hipEventRecord(stage_ev[b], stream));
+  }
+}
+
 void Ctx::d2h(void* dst, const void* src, u64 bytes) {
   if (bytes < kStageDirect) {
     if (bytes) SK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
@@ -447,34 +489,41 @@ void Engine::stage_sym(const std::string& pred, u64 n, u32 arity, const u32* con
   R.st_rows += n;
 }
 
-void Engine::stage_sym_resize(const std::string& pred, u64 total, bool weighted) {
+void Engine::stage_dev(const std::string& pred, u64 total, bool weighted, const std::vector<DevSeg>& segs) {
   int r = P_.rel_of(pred);
   if (r < 0) throw mkerr(SK_ERR_UNKNOWN_PRED, pred, "unknown predicate");
   DRel& R = R_[r];
   if (weighted != (R.sem == SK_TROP))
     throw mkerr(SK_ERR_SEMIRING_MISMATCH, pred, "fact annotation differs from the predicate's semiring");
-  if (!R.st_kind.empty() || R.st_rows) throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "relation already staged");
-  R.st_sym.assign(R.arity, SymCol());
-  for (int j = 0; j < R.arity; j++) R.st_sym[j].resize(total);
-  if (weighted) {
-    R.st_w.resize(total);
-    R.st_inf.assign(total, 0);
+  if (!R.st_kind.empty() || R.st_rows || R.arity < 1)
+    throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "relation already staged");
+  Exec x = ctx_->exec();
+  R.st_dcol.clear();
+  R.st_dcol.resize(R.arity);
+  for (int j = 0; j < R.arity; j++) R.st_dcol[j].reset(x.pool, total ? total : 1);
+  if (weighted) R.st_dw.reset(x.pool, total ? total : 1);
+  u64 at = 0;
+  for (const DevSeg& sg : segs) {
+    if (sg.at != at) throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "staged pieces must lie end to end");
+    fill_u32(x, R.st_dcol[0].get() + sg.at, sg.pid, sg.n);
+    at += sg.n;
   }
+  if (at != total) throw mkerr(SK_ERR_INVALID_ARGUMENT, pred, "staged pieces do not cover the relation");
+  std::vector<Ctx::HostSeg> hs;
+  for (int j = 1; j < R.arity; j++) {
+    hs.clear();
+    for (const DevSeg& sg : segs) hs.push_back({sg.cols[j - 1], sg.n * 4});
+    ctx_->h2d_gather(R.st_dcol[j].get(), hs);
+  }
+  if (weighted) {
+    hs.clear();
+    for (const DevSeg& sg : segs) hs.push_back({sg.w, sg.n * 8});
+    ctx_->h2d_gather(R.st_dw.get(), hs);
+  }
+  R.st_dev = true;
   R.st_weighted = weighted;
   R.st_rows = total;
   R.st_narrow = weighted ? 1 : -1;
-}
-
-void Engine::stage_sym_fill(const std::string& pred, u64 at, u64 n, const u32* const* cols, const i64* w,
-                            bool* narrow) {
-  DRel& R = R_[P_.rel_of(pred)];
-  for (int j = 0; j < R.arity; j++) std::memcpy(R.st_sym[j].data() + at, cols[j], n * 4);
-  bool nar = true;
-  if (w) {
-    std::memcpy(R.st_w.data() + at, w, n * 8);
-    for (u64 i = 0; i < n; i++) nar = nar && (u64)w[i] < 0xFFFFFFFFull;
-  }
-  if (narrow) *narrow = nar;
 }
 
 // =================================================================== load
@@ -524,6 +573,9 @@ void Engine::upload_rel(int r) {
   u64 n = R.st_rows;
   if (n == 0) {
     R.full.n = 0;
+    R.st_dcol.clear();
+    R.st_dw.release();
+    R.st_dev = false;
     return;
   }
   bool trop = R.sem == SK_TROP;
@@ -541,7 +593,11 @@ void Engine::upload_rel(int r) {
   std::vector<DBuf<u32>> dcols(R.arity);
   Cols cols{};
   u64 nsym = R.st_sym.empty() ? 0 : R.st_sym[0].size();
-  for (int j = 0; j < R.arity; j++) {
+  for (int j = 0; j < R.arity && R.st_dev; j++) {
+    dcols[j] = std::move(R.st_dcol[j]);
+    cols.c[j] = dcols[j].get();
+  }
+  for (int j = 0; j < R.arity && !R.st_dev; j++) {
     dcols[j].reset(x.pool, n);
     cols.c[j] = dcols[j].get();
     if (nsym) ctx_->h2d(dcols[j].get(), R.st_sym[j].data(), nsym * 4);
@@ -562,13 +618,18 @@ void Engine::upload_rel(int r) {
   DBuf<i64> w1, w2;
   DBuf<u8> f1, f2;
   if (trop) {
-    w1.reset(x.pool, n);
+    if (R.st_dev) {
+      w1 = std::move(R.st_dw);
+    } else {
+      w1.reset(x.pool, n);
+      ctx_->h2d(w1.get(), R.st_w.data(), n * 8);
+    }
     w2.reset(x.pool, n);
-    ctx_->h2d(w1.get(), R.st_w.data(), n * 8);
     if (has_inf_) {
       f1.reset(x.pool, n);
       f2.reset(x.pool, n);
-      ctx_->h2d(f1.get(), R.st_inf.data(), n);
+      if (R.st_dev) memset_async(x, f1.get(), 0, n);
+      else ctx_->h2d(f1.get(), R.st_inf.data(), n);
     }
   }
   with_key(R.wide, [&](auto* tag) {
@@ -598,6 +659,9 @@ void Engine::upload_rel(int r) {
   std::vector<SymCol>().swap(R.st_sym);
   std::vector<i64>().swap(R.st_w);
   std::vector<u8>().swap(R.st_inf);
+  R.st_dcol.clear();
+  R.st_dw.release();
+  R.st_dev = false;
   R.st_rows = 0;
   R.st_narrow = -1;
 }

@@ -186,6 +186,10 @@ struct DRel {
   std::vector<u32> st_kind;  // row-major kinds (empty when all Sym)
   std::vector<i64> st_val;   // row-major values
   std::vector<SymCol> st_sym;  // all-Sym column stage
+  // batched staging straight to the device (stage_dev): u32 code columns and weights
+  bool st_dev = false;
+  std::vector<DBuf<u32>> st_dcol;
+  DBuf<i64> st_dw;
   std::vector<i64> st_w;
   std::vector<u8> st_inf;
   bool st_weighted = false;
@@ -348,6 +352,14 @@ struct Ctx {
   // are packed into the pinned buffers by several host threads while the
   // previous chunk is in flight; `src` may be reused on return.
   void h2d(void* dst, const void* src, u64 bytes);
+  // Host-to-device copy of host pieces laid end to end from dst (the same
+  // staging as h2d); the pieces may be reused once it has synchronised the
+  // stream (large copies return with every piece read).
+  struct HostSeg {
+    const void* p;
+    u64 bytes;
+  };
+  void h2d_gather(void* dst, const std::vector<HostSeg>& segs);
 };
 
 struct Chunk {
@@ -401,12 +413,20 @@ class Engine {
     int r = P_.rel_of(pred);
     if (r >= 0) R_[r].st_reserve = rows;
   }
-  // Batched staging: size a relation's all-Sym stage for `total` rows, then
-  // fill disjoint row ranges (from several threads at once), recording
-  // whether every weight fits a narrow cell.  `max_sym` bounds every staged
-  // Sym id, so the dictionary needs no scan of the stage.
-  void stage_sym_resize(const std::string& pred, u64 total, bool weighted);
-  void stage_sym_fill(const std::string& pred, u64 at, u64 n, const u32* const* cols, const i64* w, bool* narrow);
+  // Batched staging straight into device columns: `total` rows of an
+  // all-Sym relation whose column 0 is the program id, given as pieces laid
+  // end to end (piece s: rows [at, at + n) of program pid, its other columns
+  // and weights in host memory that must stay valid until load()).  The
+  // callers' arrays are gathered through the pinned staging buffers; no host
+  // copy of the relation is made.  set_max_sym_hint bounds every Sym id, so
+  // the dictionary needs no scan of the stage.
+  struct DevSeg {
+    u32 pid;
+    u64 at, n;
+    const u32* const* cols;  // arity - 1 columns
+    const i64* w;
+  };
+  void stage_dev(const std::string& pred, u64 total, bool weighted, const std::vector<DevSeg>& segs);
   void set_max_sym_hint(u64 max_sym) { max_sym_hint_ = max_sym; }
   // Some staged weight does not fit a narrow cell.
   void mark_not_narrow() {

@@ -74,6 +74,17 @@ static unsigned grid_for(u64 n, u64 per = kBlock) {
   return (unsigned)g;
 }
 
+__global__ void k_fill_u32(u32* p, u32 v, u64 n) {
+  const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void fill_u32(const Exec& x, u32* p, u32 v, u64 n) {
+  if (!n) return;
+  k_fill_u32<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(p, v, n);
+  SK_HIP(hipGetLastError());
+}
+
 void fill_u8(const Exec& x, u8* p, u8 v, u64 n) {
   if (!n) return;
   k_fill_u8<<<grid_for(n, kBlock * 8), kBlock, 0, x.stream>>>(p, v, n);
