@@ -1289,10 +1289,18 @@ bool Engine::try_pull(const RulePlan& pl) {
   }
   if (pull_mode_ == 1 && cand < pull_threshold(E.full.n)) return false;
   const PullDesc pd = pull_desc(*pr, *dn, hr);
-  // algorithmic bytes: every index row of E (j, weight, its source's 16-byte
-  // row record) once, every candidate's delta cell (lane byte, weight) once
-  const double bytes =
-      (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 16.0) + (double)cand * (1.0 + (trop ? 8.0 : 0.0));
+  // algorithmic bytes: every index row of E (j, weight, its source's row
+  // record) once, every candidate's delta entry once, and every destination
+  // row's cells once.  Packed (narrow cells below 2^26): 8-byte records and
+  // one 4-byte word per candidate; otherwise 16-byte records and the lane
+  // byte and weight per candidate.
+  const u64 drows = get_index(pr->E, false, {pr->eR, pr->eJ}, true, true).pull_rows;
+  const double cell_b = trop ? (dn->cd.narrow ? 4.0 : 8.0) : 0.125;
+  const double bytes = pd.dpack ? (double)E.full.n * 20.0 + (double)cand * 4.0 +
+                                      (double)drows * cell_b * (double)(1u << dn->lbits)
+                                : (double)E.full.n * (4.0 + (trop ? 8.0 : 0.0) + 16.0) +
+                                      (double)cand * (1.0 + (trop ? 8.0 : 0.0)) +
+                                      (double)drows * cell_b * (double)(1u << dn->lbits);
   pull_apply(x, pd, dn->cd, trop, bytes, cand);
   round_pulls_++;
   return true;
@@ -1308,7 +1316,7 @@ PullDesc Engine::pull_desc(PullRule& pr, DenseRel& dn, int hr) {
   const u64 nv = D_.nvals();
   // the chunk list is an index of E, kept with it across runs
   if (ixr.pull_version != ixr.csr_version) {
-    ixr.pull_n = pull_chunks(x, ixr.csr.get(), nv, &ixr.pull);
+    ixr.pull_n = pull_chunks(x, ixr.csr.get(), nv, &ixr.pull, &ixr.pull_rows);
     ixr.pull_version = ixr.csr_version;
   }
   PullDesc pd;

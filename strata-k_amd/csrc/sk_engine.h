@@ -145,6 +145,7 @@ struct Index {
   u64 adj_version = ~0ull;
   DBuf<PullChunk> pull;  // pull-round chunk list over the CSR's rows
   u64 pull_n = 0, pull_version = ~0ull;
+  u64 pull_rows = 0;  // destination rows with an index row
 };
 
 

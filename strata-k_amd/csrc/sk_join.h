@@ -203,8 +203,9 @@ u64 popcount_bits(const Exec& x, const u32* bitmap, u64 nwords);
 
 // Chunks of at most `per` index rows per head row, from a CSR over the head
 // row column (nrows + 1 offsets).  Returns the chunk count.
-// Chunk list of a CSR over nrows destination rows (host-side greedy packing).
-u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out);
+// Chunk list of a CSR over nrows destination rows (host-side greedy packing);
+// *rows_used (when given) receives the number of rows with an entry.
+u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out, u64* rows_used = nullptr);
 // Σ over the delta rows of the push join's output count (the push cost).
 template <class K>
 u64 push_cost(const Exec& x, const K* dkeys, u64 n, int arity, int bits, int col, const u64* csr);

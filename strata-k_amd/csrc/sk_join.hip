@@ -1529,10 +1529,14 @@ void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop,
   SK_HIP(hipGetLastError());
 }
 
-u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out) {
+u64 pull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<PullChunk>* out, u64* rows_used) {
   std::vector<u64> h(nrows + 1);
   SK_HIP(hipMemcpyAsync(h.data(), csr, (nrows + 1) * sizeof(u64), hipMemcpyDeviceToHost, x.stream));
   SK_HIP(hipStreamSynchronize(x.stream));
+  if (rows_used) {
+    *rows_used = 0;
+    for (u64 r = 0; r < nrows; r++) *rows_used += h[r + 1] > h[r];
+  }
   std::vector<PullChunk> v;
   PullChunk cur{};
   bool open = false;

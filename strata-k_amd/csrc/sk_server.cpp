@@ -122,6 +122,7 @@ struct sk_server {
   DBuf<i64> iw;
   DBuf<PullChunk> ichunks;
   u64 ichunks_n = 0;
+  u64 irows = 0;  // nodes with an in-edge
   std::vector<u8> edge_node;  // host bitmap (byte per node) of ids used by the EDB
   u64 edge_universe = 0;
   u32 max_batch = 64;
@@ -310,8 +311,9 @@ void pull_group(sk_server* s, const Exec& x, u32 g) {
   pd.delta_first = 1;
   pd.nrows = s->n_nodes;
   // algorithmic bytes: every in-edge (source, weight, the source row's
-  // record) once, every candidate's packed delta word once
-  const double bytes = (double)s->n_edges * (4.0 + 8.0 + 8.0) + (double)s->gcost[g] * 4.0;
+  // record) once, every candidate's packed delta word once, every
+  // destination row's 64 narrow cells once
+  const double bytes = (double)s->n_edges * (4.0 + 8.0 + 8.0) + (double)s->gcost[g] * 4.0 + (double)s->irows * 256.0;
   pull_apply(x, pd, group_cd(s, g), true, bytes, s->gcost[g]);
   s->pulls++;
 }
@@ -450,7 +452,7 @@ sk_status sk_server_create(sk_ctx* ctx, const uint32_t* src, const uint32_t* dst
       unpack_cols<u64>(x, sk, m, 2, s->bits, ip);
       s->iw.reset(pool, mm);
       if (m) SK_HIP(hipMemcpyAsync(s->iw.get(), sw, m * 8, hipMemcpyDeviceToDevice, c.stream));
-      s->ichunks_n = pull_chunks(x, icsr.get(), s->n_nodes, &s->ichunks);
+      s->ichunks_n = pull_chunks(x, icsr.get(), s->n_nodes, &s->ichunks, &s->irows);
       SK_HIP(hipStreamSynchronize(c.stream));
     }
     // lane groups from the byte budget (default: half the pool's budget);
