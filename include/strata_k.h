@@ -422,6 +422,11 @@ sk_status sk_batch_add(sk_batch* b, const sk_program* prog, uint32_t n_edb,
                        const char* const* edb_preds, const uint64_t* edb_n,
                        const uint32_t* const* const* edb_cols, const int64_t* const* edb_w,
                        uint32_t* out_index);
+/* Device-driven rounds (hipGraph) for the batch's strata that allow them:
+ * the Bool groups' word-parallel pull rounds and the Trop groups' push rounds
+ * (default on).  Set before sk_batch_load.  Results, statuses and round
+ * counts are identical either way. */
+sk_status sk_batch_set_graph(sk_batch* b, int enabled);
 sk_status sk_batch_load(sk_batch* b);
 sk_status sk_batch_execute(sk_batch* b);
 /* Per-program status, round counts and results. */

@@ -75,6 +75,7 @@ struct sk_batch {
   };
   std::vector<Group> groups;
   bool loaded = false;
+  bool graph = true;  // device-driven rounds where a stratum allows them
   double exec_ms = 0;
   EvalErr err;
   // per-kernel records of the last execute, summed over the groups (when
@@ -167,6 +168,12 @@ sk_status sk_batch_create(sk_ctx* ctx, sk_batch** out) {
   return SK_OK;
 }
 
+sk_status sk_batch_set_graph(sk_batch* b, int enabled) {
+  if (!b || b->loaded) return SK_ERR_INVALID_ARGUMENT;
+  b->graph = enabled != 0;
+  return SK_OK;
+}
+
 void sk_batch_free(sk_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->c.device);
@@ -234,6 +241,7 @@ sk_status sk_batch_load(sk_batch* b) {
       g.eng = std::make_unique<Engine>(g.gc.get(), g.fused);
       g.eng->validate();
       g.eng->set_partitions((u32)g.members.size());
+      g.eng->set_graph(b->graph);
       const size_t nm = g.members.size();
       // per program: its universe (distinct Sym values of its EDB, a bitmap
       // over its largest id) and its largest id, on up to 8 host threads

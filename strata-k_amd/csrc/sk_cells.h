@@ -127,6 +127,10 @@ struct CellDesc {
   u8* pact = nullptr;
   int pact_shift = 0;
   const u32* pact_list = nullptr;
+  // A touched compaction's write pass (k_cells_write) marks in `pact` the
+  // program of every word holding a delta cell when pact_wshift >= 0:
+  // program rank = word index >> pact_wshift (a word never spans programs).
+  int pact_wshift = -1;
   // Device-driven Trop rounds: the compaction kernels do nothing once the
   // rounds are done or left.
   const LoopState* gate = nullptr;
@@ -320,6 +324,22 @@ void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bi
 template <class K>
 u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                        u64* push_cost, const DevErr* status, DevErr* status_out);
+// A device-driven batched Trop round's compaction (cd.gate, cd.pact =
+// active_next, cd.pact_wshift set): the touched compaction into the
+// capacity buffers with a snapshot of the device status, marking each
+// program holding a delta cell.  A status (widen, overflow) or any
+// program's overflow record leaves (2) before the write pass, so the
+// touched cells and the previous delta stay for the host to finish the
+// round; otherwise the round's per-program bookkeeping follows: every
+// program active in the previous round gets one round (piters[p] += 1),
+// leaving (3) once some piters[p] > capleft[p], active_prev takes
+// active_next (cleared), iters += 1, ls->n = the new delta's size, and done
+// on an empty delta.  cnt / off hold ncells / (kBlock * 32) entries, tot 8.
+template <class K>
+void cells_loop_round_trop_parts(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u32* cnt,
+                                 u64* off, u64* tot, const DevErr* status, LoopState* ls, u8* active_prev,
+                                 u8* active_next, u64* piters, const u64* capleft, const DevErr* part_derr,
+                                 u32 nparts);
 // Touched compaction (from cell 0) writing row records (cd.rowrec8 and/or
 // cd.rowrec, rows of 2^rlbits <= 64 cells) and, per listed cell, the packed
 // word (*dpack; narrow cells) or the weight and the lane (*w, *lane), no keys.

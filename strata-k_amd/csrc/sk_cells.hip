@@ -369,6 +369,11 @@ __global__ __launch_bounds__(kBlock) void k_cells_write(CellDesc cd, u64 w0, u64
       }
     }
   }
+  if (touched_only && cd.pact && cd.pact_wshift >= 0 && b) {
+    const u64 pr = wi >> cd.pact_wshift;
+    const u32 pid = cd.pact_list ? cd.pact_list[pr] : (u32)pr;
+    if (!cd.pact[pid]) cd.pact[pid] = 1;
+  }
   if (touched_only && cd.grp_mask) {
     // per-group lanes and push cost of this block's rows (one group)
     __shared__ u64 gm[kBlock / 64], gp[kBlock / 64];
@@ -581,6 +586,77 @@ void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bi
   SK_HIP(hipGetLastError());
 }
 
+// Between the count and the write pass of a device-driven batched Trop
+// round: a device status (widen, overflow) or any program's overflow record
+// leaves the rounds (2) before the write pass consumes the touched cells, so
+// the host resumes the round after its join with the previous delta in place.
+__global__ __launch_bounds__(kBlock) void k_loop_check_parts(LoopState* ls, const u64* __restrict__ tot,
+                                                             const DevErr* __restrict__ part_derr, u32 nparts) {
+  __shared__ u32 s_ovf;
+  if (ls->done | ls->leave) return;  // uniform over the block
+  if (threadIdx.x == 0) s_ovf = 0;
+  __syncthreads();
+  for (u32 p = threadIdx.x; p < nparts; p += kBlock)
+    if (part_derr[p].code) s_ovf = 1;
+  __syncthreads();
+  const DevErr* st = (const DevErr*)(tot + 2);
+  if (threadIdx.x == 0 && (st->code || st->widen || s_ovf)) {
+    ls->st = *st;
+    ls->leave = 2;
+  }
+}
+
+// The round's per-program bookkeeping (after the write pass): every program
+// active in the previous round gets one round, active_prev takes
+// active_next (cleared), iters += 1, then leave (3) once some program
+// passed its cap, else done on an empty delta.
+__global__ __launch_bounds__(kBlock) void k_loop_book_trop_parts(LoopState* ls, const u64* __restrict__ tot,
+                                                                 u8* __restrict__ active_prev,
+                                                                 u8* __restrict__ active_next,
+                                                                 u64* __restrict__ piters,
+                                                                 const u64* __restrict__ capleft, u32 nparts) {
+  __shared__ u32 s_cap;
+  if (ls->done | ls->leave) return;  // uniform over the block
+  if (threadIdx.x == 0) s_cap = 0;
+  __syncthreads();
+  for (u32 p = threadIdx.x; p < nparts; p += kBlock) {
+    if (active_prev[p]) {
+      piters[p] += 1;
+      if (piters[p] > capleft[p]) s_cap = 1;
+    }
+    active_prev[p] = active_next[p];
+    active_next[p] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ls->iters += 1;
+    ls->n = tot[0];
+    if (s_cap) ls->leave = 3;
+    else if (tot[0] == 0) ls->done = 1;
+  }
+}
+
+template <class K>
+void cells_loop_round_trop_parts(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u32* cnt,
+                                 u64* off, u64* tot, const DevErr* status, LoopState* ls, u8* active_prev,
+                                 u8* active_next, u64* piters, const u64* capleft, const DevErr* part_derr,
+                                 u32 nparts) {
+  static_assert(sizeof(DevErr) == 3 * sizeof(u64), "status record is three words");
+  const u64 nwords = (ncells + 31) / 32;
+  const u64 nblocks = (nwords + kBlock - 1) / kBlock;
+  if (!cells_into_fits(ncells) || !cd.gate || !cd.pact || cd.pact_wshift < 0 || cd.pact != active_next)
+    throw std::runtime_error("cells_loop_round_trop_parts: layout");
+  k_cells_count<<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, cnt, nullptr);
+  SK_HIP(hipGetLastError());
+  scan_u32_one(x, cnt, off, nblocks, tot, nullptr, (const u64*)status, 3);
+  k_loop_check_parts<<<1, kBlock, 0, x.stream>>>(ls, tot, part_derr, nparts);
+  SK_HIP(hipGetLastError());
+  k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, off, bits, keys, w, nullptr);
+  SK_HIP(hipGetLastError());
+  k_loop_book_trop_parts<<<1, kBlock, 0, x.stream>>>(ls, tot, active_prev, active_next, piters, capleft, nparts);
+  SK_HIP(hipGetLastError());
+}
+
 bool cells_into_fits(u64 ncells) {
   const u64 nwords = (ncells + 31) / 32;
   return (nwords + kBlock - 1) / kBlock <= kScanOneMax;
@@ -746,7 +822,10 @@ void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
                                      DevErr*);                                                                    \
   template void cells_mark_touched<K>(const Exec&, const CellDesc&, const K*, u64, int, int);                 \
   template void cells_loop_round_trop<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u8*, u32*, u64*, u64*, \
-                                         u64*, const DevErr*, LoopState*, u64, u64);
+                                         u64*, const DevErr*, LoopState*, u64, u64);                             \
+  template void cells_loop_round_trop_parts<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u32*, u64*, u64*,  \
+                                               const DevErr*, LoopState*, u8*, u8*, u64*, const u64*,             \
+                                               const DevErr*, u32);
 SK_CELLS_INST(u64)
 SK_CELLS_INST(u128)
 

@@ -1543,6 +1543,243 @@ int Engine::graph_loop_rel(u32 k) {
   return rel;
 }
 
+// The push round of a delta variant whose plan is a scan of the stratum's
+// Trop relation's delta, then one join to a fixed relation writing the head
+// into cells, as launch descriptors (the same ones exec_plan builds), or
+// false for any other shape.
+bool Engine::capture_push(const RulePlan& pl, PushCapture* pc) {
+  const Rule& r = P_.rules[pl.rule];
+  if (pl.tri || pl.unit_source || pl.head_err || r.agg || !pl.trop || pl.steps.size() != 2 || has_inf_) return false;
+  const StepPlan& s0 = pl.steps[0];
+  const StepPlan& s1 = pl.steps[1];
+  if (s0.kind != 0 || !s0.delta || s0.never || s0.err || !s0.scan_const_kind.empty() || !s0.scan_eq.empty())
+    return false;
+  if (s1.kind != 1 || s1.delta || s1.never || s1.err || in_idb_[s1.rel] || !s1.filt.empty() || s1.nprobe < 1)
+    return false;
+  for (int j = 0; j < s1.nprobe; j++)
+    if (s1.probe_slot[j] < 0) return false;
+  const int hr = r.head.rel;
+  DenseRel* dh = dense_of(hr);
+  if (!dh || s0.rel != hr || R_[s1.rel].sem != SK_TROP) return false;
+  ScanDesc& sd = pc->sd;
+  sd = ScanDesc{};
+  sd.arity = R_[s0.rel].arity;
+  sd.bits = R_[s0.rel].bits;
+  pc->tcols = (int)s0.out_slots.size();
+  for (int c = 0; c < pc->tcols; c++)
+    for (auto& nv : s0.newvars)
+      if (nv.second == s0.out_slots[c]) sd.out_col[c] = nv.first;
+  ProbeDesc& pd = pc->pd;
+  pd = ProbeDesc{};
+  pd.arity = R_[s1.rel].arity;
+  pd.bits = R_[s1.rel].bits;
+  pd.nprobe = s1.nprobe;
+  for (int j = 0; j < s1.nprobe; j++) pd.psrc[j] = col_index(s1.in_slots, s1.probe_slot[j]);
+  OutDesc& o = pc->o;
+  o = OutDesc{};
+  o.ncols = (int)r.head.args.size();
+  o.bits = R_[hr].bits;
+  for (int c = 0; c < o.ncols; c++) {
+    const Term& t = r.head.args[c];
+    if (t.kind != SK_T_VAR) return false;
+    const int ci = col_index(s1.in_slots, (int)t.a);
+    if (ci >= 0) {
+      o.kind[c] = 0;
+      o.idx[c] = ci;
+    } else {
+      o.kind[c] = 1;
+      for (auto& nv : s1.newvars)
+        if (nv.second == (int)t.a) o.idx[c] = nv.first;
+    }
+  }
+  o.cd = dh->cd;
+  const bool want_csr = s1.nprobe == 1;
+  Index& ix = get_index(s1.rel, false, s1.perm, want_csr, false);
+  const RelData& d = ix.direct ? R_[s1.rel].full : ix.data;
+  pd.sn = d.n;
+  pc->erel = s1.rel;
+  pc->skeys = d.k.ptr();
+  pc->sw = d.w.get();
+  pc->csr = want_csr ? ix.csr.get() : nullptr;
+  pc->part_col = col_index(s1.in_slots, (int)r.var_count - 1);
+  return pc->part_col >= 0 && pc->sw && d.n > 0;
+}
+
+// The relation of a batched Trop stratum whose remaining rounds may run on
+// the device, or -1: one derived relation, dense with capacity compaction
+// (program ids leading the cell order, a word never spanning programs),
+// whose only delta variant in the round just run is a push round of the
+// capture_push shape.
+int Engine::trop_part_graph_rel(u32 k, int* rule, int* dpos) {
+  if (!graph_enabled_ || !nparts_ || has_inf_ || round_variants_ != 1) return -1;
+  int rel = -1;
+  for (size_t i = 0; i < R_.size(); i++)
+    if (in_idb_[i]) {
+      if (rel >= 0) return -1;
+      rel = (int)i;
+    }
+  if (rel < 0 || R_[rel].delta.n == 0 || R_[rel].sem != SK_TROP) return -1;
+  DenseRel* dn = dense_of(rel);
+  if (!dn || !dn->into || dn->keep_full || !dn->cd.trop || !cells_into_fits(dn->ncells)) return -1;
+  if (dn->cd.shift[0] < 5) return -1;
+  for (int c = 1; c < dn->cd.ncols; c++)
+    if (dn->cd.shift[c] >= dn->cd.shift[0]) return -1;
+  int found = 0;
+  for (size_t ri = 0; ri < P_.rules.size(); ri++) {
+    const Rule& r = P_.rules[ri];
+    if (r.stratum != k || r.agg) continue;
+    for (size_t j = 0; j < r.body.size(); j++) {
+      const Atom& a = r.body[j];
+      if (a.neg || a.rel < 0 || !in_idb_[a.rel]) continue;
+      found++;
+      *rule = (int)ri;
+      *dpos = (int)j;
+    }
+  }
+  return found == 1 ? rel : -1;
+}
+
+// Runs a batched Trop stratum's next rounds on the device: a hipGraph of
+// kGraphRounds rounds (the push round from device-side counts,
+// push_round_dev; the capacity compaction marking each program with a
+// delta cell and the per-program bookkeeping, cells_loop_round_trop_parts)
+// relaunched until the rounds are done or leave.  Per-program round counts
+// are added to part_iters_ and active_prev is left as the device has it.
+// Returns 0 converged; 2 the round's join ran but a status or a program's
+// overflow stopped it before its compaction (*st; the previous delta is in
+// place, n restored); 3 a program passed its cap after the round (the new
+// delta is in place).
+int Engine::run_trop_part_graph_loop(u32 k, int rel, const RulePlan& pl, u64* iters, std::vector<u8>& active_prev,
+                                     DevErr* st) {
+  constexpr int kGraphRounds = 4;
+  DRel& H = R_[rel];
+  DenseRel& dn = *dense_of(rel);
+  PushCapture pc;
+  if (!capture_push(pl, &pc)) return -1;
+  Exec x = ctx_->exec();
+  const u32 np = nparts_;
+  const u64 cap = std::max<u64>(dn.ncells, 1);
+  // the rounds' compactions write up to ncells delta entries in place; a
+  // delta that a dropped program shrank (drop_dead_from_deltas sizes it to
+  // its rows) moves to buffers of that capacity first
+  const u64 kcap = H.wide ? H.delta.k.b.n : H.delta.k.a.n;
+  if (kcap < cap || H.delta.w.n < cap) {
+    RelData nd;
+    nd.n = H.delta.n;
+    nd.k.ensure(x.pool, cap, H.wide);
+    nd.w.reset(x.pool, cap);
+    if (nd.n) {
+      SK_HIP(hipMemcpyAsync(nd.k.ptr(), H.delta.k.ptr(), nd.n * (H.wide ? 16 : 8), hipMemcpyDeviceToDevice,
+                            x.stream));
+      SK_HIP(hipMemcpyAsync(nd.w.get(), H.delta.w.get(), nd.n * 8, hipMemcpyDeviceToDevice, x.stream));
+    }
+    H.delta = std::move(nd);
+  }
+  std::vector<DBuf<u32>> tcols(pc.tcols);
+  Table t;
+  t.ncols = pc.tcols;
+  for (int c = 0; c < pc.tcols; c++) {
+    tcols[c].reset(x.pool, cap);
+    t.col[c] = tcols[c].get();
+  }
+  DBuf<i64> tw(x.pool, cap);
+  t.w = tw.get();
+  t.part_col = pc.part_col;
+  t.perr = part_derr_.get();
+  DBuf<u64> lo(x.pool, cap), off(x.pool, cap), bsum(x.pool, kDevScanBlocks), jtot(x.pool, 1);
+  DBuf<u32> cnt(x.pool, cap);
+  const u64 nb = (((dn.ncells + 31) / 32) + kBlock - 1) / kBlock;
+  DBuf<u32> ccnt(x.pool, nb ? nb : 1);
+  DBuf<u64> coff(x.pool, nb ? nb : 1), ctot(x.pool, 8);
+  DBuf<LoopState> ls(x.pool, 1);
+  DBuf<u8> ap(x.pool, np), an(x.pool, np);
+  DBuf<u64> pit(x.pool, np), capl(x.pool, np);
+  LoopState h0{};
+  h0.n = H.delta.n;
+  std::vector<u64> hcap(np);
+  for (u32 p = 0; p < np; p++) {
+    const u64 lim = part_universe_[p] + 2, used = part_iters_[p][k];
+    hcap[p] = part_dead_[p] ? ~0ull : (lim > used ? lim - used : 0);
+  }
+  if (active_prev.size() != np) active_prev.assign(np, 0);
+  SK_HIP(hipMemcpyAsync(ls.get(), &h0, sizeof h0, hipMemcpyHostToDevice, x.stream));
+  SK_HIP(hipMemcpyAsync(ap.get(), active_prev.data(), np, hipMemcpyHostToDevice, x.stream));
+  SK_HIP(hipMemcpyAsync(capl.get(), hcap.data(), np * 8, hipMemcpyHostToDevice, x.stream));
+  memset_async(x, an.get(), 0, np);
+  memset_async(x, pit.get(), 0, np * 8);
+  SK_HIP(hipStreamSynchronize(x.stream));
+  CellDesc cdl = dn.cd;
+  cdl.gate = ls.get();
+  cdl.pact = an.get();
+  cdl.pact_list = cdl.dom_list[0];
+  cdl.pact_wshift = cdl.shift[0] - 5;
+  OutDesc o = pc.o;
+  o.cd = dn.cd;
+  Exec xg = x;
+  xg.timer = nullptr;  // no event records inside the capture
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  SK_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    with_key(H.wide, [&](auto* tagd) {
+      using KD = std::remove_pointer_t<decltype(tagd)>;
+      with_key(R_[pc.erel].wide, [&](auto* tage) {
+        using KE = std::remove_pointer_t<decltype(tage)>;
+        for (int r = 0; r < kGraphRounds; r++) {
+          push_round_dev<KD, KE>(xg, (const KD*)H.delta.k.ptr(), H.delta.w.get(), H.arity, H.bits, pc.sd, t, pc.pd,
+                                 (const KE*)pc.skeys, pc.sw, pc.csr, o, lo.get(), cnt.get(), off.get(), bsum.get(),
+                                 jtot.get(), cap, ls.get());
+          cells_loop_round_trop_parts<KD>(xg, cdl, dn.ncells, H.bits, (KD*)H.delta.k.ptr(), H.delta.w.get(),
+                                          ccnt.get(), coff.get(), ctot.get(), ctx_->derr, ls.get(), ap.get(),
+                                          an.get(), pit.get(), capl.get(), part_derr_.get(), np);
+        }
+      });
+    });
+  } catch (...) {
+    hipGraph_t bad = nullptr;
+    (void)hipStreamEndCapture(x.stream, &bad);
+    if (bad) (void)hipGraphDestroy(bad);
+    throw;
+  }
+  SK_HIP(hipStreamEndCapture(x.stream, &g));
+  hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    SK_HIP(ie);
+  }
+  LoopState hl{};
+  try {
+    for (;;) {
+      {
+        Scope sc(x.timer, "graph_rounds_trop", KC_JOIN, 0.0, kGraphRounds);
+        SK_HIP(hipGraphLaunch(ge, x.stream));
+      }
+      u64 hv[3] = {0, 0, 0};
+      read_u64s(x, (const u64*)ls.get(), hv, 3);
+      if ((u32)hv[0] || (u32)(hv[2] >> 32)) break;  // done, or leave
+    }
+  } catch (...) {
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    throw;
+  }
+  SK_HIP(hipGraphExecDestroy(ge));
+  SK_HIP(hipGraphDestroy(g));
+  std::vector<u64> pi(np);
+  SK_HIP(hipMemcpyAsync(&hl, ls.get(), sizeof hl, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipMemcpyAsync(pi.data(), pit.get(), np * 8, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipMemcpyAsync(active_prev.data(), ap.get(), np, hipMemcpyDeviceToHost, x.stream));
+  SK_HIP(hipStreamSynchronize(x.stream));
+  *iters += hl.iters;
+  for (u32 p = 0; p < np; p++) part_iters_[p][k] += pi[p];
+  if (hl.done) return 0;
+  // the delta in the capacity buffers: the previous one (2) or the new one (3)
+  H.delta.n = hl.n;
+  H.dversion++;
+  *st = hl.st;
+  return (int)hl.leave;
+}
+
 // Runs the stratum's remaining delta rounds on the device: a hipGraph of
 // kGraphRounds rounds (each: every bit pull, then the bitmap-only
 // compaction and the round's bookkeeping, see cells_loop_round) is launched until the
@@ -3020,6 +3257,7 @@ void Engine::run_stratum(u32 k) {
     return it->second;
   };
   bool bootstrap = true;
+  bool joined = false;  // the round's joins already ran on the device (device-driven rounds left mid-round)
   for (;;) {
     std::map<int, std::vector<Chunk>> chunks;
     round_variants_ = round_bitpulls_ = round_pulls_ = 0;
@@ -3041,7 +3279,8 @@ void Engine::run_stratum(u32 k) {
         }
       }
     };
-    run_rules();
+    if (!joined) run_rules();
+    joined = false;
     const bool check_err = is_trop && !nparts_;
     const bool need_status = check_err || any_narrow();
     if (need_status && !fold) {
@@ -3134,6 +3373,35 @@ void Engine::run_stratum(u32 k) {
         raise_device_error(gst);
       }
       // rc == 1: the next round pushes, on the host
+    }
+    int pri = -1, pdp = -1;
+    const int prel = is_trop && nparts_ ? trop_part_graph_rel(k, &pri, &pdp) : -1;
+    if (prel >= 0) {
+      DevErr gst{};
+      const int rc = run_trop_part_graph_loop(k, prel, plan_of(pri, pdp), &iters, active_prev, &gst);
+      if (rc == 0) break;
+      if (rc == 2) {
+        // the round's join ran; the host finishes it (widening first when
+        // the status says so) with the previous delta in place
+        joined = true;
+        continue;
+      }
+      if (rc == 3) {
+        // a program passed its cap (crates/strata-eval/src/seminaive.rs:98-104)
+        bool newly_dead = false;
+        for (u32 p = 0; p < nparts_; p++)
+          if (!part_dead_[p] && part_iters_[p][k] > part_universe_[p] + 2) {
+            part_dead_[p] = 1;
+            newly_dead = true;
+            part_err_[p] = mkerr(SK_ERR_NEGATIVE_WEIGHT_CYCLE, "", "negative-weight cycle: tropical stratum diverges");
+            active_prev[p] = 0;
+          }
+        if (newly_dead) drop_dead_from_deltas();
+        bool more = false;
+        for (size_t i = 0; i < R_.size(); i++)
+          if (in_idb_[i] && R_[i].delta.n) more = true;
+        if (!more) break;
+      }
     }
   }
   iters_[k] = iters;

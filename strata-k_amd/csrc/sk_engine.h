@@ -507,6 +507,23 @@ class Engine {
   PullDesc pull_desc(PullRule& pr, DenseRel& dn, int hr);
   int trop_graph_rel(u32 k);
   int run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st);
+  // Batched Trop strata: descriptors of a delta variant's push round, and
+  // the remaining rounds device-driven (see run_trop_part_graph_loop).
+  struct PushCapture {
+    ScanDesc sd;
+    int tcols = 0;  // table columns after the delta scan
+    int erel = -1;  // the probed (fixed) relation
+    ProbeDesc pd;
+    OutDesc o;
+    const void* skeys = nullptr;
+    const i64* sw = nullptr;
+    const u64* csr = nullptr;
+    int part_col = -1;
+  };
+  bool capture_push(const RulePlan& pl, PushCapture* pc);
+  int trop_part_graph_rel(u32 k, int* rule, int* dpos);
+  int run_trop_part_graph_loop(u32 k, int rel, const RulePlan& pl, u64* iters, std::vector<u8>& active_prev,
+                               DevErr* st);
   void run_graph_loop(u32 k, int rel, u64* iters, std::vector<u8>& active_prev);
   void plan_bitpulls(u32 k, int i, DenseRel& dn);
   u64 finalize(int rel, std::vector<Chunk>& chunks);

@@ -211,4 +211,18 @@ template <class K>
 u64 push_cost(const Exec& x, const K* dkeys, u64 n, int arity, int bits, int col, const u64* csr);
 void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop, double bytes, u64 items);
 
+// One device-driven push round (graph capture) of a Trop rule that scans a
+// delta and joins it to a fixed relation, the head applied to cells: the
+// delta's ls->n rows (packed keys KD, weights) are scanned into table `t`
+// (sd.out_col), joined (count, device-sized scan, load-balanced emit) and
+// applied to o.cd.  t, lo, cnt and off hold `cap` rows (at least any delta),
+// bsum 1024 words, jtot one.  Every kernel returns at once when the rounds
+// are done or have left (ls).
+constexpr u32 kDevScanBlocks = 1024;
+template <class KD, class KE>
+void push_round_dev(const Exec& x, const KD* dkeys, const i64* dw, int darity, int dbits, const ScanDesc& sd,
+                    const Table& t, const ProbeDesc& p, const KE* skeys, const i64* sw, const u64* csr,
+                    const OutDesc& o, u64* lo, u32* cnt, u64* off, u64* bsum, u64* jtot, u64 cap,
+                    const LoopState* ls);
+
 }  // namespace sk

@@ -154,10 +154,9 @@ __device__ __forceinline__ bool filt_ok(K key, const ProbeDesc& p) {
 }
 
 template <class K>
-__global__ void k_join_count(Table in, ProbeDesc p, const K* __restrict__ skeys, const u64* __restrict__ csr,
-                             u64* __restrict__ lo_out, u32* __restrict__ cnt) {
-  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= in.n) return;
+__device__ __forceinline__ void join_count_row(const Table& in, const ProbeDesc& p, const K* __restrict__ skeys,
+                                               const u64* __restrict__ csr, u64 r, u64* __restrict__ lo_out,
+                                               u32* __restrict__ cnt) {
   u64 lo, hi;
   if (p.nprobe == 0) {
     lo = 0;
@@ -193,6 +192,14 @@ __global__ void k_join_count(Table in, ProbeDesc p, const K* __restrict__ skeys,
   }
   lo_out[r] = lo;
   cnt[r] = (u32)c;
+}
+
+template <class K>
+__global__ void k_join_count(Table in, ProbeDesc p, const K* __restrict__ skeys, const u64* __restrict__ csr,
+                             u64* __restrict__ lo_out, u32* __restrict__ cnt) {
+  u64 r = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= in.n) return;
+  join_count_row<K>(in, p, skeys, csr, r, lo_out, cnt);
 }
 
 // Output value for output column c of emitted row (table row r, relation row key).
@@ -336,15 +343,13 @@ __global__ void k_join_emit_rows(Table in, ProbeDesc p, OutDesc o, const K* __re
 // lookup is compiled only into that variant, which keeps the common one at
 // its register budget.
 template <class K, bool TROP, bool PDOM>
-__global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
-                                                       const i64* __restrict__ sw, const u8* __restrict__ sf,
-                                                       const u64* __restrict__ lo, const u64* __restrict__ off,
-                                                       u64 total, DevErr* err) {
+__device__ __forceinline__ void join_apply_tile(const Table& in, const ProbeDesc& p, const OutDesc& o,
+                                                const K* __restrict__ skeys, const i64* __restrict__ sw,
+                                                const u8* __restrict__ sf, const u64* __restrict__ lo,
+                                                const u64* __restrict__ off, const u64 o0, const u64 o1,
+                                                const u64 nrows, DevErr* err) {
   __shared__ u64 s_off[kEmitLds];
   __shared__ u64 s_r[2];
-  const u64 o0 = (u64)blockIdx.x * kEmitTile;
-  const u64 o1 = min(o0 + kEmitTile, total);
-  const u64 nrows = in.n;
   if (threadIdx.x < 2) {
     u64 q = threadIdx.x == 0 ? o0 : o1 - 1;
     u64 a = 0, b = nrows;
@@ -546,6 +551,15 @@ __global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, Ou
 #pragma unroll
   for (int k = 0; k < kEmitItems; k++)
     if (ok[k]) cell_apply_t<TROP>(o.cd, cell[k], res[k], rf[k], seen[k]);
+}
+
+template <class K, bool TROP, bool PDOM>
+__global__ __launch_bounds__(kBlock) void k_join_apply(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
+                                                       const i64* __restrict__ sw, const u8* __restrict__ sf,
+                                                       const u64* __restrict__ lo, const u64* __restrict__ off,
+                                                       u64 total, DevErr* err) {
+  const u64 o0 = (u64)blockIdx.x * kEmitTile;
+  join_apply_tile<K, TROP, PDOM>(in, p, o, skeys, sw, sf, lo, off, o0, min(o0 + kEmitTile, total), in.n, err);
 }
 
 template <class K, class KO, bool FINAL, bool TROP>
@@ -1388,14 +1402,26 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
   }
 }
 
+// Pairs per lane per step of the packed pull kernel (their loads are issued
+// together before the LDS minimum).
+#ifndef SK_PULL_PAIRS_PACKED
+#define SK_PULL_PAIRS_PACKED 4
+#endif
+constexpr int kPullPairsP = SK_PULL_PAIRS_PACKED;
+// LDS minima of one destination row are kPullAccStride words apart.
+#ifndef SK_PULL_ACC_STRIDE
+#define SK_PULL_ACC_STRIDE 64
+#endif
+constexpr u32 kPullAccStride = SK_PULL_ACC_STRIDE;
+
 // Per wave of the packed pull kernel (narrow Trop cells below 2^26).
 struct PullLdsP {
   u64 base[64];               // delta-list position of the index row's source row's run
   u32 w[64];                  // edge weight
   u32 excl[64];               // exclusive prefix of the index rows' pair counts
   u32 slot[64];               // which of the chunk's destination rows the index row feeds
-  u32 acc[kPullRows * 64];    // running minimum per (destination row, lane value); kCellTop32 = none
-  u8 start[64 * kPullPairs];  // 1 + the index row whose pairs start at this position of the step
+  u32 acc[kPullRows * kPullAccStride];  // running minimum per (destination row, lane value); kCellTop32 = none
+  u8 start[64 * kPullPairsP];  // 1 + the index row whose pairs start at this position of the step
 };
 
 // The pull round of k_pull<true> over a packed delta (CellDesc::dpack,
@@ -1420,7 +1446,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
   const u32 nl = 1u << lb;
   const u32 ncell = nr << lb;  // <= kPullRows * 64
   const u64 lim = cd.narrow_lim;
-  for (u32 i = lane; i < kPullRows * 64; i += 64) L.acc[i] = kCellTop32;
+  for (u32 i = lane; i < kPullRows * kPullAccStride; i += 64) L.acc[i] = kCellTop32;
   u64 seen[kPullRows];
 #pragma unroll
   for (u32 u = 0; u < kPullRows; u++) {
@@ -1456,25 +1482,25 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
     L.excl[lane] = excl;
     L.slot[lane] = slot;
 #pragma unroll
-    for (int u = 0; u < kPullPairs; u++) L.start[u * 64 + lane] = 0;
+    for (int u = 0; u < kPullPairsP; u++) L.start[u * 64 + lane] = 0;
     const u64 le = ~0ull >> (63 - lane);  // lanes at or below this one
     u32 carry = 0;  // the edge holding the step's first pair (pair 0 starts an edge)
-    for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairs) {
+    for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairsP) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if (cnt && excl >= p0 && excl < p0 + 64 * kPullPairs) L.start[excl - p0] = (u8)(lane + 1);
+      if (cnt && excl >= p0 && excl < p0 + 64 * kPullPairsP) L.start[excl - p0] = (u8)(lane + 1);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      u32 rs[kPullPairs];
+      u32 rs[kPullPairsP];
 #pragma unroll
-      for (int u = 0; u < kPullPairs; u++) rs[u] = L.start[u * 64 + lane];
+      for (int u = 0; u < kPullPairsP; u++) rs[u] = L.start[u * 64 + lane];
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int u = 0; u < kPullPairs; u++) L.start[u * 64 + lane] = 0;
-      u32 jj[kPullPairs], kk[kPullPairs], vv[kPullPairs];
+      for (int u = 0; u < kPullPairsP; u++) L.start[u * 64 + lane] = 0;
+      u32 jj[kPullPairsP], kk[kPullPairsP], vv[kPullPairsP];
 #pragma unroll
-      for (int u = 0; u < kPullPairs; u++) {
+      for (int u = 0; u < kPullPairsP; u++) {
         const u64 sm = __ballot(rs[u] != 0);
         const u64 mine = sm & le;
         const int h = mine ? 63 - __builtin_clzll(mine) : 0;
@@ -1486,12 +1512,12 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
         kk[u] = pidx - L.excl[j];
       }
 #pragma unroll
-      for (int u = 0; u < kPullPairs; u++) {
+      for (int u = 0; u < kPullPairsP; u++) {
         vv[u] = kCellTop32;
         if (p0 + (u32)u * 64 + lane < total) vv[u] = pd.dpack[L.base[jj[u]] + kk[u]];
       }
 #pragma unroll
-      for (int u = 0; u < kPullPairs; u++) {
+      for (int u = 0; u < kPullPairsP; u++) {
         if (p0 + (u32)u * 64 + lane >= total) continue;
         const u32 j = jj[u];
         const u64 cand = (u64)(vv[u] >> 6) + (u64)L.w[j];
@@ -1499,7 +1525,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
           wide = true;
           continue;
         }
-        atomicMin(&L.acc[L.slot[j] * 64 + (vv[u] & 63u)], (u32)cand);
+        atomicMin(&L.acc[L.slot[j] * kPullAccStride + (vv[u] & 63u)], (u32)cand);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1510,13 +1536,184 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
     const u32 i = lane + 64 * u;
     if (i >= ncell) continue;
     const u32 sl = i >> lb, xl = i & (nl - 1);
-    const u32 v = L.acc[sl * 64 + xl];
+    const u32 v = L.acc[sl * kPullAccStride + xl];
     if (v == kCellTop32) continue;
     const u64 cl = ((u64)c.row[sl] << lb) + xl;
     cell_apply_t<true>(cd, cl, (i64)v, 0, seen[u]);
   }
   if (__ballot(wide) && lane == 0) atomicOr(cd.widen, 1u);
 }
+
+// ============================================================ device-driven push rounds
+//
+// The push join of one delta variant with the delta's size and the join's
+// output count on the device, for rounds captured in a graph: each grid is
+// fixed and strides over the device-side count, and every kernel returns at
+// once when the rounds are done or have left (LoopState).
+
+constexpr u32 kDevGrid = 4096;      // blocks of the striding row and tile kernels
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_scan_delta_dev(const K* __restrict__ keys, const i64* __restrict__ w,
+                                                           const LoopState* __restrict__ ls, int arity, int bits,
+                                                           ScanDesc sd, Table out) {
+  if (ls->done | ls->leave) return;
+  const u64 n = ls->n;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (u64)gridDim.x * kBlock) {
+    const K key = keys[i];
+    for (int c = 0; c < out.ncols; c++) out.col[c][i] = col_of<K>(key, arity, bits, sd.out_col[c]);
+    if (out.w) out.w[i] = w[i];
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_join_count_dev(Table in, ProbeDesc p, const K* __restrict__ skeys,
+                                                           const u64* __restrict__ csr, u64* __restrict__ lo_out,
+                                                           u32* __restrict__ cnt, const LoopState* __restrict__ ls) {
+  if (ls->done | ls->leave) return;
+  const u64 n = ls->n;
+  for (u64 r = (u64)blockIdx.x * kBlock + threadIdx.x; r < n; r += (u64)gridDim.x * kBlock)
+    join_count_row<K>(in, p, skeys, csr, r, lo_out, cnt);
+}
+
+// Exclusive scan of cnt[0, ls->n) in three launches of fixed grids: block b
+// sums its share, one block scans the shares (writing the total), block b
+// rescans its share from its offset.
+__device__ __forceinline__ u64 dscan_block_sum(u64 v, u64* tmp) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  u64 t = 0;
+#pragma unroll
+  for (int q = 0; q < kBlock / 64; q++) t += tmp[q];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dscan_reduce(const u32* __restrict__ cnt, const LoopState* __restrict__ ls,
+                                                         u64* __restrict__ bsum) {
+  __shared__ u64 tmp[kBlock / 64];
+  if (ls->done | ls->leave) return;
+  const u64 n = ls->n, per = (n + gridDim.x - 1) / gridDim.x;
+  const u64 a = (u64)blockIdx.x * per, e = a + per < n ? a + per : n;
+  u64 v = 0;
+  for (u64 i = a + threadIdx.x; i < e; i += kBlock) v += cnt[i];
+  const u64 t = dscan_block_sum(v, tmp);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dscan_sums(u64* __restrict__ bsum, u32 nb, u64* __restrict__ total,
+                                                       const LoopState* __restrict__ ls) {
+  __shared__ u64 part[kBlock];
+  if (ls->done | ls->leave) return;
+  // thread t scans entries [t * per, (t + 1) * per) serially, then the block
+  // scans the threads' sums
+  const u32 per = (nb + kBlock - 1) / kBlock;
+  const u32 a = threadIdx.x * per, e = a + per < nb ? a + per : nb;
+  u64 sum = 0;
+  for (u32 i = a; i < e; i++) sum += bsum[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (u32 o = 1; o < kBlock; o <<= 1) {
+    const u64 y = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += y;
+    __syncthreads();
+  }
+  u64 run = part[threadIdx.x] - sum;
+  for (u32 i = a; i < e; i++) {
+    const u64 v = bsum[i];
+    bsum[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == kBlock - 1) *total = part[kBlock - 1];
+}
+
+__global__ __launch_bounds__(kBlock) void k_dscan_down(const u32* __restrict__ cnt, const LoopState* __restrict__ ls,
+                                                       const u64* __restrict__ boff, u64* __restrict__ out) {
+  __shared__ u64 tmp[kBlock / 64];
+  if (ls->done | ls->leave) return;
+  const u64 n = ls->n, per = (n + gridDim.x - 1) / gridDim.x;
+  const u64 a = (u64)blockIdx.x * per, e = a + per < n ? a + per : n;
+  u64 carry = boff[blockIdx.x];
+  const u32 lane = threadIdx.x & 63;
+  for (u64 t0 = a; t0 < e; t0 += kBlock) {
+    const u64 i = t0 + threadIdx.x;
+    const u64 v = i < e ? cnt[i] : 0;
+    u64 inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const u64 y = __shfl_up(inc, o, 64);
+      if (lane >= (u32)o) inc += y;
+    }
+    if (lane == 63) tmp[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    u64 wbase = 0, tile = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; q++) {
+      if (q < (int)(threadIdx.x >> 6)) wbase += tmp[q];
+      tile += tmp[q];
+    }
+    if (i < e) out[i] = carry + wbase + inc - v;
+    carry += tile;
+    __syncthreads();
+  }
+}
+
+template <class K, bool TROP, bool PDOM>
+__global__ __launch_bounds__(kBlock) void k_join_apply_dev(Table in, ProbeDesc p, OutDesc o, const K* __restrict__ skeys,
+                                                           const i64* __restrict__ sw, const u64* __restrict__ lo,
+                                                           const u64* __restrict__ off,
+                                                           const u64* __restrict__ total_dev,
+                                                           const LoopState* __restrict__ ls, DevErr* err) {
+  if (ls->done | ls->leave) return;
+  const u64 total = *total_dev, nrows = ls->n;
+  for (u64 t = blockIdx.x; t * kEmitTile < total; t += gridDim.x) {
+    const u64 o0 = t * kEmitTile;
+    join_apply_tile<K, TROP, PDOM>(in, p, o, skeys, sw, nullptr, lo, off, o0, min(o0 + kEmitTile, total), nrows, err);
+    __syncthreads();  // the tile's LDS row offsets are rewritten by the next tile
+  }
+}
+
+template <class KD, class KE>
+void push_round_dev(const Exec& x, const KD* dkeys, const i64* dw, int darity, int dbits, const ScanDesc& sd,
+                    const Table& t, const ProbeDesc& p, const KE* skeys, const i64* sw, const u64* csr,
+                    const OutDesc& o, u64* lo, u32* cnt, u64* off, u64* bsum, u64* jtot, u64 cap,
+                    const LoopState* ls) {
+  if (!o.cd.on || !o.cd.trop || p.nfilt || !t.w || t.inf) throw std::runtime_error("push_round_dev: shape");
+  const unsigned rg = (unsigned)std::min<u64>(gridj(cap), kDevGrid);
+  k_scan_delta_dev<KD><<<rg, kBlock, 0, x.stream>>>(dkeys, dw, ls, darity, dbits, sd, t);
+  SK_HIP(hipGetLastError());
+  k_join_count_dev<KE><<<rg, kBlock, 0, x.stream>>>(t, p, skeys, csr, lo, cnt, ls);
+  SK_HIP(hipGetLastError());
+  k_dscan_reduce<<<kDevScanBlocks, kBlock, 0, x.stream>>>(cnt, ls, bsum);
+  SK_HIP(hipGetLastError());
+  k_dscan_sums<<<1, kBlock, 0, x.stream>>>(bsum, kDevScanBlocks, jtot, ls);
+  SK_HIP(hipGetLastError());
+  k_dscan_down<<<kDevScanBlocks, kBlock, 0, x.stream>>>(cnt, ls, bsum, off);
+  SK_HIP(hipGetLastError());
+  bool pdom = false;
+  for (int j = 0; j < o.cd.ncols; j++) pdom = pdom || o.cd.pdom_off[j] != nullptr;
+  if (pdom)
+    k_join_apply_dev<KE, true, true><<<kDevGrid, kBlock, 0, x.stream>>>(t, p, o, skeys, sw, lo, off, jtot, ls, x.err);
+  else
+    k_join_apply_dev<KE, true, false><<<kDevGrid, kBlock, 0, x.stream>>>(t, p, o, skeys, sw, lo, off, jtot, ls, x.err);
+  SK_HIP(hipGetLastError());
+}
+
+template void push_round_dev<u64, u64>(const Exec&, const u64*, const i64*, int, int, const ScanDesc&, const Table&,
+                                       const ProbeDesc&, const u64*, const i64*, const u64*, const OutDesc&, u64*,
+                                       u32*, u64*, u64*, u64*, u64, const LoopState*);
+template void push_round_dev<u128, u64>(const Exec&, const u128*, const i64*, int, int, const ScanDesc&, const Table&,
+                                        const ProbeDesc&, const u64*, const i64*, const u64*, const OutDesc&, u64*,
+                                        u32*, u64*, u64*, u64*, u64, const LoopState*);
+template void push_round_dev<u64, u128>(const Exec&, const u64*, const i64*, int, int, const ScanDesc&, const Table&,
+                                        const ProbeDesc&, const u128*, const i64*, const u64*, const OutDesc&, u64*,
+                                        u32*, u64*, u64*, u64*, u64, const LoopState*);
+template void push_round_dev<u128, u128>(const Exec&, const u128*, const i64*, int, int, const ScanDesc&,
+                                         const Table&, const ProbeDesc&, const u128*, const i64*, const u64*,
+                                         const OutDesc&, u64*, u32*, u64*, u64*, u64*, u64, const LoopState*);
 
 void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop, double bytes, u64 items) {
   if (!p.nchunks) return;

@@ -177,6 +177,7 @@ SIGNATURES = [
     ("sk_batch_create", C.c_int, [_VP, C.POINTER(_VP)]),
     ("sk_batch_free", None, [_VP]),
     ("sk_batch_add", C.c_int, [_VP, C.POINTER(SkProgram), _U32, _VP, _VP, _VP, _VP, C.POINTER(_U32)]),
+    ("sk_batch_set_graph", C.c_int, [_VP, C.c_int]),
     ("sk_batch_load", C.c_int, [_VP]),
     ("sk_batch_execute", C.c_int, [_VP]),
     ("sk_batch_status", C.c_int, [_VP, _U32, C.POINTER(SkErrorInfo)]),
@@ -978,6 +979,12 @@ class Batch:
         self.progs.append(prog)
         self._keep.append(keep)  # the library reads the facts at sk_batch_load
         return idx.value
+
+    def set_graph(self, enabled: bool):
+        """Device-driven rounds where a stratum allows them (default on)."""
+        code = self.L.sk_batch_set_graph(self.h, 1 if enabled else 0)
+        if code:
+            _raise(code)
 
     def load(self):
         code = self.L.sk_batch_load(self.h)

@@ -106,6 +106,75 @@ def test_batch_isolates_errors(ctx):
     b.free()
 
 
+def _batch_run(ctx, progs, graph):
+    b = sk.Batch(ctx)
+    b.set_graph(graph)
+    for prog, bulk in progs:
+        b.add(prog, bulk)
+    ctx.set_profiling(True)
+    b.load()
+    b.execute()
+    recs = {r["name"] for r in b.kernel_records()}
+    ctx.set_profiling(False)
+    out = []
+    for i, (prog, bulk) in enumerate(progs):
+        code, info = b.status(i)
+        rel = "cost" if any(p.name == "cost" for p in prog.predicates) else "path"
+        if code:
+            out.append((code, info["lhs"], info["rhs"]))
+            continue
+        cols, w = b.relation_sym(i, rel)
+        out.append((0, [c.copy() for c in cols], None if w is None else w.copy(), b.iterations(i)))
+    b.free()
+    return out, recs
+
+
+def _same_batch_results(a, b):
+    assert len(a) == len(b)
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert x[0] == y[0], i
+        if x[0]:
+            assert x == y, i
+            continue
+        assert all((c == d).all() for c, d in zip(x[1], y[1])), i
+        assert (x[2] is None) == (y[2] is None) and (x[2] is None or (x[2] == y[2]).all()), i
+        assert x[3] == y[3], i
+
+
+def test_batch_trop_graph_rounds_equal_host_rounds(ctx):
+    """The batched Trop group's device-driven rounds (a hipGraph of push
+    rounds sized on the device, per-program round counts and caps) against
+    the host-driven rounds: every status, tuple, weight and per-program round
+    count is identical.  One batch runs narrow cells and holds a program whose
+    sums pass 2^32 (the rounds leave, the cells widen, the round is finished
+    on the host); the other holds a negative cycle and an overflow."""
+    base = [(p.program, p.bulk) for p in wl.batch_programs(seed=46, count=24, n_lo=300, n_hi=4000, m_lo=1200,
+                                                            m_hi=20000, cost_sources=2)]
+    chain = np.arange(60, dtype=np.uint32)
+    widen = {"edge": ([chain, chain + 1], np.full(60, 1 << 30, np.int64)),
+             "src": ([np.array([0], np.uint32)], np.zeros(1, np.int64))}
+    a = base + [(cost_program(), widen)]
+    neg = {"edge": ([np.array([0, 1, 1], np.uint32), np.array([1, 0, 2], np.uint32)], np.array([1, -2, 5], np.int64)),
+           "src": ([np.array([0], np.uint32)], np.array([0], np.int64))}
+    ovf = {"edge": ([np.array([0, 1, 2], np.uint32), np.array([1, 2, 3], np.uint32)],
+                    np.array([2**62, 2**62, 2**62], np.int64)),
+           "src": ([np.array([0], np.uint32)], np.array([0], np.int64))}
+    bb = base + [(cost_program(), neg), (cost_program(), ovf)]
+    for progs in (a, bb):
+        on, recs_on = _batch_run(ctx, progs, True)
+        off, recs_off = _batch_run(ctx, progs, False)
+        assert "graph_rounds_trop" in recs_on and "graph_rounds_trop" not in recs_off
+        _same_batch_results(on, off)
+    assert on[-2][0] == 6 and on[-1][0] == 2  # NegativeWeightCycle, Overflow
+    # and the batch's answers against the oracle
+    for (prog, bulk), r in zip(a, _batch_run(ctx, a, True)[0]):
+        rel = "cost" if any(p.name == "cost" for p in prog.predicates) else "path"
+        ocols, ow, oit = oracle_cols(prog, bulk, rel)
+        assert r[0] == 0 and all((c == oc).all() for c, oc in zip(r[1], ocols)) and r[3] == oit
+        if ow is not None:
+            assert (r[2] == ow).all()
+
+
 def test_batch_add_refuses_semiring_mismatch(ctx):
     """Weights go with Trop predicates only, as a plain run's staging
     requires: weights on a Bool predicate, or none on a Trop one, are
