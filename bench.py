@@ -589,7 +589,7 @@ def bench_sort(args):
         t = float(np.median(ms))
         # every candidate pinned in turn, the same keys (stderr)
         per_cand = {}
-        for c in range(8):  # every tuner candidate (sk_ctx_force_radix)
+        for c in range(sk.radix_candidates()):  # every tuner candidate (sk_ctx_force_radix)
             ctx.force_radix(c)
             cm = []
             for _ in range(3):
@@ -606,8 +606,10 @@ def bench_sort(args):
         rec = 8 + (8 if weighted else 0)
         alg_bytes = n * 8 + passes * 2 * rec * n
         gbs = alg_bytes / (t * 1e-3) / 1e9
-        out.append({"weighted": weighted, "n": n, "key_bits": 44, "digit_bits": choice["digit_bits"],
-                    "items_per_thread": choice["items_per_thread"], "passes": passes, "ms_per_sort": round(t, 3),
+        out.append({"weighted": weighted, "n": n, "key_bits": 44, "candidate": choice["candidate"],
+                    "digit_bits": choice["digit_bits"], "items_per_thread": choice["items_per_thread"],
+                    "block": choice["block"], "single_sweep": choice["sweep"], "passes": passes,
+                    "ms_per_sort": round(t, 3),
                     "keys_per_s": round(n / (t * 1e-3), 1), "alg_gbps": round(gbs, 1),
                     "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBPS, 4), "sorted_equals_torch": ok,
                     "ms_per_sort_by_candidate": per_cand})
@@ -843,6 +845,16 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+    if not args.sort:
+        # the radix autotuner times every size class once per device and
+        # shares the timings with every later context of the process: do it
+        # here, before any timed region
+        import strata_k as sk
+        t = time.perf_counter()
+        tctx = sk.Context(0)
+        tctx.set_autotune(2)
+        tctx.close()
+        log(f"radix autotuner: every class timed in {time.perf_counter() - t:.2f}s")
     if args.sort:
         bench_sort(args)
         return

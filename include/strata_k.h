@@ -391,20 +391,35 @@ sk_status sk_radix_sort_device(sk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, 
                                int64_t* w_alt, uint64_t n, int key_bits, int* result_in_alt);
 
 /* Radix sort autotuner of a context (no reference counterpart).  Each sort
- * of at least 2^16 records picks its digit width (6, 7 or 8 bits) and tile
- * (items per thread) from one-pass timings of every candidate, measured on
- * the device per key type, weights and size class (2^16 ... 2^24 records).
+ * of at least 2^16 records picks a candidate configuration: digit width (6,
+ * 7 or 8 bits), tile (items per thread), threads per block, and the pass
+ * structure (a histogram and a scatter launch per pass, or single-sweep
+ * passes with one histogram launch for every pass).  The choice comes from
+ * two-pass timings of every candidate, measured on the device per key type,
+ * weights and size class (2^16 ... 2^24 records).
  * mode 0: a fixed default (8-bit digits); 1 (the default): a class is timed
  * on its first sort; 2: time every class now. */
 sk_status sk_ctx_set_autotune(sk_ctx* ctx, int mode);
-/* Pin every sort of the context to candidate c (0 .. 7: digit bits 8, 8, 7,
- * 7, 6, 6 with 16 / 8 items per thread, then 8 and 6 with 32; half the
- * items for 128-bit keys), or -1 to return to the tuner.  A testing knob. */
+/* Pin every sort of the context to candidate c (0 .. sk_radix_candidates()
+ * - 1, described by sk_radix_candidate), or -1 to return to the tuner.  A
+ * testing knob. */
 sk_status sk_ctx_force_radix(sk_ctx* ctx, int candidate);
 /* The configuration a sort of n records with key_bits significant bits would
  * take now; *est_ms the tuner's estimate (-1 when the class is untimed). */
 sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits,
                               int* digit_bits, int* items, double* est_ms);
+/* The candidate index behind sk_ctx_radix_choice's answer. */
+sk_status sk_ctx_radix_pick(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits, int* candidate,
+                            double* est_ms);
+/* Number of radix candidates, and candidate c's digit bits, items per thread
+ * (for 128-bit keys when wide), threads per block and pass structure
+ * (*sweep = 1: single-sweep passes). */
+int sk_radix_candidates(void);
+/* The tuner's measured time of one pass (ms) of every candidate for size
+ * class cls (0 .. 4: 2^16, 2^18, ..., 2^24 records), written to ms[0 ..
+ * n_ms - 1]; SK_ERR_INVALID_ARGUMENT when the class is untimed. */
+sk_status sk_ctx_radix_timing(sk_ctx* ctx, int wide, int weighted, int cls, float* ms, int n_ms);
+sk_status sk_radix_candidate(int c, int wide, int* digit_bits, int* items, int* block, int* sweep);
 
 /* ------------------------------------------------------------ batched runs */
 

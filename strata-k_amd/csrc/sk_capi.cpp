@@ -112,7 +112,8 @@ void sk_ctx_destroy(sk_ctx* ctx) {
 
 sk_status sk_ctx_synchronize(sk_ctx* ctx) {
   if (!ctx) return SK_ERR_INVALID_ARGUMENT;
-  return hipStreamSynchronize(ctx->c.stream) == hipSuccess ? SK_OK : SK_ERR_DEVICE;
+  if (hipStreamSynchronize(ctx->c.stream) != hipSuccess) return SK_ERR_DEVICE;
+  return ctx->c.fault_h && *ctx->c.fault_h ? SK_ERR_DEVICE : SK_OK;
 }
 
 void sk_ctx_error(const sk_ctx* ctx, sk_error_info* out) {

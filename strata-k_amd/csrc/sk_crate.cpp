@@ -354,9 +354,9 @@ sk_status sk_ctx_force_radix(sk_ctx* ctx, int candidate) {
   return SK_OK;
 }
 
-sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits, int* digit_bits,
-                              int* items, double* est_ms) {
-  if (!ctx || !digit_bits || !items || key_bits <= 0 || key_bits > 128) return SK_ERR_INVALID_ARGUMENT;
+sk_status sk_ctx_radix_pick(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits, int* candidate,
+                            double* est_ms) {
+  if (!ctx || !candidate || key_bits <= 0 || key_bits > 128) return SK_ERR_INVALID_ARGUMENT;
   RadixTune& t = ctx->c.tune;
   int cls = 0;
   while (cls < RadixTune::kClasses - 1 && (n >> (RadixTune::kMinLog2 + 2 * cls + 1)) > 0) cls++;
@@ -376,9 +376,39 @@ sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, i
   } else if (t.forced > 0) {
     best = t.forced;
   }
+  *candidate = best;
+  if (est_ms) *est_ms = bt;
+  return SK_OK;
+}
+
+sk_status sk_ctx_radix_choice(sk_ctx* ctx, int wide, int weighted, uint64_t n, int key_bits, int* digit_bits,
+                              int* items, double* est_ms) {
+  if (!ctx || !digit_bits || !items) return SK_ERR_INVALID_ARGUMENT;
+  int best = 0;
+  const sk_status s = sk_ctx_radix_pick(ctx, wide, weighted, n, key_bits, &best, est_ms);
+  if (s != SK_OK) return s;
   *digit_bits = radix_candidate_digit(best);
   *items = radix_candidate_items(best, wide != 0);
-  if (est_ms) *est_ms = bt;
+  return SK_OK;
+}
+
+int sk_radix_candidates(void) { return radix_candidates(); }
+
+sk_status sk_ctx_radix_timing(sk_ctx* ctx, int wide, int weighted, int cls, float* ms, int n_ms) {
+  if (!ctx || !ms || cls < 0 || cls >= RadixTune::kClasses || n_ms < 0) return SK_ERR_INVALID_ARGUMENT;
+  const RadixTune& t = ctx->c.tune;
+  const int kw = wide ? 1 : 0, hw = weighted ? 1 : 0;
+  if (!t.tuned[kw][hw][cls]) return SK_ERR_INVALID_ARGUMENT;
+  for (int c = 0; c < n_ms && c < radix_candidates(); c++) ms[c] = t.pass_ms[kw][hw][cls][c];
+  return SK_OK;
+}
+
+sk_status sk_radix_candidate(int c, int wide, int* digit_bits, int* items, int* block, int* sweep) {
+  if (c < 0 || c >= radix_candidates() || !digit_bits || !items || !block || !sweep) return SK_ERR_INVALID_ARGUMENT;
+  *digit_bits = radix_candidate_digit(c);
+  *items = radix_candidate_items(c, wide != 0);
+  *block = radix_candidate_block(c);
+  *sweep = radix_candidate_sweep(c) ? 1 : 0;
   return SK_OK;
 }
 

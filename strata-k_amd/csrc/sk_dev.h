@@ -192,11 +192,13 @@ struct Scope {
 // Radix sort autotuner state of a context: per key type (u64 / u128), with
 // or without weights, and size class (2^16, 2^18, ..., 2^24 records), the
 // measured time of one pass of every candidate configuration (digit bits x
-// items per thread).  A sort takes the candidate with the least
-// passes x time for its key width; a class is timed on first use (`lazy`)
-// or all at once by radix_autotune.  `forced` >= 0 pins one candidate.
+// items per thread x threads per block, reduce-then-scan or single-sweep
+// passes).  A sort takes the candidate with the least passes x time for its
+// key width; a class is timed on first use (`lazy`) or all at once by
+// radix_autotune, and the timings are shared by every context of the
+// process on the same device.  `forced` >= 0 pins one candidate.
 struct RadixTune {
-  static constexpr int kClasses = 5, kMinLog2 = 16, kMaxCands = 8;
+  static constexpr int kClasses = 5, kMinLog2 = 16, kMaxCands = 20;
   bool lazy = true;
   int forced = -1;
   bool tuned[2][2][kClasses] = {};
@@ -213,7 +215,15 @@ struct Exec {
   u64* host = nullptr;          // pinned words for small device-to-host reads
   hipEvent_t read_ev = nullptr;  // polled behind those reads
   RadixTune* tune = nullptr;    // radix configuration choices (null: the default)
+  // Pinned host word (device-visible pointer and host pointer) that a
+  // single-sweep radix pass sets when a look-back exceeds its spin bound;
+  // read_u64s and the engine's status reads raise DeviceError when it is set.
+  u32* fault_d = nullptr;
+  const volatile u32* fault_h = nullptr;
 };
+
+// Throws DeviceError when a kernel has set the context's fault word.
+void check_fault(const Exec& x);
 
 // ------------------------------------------------------ primitive wrappers
 
@@ -243,6 +253,8 @@ void radix_autotune(const Exec& x, RadixTune* t);
 int radix_candidates();
 int radix_candidate_digit(int c);
 int radix_candidate_items(int c, bool wide);
+int radix_candidate_block(int c);
+bool radix_candidate_sweep(int c);
 
 // Collapse runs of equal keys of a sorted array.  Bool: keep one.  With
 // weights: keep the minimum (+inf above every finite).  Returns the count.

@@ -120,6 +120,9 @@ void Ctx::init(int dev, u64 budget) {
   SK_HIP(hipMalloc((void**)&derr, sizeof(DevErr)));
   SK_HIP(hipMemsetAsync(derr, 0, sizeof(DevErr), stream));
   SK_HIP(hipHostMalloc((void**)&host_words, 64, hipHostMallocDefault));
+  SK_HIP(hipHostMalloc((void**)&fault_h, 64, hipHostMallocMapped));
+  *fault_h = 0;
+  SK_HIP(hipHostGetDevicePointer((void**)&fault_d, fault_h, 0));
   SK_HIP(hipEventCreateWithFlags(&read_ev, hipEventDisableTiming));
   asp_scratch = asp_scratch_address(dev);
   SK_HIP(hipStreamSynchronize(stream));
@@ -271,10 +274,12 @@ void Ctx::release() {
   pool.reset();
   if (derr) (void)hipFree(derr);
   if (host_words) (void)hipHostFree(host_words);
+  if (fault_h) (void)hipHostFree(fault_h);
   if (read_ev) (void)hipEventDestroy(read_ev);
   if (stream) (void)hipStreamDestroy(stream);
   derr = nullptr;
   host_words = nullptr;
+  fault_h = fault_d = nullptr;
   read_ev = nullptr;
   stream = nullptr;
 }
@@ -2355,6 +2360,7 @@ DevErr Engine::read_device_status() {
   DevErr h{};
   SK_HIP(hipMemcpyAsync(&h, ctx_->derr, sizeof(DevErr), hipMemcpyDeviceToHost, x.stream));
   SK_HIP(hipStreamSynchronize(x.stream));
+  check_fault(x);
   return h;
 }
 

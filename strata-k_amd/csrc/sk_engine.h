@@ -328,6 +328,9 @@ struct Ctx {
   DevErr* derr = nullptr;
   u64* host_words = nullptr;
   hipEvent_t read_ev = nullptr;
+  // pinned fault word of the single-sweep radix passes (Exec::fault_*)
+  u32* fault_h = nullptr;
+  u32* fault_d = nullptr;
   EvalErr last;
   RadixTune tune;
   // 4-clique counter test knobs: every edge to the wave path, and the wave
@@ -341,7 +344,7 @@ struct Ctx {
   static constexpr u64 kStageBytes = 64ull << 20;
   u8* stage[2] = {nullptr, nullptr};
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
-  Exec exec() { return Exec{pool.get(), stream, &timer, derr, host_words, read_ev, &tune}; }
+  Exec exec() { return Exec{pool.get(), stream, &timer, derr, host_words, read_ev, &tune, fault_d, fault_h}; }
   void init(int dev, u64 budget);  // throws DeviceError
   void release();
   // Device-to-host copy into caller memory.  Small copies are enqueued on

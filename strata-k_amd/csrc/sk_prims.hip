@@ -9,6 +9,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace sk {
 
@@ -113,6 +116,12 @@ void read_u64s(const Exec& x, const u64* d, u64* h, int n) {
     if (q != hipErrorNotReady) SK_HIP(q);
   }
   for (int i = 0; i < n; i++) h[i] = x.host[i];
+  check_fault(x);
+}
+
+void check_fault(const Exec& x) {
+  if (x.fault_h && *x.fault_h)
+    throw DeviceError{hipErrorLaunchFailure, "radix sort: a single-sweep look-back exceeded its spin bound"};
 }
 
 // ===================================================================== scan
@@ -289,6 +298,18 @@ __device__ __forceinline__ u32 digit_of(K k, int shift) {
   return (u32)(k >> shift) & ((1u << D) - 1u);
 }
 
+// Adds the valid lanes' digits to the LDS histogram h: one atomic of 64 when
+// the whole wave holds one digit (sorted or clustered keys would otherwise
+// send the wave to one address), else one atomic per lane.  The test costs
+// one compare; `nvalid` (wave-uniform) counts the valid lanes, a prefix of
+// the wave.  Must be reached by every lane.
+__device__ __forceinline__ void hist_add(u32* h, u32 d, bool valid, u32 nvalid) {
+  const u32 d0 = (u32)__builtin_amdgcn_readfirstlane((int)d);
+  const bool uni = nvalid == 64 && __ballot(d != d0) == 0;
+  const bool act = uni ? lane_id() == 0 : valid;
+  if (act) atomicAdd(&h[uni ? d0 : d], uni ? 64u : 1u);
+}
+
 template <class K, int D>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, u64 n, u64 chunk,
                                                        int shift, u32* __restrict__ counts, u32 G) {
@@ -309,12 +330,13 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
       const u64 i = base + (u64)q * kBlock + threadIdx.x;
       kv[q] = keys[i < hi ? i : hi - 1];
     }
-    // one LDS atomic per key: cheaper than matching equal digits across the
-    // wave first (D ballots per key)
+    // one LDS atomic per key, or one per wave when the wave holds one digit
+    // (no D-ballot match)
 #pragma unroll
     for (int q = 0; q < kHistBatch; q++) {
-      const bool valid = base + (u64)q * kBlock + threadIdx.x < hi;
-      if (valid) atomicAdd(&h[w][digit_of<K, D>(kv[q], shift)], 1u);
+      const u64 i0 = base + (u64)q * kBlock + (threadIdx.x & ~63u);
+      const u32 nvalid = i0 >= hi ? 0u : (u32)min((u64)64, hi - i0);
+      hist_add(h[w], digit_of<K, D>(kv[q], shift), lane_id() < nvalid, nvalid);
     }
   }
   __syncthreads();
@@ -451,14 +473,351 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
   }
 }
 
-// The candidate configurations the tuner times: digit bits x items per thread
+// ============================================================ single-sweep radix
+//
+// The pass structure of the candidates marked `sweep`:
+//   1. k_os_hist reads the keys once and counts the digits of every pass
+//      (per block in LDS, then one global add per (pass, digit) and block);
+//   2. k_os_scan turns each pass's counts into its bucket starts;
+//   3. each pass is one launch of k_os_pass, whose tiles take their ids in
+//      launch order from a counter and rank their records in LDS.  A tile
+//      finds how many records of each digit the earlier tiles hold by
+//      looking back over their published counts:
+//        - at once it publishes its own digit counts (status AGG); tile 0
+//          publishes them as a prefix (status PRE);
+//        - one thread per digit walks back from the predecessor, adding AGG
+//          counts, until it reads a PRE, then publishes PRE with the sum plus
+//          its own count.  kOsWindow predecessors are read per step.
+//      A tile waits only on tiles with smaller ids, which took their ids
+//      earlier and are running, and those publish AGG without waiting on
+//      anything, so every walk ends.  The spin bound only guards against a
+//      defect: past it the tile stops waiting and sets the context's fault
+//      word, which the host turns into an error (an undercounted prefix keeps
+//      every write in bounds).
+// Status and count share one 64-bit word, written and read with relaxed
+// device-scope atomics: nothing else is ordered by them, so no fence and no
+// L2 write-back is needed; a publication is one store and a look-back step
+// one load served below the XCDs' L2s.  Each pass clears, row by row, the
+// words the next pass publishes into (two word arrays alternate).
+
+constexpr u64 kOsAgg = 1ull << 62, kOsPre = 2ull << 62, kOsVal = (1ull << 62) - 1;
+#ifndef SK_OS_WINDOW
+#define SK_OS_WINDOW 4
+#endif
+constexpr int kOsWindow = SK_OS_WINDOW;
+constexpr u32 kOsSpinBound = 1u << 22;   // sleeping retries of one look-back (~0.1 s) before it gives up
+constexpr u32 kOsMaxSlots = 4096;        // passes x buckets counted by k_os_hist (128-bit keys at 8 bits: 16 x 256)
+constexpr u64 kOsMaxTiles = 1ull << 16;  // tiles of a pass (their published words: tiles x buckets x 16 B)
+constexpr int kOsHistBlock = 1024;
+
+struct OsPass {
+  u64* flags;         // this pass: tiles x buckets words
+  u64* flags_next;    // the next pass's words, cleared row by row
+  u32* tile_ctr;      // this pass's tile counter
+  const u64* gstart;  // this pass's bucket starts
+  u32* fault;         // pinned host word (device pointer)
+};
+
+// Digit counts of every pass: per block in LDS (hist_add), then one global
+// add per (pass, bucket) and block.
+template <class K, int D>
+__global__ __launch_bounds__(kOsHistBlock) void k_os_hist(const K* __restrict__ keys, u64 n, int lo_bit, int P,
+                                                          unsigned long long* __restrict__ counts) {
+  constexpr u32 NB = 1u << D;
+  __shared__ u32 h[kOsMaxSlots];
+  const u32 slots = (u32)P * NB;
+  const u32 lane = lane_id();
+  for (u32 i = threadIdx.x; i < slots; i += kOsHistBlock) h[i] = 0;
+  __syncthreads();
+  constexpr int kB = 4;  // keys per thread in flight
+  const u64 step = (u64)gridDim.x * kOsHistBlock * kB;
+  for (u64 base = (u64)blockIdx.x * kOsHistBlock * kB; base < n; base += step) {
+    K kv[kB];
+    u32 nv[kB];
+#pragma unroll
+    for (int q = 0; q < kB; q++) {
+      const u64 i = base + (u64)q * kOsHistBlock + threadIdx.x;
+      kv[q] = keys[i < n ? i : n - 1];
+      const u64 i0 = i - lane;  // the wave's first position: valid lanes are a prefix
+      nv[q] = i0 >= n ? 0u : (u32)min((u64)64, n - i0);
+    }
+    for (int p = 0; p < P; p++) {
+      const int sh = lo_bit + D * p;
+#pragma unroll
+      for (int q = 0; q < kB; q++) hist_add(h + p * NB, digit_of<K, D>(kv[q], sh), lane < nv[q], nv[q]);
+    }
+  }
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < slots; i += kOsHistBlock)
+    if (h[i]) atomicAdd(&counts[i], (unsigned long long)h[i]);
+}
+
+// Bucket starts of pass blockIdx.x: exclusive scan of its NB counts (one
+// thread per bucket; NB is a multiple of 64).
+template <int D>
+__global__ __launch_bounds__(256) void k_os_scan(const u64* __restrict__ counts, u64* __restrict__ gstart) {
+  constexpr u32 NB = 1u << D;
+  __shared__ u64 part[4];
+  const u32 d = threadIdx.x;
+  const u64 v = counts[(u64)blockIdx.x * NB + d];
+  const u64 inc = wave_incl_scan(v);
+  if (lane_id() == 63) part[wave_id()] = inc;
+  __syncthreads();
+  u64 off = 0;
+  for (u32 q = 0; q < wave_id(); q++) off += part[q];
+  gstart[(u64)blockIdx.x * NB + d] = off + inc - v;
+}
+
+// 512-thread blocks are held to 128 VGPRs (four waves per SIMD), so that two
+// blocks fit a CU whenever their LDS does.
+template <class K, bool HASW, int D, int IT, int NT>
+__global__ __launch_bounds__(NT, NT == 512 ? 4 : 1) void k_os_pass(const K* __restrict__ kin, K* __restrict__ kout,
+                                                const i64* __restrict__ win, i64* __restrict__ wout,
+                                                const u8* __restrict__ fin, u8* __restrict__ fout, u64 n, int shift,
+                                                OsPass ps) {
+  constexpr u32 NB = 1u << D;
+  constexpr int NW = NT / 64;
+  constexpr u32 TILE = (u32)NT * IT;
+  static_assert(NB <= (u32)NT && NB <= 256 && NB % 64 == 0, "one thread per bucket");
+  static_assert(sizeof(K) >= sizeof(i64), "weights reuse the key stage");
+  static_assert(TILE <= 65535, "16-bit per-wave offsets");
+  __shared__ K s_buf[TILE];
+  __shared__ u8 s_dig[HASW ? TILE : 1];
+  __shared__ u16 s_wcnt[NW][NB];  // per-wave digit counts, then offsets (< TILE <= 2^16)
+  __shared__ u32 s_start[NB];
+  __shared__ u64 s_gbase[NB];
+  __shared__ u64 s_part[4];
+  __shared__ u32 s_tile;
+
+  // per-wave peer masks of the ranking, in the staging buffer (idle until
+  // the keys are staged): entry d holds the lanes of the current step whose
+  // digit is d
+  u64 (*s_peer)[NB] = reinterpret_cast<u64 (*)[NB]>(s_buf);
+  static_assert((u64)NW * NB * 8 <= (u64)TILE * sizeof(K), "peer masks fit the stage");
+  const u32 tid = threadIdx.x, lane = lane_id(), wv = wave_id();
+  if (tid == 0) s_tile = atomicAdd(ps.tile_ctr, 1u);
+  for (u32 i = tid; i < NW * NB; i += NT) {
+    (&s_wcnt[0][0])[i] = 0;
+    (&s_peer[0][0])[i] = 0;
+  }
+  __syncthreads();
+  const u32 tile = s_tile;
+  const u64 tb = (u64)tile * TILE;
+  const u64 hi = min(n, tb + TILE);  // > tb: the grid has exactly ceil(n / TILE) tiles
+  const u64 wb = tb + (u64)wv * 64 * IT;
+  // valid positions of the wave's stretch (wave-uniform): item it of this
+  // lane is position it * 64 + lane of the stretch
+  const u32 nvw = hi > wb ? (u32)min(hi - wb, (u64)IT * 64) : 0u;
+  const bool has_f = HASW && fin != nullptr;
+  const K* kw = kin + wb;
+  K key[IT];
+  u32 rk[IT];
+  // every key load of the wave's stretch first, at clamped addresses
+  if (nvw == IT * 64) {
+#pragma unroll
+    for (int it = 0; it < IT; it++) key[it] = kw[it * 64 + lane];
+  } else {
+    const u32 last = nvw ? nvw - 1 : 0;  // a wave past the tile's end reads a valid record it never uses
+    const K* kc = nvw ? kw : kin + (hi - 1);
+#pragma unroll
+    for (int it = 0; it < IT; it++) key[it] = kc[min((u32)(it * 64) + lane, last)];
+  }
+  // stable rank within the wave's digit group; per-wave digit counts in LDS.
+  // Per step every valid lane ORs its bit into its digit's peer mask, and
+  // reads the mask and the digit's count so far; the lowest lane of each
+  // mask then adds the mask's size to the count and clears the mask.  A
+  // wave's LDS operations complete in issue order; the fences keep the
+  // compiler from reordering them.
+  const u64 mybit = 1ull << lane;
+  const u64 below = lanemask_lt();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const bool valid = (u32)(it * 64) + lane < nvw;
+    const u32 d = digit_of<K, D>(key[it], shift);
+    if (valid) atomicOr((unsigned long long*)&s_peer[wv][d], (unsigned long long)mybit);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    u64 m = 0;
+    u32 old = 0;
+    if (valid) {
+      m = s_peer[wv][d];
+      old = s_wcnt[wv][d];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (m & below) == 0) {  // the lowest lane of its digit
+      s_peer[wv][d] = 0;
+      s_wcnt[wv][d] = (u16)(old + (u32)__popcll(m));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    rk[it] = old + (u32)__popcll(m & below);
+  }
+  __syncthreads();
+  // the tile's digit totals, published at once; per-wave offsets
+  u32 tot = 0;
+  if (tid < NB) {
+    u32 run = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) {
+      const u32 c = s_wcnt[q][tid];
+      s_wcnt[q][tid] = (u16)run;
+      run += c;
+    }
+    tot = run;
+    __hip_atomic_store(&ps.flags[(u64)tile * NB + tid], (tile ? kOsAgg : kOsPre) | (u64)tot, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    ps.flags_next[(u64)tile * NB + tid] = 0;
+  }
+  // exclusive scan of the totals over the digits (waves 0 .. NB/64 - 1)
+  u64 inc = 0;
+  if (tid < NB) {
+    inc = wave_incl_scan((u64)tot);
+    if (lane == 63) s_part[wv] = inc;
+  }
+  __syncthreads();
+  if (tid < NB) {
+    u64 off = 0;
+    for (u32 q = 0; q < wv; q++) off += s_part[q];
+    s_start[tid] = (u32)(off + inc - tot);
+    // look back over the earlier tiles for this digit
+    u64 excl = 0;
+    if (tile) {
+      long j = (long)tile - 1;
+      u32 spins = 0;
+      for (;;) {
+        u64 v[kOsWindow];
+#pragma unroll
+        for (int q = 0; q < kOsWindow; q++)
+          v[q] = j - q >= 0 ? __hip_atomic_load(&ps.flags[(u64)(j - q) * NB + tid], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : kOsPre;
+        int q = 0;
+        bool done = false;
+        for (; q < kOsWindow; q++) {
+          const u64 st = v[q] >> 62;
+          if (!st) break;
+          excl += v[q] & kOsVal;
+          if (st == 2) {
+            done = true;
+            break;
+          }
+        }
+        if (done) break;
+        j -= q;
+        if (q < kOsWindow) {
+          if (++spins > kOsSpinBound) {
+            __hip_atomic_store(ps.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __hip_atomic_store(&ps.flags[(u64)tile * NB + tid], kOsPre | (excl + tot), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_gbase[tid] = ps.gstart[tid] + excl;
+  }
+  __syncthreads();
+  // stage the keys in digit order, then write each digit's run
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    if ((u32)(it * 64) + lane < nvw) {
+      const u32 d = digit_of<K, D>(key[it], shift);
+      s_buf[s_start[d] + s_wcnt[wv][d] + rk[it]] = key[it];
+    }
+  }
+  __syncthreads();
+  const u32 tn = (u32)(hi - tb);
+  if (tn == TILE) {
+    // a full tile: every thread writes IT records, four at a time with their
+    // LDS reads in flight together
+#pragma unroll 4
+    for (int it = 0; it < IT; it++) {
+      const u32 p = tid + (u32)it * NT;
+      const K k = s_buf[p];
+      const u32 d = digit_of<K, D>(k, shift);
+      kout[s_gbase[d] + (p - s_start[d])] = k;
+      if constexpr (HASW) s_dig[p] = (u8)d;
+    }
+  } else {
+    for (u32 p = tid; p < tn; p += NT) {
+      const K k = s_buf[p];
+      const u32 d = digit_of<K, D>(k, shift);
+      kout[s_gbase[d] + (p - s_start[d])] = k;
+      if constexpr (HASW) s_dig[p] = (u8)d;
+    }
+  }
+  if constexpr (HASW) {
+    // the weights (and flags) are read only now, straight into their ranked
+    // stage positions: no registers hold them across the ranking and the
+    // look-back, which leaves room for a second tile per CU
+    __syncthreads();
+    i64* s_w = reinterpret_cast<i64*>(s_buf);
+    const i64* ww = win + wb;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const u32 li = (u32)(it * 64) + lane;
+      if (li < nvw) {
+        const u32 d = digit_of<K, D>(key[it], shift);
+        s_w[s_start[d] + s_wcnt[wv][d] + rk[it]] = ww[li];
+      }
+    }
+    __syncthreads();
+    if (tn == TILE) {
+#pragma unroll 4
+      for (int it = 0; it < IT; it++) {
+        const u32 p = tid + (u32)it * NT;
+        const u32 d = s_dig[p];
+        wout[s_gbase[d] + (p - s_start[d])] = s_w[p];
+      }
+    } else {
+      for (u32 p = tid; p < tn; p += NT) {
+        const u32 d = s_dig[p];
+        wout[s_gbase[d] + (p - s_start[d])] = s_w[p];
+      }
+    }
+    if (has_f) {
+      __syncthreads();
+      u8* s_f = reinterpret_cast<u8*>(s_buf);
+      const u8* fw = fin + wb;
+#pragma unroll
+      for (int it = 0; it < IT; it++) {
+        const u32 li = (u32)(it * 64) + lane;
+        if (li < nvw) {
+          const u32 d = digit_of<K, D>(key[it], shift);
+          s_f[s_start[d] + s_wcnt[wv][d] + rk[it]] = fw[li];
+        }
+      }
+      __syncthreads();
+      for (u32 p = tid; p < tn; p += NT) {
+        const u32 d = s_dig[p];
+        fout[s_gbase[d] + (p - s_start[d])] = s_f[p];
+      }
+    }
+  }
+}
+
+// The candidate configurations the tuner times: digit bits x items per
+// thread x threads per block, by reduce-then-scan passes (a histogram launch
+// and a scatter launch per pass, kBlock threads) or single-sweep passes
 // (u128 keys take half the items, so that a tile with weights fits LDS).
 struct RadixCand {
-  int digit, items;
+  int digit, items, block;
+  bool sweep;
 };
-constexpr RadixCand kRadixCands[] = {{8, 16}, {8, 8}, {7, 16}, {7, 8}, {6, 16}, {6, 8}, {8, 32}, {6, 32}};
+constexpr RadixCand kRadixCands[] = {
+    {8, 16, kBlock, false}, {8, 8, kBlock, false}, {7, 16, kBlock, false}, {7, 8, kBlock, false},
+    {6, 16, kBlock, false}, {6, 8, kBlock, false}, {8, 32, kBlock, false}, {6, 32, kBlock, false},
+    {8, 16, 256, true},     {8, 32, 256, true},    {8, 16, 512, true},     {8, 8, 512, true},
+    {8, 8, 1024, true},     {7, 16, 512, true},    {6, 16, 512, true},     {8, 32, 512, true},
+    {8, 16, 1024, true}};
 constexpr int kNCands = sizeof(kRadixCands) / sizeof(kRadixCands[0]);
 static_assert(kNCands <= RadixTune::kMaxCands, "tuner table width");
+// The reduce-then-scan candidate a sweep candidate falls back to when a sort
+// has more tiles than the sweep's published words allow.
+constexpr int kOsFallback[] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 6, 0, 1, 1, 2, 4, 6, 0};
+static_assert(sizeof(kOsFallback) / sizeof(int) == kNCands, "fallback per candidate");
 
 template <class K>
 constexpr int items_for(int c) {
@@ -522,9 +881,114 @@ static void radix_grid(u64 n, u64 tile, u64 gcap, u64* G, u64* chunk) {
   *G = (n + ch - 1) / ch;
 }
 
+// One single-sweep pass of candidate I: one tile per ceil(n / tile) blocks.
+template <class K, bool HASW, int I>
+static void os_pass(const Exec& x, const K* a, K* b, const i64* wa, i64* wb, const u8* fa, u8* fb, u64 n, int shift,
+                    const OsPass& ps, double rec) {
+  constexpr int D = kRadixCands[I].digit, IT = items_for<K>(I), NT = kRadixCands[I].block;
+  const u64 tiles = (n + (u64)NT * IT - 1) / ((u64)NT * IT);
+  Scope sc(x.timer, "radix_sweep", KC_SORT, 2.0 * rec * (double)n, n);
+  k_os_pass<K, HASW, D, IT, NT><<<(unsigned)tiles, NT, 0, x.stream>>>(a, b, wa, wb, fa, fb, n, shift, ps);
+  SK_HIP(hipGetLastError());
+}
+
+template <class K, bool HASW>
+static void os_pass_c(int c, const Exec& x, const K* a, K* b, const i64* wa, i64* wb, const u8* fa, u8* fb, u64 n,
+                      int shift, const OsPass& ps, double rec) {
+  switch (c) {
+#define SK_OS_CASE(I)                                                           \
+  case I:                                                                       \
+    os_pass<K, HASW, I>(x, a, b, wa, wb, fa, fb, n, shift, ps, rec);            \
+    break;
+    SK_OS_CASE(8)
+    SK_OS_CASE(9)
+    SK_OS_CASE(10)
+    SK_OS_CASE(11)
+    SK_OS_CASE(12)
+    SK_OS_CASE(13)
+    SK_OS_CASE(14)
+    SK_OS_CASE(15)
+    SK_OS_CASE(16)
+#undef SK_OS_CASE
+    default:
+      throw std::runtime_error("radix: sweep configuration");
+  }
+}
+
+template <class K>
+static void os_hist_c(int D, const Exec& x, const K* keys, u64 n, int lo_bit, int P, u64* counts) {
+  const u64 per = (u64)kOsHistBlock * 4;
+  const unsigned g = (unsigned)std::max<u64>(1, std::min<u64>(256, (n + per - 1) / per));
+  Scope sc(x.timer, "radix_sweep_hist", KC_SORT, (double)n * sizeof(K), n);
+  if (D == 8) k_os_hist<K, 8><<<g, kOsHistBlock, 0, x.stream>>>(keys, n, lo_bit, P, (unsigned long long*)counts);
+  else if (D == 7) k_os_hist<K, 7><<<g, kOsHistBlock, 0, x.stream>>>(keys, n, lo_bit, P, (unsigned long long*)counts);
+  else k_os_hist<K, 6><<<g, kOsHistBlock, 0, x.stream>>>(keys, n, lo_bit, P, (unsigned long long*)counts);
+  SK_HIP(hipGetLastError());
+}
+
+static void os_scan(int D, const Exec& x, const u64* counts, u64* gstart, int P) {
+  if (D == 8) k_os_scan<8><<<(unsigned)P, 256, 0, x.stream>>>(counts, gstart);
+  else if (D == 7) k_os_scan<7><<<(unsigned)P, 128, 0, x.stream>>>(counts, gstart);
+  else k_os_scan<6><<<(unsigned)P, 64, 0, x.stream>>>(counts, gstart);
+  SK_HIP(hipGetLastError());
+}
+
+// Single-sweep sort of candidate c: false (nothing launched) when the sort
+// has more tiles than the published words allow, or the context has no
+// fault word.
+template <class K>
+static bool radix_run_sweep(const Exec& x, int c, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt,
+                            u64 n, int key_bits, int lo_bit, K** out_keys, i64** out_w, u8** out_inf) {
+  const int D = kRadixCands[c].digit;
+  const u64 NB = 1ull << D;
+  const u64 tile = (u64)kRadixCands[c].block * items_for<K>(c);
+  const u64 tiles = (n + tile - 1) / tile;
+  const int P = (key_bits + D - 1) / D;
+  if (tiles > kOsMaxTiles || (u64)P * NB > kOsMaxSlots || !x.fault_d) return false;
+  const bool hasw = (w != nullptr);
+  const double rec = sizeof(K) + (hasw ? 8.0 : 0) + (inf ? 1.0 : 0);
+  // [counts P*NB][tile counters P][flags 0: tiles*NB] are cleared here;
+  // [flags 1: tiles*NB][bucket starts P*NB] are written before they are read
+  const u64 wc = (u64)P * NB, wt = ((u64)P + 1) / 2, wf = tiles * NB;
+  DBuf<u64> st(x.pool, wc + wt + 2 * wf + wc);
+  u64* counts = st.get();
+  u32* ctr = (u32*)(counts + wc);
+  u64* fl0 = counts + wc + wt;
+  u64* fl1 = fl0 + wf;
+  u64* gstart = fl1 + wf;
+  fill_u32(x, (u32*)counts, 0, 2 * (wc + wt + wf));
+  os_hist_c<K>(D, x, keys, n, lo_bit, P, counts);
+  os_scan(D, x, counts, gstart, P);
+  K* a = keys;
+  K* b = keys_alt;
+  i64* wa = w;
+  i64* wb = w_alt;
+  u8* fa = inf;
+  u8* fb = inf_alt;
+  for (int p = 0; p < P; p++) {
+    OsPass ps{(p & 1) ? fl1 : fl0, (p & 1) ? fl0 : fl1, ctr + p, gstart + (u64)p * NB, x.fault_d};
+    const int shift = lo_bit + D * p;
+    if (hasw) os_pass_c<K, true>(c, x, a, b, wa, wb, fa, fb, n, shift, ps, rec);
+    else os_pass_c<K, false>(c, x, a, b, nullptr, nullptr, nullptr, nullptr, n, shift, ps, rec);
+    std::swap(a, b);
+    std::swap(wa, wb);
+    std::swap(fa, fb);
+  }
+  *out_keys = a;
+  if (out_w) *out_w = wa;
+  if (out_inf) *out_inf = fa;
+  return true;
+}
+
 template <class K>
 static void radix_run(const Exec& x, int c, K* keys, K* keys_alt, i64* w, i64* w_alt, u8* inf, u8* inf_alt, u64 n,
                       int key_bits, int lo_bit, K** out_keys, i64** out_w, u8** out_inf) {
+  if (kRadixCands[c].sweep) {
+    if (radix_run_sweep<K>(x, c, keys, keys_alt, w, w_alt, inf, inf_alt, n, key_bits, lo_bit, out_keys, out_w,
+                           out_inf))
+      return;
+    c = kOsFallback[c];
+  }
   const int D = kRadixCands[c].digit;
   const u64 tile = (u64)kBlock * items_for<K>(c);
   const bool hasw = (w != nullptr);
@@ -563,10 +1027,10 @@ __global__ void k_tune_keys(u64* k, u64 n, u64 seed) {
   k[i] = z ^ (z >> 31);
 }
 
-// Times one pass of every candidate on random keys of the class's size
-// (median of three, HIP events on the context's stream) and records the
-// per-pass times; a sort then takes the candidate with the least
-// passes x time for its key width.
+// Times a two-pass sort under every candidate on random keys of the class's
+// size (median of three, HIP events on the context's stream) and records
+// half of it as the candidate's per-pass time; a sort then takes the
+// candidate with the least passes x time for its key width.
 template <class K>
 static void radix_tune_class(const Exec& x, RadixTune* t, bool hasw, int cls) {
   const u64 n = 1ull << (RadixTune::kMinLog2 + 2 * cls);
@@ -587,13 +1051,20 @@ static void radix_tune_class(const Exec& x, RadixTune* t, bool hasw, int cls) {
       K* ok;
       i64* ow;
       u8* of;
+      // fresh random keys for every sort: a sort leaves k1 ordered on the
+      // timed digits, which would favour some candidates over others
+      k_tune_keys<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, x.stream>>>(k1.get(), words,
+                                                                                      0x5eed + 64 * cls + 3 * c + r);
+      SK_HIP(hipGetLastError());
       SK_HIP(hipEventRecord(e0, x.stream));
       radix_run<K>(xq, c, (K*)k1.get(), (K*)k2.get(), hasw ? w1.get() : nullptr, hasw ? w2.get() : nullptr, nullptr,
-                   nullptr, n, kRadixCands[c].digit, 0, &ok, &ow, &of);
+                   nullptr, n, 2 * kRadixCands[c].digit, 0, &ok, &ow, &of);
       SK_HIP(hipEventRecord(e1, x.stream));
       SK_HIP(hipEventSynchronize(e1));
       SK_HIP(hipEventElapsedTime(&ms[r], e0, e1));
+      ms[r] *= 0.5f;  // per pass (a sweep's histogram shared by both)
     }
+    check_fault(x);
     std::sort(ms, ms + 3);
     t->pass_ms[sizeof(K) == 16][hasw][cls][c] = ms[1];
   }
@@ -602,6 +1073,40 @@ static void radix_tune_class(const Exec& x, RadixTune* t, bool hasw, int cls) {
   t->tuned[sizeof(K) == 16][hasw][cls] = true;
 }
 
+// Per-pass timings are a property of the device, so every context of the
+// process shares them: a class timed by one context (or by
+// sk_ctx_set_autotune) is copied into the next context that needs it
+// instead of being timed again.
+namespace {
+struct TuneRow {
+  float ms[RadixTune::kMaxCands];
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, TuneRow> g_tune;  // (device, wide, weighted, class)
+
+int current_device() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return dev;
+}
+
+void tune_publish(const RadixTune* t, int kw, int hw, int cls) {
+  TuneRow r;
+  std::memcpy(r.ms, t->pass_ms[kw][hw][cls], sizeof(r.ms));
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[std::make_tuple(current_device(), kw, hw, cls)] = r;
+}
+
+bool tune_fetch(RadixTune* t, int kw, int hw, int cls) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tune.find(std::make_tuple(current_device(), kw, hw, cls));
+  if (it == g_tune.end()) return false;
+  std::memcpy(t->pass_ms[kw][hw][cls], it->second.ms, sizeof(it->second.ms));
+  t->tuned[kw][hw][cls] = true;
+  return true;
+}
+}  // namespace
+
 template <class K>
 static int radix_choose(const Exec& x, u64 n, int key_bits, bool hasw) {
   RadixTune* t = x.tune;
@@ -609,9 +1114,10 @@ static int radix_choose(const Exec& x, u64 n, int key_bits, bool hasw) {
   if (t->forced >= 0) return t->forced;
   const int cls = size_class(n);
   const int kw = sizeof(K) == 16;
-  if (!t->tuned[kw][hasw][cls]) {
+  if (!t->tuned[kw][hasw][cls] && !tune_fetch(t, kw, hasw, cls)) {
     if (!t->lazy) return 0;
     radix_tune_class<K>(x, t, hasw, cls);
+    tune_publish(t, kw, hasw, cls);
   }
   int best = 0;
   double bt = 1e30;
@@ -642,7 +1148,9 @@ void radix_autotune(const Exec& x, RadixTune* t) {
   for (int cls = 0; cls < RadixTune::kClasses; cls++)
     for (int hasw = 0; hasw < 2; hasw++) {
       radix_tune_class<u64>(x, t, hasw != 0, cls);
+      tune_publish(t, 0, hasw, cls);
       radix_tune_class<u128>(x, t, hasw != 0, cls);
+      tune_publish(t, 1, hasw, cls);
     }
 }
 
@@ -652,6 +1160,8 @@ int radix_candidate_items(int c, bool wide) {
   return wide ? items_for<u128>(c) : items_for<u64>(c);
 }
 int radix_candidates() { return kNCands; }
+int radix_candidate_block(int c) { return c >= 0 && c < kNCands ? kRadixCands[c].block : 0; }
+bool radix_candidate_sweep(int c) { return c >= 0 && c < kNCands && kRadixCands[c].sweep; }
 
 // ============================================================ run collapse
 

@@ -10,5 +10,5 @@ from .core_ir import (AGG_OPS, BOOL, INF, TROP, Agg, Const, CoreAtom, CorePred, 
                       triangle_program, v)
 from .engine import (Batch, Context, Db, EngineError, EvalError, GpuError, Relation, Run, Server,
                      asp_simplify_filter, binary_plan_intermediate, count_4cliques, count_triangles, default_context,
-                     intern_terms, load_library, program_from_json, shortest_paths_trop, sort_min_pairs, sort_u64,
+                     intern_terms, load_library, program_from_json, radix_candidate, radix_candidates, shortest_paths_trop, sort_min_pairs, sort_u64,
                      sort_unique_pairs, sort_unique_u64, transitive_closure_bool)

@@ -202,6 +202,12 @@ SIGNATURES = [
     ("sk_radix_sort_device", C.c_int, [_VP, _VP, _VP, _VP, _VP, _U64, C.c_int, C.POINTER(C.c_int)]),
     ("sk_ctx_set_autotune", C.c_int, [_VP, C.c_int]),
     ("sk_ctx_force_radix", C.c_int, [_VP, C.c_int]),
+    ("sk_ctx_radix_pick", C.c_int, [_VP, C.c_int, C.c_int, _U64, C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]),
+    ("sk_radix_candidates", C.c_int, []),
+    ("sk_ctx_radix_timing", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+    ("sk_radix_candidate", C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sk_ctx_radix_choice", C.c_int, [_VP, C.c_int, C.c_int, _U64, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("sk_batch_kernel_count", _U32, [_VP]),
@@ -360,6 +366,22 @@ class Db:
 
 # ------------------------------------------------------------------ context
 
+def radix_candidates() -> int:
+    """Number of radix sort candidate configurations."""
+    return int(load_library().sk_radix_candidates())
+
+
+def radix_candidate(c: int, wide: bool = False) -> dict:
+    """Candidate c: digit bits, items per thread (128-bit keys when wide),
+    threads per block, and whether its passes are single-sweep."""
+    d, it, b, sw = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    code = load_library().sk_radix_candidate(int(c), 1 if wide else 0, C.byref(d), C.byref(it), C.byref(b),
+                                             C.byref(sw))
+    if code:
+        _raise(code)
+    return {"digit_bits": d.value, "items_per_thread": it.value, "block": b.value, "sweep": bool(sw.value)}
+
+
 class Context:
     """One device, one stream, one memory pool (sk_ctx)."""
 
@@ -435,7 +457,24 @@ class Context:
                                           C.byref(d), C.byref(it), C.byref(est))
         if code:
             _raise(code)
-        return {"digit_bits": d.value, "items_per_thread": it.value, "est_ms": est.value}
+        pick = C.c_int()
+        code = self.L.sk_ctx_radix_pick(self.h, 1 if wide else 0, 1 if weighted else 0, int(n), int(key_bits),
+                                        C.byref(pick), None)
+        if code:
+            _raise(code)
+        desc = radix_candidate(pick.value, wide)
+        return {"candidate": pick.value, "digit_bits": d.value, "items_per_thread": it.value,
+                "block": desc["block"], "sweep": desc["sweep"], "est_ms": est.value}
+
+    def radix_timing(self, cls: int, wide: bool = False, weighted: bool = False) -> list:
+        """The tuner's per-pass time (ms) of every candidate for size class
+        cls (records 2^(16 + 2 cls))."""
+        nc = radix_candidates()
+        buf = (C.c_float * nc)()
+        code = self.L.sk_ctx_radix_timing(self.h, 1 if wide else 0, 1 if weighted else 0, int(cls), buf, nc)
+        if code:
+            _raise(code)
+        return [round(float(v), 4) for v in buf]
 
     def radix_sort_device(self, keys_ptr: int, alt_ptr: int, n: int, key_bits: int, w_ptr: int = 0,
                           walt_ptr: int = 0) -> bool:

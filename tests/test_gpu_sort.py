@@ -27,7 +27,7 @@ class Lcg:
 SIZES = [0, 1, 2, 3, 5, 7, 8, 100, 1000, 4097]
 
 
-@pytest.fixture(params=[-1, 0, 1, 2, 3, 4, 5, 6, 7], ids=lambda c: f"cand{c}")
+@pytest.fixture(params=[-1] + list(range(sk.radix_candidates())), ids=lambda c: f"cand{c}")
 def radix_ctx(ctx, request):
     ctx.force_radix(request.param)
     yield ctx
@@ -138,12 +138,10 @@ def test_tuner_picks_a_candidate(ctx):
         for wide in (False, True):
             for weighted in (False, True):
                 c = ctx.radix_choice(1 << 20, 40, wide=wide, weighted=weighted)
-                # the tuner's candidates (digit bits, items per thread for u64
-                # keys; half the items for u128 keys)
-                cands = [(8, 16), (8, 8), (7, 16), (7, 8), (6, 16), (6, 8), (8, 32), (6, 32)]
-                if wide:
-                    cands = [(d, i // 2) for d, i in cands]
-                assert (c["digit_bits"], c["items_per_thread"]) in cands, c
+                desc = sk.radix_candidate(c["candidate"], wide)
+                assert 0 <= c["candidate"] < sk.radix_candidates(), c
+                assert (c["digit_bits"], c["items_per_thread"], c["block"], c["sweep"]) == (
+                    desc["digit_bits"], desc["items_per_thread"], desc["block"], desc["sweep"]), c
                 assert c["est_ms"] > 0, c
         keys = np.random.default_rng(3).integers(0, 1 << 40, size=1 << 20, dtype=np.uint64)
         assert (sk.sort_u64(keys, ctx) == np.sort(keys)).all()
