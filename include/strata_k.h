@@ -396,7 +396,7 @@ sk_status sk_radix_sort_device(sk_ctx* ctx, uint64_t* keys, uint64_t* keys_alt, 
  * structure (a histogram and a scatter launch per pass, or single-sweep
  * passes with one histogram launch for every pass).  The choice comes from
  * two-pass timings of every candidate, measured on the device per key type,
- * weights and size class (2^16 ... 2^24 records).
+ * weights and size class (2^16 ... 2^26 records).
  * mode 0: a fixed default (8-bit digits); 1 (the default): a class is timed
  * on its first sort; 2: time every class now. */
 sk_status sk_ctx_set_autotune(sk_ctx* ctx, int mode);
@@ -416,7 +416,7 @@ sk_status sk_ctx_radix_pick(sk_ctx* ctx, int wide, int weighted, uint64_t n, int
  * (*sweep = 1: single-sweep passes). */
 int sk_radix_candidates(void);
 /* The tuner's measured time of one pass (ms) of every candidate for size
- * class cls (0 .. 4: 2^16, 2^18, ..., 2^24 records), written to ms[0 ..
+ * class cls (0 .. 5: 2^16, 2^18, ..., 2^26 records), written to ms[0 ..
  * n_ms - 1]; SK_ERR_INVALID_ARGUMENT when the class is untimed. */
 sk_status sk_ctx_radix_timing(sk_ctx* ctx, int wide, int weighted, int cls, float* ms, int n_ms);
 sk_status sk_radix_candidate(int c, int wide, int* digit_bits, int* items, int* block, int* sweep);

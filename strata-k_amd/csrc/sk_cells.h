@@ -37,7 +37,7 @@ constexpr u64 kPackLim = 1ull << 26;
 // the rounds that ran.
 struct LoopState {
   u32 done;
-  u32 stamp;
+  u32 stamp;  // Bool rounds: the compaction stamp; Trop rounds: the next round's kind (0 pull, 1 push)
   u64 iters;
   u32 any;    // the round's compaction found a touched cell
   u32 leave;  // Trop rounds: 1 the push join is cheaper next round, 2 a device
@@ -314,13 +314,14 @@ void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* 
 // compaction into the capacity buffers with the push cost and a snapshot of
 // the device status (as cells_compact_into, without the host read), then
 // the round's bookkeeping: a status sets leave = 2; otherwise iters += 1,
-// n and pc recorded, leave = 3 past cap_left rounds, done on an empty delta,
-// leave = 1 when pc < push_limit.  cnt / off / pcb hold
-// ncells / (kBlock * 32) entries, tot 8.
+// n and pc recorded, leave = 3 past cap_left rounds, done on an empty delta;
+// when pc < push_limit the next round pushes on the device (stamp = 1) if
+// the delta has at most push_cap rows, else leave = 1; otherwise it pulls
+// (stamp = 0).  cnt / off / pcb hold ncells / (kBlock * 32) entries, tot 8.
 template <class K>
 void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                            u32* cnt, u64* off, u64* pcb, u64* tot, const DevErr* status, LoopState* ls,
-                           u64 cap_left, u64 push_limit);
+                           u64 cap_left, u64 push_limit, u64 push_cap);
 template <class K>
 u64 cells_compact_into(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                        u64* push_cost, const DevErr* status, DevErr* status_out);

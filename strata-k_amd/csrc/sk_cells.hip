@@ -552,7 +552,13 @@ void cells_loop_round(const Exec& x, const CellDesc& cd, u64 ncells, LoopState* 
   SK_HIP(hipGetLastError());
 }
 
-__global__ void k_loop_book_trop(LoopState* ls, const u64* __restrict__ tot, u64 cap_left, u64 push_limit) {
+// The round's bookkeeping of device-driven Trop rounds: leave on a device
+// status (2) or past the round cap (3), done on an empty delta; otherwise
+// the next round pulls, or pushes when the delta's push cost is below
+// push_limit: on the device when the delta fits the push table (push_cap
+// rows), else the rounds leave for the host (1).
+__global__ void k_loop_book_trop(LoopState* ls, const u64* __restrict__ tot, u64 cap_left, u64 push_limit,
+                                 u64 push_cap) {
   if (threadIdx.x != 0 || ls->done || ls->leave) return;
   const DevErr* st = (const DevErr*)(tot + 2);
   if (st->code || st->widen) {
@@ -563,15 +569,19 @@ __global__ void k_loop_book_trop(LoopState* ls, const u64* __restrict__ tot, u64
   ls->iters += 1;
   ls->n = tot[0];
   ls->pc = tot[1];
+  ls->stamp = 0;
   if (ls->iters > cap_left) ls->leave = 3;
   else if (tot[0] == 0) ls->done = 1;
-  else if (tot[1] < push_limit) ls->leave = 1;
+  else if (tot[1] < push_limit) {
+    if (tot[0] <= push_cap) ls->stamp = 1;
+    else ls->leave = 1;
+  }
 }
 
 template <class K>
 void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bits, K* keys, i64* w, u8* inf,
                            u32* cnt, u64* off, u64* pcb, u64* tot, const DevErr* status, LoopState* ls,
-                           u64 cap_left, u64 push_limit) {
+                           u64 cap_left, u64 push_limit, u64 push_cap) {
   static_assert(sizeof(DevErr) == 3 * sizeof(u64), "status record is three words");
   const u64 nwords = (ncells + 31) / 32;
   const u64 nblocks = (nwords + kBlock - 1) / kBlock;
@@ -582,7 +592,7 @@ void cells_loop_round_trop(const Exec& x, const CellDesc& cd, u64 ncells, int bi
   scan_u32_one(x, cnt, off, nblocks, tot, pcb, (const u64*)status, 3);
   k_cells_write<K><<<(unsigned)nblocks, kBlock, 0, x.stream>>>(cd, 0, nwords, 1, off, bits, keys, w, inf);
   SK_HIP(hipGetLastError());
-  k_loop_book_trop<<<1, 64, 0, x.stream>>>(ls, tot, cap_left, push_limit);
+  k_loop_book_trop<<<1, 64, 0, x.stream>>>(ls, tot, cap_left, push_limit, push_cap);
   SK_HIP(hipGetLastError());
 }
 
@@ -822,7 +832,7 @@ void cells_widen(const Exec& x, const u32* c32, u64* c64, u64 n) {
                                      DevErr*);                                                                    \
   template void cells_mark_touched<K>(const Exec&, const CellDesc&, const K*, u64, int, int);                 \
   template void cells_loop_round_trop<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u8*, u32*, u64*, u64*, \
-                                         u64*, const DevErr*, LoopState*, u64, u64);                             \
+                                         u64*, const DevErr*, LoopState*, u64, u64, u64);                             \
   template void cells_loop_round_trop_parts<K>(const Exec&, const CellDesc&, u64, int, K*, i64*, u32*, u64*, u64*,  \
                                                const DevErr*, LoopState*, u8*, u8*, u64*, const u64*,             \
                                                const DevErr*, u32);

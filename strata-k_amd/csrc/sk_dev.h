@@ -190,7 +190,7 @@ struct Scope {
 };
 
 // Radix sort autotuner state of a context: per key type (u64 / u128), with
-// or without weights, and size class (2^16, 2^18, ..., 2^24 records), the
+// or without weights, and size class (2^16, 2^18, ..., 2^26 records), the
 // measured time of one pass of every candidate configuration (digit bits x
 // items per thread x threads per block, reduce-then-scan or single-sweep
 // passes).  A sort takes the candidate with the least passes x time for its
@@ -198,7 +198,7 @@ struct Scope {
 // radix_autotune, and the timings are shared by every context of the
 // process on the same device.  `forced` >= 0 pins one candidate.
 struct RadixTune {
-  static constexpr int kClasses = 5, kMinLog2 = 16, kMaxCands = 20;
+  static constexpr int kClasses = 6, kMinLog2 = 16, kMaxCands = 20;
   bool lazy = true;
   int forced = -1;
   bool tuned[2][2][kClasses] = {};

@@ -1376,7 +1376,7 @@ int Engine::trop_graph_rel(u32 k) {
 // its size and push cost are in place), 2 a device status to handle (*st),
 // 3 the round cap exceeded.  *iters counts the rounds run, as the host loop
 // counts them.
-int Engine::run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st) {
+int Engine::run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st, const RulePlan* pl) {
   (void)k;
   constexpr int kGraphRounds = 4;
   DRel& H = R_[rel];
@@ -1388,14 +1388,48 @@ int Engine::run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st)
   DBuf<LoopState> ls(x.pool, 1);
   DBuf<u32> cnt(x.pool, nb);
   DBuf<u64> off(x.pool, nb), pcb(x.pool, nb), tot(x.pool, 8);
+  const u64 cap_left = cap > *iters ? cap - *iters : 0;
+  const u64 push_limit = pull_mode_ == 2 ? 0 : pull_threshold(R_[pr.E].full.n);
+  // Rounds whose push cost is below push_limit push on the device when the
+  // delta variant has the capture_push shape and the delta fits the push
+  // table (push_cap rows); otherwise such a round leaves for the host.
+  PushCapture pc;
+  const bool dev_push = pl && pull_mode_ != 2 && capture_push(*pl, &pc, false);
+  const u64 push_cap = dev_push ? std::min<u64>(std::max<u64>(dn.ncells, 1), kDevPushRows) : 0;
+  std::vector<DBuf<u32>> tcols;
+  Table t;
+  DBuf<i64> tw;
+  DBuf<u64> plo, poff, pbsum, pjtot;
+  DBuf<u32> pcnt;
+  OutDesc o;
+  if (dev_push) {
+    tcols.resize(pc.tcols);
+    t.ncols = pc.tcols;
+    for (int c = 0; c < pc.tcols; c++) {
+      tcols[c].reset(x.pool, push_cap);
+      t.col[c] = tcols[c].get();
+    }
+    tw.reset(x.pool, push_cap);
+    t.w = tw.get();
+    plo.reset(x.pool, push_cap);
+    poff.reset(x.pool, push_cap);
+    pcnt.reset(x.pool, push_cap);
+    pbsum.reset(x.pool, kDevScanBlocks);
+    pjtot.reset(x.pool, 1);
+    o = pc.o;
+    o.cd = dn.cd;
+  }
   LoopState h0{};
+  h0.n = H.delta.n;
+  // the first round pushes when the host's round left a delta that should
+  const bool cost_ok = pr.cost_dv == H.dversion;
+  h0.pc = cost_ok ? pr.cost : ~0ull;
+  h0.stamp = dev_push && cost_ok && pr.cost < push_limit && H.delta.n <= push_cap ? 1u : 0u;
   SK_HIP(hipMemcpyAsync(ls.get(), &h0, sizeof h0, hipMemcpyHostToDevice, x.stream));
   SK_HIP(hipStreamSynchronize(x.stream));
   CellDesc cdl = dn.cd;
   cdl.gate = ls.get();
   pd.gate = ls.get();
-  const u64 cap_left = cap > *iters ? cap - *iters : 0;
-  const u64 push_limit = pull_mode_ == 2 ? 0 : pull_threshold(R_[pr.E].full.n);
   const bool hf = has_inf_;
   Exec xg = x;
   xg.timer = nullptr;  // no event records inside the capture
@@ -1403,15 +1437,23 @@ int Engine::run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st)
   hipGraphExec_t ge = nullptr;
   SK_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
   try {
-    for (int r = 0; r < kGraphRounds; r++) {
-      pull_apply(xg, pd, cdl, true, 0, 0);
-      with_key(H.wide, [&](auto* tag) {
-        using K = std::remove_pointer_t<decltype(tag)>;
+    with_key(H.wide, [&](auto* tag) {
+      using K = std::remove_pointer_t<decltype(tag)>;
+      for (int r = 0; r < kGraphRounds; r++) {
+        pull_apply(xg, pd, cdl, true, 0, 0);  // returns at once in a push round
+        if (dev_push) {
+          with_key(R_[pc.erel].wide, [&](auto* tage) {
+            using KE = std::remove_pointer_t<decltype(tage)>;
+            push_round_dev<K, KE>(xg, (const K*)H.delta.k.ptr(), H.delta.w.get(), H.arity, H.bits, pc.sd, t, pc.pd,
+                                  (const KE*)pc.skeys, pc.sw, pc.csr, o, plo.get(), pcnt.get(), poff.get(),
+                                  pbsum.get(), pjtot.get(), push_cap, ls.get());
+          });
+        }
         cells_loop_round_trop<K>(xg, cdl, dn.ncells, H.bits, (K*)H.delta.k.ptr(), H.delta.w.get(),
                                  hf ? H.delta.inf.get() : nullptr, cnt.get(), off.get(), pcb.get(), tot.get(),
-                                 ctx_->derr, ls.get(), cap_left, push_limit);
-      });
-    }
+                                 ctx_->derr, ls.get(), cap_left, push_limit, push_cap);
+      }
+    });
   } catch (...) {
     hipGraph_t bad = nullptr;
     (void)hipStreamEndCapture(x.stream, &bad);
@@ -1552,7 +1594,7 @@ int Engine::graph_loop_rel(u32 k) {
 // Trop relation's delta, then one join to a fixed relation writing the head
 // into cells, as launch descriptors (the same ones exec_plan builds), or
 // false for any other shape.
-bool Engine::capture_push(const RulePlan& pl, PushCapture* pc) {
+bool Engine::capture_push(const RulePlan& pl, PushCapture* pc, bool parts) {
   const Rule& r = P_.rules[pl.rule];
   if (pl.tri || pl.unit_source || pl.head_err || r.agg || !pl.trop || pl.steps.size() != 2 || has_inf_) return false;
   const StepPlan& s0 = pl.steps[0];
@@ -1606,8 +1648,8 @@ bool Engine::capture_push(const RulePlan& pl, PushCapture* pc) {
   pc->skeys = d.k.ptr();
   pc->sw = d.w.get();
   pc->csr = want_csr ? ix.csr.get() : nullptr;
-  pc->part_col = col_index(s1.in_slots, (int)r.var_count - 1);
-  return pc->part_col >= 0 && pc->sw && d.n > 0;
+  pc->part_col = parts ? col_index(s1.in_slots, (int)r.var_count - 1) : -1;
+  return (!parts || pc->part_col >= 0) && pc->sw && d.n > 0;
 }
 
 // The relation of a batched Trop stratum whose remaining rounds may run on
@@ -1701,6 +1743,7 @@ int Engine::run_trop_part_graph_loop(u32 k, int rel, const RulePlan& pl, u64* it
   DBuf<u64> pit(x.pool, np), capl(x.pool, np);
   LoopState h0{};
   h0.n = H.delta.n;
+  h0.stamp = 1;  // every round pushes
   std::vector<u64> hcap(np);
   for (u32 p = 0; p < np; p++) {
     const u64 lim = part_universe_[p] + 2, used = part_iters_[p][k];
@@ -3368,7 +3411,8 @@ void Engine::run_stratum(u32 k) {
     const int trel = is_trop ? trop_graph_rel(k) : -1;
     if (trel >= 0) {
       DevErr gst{};
-      const int rc = run_trop_graph_loop(k, trel, cap, &iters, &gst);
+      const PullRule& tpr = dense_of(trel)->pulls[0];
+      const int rc = run_trop_graph_loop(k, trel, cap, &iters, &gst, &plan_of(tpr.rule, tpr.dpos));
       if (rc == 0) break;
       if (rc == 3) throw mkerr(SK_ERR_NEGATIVE_WEIGHT_CYCLE, "", "negative-weight cycle: tropical stratum diverges");
       if (rc == 2) {

@@ -509,7 +509,7 @@ class Engine {
   int graph_loop_rel(u32 k);
   PullDesc pull_desc(PullRule& pr, DenseRel& dn, int hr);
   int trop_graph_rel(u32 k);
-  int run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st);
+  int run_trop_graph_loop(u32 k, int rel, u64 cap, u64* iters, DevErr* st, const RulePlan* pl);
   // Batched Trop strata: descriptors of a delta variant's push round, and
   // the remaining rounds device-driven (see run_trop_part_graph_loop).
   struct PushCapture {
@@ -523,7 +523,7 @@ class Engine {
     const u64* csr = nullptr;
     int part_col = -1;
   };
-  bool capture_push(const RulePlan& pl, PushCapture* pc);
+  bool capture_push(const RulePlan& pl, PushCapture* pc, bool parts = true);
   int trop_part_graph_rel(u32 k, int* rule, int* dpos);
   int run_trop_part_graph_loop(u32 k, int rel, const RulePlan& pl, u64* iters, std::vector<u8>& active_prev,
                                DevErr* st);

@@ -219,6 +219,9 @@ void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop,
 // bsum 1024 words, jtot one.  Every kernel returns at once when the rounds
 // are done or have left (ls).
 constexpr u32 kDevScanBlocks = 1024;
+// Rows of the push table of device-driven plain Trop rounds: a delta with
+// more rows leaves its push round to the host.
+constexpr u64 kDevPushRows = 1ull << 24;
 template <class KD, class KE>
 void push_round_dev(const Exec& x, const KD* dkeys, const i64* dw, int darity, int dbits, const ScanDesc& sd,
                     const Table& t, const ProbeDesc& p, const KE* skeys, const i64* sw, const u64* csr,

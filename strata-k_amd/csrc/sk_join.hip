@@ -1281,7 +1281,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
   const u32 lane = threadIdx.x & 63;
   const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
-  if (pd.gate && (pd.gate->done | pd.gate->leave)) return;
+  if (pd.gate && (pd.gate->done | pd.gate->leave | pd.gate->stamp)) return;  // a push round runs instead
   PullLds& L = sh[threadIdx.x >> 6];
   const PullChunk c = pd.chunks[ch];
   const u32 nr = c.nr_off & 0xFFu;
@@ -1402,6 +1402,21 @@ __global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevEr
   }
 }
 
+// Inclusive prefix maximum of a non-negative value over the 64 lanes of a
+// wave, by DPP lane moves (no LDS): shifts by 1, 2, 4 and 8 within each row
+// of 16 lanes, then row 0's last lane broadcast into row 1 and row 2's into
+// row 3, then lane 31 into rows 2 and 3.  Lanes whose source is outside
+// their row read 0, the identity.
+__device__ __forceinline__ u32 wave_incl_max_u32(u32 v) {
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return v;
+}
+
 // Pairs per lane per step of the packed pull kernel (their loads are issued
 // together before the LDS minimum).
 #ifndef SK_PULL_PAIRS_PACKED
@@ -1414,15 +1429,23 @@ constexpr int kPullPairsP = SK_PULL_PAIRS_PACKED;
 #endif
 constexpr u32 kPullAccStride = SK_PULL_ACC_STRIDE;
 
-// Per wave of the packed pull kernel (narrow Trop cells below 2^26).
+// Waves per SIMD the packed pull kernel is compiled for (its VGPR budget).
+#ifndef SK_PULL_MIN_WAVES
+#define SK_PULL_MIN_WAVES 1
+#endif
+
+// Per wave of the packed pull kernel (narrow Trop cells below 2^26).  One
+// 16-byte record per index row of the batch, read with one LDS load per
+// pair: {delta-list position of the source row's run | the LDS offset of
+// the minima of the destination row it feeds << 48, exclusive prefix of the
+// batch's pair counts | edge weight << 32}.
 struct PullLdsP {
-  u64 base[64];               // delta-list position of the index row's source row's run
-  u32 w[64];                  // edge weight
-  u32 excl[64];               // exclusive prefix of the index rows' pair counts
-  u32 slot[64];               // which of the chunk's destination rows the index row feeds
+  ulonglong2 rec[64];
   u32 acc[kPullRows * kPullAccStride];  // running minimum per (destination row, lane value); kCellTop32 = none
-  u8 start[64 * kPullPairsP];  // 1 + the index row whose pairs start at this position of the step
+  alignas(4) u8 start[64 * kPullPairsP];  // 1 + the index row whose pairs start at this position of the step
 };
+static_assert(kPullRows * kPullAccStride < (1u << 16), "minima offsets fit 16 bits");
+static_assert(kPullPairsP == 4, "a lane's start marks are one 32-bit word");
 
 // The pull round of k_pull<true> over a packed delta (CellDesc::dpack,
 // rowrec8): a source row's delta run is one array of words
@@ -1432,12 +1455,12 @@ struct PullLdsP {
 // above narrow_lim raises the widen flag and is dropped, as a narrow cell
 // drops it.  Pairs are dealt to lanes and folded into a u32 LDS minimum per
 // (destination row, lane value) as in k_pull.
-__global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd) {
+__global__ __launch_bounds__(kBlock, SK_PULL_MIN_WAVES) void k_pull_packed(PullDesc pd, CellDesc cd) {
   __shared__ PullLdsP sh[kBlock / 64];
   const u32 lane = threadIdx.x & 63;
   const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
-  if (pd.gate && (pd.gate->done | pd.gate->leave)) return;
+  if (pd.gate && (pd.gate->done | pd.gate->leave | pd.gate->stamp)) return;  // a push round runs instead
   PullLdsP& L = sh[threadIdx.x >> 6];
   const PullChunk c = pd.chunks[ch];
   const u32 nr = c.nr_off & 0xFFu;
@@ -1445,7 +1468,9 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
   const int lb = pd.lbits;
   const u32 nl = 1u << lb;
   const u32 ncell = nr << lb;  // <= kPullRows * 64
-  const u64 lim = cd.narrow_lim;
+  // packed deltas come with narrow_lim = kPackLim; clamping keeps every sum
+  // below 2^27 (a lower limit would only widen more often, which is exact)
+  const u32 lim = (u32)min((u64)cd.narrow_lim, kPackLim);
   for (u32 i = lane; i < kPullRows * kPullAccStride; i += 64) L.acc[i] = kCellTop32;
   u64 seen[kPullRows];
 #pragma unroll
@@ -1465,7 +1490,7 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
       // bounded by the staged weights) makes every candidate it forms reach
       // lim, i.e. widen, where the wide cells redo the round exactly
       const i64 wraw = pd.w[e];
-      we = (wraw < 0 || (u64)wraw >= lim) ? (u32)lim : (u32)wraw;
+      we = (wraw < 0 || (u64)wraw >= lim) ? lim : (u32)wraw;
       if (q < pd.nrows) {
         const u64 rec = pd.rowrec8[q];
         base = rec & ((1ull << 40) - 1);
@@ -1477,55 +1502,52 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
     const u32 incl = (u32)wave_incl_scan_j(cnt);
     const u32 total = __shfl(incl, 63);
     const u32 excl = incl - cnt;
-    L.base[lane] = base;
-    L.w[lane] = we;
-    L.excl[lane] = excl;
-    L.slot[lane] = slot;
-#pragma unroll
-    for (int u = 0; u < kPullPairsP; u++) L.start[u * 64 + lane] = 0;
-    const u64 le = ~0ull >> (63 - lane);  // lanes at or below this one
+    L.rec[lane] = make_ulonglong2(base | (u64)(slot * kPullAccStride) << 48, (u64)excl | (u64)we << 32);
+    // the step's start marks: position u * 64 + l is byte u of word l, so a
+    // lane reads and clears its four marks with one LDS access each
+    u32* smark = reinterpret_cast<u32*>(L.start);
+    smark[lane] = 0;
     u32 carry = 0;  // the edge holding the step's first pair (pair 0 starts an edge)
     for (u32 p0 = 0; p0 < total; p0 += 64 * kPullPairsP) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if (cnt && excl >= p0 && excl < p0 + 64 * kPullPairsP) L.start[excl - p0] = (u8)(lane + 1);
+      if (cnt && excl >= p0 && excl < p0 + 64 * kPullPairsP) {
+        const u32 pos = excl - p0;
+        L.start[(pos & 63u) * 4 + (pos >> 6)] = (u8)(lane + 1);
+      }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      u32 rs[kPullPairsP];
-#pragma unroll
-      for (int u = 0; u < kPullPairsP; u++) rs[u] = L.start[u * 64 + lane];
+      const u32 mw = smark[lane];
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int u = 0; u < kPullPairsP; u++) L.start[u * 64 + lane] = 0;
-      u32 jj[kPullPairsP], kk[kPullPairsP], vv[kPullPairsP];
+      smark[lane] = 0;
+      ulonglong2 rr[kPullPairsP];
+      u32 vv[kPullPairsP];
 #pragma unroll
       for (int u = 0; u < kPullPairsP; u++) {
-        const u64 sm = __ballot(rs[u] != 0);
-        const u64 mine = sm & le;
-        const int h = mine ? 63 - __builtin_clzll(mine) : 0;
-        const u32 from = (u32)__shfl((int)rs[u], h, 64);
-        const u32 j = mine ? from - 1 : carry;
-        if (sm) carry = (u32)__builtin_amdgcn_readlane((int)rs[u], 63 - __builtin_clzll(sm)) - 1;
+        // the last index row started at or below this position (rows start
+        // in index order), else the row the step's previous positions were in
+        const u32 pm = wave_incl_max_u32((mw >> (8 * u)) & 0xFFu);
+        const u32 j = pm ? pm - 1 : carry;
+        carry = (u32)__builtin_amdgcn_readlane((int)j, 63);
+        rr[u] = L.rec[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kPullPairsP; u++) {
         const u32 pidx = p0 + (u32)u * 64 + lane;
-        jj[u] = j;
-        kk[u] = pidx - L.excl[j];
-      }
-#pragma unroll
-      for (int u = 0; u < kPullPairsP; u++) {
         vv[u] = kCellTop32;
-        if (p0 + (u32)u * 64 + lane < total) vv[u] = pd.dpack[L.base[jj[u]] + kk[u]];
+        if (pidx < total)
+          vv[u] = pd.dpack[(rr[u].x & ((1ull << 40) - 1)) + (pidx - (u32)rr[u].y)];
       }
+      // lanes past the batch's pairs apply a no-op minimum (kCellTop32) at
+      // an offset inside the row's minima
 #pragma unroll
       for (int u = 0; u < kPullPairsP; u++) {
-        if (p0 + (u32)u * 64 + lane >= total) continue;
-        const u32 j = jj[u];
-        const u64 cand = (u64)(vv[u] >> 6) + (u64)L.w[j];
-        if (cand >= lim) {
-          wide = true;
-          continue;
-        }
-        atomicMin(&L.acc[L.slot[j] * kPullAccStride + (vv[u] & 63u)], (u32)cand);
+        const u32 cand = (vv[u] >> 6) + (u32)(rr[u].y >> 32);  // < 2^27
+        const bool real = p0 + (u32)u * 64 + lane < total;
+        const bool fits = cand < lim;
+        wide |= real && !fits;
+        atomicMin(&L.acc[(u32)(rr[u].x >> 48) + (vv[u] & 63u)], real && fits ? cand : kCellTop32);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1549,7 +1571,8 @@ __global__ __launch_bounds__(kBlock) void k_pull_packed(PullDesc pd, CellDesc cd
 // The push join of one delta variant with the delta's size and the join's
 // output count on the device, for rounds captured in a graph: each grid is
 // fixed and strides over the device-side count, and every kernel returns at
-// once when the rounds are done or have left (LoopState).
+// once when the rounds are done or have left, or the round is not a push
+// round (LoopState::stamp != 1).
 
 constexpr u32 kDevGrid = 4096;      // blocks of the striding row and tile kernels
 
@@ -1557,7 +1580,7 @@ template <class K>
 __global__ __launch_bounds__(kBlock) void k_scan_delta_dev(const K* __restrict__ keys, const i64* __restrict__ w,
                                                            const LoopState* __restrict__ ls, int arity, int bits,
                                                            ScanDesc sd, Table out) {
-  if (ls->done | ls->leave) return;
+  if (ls->done | ls->leave | (ls->stamp ^ 1u)) return;  // push rounds only
   const u64 n = ls->n;
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += (u64)gridDim.x * kBlock) {
     const K key = keys[i];
@@ -1570,7 +1593,7 @@ template <class K>
 __global__ __launch_bounds__(kBlock) void k_join_count_dev(Table in, ProbeDesc p, const K* __restrict__ skeys,
                                                            const u64* __restrict__ csr, u64* __restrict__ lo_out,
                                                            u32* __restrict__ cnt, const LoopState* __restrict__ ls) {
-  if (ls->done | ls->leave) return;
+  if (ls->done | ls->leave | (ls->stamp ^ 1u)) return;  // push rounds only
   const u64 n = ls->n;
   for (u64 r = (u64)blockIdx.x * kBlock + threadIdx.x; r < n; r += (u64)gridDim.x * kBlock)
     join_count_row<K>(in, p, skeys, csr, r, lo_out, cnt);
@@ -1594,7 +1617,7 @@ __device__ __forceinline__ u64 dscan_block_sum(u64 v, u64* tmp) {
 __global__ __launch_bounds__(kBlock) void k_dscan_reduce(const u32* __restrict__ cnt, const LoopState* __restrict__ ls,
                                                          u64* __restrict__ bsum) {
   __shared__ u64 tmp[kBlock / 64];
-  if (ls->done | ls->leave) return;
+  if (ls->done | ls->leave | (ls->stamp ^ 1u)) return;  // push rounds only
   const u64 n = ls->n, per = (n + gridDim.x - 1) / gridDim.x;
   const u64 a = (u64)blockIdx.x * per, e = a + per < n ? a + per : n;
   u64 v = 0;
@@ -1606,7 +1629,7 @@ __global__ __launch_bounds__(kBlock) void k_dscan_reduce(const u32* __restrict__
 __global__ __launch_bounds__(kBlock) void k_dscan_sums(u64* __restrict__ bsum, u32 nb, u64* __restrict__ total,
                                                        const LoopState* __restrict__ ls) {
   __shared__ u64 part[kBlock];
-  if (ls->done | ls->leave) return;
+  if (ls->done | ls->leave | (ls->stamp ^ 1u)) return;  // push rounds only
   // thread t scans entries [t * per, (t + 1) * per) serially, then the block
   // scans the threads' sums
   const u32 per = (nb + kBlock - 1) / kBlock;
@@ -1633,7 +1656,7 @@ __global__ __launch_bounds__(kBlock) void k_dscan_sums(u64* __restrict__ bsum, u
 __global__ __launch_bounds__(kBlock) void k_dscan_down(const u32* __restrict__ cnt, const LoopState* __restrict__ ls,
                                                        const u64* __restrict__ boff, u64* __restrict__ out) {
   __shared__ u64 tmp[kBlock / 64];
-  if (ls->done | ls->leave) return;
+  if (ls->done | ls->leave | (ls->stamp ^ 1u)) return;  // push rounds only
   const u64 n = ls->n, per = (n + gridDim.x - 1) / gridDim.x;
   const u64 a = (u64)blockIdx.x * per, e = a + per < n ? a + per : n;
   u64 carry = boff[blockIdx.x];
@@ -1667,7 +1690,7 @@ __global__ __launch_bounds__(kBlock) void k_join_apply_dev(Table in, ProbeDesc p
                                                            const u64* __restrict__ off,
                                                            const u64* __restrict__ total_dev,
                                                            const LoopState* __restrict__ ls, DevErr* err) {
-  if (ls->done | ls->leave) return;
+  if (ls->done | ls->leave | (ls->stamp ^ 1u)) return;  // push rounds only
   const u64 total = *total_dev, nrows = ls->n;
   for (u64 t = blockIdx.x; t * kEmitTile < total; t += gridDim.x) {
     const u64 o0 = t * kEmitTile;
