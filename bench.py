@@ -17,10 +17,16 @@ The same JSON line also carries, as extra keys:
   served       config 5: Poisson queries on a resident RMAT-24 graph (p50/p99,
                queries/s, batched vs one at a time, answer contents compared
                with plain runs);
+  bool_tc      config 1: Bool transitive closure, 2,000 nodes / 10,000 edges;
+  triangles    config 3: triangles of RMAT scale 22, listed and counted;
+  sort         the radix sort primitive on 2^26 device-resident keys, alone
+               and with i64 weights, at the autotuner's pick (checked against
+               torch.sort);
   cpu_baseline the oracle (C restatement of strata-eval) on host threads, one
                source per run; cpu_scan_config1 the faithful scan-mode
                interpreter's labelled extrapolation to config 1.
---no-extras leaves out batched and served, --no-cpu the CPU baselines.
+--no-extras leaves out batched, served, bool_tc, triangles and sort; --no-cpu
+the CPU baselines.
 
 Also: --config 1|3|4|5 for the other workloads (TC, triangles, batched,
 served) used by the documentation; they print their own JSON line.
@@ -228,6 +234,9 @@ def bench_cfg2(args):
     if not args.no_extras:
         line["batched"] = cfg4_summary(args)
         line["served"] = cfg5_summary(args)
+        line["bool_tc"] = cfg1_summary(args)
+        line["triangles"] = cfg3_summary(args)
+        line["sort"] = sort_summary(args)
     if not args.no_cpu:
         threads = max(1, min(16, host_info()["usable_cpus"]))
         log(f"timing the CPU baseline (oracle, {threads} sources on {threads} threads) ...")
@@ -263,6 +272,37 @@ def cfg2_end_to_end(w, n_cost):
             "includes": "prepare, H2D of the host EDB, device sort, fixpoint, D2H of the result"}
 
 
+def cfg1_summary(args):
+    """Config 1 (Bool TC, 2,000 nodes / 10,000 edges) as extra keys."""
+    log("config 1: Bool TC ...")
+    o = bench_cfg1(argparse.Namespace(**vars(args)), emit=False)
+    return {k: o[k] for k in ("config", "derived_tuples", "rounds", "ms_per_step", "tuples_per_s",
+                              "kernel_ms_per_step", "launches_per_step")}
+
+
+def cfg3_summary(args):
+    """Config 3 (triangles, RMAT scale 22) as extra keys: listing and count
+    per run, the dominant kernel's roofline."""
+    log("config 3: triangles on RMAT scale 22 ...")
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup, a.scale = 3, 1, 22
+    o = bench_cfg3(a, emit=False)
+    return {k: o[k] for k in ("config", "params", "tri", "ntri", "tri_equals_ntri", "ms_per_step", "tuples_per_s",
+                              "kernel_ms_per_step", "roofline")}
+
+
+def sort_summary(args):
+    """The radix sort primitive on 2^26 device-resident keys (44 significant
+    bits), alone and with i64 weights, at the tuner's pick."""
+    log("radix sort, 2^26 keys ...")
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup, a.sort_log2 = 5, 1, 26
+    r = bench_sort(a, emit=False)["results"]
+    return [{k: x[k] for k in ("weighted", "n", "key_bits", "candidate", "digit_bits", "items_per_thread", "block",
+                               "single_sweep", "passes", "ms_per_sort", "alg_gbps", "frac_of_hbm_peak",
+                               "sorted_equals_torch")} for x in r]
+
+
 def cfg4_summary(args):
     """Config 4 (512 batched programs against 512 plain runs) as extra keys."""
     a = argparse.Namespace(**vars(args))
@@ -293,7 +333,9 @@ def cfg5_summary(args):
     return out
 
 
-def bench_cfg1(args):
+def bench_cfg1(args, emit=True):
+    """Config 1: Bool transitive closure of a uniform 2,000-node / 10,000-edge
+    graph, fixpoints on the resident EDB."""
     import strata_k as sk
     from strata_k import workloads as wl
     w = wl.config1()
@@ -304,14 +346,21 @@ def bench_cfg1(args):
     ms, recs = run_steps(ctx, run, args.steps, args.warmup)
     n = run.size("path")
     top = max((r for r in recs if r["bytes"] > 0), key=lambda r: r["ms"], default=None)
-    print(json.dumps({"config": "bool_tc_uniform_2000_10000", "derived_tuples": n, "rounds": run.iterations()[0],
-                      "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
-                      "kernel_ms_per_step": round(sum(r["ms"] for r in recs) / args.steps, 3),
-                      "launches_per_step": sum(r["launches"] for r in recs) // args.steps,
-                      "roofline": roofline_of(top)}), flush=True)
+    out = {"config": "bool_tc_uniform_2000_10000", "derived_tuples": n, "rounds": run.iterations()[0],
+           "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
+           "kernel_ms_per_step": round(sum(r["ms"] for r in recs) / args.steps, 3),
+           "launches_per_step": sum(r["launches"] for r in recs) // args.steps,
+           "roofline": roofline_of(top)}
+    run.free()
+    ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def bench_cfg3(args):
+def bench_cfg3(args, emit=True):
+    """Config 3: triangles of an RMAT scale-22 graph (64 M undirected draws),
+    oriented by (degree, id): the tri relation listed and counted."""
     import strata_k as sk
     from strata_k import workloads as wl
     t = time.time()
@@ -327,11 +376,17 @@ def bench_cfg3(args):
     top = max((r for r in recs if r["bytes"] > 0), key=lambda r: r["ms"], default=None)
     if args.kernels:
         log(json.dumps(sorted(recs, key=lambda r: -r["ms"]), indent=1))
-    print(json.dumps({"config": "wcoj_triangles_rmat", "params": w.params, "tri": n, "ntri": str(ntri),
-                      "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
-                      "kernel_ms_per_step": {r["name"]: round(r["ms"] / args.steps, 2)
-                                             for r in sorted(recs, key=lambda r: -r["ms"])[:8]},
-                      "roofline": roofline_of(top)}), flush=True)
+    count = int(ntri[0][0][0].value) if ntri else None
+    out = {"config": "wcoj_triangles_rmat", "params": w.params, "tri": n, "ntri": count,
+           "tri_equals_ntri": count == n, "ms_per_step": round(ms, 3), "tuples_per_s": round(n / (ms / 1e3), 1),
+           "kernel_ms_per_step": {r["name"]: round(r["ms"] / args.steps, 2)
+                                  for r in sorted(recs, key=lambda r: -r["ms"])[:8]},
+           "roofline": roofline_of(top)}
+    run.free()
+    ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_cfg4(args, emit=True):
@@ -545,7 +600,7 @@ def bench_cfg5(args, emit=True):
     return out
 
 
-def bench_sort(args):
+def bench_sort(args, emit=True):
     """Device-resident LSD radix sort (the engine's sort primitive) of 2^26
     packed two-column keys (44 significant bits: 22-bit ids), keys alone and
     with i64 weights, under the context's autotuner.  Algorithmic bytes of a
@@ -615,7 +670,10 @@ def bench_sort(args):
                     "ms_per_sort_by_candidate": per_cand})
         del keys, alt, w, walt, src
     ctx.close()
-    print(json.dumps({"metric": "radix sort keys/s", "unit": "keys/s", "results": out}), flush=True)
+    line = {"metric": "radix sort keys/s", "unit": "keys/s", "results": out}
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 # The reference's published chain-union figures (crates/strata-gpu/README.md:56-58,
@@ -814,7 +872,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="config 2: leave out the batched (config 4) and served (config 5) measurements")
+                    help="config 2: leave out the batched (config 4), served (config 5), config 1, config 3 and sort "
+                         "measurements")
     ap.add_argument("--kernels", action="store_true")
     ap.add_argument("--programs", type=int, default=512, help="config 4: number of programs")
     ap.add_argument("--queries", type=int, default=64, help="config 5: number of queries")
@@ -843,6 +902,13 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (per-GPU shards would be "
               "independent source sets); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.sort or args.copy_probe or (args.config == 2 and not args.no_extras):
+        # torch (device buffers and the reference sort of the sort leg) brings
+        # its own HIP runtime, which finds the device only when it starts
+        # before the engine's: start it first
+        import torch
+        torch.zeros(1, device="cuda:0")
+        torch.cuda.synchronize()
     if args.copy_probe:
         copy_probe(args.copy_probe)
     if not args.sort:
