@@ -1246,6 +1246,15 @@ double tri_list_len(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, co
 // what changes is that each head cell is written once per chunk instead of
 // once per candidate.
 
+// Threads per block of the pull kernels (k_pull, k_pull_packed, k_bitpull):
+// one wave, one chunk.  With four waves a block held its LDS and slot until
+// its slowest chunk was done (chunks range from one index row to 256):
+// config 2, 0.759 -> 0.714 ms per packed pull launch.
+#ifndef SK_PULL_BLOCK
+#define SK_PULL_BLOCK 64
+#endif
+constexpr int kPullBlock = SK_PULL_BLOCK;
+
 // Pairs per lane and step of the pull kernel.
 constexpr int kPullPairs = 2;
 
@@ -1276,10 +1285,10 @@ struct PullLds {
 // cells are peeked at the start, overlapping the loads.
 
 template <bool TROP>
-__global__ __launch_bounds__(kBlock) void k_pull(PullDesc pd, CellDesc cd, DevErr* err) {
-  __shared__ PullLds sh[kBlock / 64];
+__global__ __launch_bounds__(kPullBlock) void k_pull(PullDesc pd, CellDesc cd, DevErr* err) {
+  __shared__ PullLds sh[kPullBlock / 64];
   const u32 lane = threadIdx.x & 63;
-  const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const u64 ch = (u64)blockIdx.x * (kPullBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
   if (pd.gate && (pd.gate->done | pd.gate->leave | pd.gate->stamp)) return;  // a push round runs instead
   PullLds& L = sh[threadIdx.x >> 6];
@@ -1429,6 +1438,7 @@ constexpr int kPullPairsP = SK_PULL_PAIRS_PACKED;
 #endif
 constexpr u32 kPullAccStride = SK_PULL_ACC_STRIDE;
 
+
 // Waves per SIMD the packed pull kernel is compiled for (its VGPR budget).
 #ifndef SK_PULL_MIN_WAVES
 #define SK_PULL_MIN_WAVES 1
@@ -1455,10 +1465,10 @@ static_assert(kPullPairsP == 4, "a lane's start marks are one 32-bit word");
 // above narrow_lim raises the widen flag and is dropped, as a narrow cell
 // drops it.  Pairs are dealt to lanes and folded into a u32 LDS minimum per
 // (destination row, lane value) as in k_pull.
-__global__ __launch_bounds__(kBlock, SK_PULL_MIN_WAVES) void k_pull_packed(PullDesc pd, CellDesc cd) {
-  __shared__ PullLdsP sh[kBlock / 64];
+__global__ __launch_bounds__(kPullBlock, SK_PULL_MIN_WAVES) void k_pull_packed(PullDesc pd, CellDesc cd) {
+  __shared__ PullLdsP sh[kPullBlock / 64];
   const u32 lane = threadIdx.x & 63;
-  const u64 ch = (u64)blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const u64 ch = (u64)blockIdx.x * (kPullBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (ch >= pd.nchunks) return;  // whole waves leave; no block barrier below
   if (pd.gate && (pd.gate->done | pd.gate->leave | pd.gate->stamp)) return;  // a push round runs instead
   PullLdsP& L = sh[threadIdx.x >> 6];
@@ -1741,11 +1751,11 @@ template void push_round_dev<u128, u128>(const Exec&, const u128*, const i64*, i
 void pull_apply(const Exec& x, const PullDesc& p, const CellDesc& cd, bool trop, double bytes, u64 items) {
   if (!p.nchunks) return;
   Scope sc(x.timer, "join_pull", KC_JOIN, bytes, items);
-  const u64 waves_per_block = kBlock / 64;
+  const u64 waves_per_block = kPullBlock / 64;
   const unsigned nb = (unsigned)((p.nchunks + waves_per_block - 1) / waves_per_block);
-  if (trop && p.dpack) k_pull_packed<<<nb, kBlock, 0, x.stream>>>(p, cd);
-  else if (trop) k_pull<true><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
-  else k_pull<false><<<nb, kBlock, 0, x.stream>>>(p, cd, x.err);
+  if (trop && p.dpack) k_pull_packed<<<nb, kPullBlock, 0, x.stream>>>(p, cd);
+  else if (trop) k_pull<true><<<nb, kPullBlock, 0, x.stream>>>(p, cd, x.err);
+  else k_pull<false><<<nb, kPullBlock, 0, x.stream>>>(p, cd, x.err);
   SK_HIP(hipGetLastError());
 }
 
@@ -1867,11 +1877,11 @@ __device__ __forceinline__ u32 bitpull_batch(const BitPullDesc& bd, u32 stamp, u
   return (u32)__popcll(m);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bitpull(BitPullDesc bd, CellDesc cd) {
-  __shared__ u32 s_act[kBlock];
+__global__ __launch_bounds__(kPullBlock) void k_bitpull(BitPullDesc bd, CellDesc cd) {
+  __shared__ u32 s_act[kPullBlock];
   const u32 lane = threadIdx.x & 63;
   const u32 wv = threadIdx.x >> 6;
-  const u64 ch = (u64)blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(wv);
+  const u64 ch = (u64)blockIdx.x * (kPullBlock / 64) + __builtin_amdgcn_readfirstlane(wv);
   if (ch >= bd.nchunks) return;  // whole waves leave; no block barrier below
   if (bd.loop && bd.loop->done) return;
   const u32 stamp = bd.loop ? bd.loop->stamp : bd.stamp;
@@ -1966,8 +1976,8 @@ u64 bitpull_chunks(const Exec& x, const u64* csr, u64 nrows, DBuf<BitChunk>* out
 void bitpull_apply(const Exec& x, const BitPullDesc& p, const CellDesc& cd, double bytes, u64 items) {
   if (!p.nchunks) return;
   Scope sc(x.timer, "join_bitpull", KC_JOIN, bytes, items);
-  const unsigned nb = (unsigned)((p.nchunks + kWaves - 1) / kWaves);
-  k_bitpull<<<nb, kBlock, 0, x.stream>>>(p, cd);
+  const unsigned nb = (unsigned)((p.nchunks + kPullBlock / 64 - 1) / (kPullBlock / 64));
+  k_bitpull<<<nb, kPullBlock, 0, x.stream>>>(p, cd);
   SK_HIP(hipGetLastError());
 }
 
