@@ -963,6 +963,12 @@ __device__ __forceinline__ bool tri_member(const u32* __restrict__ B, u64 nb, u3
 #define SK_TRI_LDS 1024
 #endif
 constexpr u64 kTriLds = SK_TRI_LDS;  // longer lists up to this length are searched in LDS
+// Threads per block of the heavy-row kernels: one wave, so that a block's
+// slot and LDS are not held by its three siblings' shorter rows.
+#ifndef SK_TRI_BLOCK
+#define SK_TRI_BLOCK 64
+#endif
+constexpr int kTriBlock = SK_TRI_BLOCK;
 
 __global__ void k_tri_heavy_words(const u32* __restrict__ hshort, u64 nh, u32* __restrict__ nw) {
   const u64 h = (u64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -985,7 +991,7 @@ __device__ __forceinline__ bool tri_member_lds(const u32* B, u32 nb, u32 x) {
 // when the longer list fits kTriLds values (loaded once, coalesced), else in
 // global memory.  Each step's ballot is the row's next hit word.
 template <class K>
-__global__ __launch_bounds__(kBlock) void k_tri_count_heavy(const K* __restrict__ d_keys, TriDesc t,
+__global__ __launch_bounds__(kTriBlock) void k_tri_count_heavy(const K* __restrict__ d_keys, TriDesc t,
                                                             const u64* __restrict__ csr2,
                                                             const u32* __restrict__ adj2,
                                                             const u64* __restrict__ csr3,
@@ -993,8 +999,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_count_heavy(const K* __restrict_
                                                             const u32* __restrict__ heavy, u64 nh,
                                                             const u64* __restrict__ hwoff, u64* __restrict__ hmask,
                                                             u64* __restrict__ hrec, u32* __restrict__ cnt) {
-  __shared__ u32 sB[kBlock / 64][kTriLds];
-  const u64 w = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  __shared__ u32 sB[kTriBlock / 64][kTriLds];
+  const u64 w = (u64)blockIdx.x * (kTriBlock / 64) + (threadIdx.x >> 6);
   if (w >= nh) return;  // whole waves; no block barrier below
   const u32 lane = threadIdx.x & 63;
   u32* L = sB[threadIdx.x >> 6];
@@ -1044,13 +1050,13 @@ __global__ void k_tri_heavy_off(const u32* __restrict__ heavy, u64 nh, const u64
 // row's hit words one at a time: a set bit l of word c is the common value
 // A[64 c + l], written at its rank among the row's hits (ascending, as A is).
 template <class KO>
-__global__ __launch_bounds__(kBlock) void k_tri_emit_heavy(const u32* __restrict__ adj2, const u32* __restrict__ adj3,
+__global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restrict__ adj2, const u32* __restrict__ adj3,
                                                            u64 nh, const u64* __restrict__ hrec,
                                                            const u32* __restrict__ hshort,
                                                            const u64* __restrict__ hwoff,
                                                            const u64* __restrict__ hoff,
                                                            const u64* __restrict__ hmask, OutDesc o) {
-  const u64 g = (u64)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const u64 g = (u64)blockIdx.x * (kTriBlock / 64) + (threadIdx.x >> 6);
   const u64 h0 = g * 64;
   if (h0 >= nh) return;  // whole waves
   const u32 lane = threadIdx.x & 63;
@@ -1151,7 +1157,7 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
   read_u64s(x, hstat.get(), hs, 2);
   const u64 nh = hs[0];
   const double heavy_vals = (double)hs[1], light_vals = std::max(0.0, lb - heavy_vals);
-  const unsigned hgrid = (unsigned)((nh + kBlock / 64 - 1) / (kBlock / 64));
+  const unsigned hgrid = (unsigned)((nh + kTriBlock / 64 - 1) / (kTriBlock / 64));
   if (x.timer) x.timer->credit("leapfrog_count", KC_LEAPFROG, (double)nd * sizeof(K) + light_vals * 4.0, 0);
   // the heavy rows' hit words: ceil(shorter list / 64) per row
   DBuf<u32> hnw(x.pool, nh ? nh : 1);
@@ -1163,7 +1169,7 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
     scan_u32(x, hnw.get(), hwoff.get(), nh, hwtot.get());
     hmask.reset(x.pool, std::max<u64>(read_u64(x, hwtot.get()), 1));
     Scope sc(x.timer, "leapfrog_count_heavy", KC_LEAPFROG, (double)nh * 4.0 + heavy_vals * 4.0, nh);
-    k_tri_count_heavy<K><<<hgrid, kBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
+    k_tri_count_heavy<K><<<hgrid, kTriBlock, 0, x.stream>>>(d_keys, t, csr2, adj2, csr3, adj3, heavy.get(), nh,
                                                          hwoff.get(), hmask.get(), hrec.get(), cnt.get());
     SK_HIP(hipGetLastError());
   }
@@ -1189,12 +1195,12 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
     // per heavy row its record (16), word and output offsets (16), list
     // length (4); per hit its shorter-list value (4) and the output written
     Scope sc(x.timer, "leapfrog_emit_heavy", KC_LEAPFROG, 0.0, nh);
-    const unsigned egrid = (unsigned)((nh + 64 * (kBlock / 64) - 1) / (64 * (kBlock / 64)));
+    const unsigned egrid = (unsigned)((nh + 64 * (kTriBlock / 64) - 1) / (64 * (kTriBlock / 64)));
     if (k128_out)
-      k_tri_emit_heavy<u128><<<egrid, kBlock, 0, x.stream>>>(adj2, adj3, nh, hrec.get(), hshort.get(), hwoff.get(),
+      k_tri_emit_heavy<u128><<<egrid, kTriBlock, 0, x.stream>>>(adj2, adj3, nh, hrec.get(), hshort.get(), hwoff.get(),
                                                              hoff.get(), hmask.get(), *o);
     else
-      k_tri_emit_heavy<u64><<<egrid, kBlock, 0, x.stream>>>(adj2, adj3, nh, hrec.get(), hshort.get(), hwoff.get(),
+      k_tri_emit_heavy<u64><<<egrid, kTriBlock, 0, x.stream>>>(adj2, adj3, nh, hrec.get(), hshort.get(), hwoff.get(),
                                                             hoff.get(), hmask.get(), *o);
     SK_HIP(hipGetLastError());
   }
