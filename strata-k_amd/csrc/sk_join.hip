@@ -31,6 +31,34 @@ __device__ __forceinline__ u64 wave_incl_scan_j(u64 v) {
   return v;
 }
 
+// Inclusive prefix maximum of a non-negative value over the 64 lanes of a
+// wave, by DPP lane moves (no LDS): shifts by 1, 2, 4 and 8 within each row
+// of 16 lanes, then row 0's last lane broadcast into row 1 and row 2's into
+// row 3, then lane 31 into rows 2 and 3.  Lanes whose source is outside
+// their row read 0, the identity.
+__device__ __forceinline__ u32 wave_incl_max_u32(u32 v) {
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return v;
+}
+
+// Inclusive prefix sum of a u32 over the 64 lanes of a wave, by the same DPP
+// lane moves as wave_incl_max_u32 (lanes whose source is outside their row
+// read 0, the identity).
+__device__ __forceinline__ u32 wave_incl_sum_u32(u32 v) {
+  v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+
 static unsigned gridj(u64 n, u64 per = kBlock) {
   u64 g = (n + per - 1) / per;
   if (g == 0) g = 1;
@@ -1045,10 +1073,41 @@ __global__ void k_tri_heavy_off(const u32* __restrict__ heavy, u64 nh, const u64
   if (h < nh) hoff[h] = off[heavy[h]];
 }
 
-// One wave per 64 heavy rows: lane l loads row l's record, word offset,
-// output offset and shorter-list length in one step; then, row by row, the
-// row's hit words one at a time: a set bit l of word c is the common value
-// A[64 c + l], written at its rank among the row's hits (ascending, as A is).
+// The r-th (from 0) set bit of m, which has more than r set bits: halve the
+// window six times, keeping the half that holds it.
+__device__ __forceinline__ u32 select_bit64(u64 m, u32 r) {
+  u32 pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const u32 c = (u32)__popcll(m & ((1ull << w) - 1));
+    if (r >= c) {
+      r -= c;
+      m >>= w;
+      pos += (u32)w;
+    }
+  }
+  return pos;
+}
+
+// A hit word of the heavy emit, as its lanes read it: the word, the output
+// index of its hit q (of the chunk's hits) less q, where in adj2/adj3 its 64
+// list values start (bits 0-39; bit 63: adj3) with the chunk's hits before it
+// (bits 40-52), and its row's binding (u, v).
+struct TriWord {
+  u64 m, ob, ao, uv;
+};
+
+// One wave per 64 heavy rows, over their hit words as one range (rows' words
+// are contiguous, as hwoff is a scan in heavy-row order), 64 words at a time:
+// lane j takes word j, finds its row (each row's first word marked in LDS with
+// the row, spread by a prefix maximum; a row begun in an earlier chunk carries
+// over with its hits so far), and the hits before it in its row (prefix sum of
+// the words' bit counts).  Then the chunk's hits, 64 at a time, one per lane:
+// hit q's word by the same marking (each word marks the rank of its first
+// hit), its bit by selecting set bit q - (hits before the word).  A hit is the
+// common value A[64 c + bit] of its row's word c, written at the row's output
+// offset plus its rank among the row's hits (ascending, as A is); a row's hits
+// fill consecutive outputs, so 64 lanes write mostly consecutive keys.
 template <class KO>
 __global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restrict__ adj2, const u32* __restrict__ adj3,
                                                            u64 nh, const u64* __restrict__ hrec,
@@ -1056,13 +1115,17 @@ __global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restr
                                                            const u64* __restrict__ hwoff,
                                                            const u64* __restrict__ hoff,
                                                            const u64* __restrict__ hmask, OutDesc o) {
+  __shared__ TriWord sw[kTriBlock / 64][64];
+  __shared__ u32 smk[kTriBlock / 64][64];
   const u64 g = (u64)blockIdx.x * (kTriBlock / 64) + (threadIdx.x >> 6);
   const u64 h0 = g * 64;
-  if (h0 >= nh) return;  // whole waves
+  if (h0 >= nh) return;  // whole waves; no block barrier below
   const u32 lane = threadIdx.x & 63;
-  const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  u64 ra = 0, ruv = 0, rw = 0, ro = 0, rn = 0;
-  if (h0 + lane < nh) {
+  TriWord* W = sw[threadIdx.x >> 6];
+  u32* mk = smk[threadIdx.x >> 6];
+  const u32 nrows = (u32)min((u64)64, nh - h0);
+  u64 ra = 0, ruv = 0, rw = 0, ro = 0, rn = 0;  // lane l: row h0 + l
+  if (lane < nrows) {
     const u64 h = h0 + lane;
     ra = hrec[2 * h];
     ruv = hrec[2 * h + 1];
@@ -1070,27 +1133,72 @@ __global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restr
     ro = hoff[h];
     rn = hshort[h];
   }
-  const u32 nrows = (u32)min((u64)64, nh - h0);
-  for (u32 r = 0; r < nrows; r++) {
-    const u64 a = __shfl(ra, (int)r, 64), uv = __shfl(ruv, (int)r, 64);
-    const u64 woff = __shfl(rw, (int)r, 64), na = __shfl(rn, (int)r, 64);
-    u64 oi = __shfl(ro, (int)r, 64);
-    const u32* A = ((a >> 63) ? adj3 : adj2) + (a & ~(1ull << 63));
-    const u32 u = (u32)(uv >> 32), v = (u32)uv;
-    const u64 nw = (na + 63) / 64;
-    for (u64 cw = 0; cw < nw; cw++) {
-      const u64 m = hmask[woff + cw];
-      if ((m >> lane) & 1ull) {
-        const u32 x = A[cw * 64 + lane];
+  const u64 w0 = __shfl(rw, 0, 64);
+  const u64 w1 = __shfl(rw + (rn + 63) / 64, (int)nrows - 1, 64);
+  mk[lane] = 0;
+  u32 crow = 0;   // the row of the previous chunk's last word
+  u64 chits = 0;  // and its hits up to the end of that chunk
+  for (u64 c0 = w0; c0 < w1; c0 += 64) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // a row without words starts where the next row does: the later row wins
+    if (lane < nrows && rw >= c0 && rw < c0 + 64) atomicMax(&mk[rw - c0], lane + 1);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const u32 rmk = mk[lane];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    mk[lane] = 0;
+    const u32 pr = wave_incl_max_u32(rmk);
+    const u32 rj = pr ? pr - 1 : crow;
+    const u64 wi = c0 + lane;
+    const u64 m = wi < w1 ? hmask[wi] : 0ull;
+    const u32 pc = (u32)__popcll(m);
+    const u32 incl = wave_incl_sum_u32(pc);
+    const u32 ex = incl - pc;
+    const u64 rs = __shfl(rw, (int)rj, 64);  // the row's first word
+    const u32 ps = __shfl(ex, (int)(rs > c0 ? rs - c0 : 0), 64);
+    const u64 before = (u64)(ex - ps) + (rs < c0 ? chits : 0ull);  // hits before this word in its row
+    const u64 a = __shfl(ra, (int)rj, 64);
+    TriWord tw;
+    tw.m = m;
+    tw.ob = __shfl(ro, (int)rj, 64) + before - ex;
+    tw.ao = (a & (1ull << 63)) | ((u64)ex << 40) | ((a & ((1ull << 40) - 1)) + (wi - rs) * 64);
+    tw.uv = __shfl(ruv, (int)rj, 64);
+    W[lane] = tw;
+    crow = (u32)__builtin_amdgcn_readlane((int)rj, 63);
+    chits = __shfl(before + pc, 63, 64);
+    const u32 nhit = (u32)__builtin_amdgcn_readlane((int)incl, 63);
+    u32 cw = 0;  // the word of the previous batch's last hit
+    for (u32 t = 0; t < nhit; t += 64) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // words with hits start at distinct ranks
+      if (pc && ex >= t && ex < t + 64) mk[ex - t] = lane + 1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const u32 hmk = mk[lane];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      mk[lane] = 0;
+      const u32 pw = wave_incl_max_u32(hmk);
+      const u32 j = pw ? pw - 1 : cw;
+      cw = (u32)__builtin_amdgcn_readlane((int)j, 63);
+      const u32 q = t + lane;
+      if (q < nhit) {
+        const TriWord r = W[j];
+        const u32 bit = select_bit64(r.m, q - (u32)((r.ao >> 40) & 0x1FFFu));
+        const u32* A = (r.ao >> 63) ? adj3 : adj2;
+        const u32 x = A[(r.ao & ((1ull << 40) - 1)) + bit];
+        const u32 u = (u32)(r.uv >> 32), v = (u32)r.uv;
         KO key = 0;
         for (int c = 0; c < o.ncols; c++) {
           // binding slots: 0 = u, 1 = v, 2 = w
           const u32 val = o.kind[c] != 0 ? o.cnst[c] : o.idx[c] == 0 ? u : o.idx[c] == 1 ? v : x;
           key = (key << o.bits) | (KO)val;
         }
-        ((KO*)o.keys)[oi + (u64)__popcll(m & below)] = key;
+        ((KO*)o.keys)[r.ob + q] = key;
       }
-      oi += (u64)__popcll(m);
     }
   }
 }
@@ -1417,20 +1525,6 @@ __global__ __launch_bounds__(kPullBlock) void k_pull(PullDesc pd, CellDesc cd, D
   }
 }
 
-// Inclusive prefix maximum of a non-negative value over the 64 lanes of a
-// wave, by DPP lane moves (no LDS): shifts by 1, 2, 4 and 8 within each row
-// of 16 lanes, then row 0's last lane broadcast into row 1 and row 2's into
-// row 3, then lane 31 into rows 2 and 3.  Lanes whose source is outside
-// their row read 0, the identity.
-__device__ __forceinline__ u32 wave_incl_max_u32(u32 v) {
-  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
-  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
-  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
-  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
-  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
-  v = max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
-  return v;
-}
 
 // Pairs per lane per step of the packed pull kernel (their loads are issued
 // together before the LDS minimum).
