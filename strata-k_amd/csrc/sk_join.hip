@@ -1003,21 +1003,72 @@ __global__ void k_tri_heavy_words(const u32* __restrict__ hshort, u64 nh, u32* _
   if (h < nh) nw[h] = (u32)((hshort[h] + 63) / 64);
 }
 
-// x in the sorted, duplicate-free list B[0, nb) held in LDS
-__device__ __forceinline__ bool tri_member_lds(const u32* B, u32 nb, u32 x) {
-  u32 lo = 0, hi = nb;
-  while (lo < hi) {
-    const u32 mid = (lo + hi) >> 1;
-    if (B[mid] < x) lo = mid + 1;
-    else hi = mid;
+// A wave-uniform 64-bit value as the compiler's scalar (read from lane 0).
+__device__ __forceinline__ u64 uni_u64(u64 x) {
+  const u32 lo = (u32)__builtin_amdgcn_readfirstlane((int)(u32)x);
+  const u32 hi = (u32)__builtin_amdgcn_readfirstlane((int)(u32)(x >> 32));
+  return ((u64)hi << 32) | lo;
+}
+
+// Membership of G values in the sorted, duplicate-free list L[0, nb) held in
+// LDS (nb >= 1), searched together: the step sizes depend on nb alone, so every
+// lane takes the same ceil(log2 nb) steps with no divergence, each one LDS read,
+// a compare and a select per value.  b tracks the last position whose value is
+// <= x (position 0 when there is none); x is in L exactly when L[b] == x.
+template <int G>
+__device__ __forceinline__ void tri_member_lds_n(const u32* L, u32 nb, const u32 (&x)[G], bool (&hit)[G]) {
+  const char* Lb = reinterpret_cast<const char*>(L);
+  u32 b[G];  // in bytes
+#pragma unroll
+  for (int g = 0; g < G; g++) b[g] = 0;
+  for (u32 n = nb; n > 1;) {
+    const u32 h = n >> 1;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const u32 p = b[g] + 4 * h;
+      b[g] = *reinterpret_cast<const u32*>(Lb + p) <= x[g] ? p : b[g];
+    }
+    n -= h;
   }
-  return lo < nb && B[lo] == x;
+#pragma unroll
+  for (int g = 0; g < G; g++) hit[g] = *reinterpret_cast<const u32*>(Lb + b[g]) == x[g];
+}
+
+// G chunks of 64 values of the shorter list, from chunk k0: their values are
+// loaded together, searched together, and each chunk's ballot is its hit word.
+template <int G>
+__device__ __forceinline__ u32 tri_heavy_chunks(const u32* L, const u32* __restrict__ A, u64 na,
+                                                const u32* __restrict__ B, u64 nb, bool lds, u64 k0, u32 lane,
+                                                u64* __restrict__ words) {
+  u32 x[G];
+  bool in[G], hit[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const u64 i = (k0 + g) * 64 + lane;
+    in[g] = i < na;
+    x[g] = in[g] ? A[i] : 0u;
+  }
+  if (lds) {
+    tri_member_lds_n<G>(L, (u32)nb, x, hit);
+  } else {
+#pragma unroll
+    for (int g = 0; g < G; g++) hit[g] = tri_member(B, nb, x[g]);
+  }
+  u32 c = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const u64 m = __ballot(in[g] && hit[g]);
+    if (lane == 0) words[k0 + g] = m;
+    c += (u32)__popcll(m);
+  }
+  return c;
 }
 
 // One wave per heavy row: lane l tests values l, l + 64, ... of the shorter
-// list against the longer one by binary search (the lists are sets), in LDS
-// when the longer list fits kTriLds values (loaded once, coalesced), else in
-// global memory.  Each step's ballot is the row's next hit word.
+// list against the longer one, in LDS when the longer list fits kTriLds
+// values (its loads all issued before the first LDS write), else by binary
+// search in global memory.  Up to four chunks of 64 values go through the
+// search together; each chunk's ballot is the row's next hit word.
 template <class K>
 __global__ __launch_bounds__(kTriBlock) void k_tri_count_heavy(const K* __restrict__ d_keys, TriDesc t,
                                                             const u64* __restrict__ csr2,
@@ -1032,30 +1083,52 @@ __global__ __launch_bounds__(kTriBlock) void k_tri_count_heavy(const K* __restri
   if (w >= nh) return;  // whole waves; no block barrier below
   const u32 lane = threadIdx.x & 63;
   u32* L = sB[threadIdx.x >> 6];
-  const u64 e = heavy[w];
+  // the row's values are the same in every lane: held as scalars, so that the
+  // chunk and search loops below branch on scalars
+  const u64 e = (u32)__builtin_amdgcn_readfirstlane((int)heavy[w]);
+  u64* words = (u64*)uni_u64((u64)(hmask + hwoff[w]));
   u32 u, v;
   const u32 *A, *B;
   u64 na, nb;
   bool in3 = false;
   tri_heavy_lists<K>(d_keys, e, t, csr2, adj2, csr3, adj3, &u, &v, &A, &na, &B, &nb, &in3);
+  in3 = __builtin_amdgcn_readfirstlane((int)in3) != 0;
+  {
+    const u32* Ab = in3 ? adj3 : adj2;  // the lists' arrays: offsets as scalars
+    const u32* Bb = in3 ? adj2 : adj3;
+    A = Ab + uni_u64((u64)(A - Ab));
+    B = Bb + uni_u64((u64)(B - Bb));
+  }
+  na = uni_u64(na);
+  nb = uni_u64(nb);
   const bool lds = nb <= kTriLds;
   if (lds) {
-    for (u64 i = lane; i < nb; i += 64) L[i] = B[i];
+    constexpr int kPer = (int)(kTriLds / 64);
+    const u32 nb32 = (u32)nb, nbk = (nb32 + 63) / 64;
+    u32 tv[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const u32 i = (u32)k * 64 + lane;
+      tv[k] = 0;
+      if ((u32)k < nbk && i < nb32) tv[k] = B[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const u32 i = (u32)k * 64 + lane;
+      if ((u32)k < nbk && i < nb32) L[i] = tv[k];
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  u64* words = hmask + hwoff[w];
+  const u64 nc = (na + 63) / 64;
   u32 c = 0;
-  for (u64 base = 0; base < na; base += 64) {
-    const u64 i = base + lane;
-    bool hit = false;
-    if (i < na) {
-      const u32 x = A[i];
-      hit = lds ? tri_member_lds(L, (u32)nb, x) : tri_member(B, nb, x);
+  for (u64 k0 = 0; k0 < nc; k0 += 4) {
+    switch (min(nc - k0, (u64)4)) {
+      case 4: c += tri_heavy_chunks<4>(L, A, na, B, nb, lds, k0, lane, words); break;
+      case 3: c += tri_heavy_chunks<3>(L, A, na, B, nb, lds, k0, lane, words); break;
+      case 2: c += tri_heavy_chunks<2>(L, A, na, B, nb, lds, k0, lane, words); break;
+      default: c += tri_heavy_chunks<1>(L, A, na, B, nb, lds, k0, lane, words); break;
     }
-    const u64 m = __ballot(hit);
-    if (lane == 0) words[base / 64] = m;
-    c += (u32)__popcll(m);
   }
   if (lane == 0) {
     cnt[e] = c;
