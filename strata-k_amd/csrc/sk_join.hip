@@ -914,37 +914,92 @@ __device__ __forceinline__ void intersect_each(const u32* __restrict__ A, u64 na
 // A driver row whose two lists hold more than kTriHeavy values together is
 // left to a wave of its own (k_tri_count_heavy, k_tri_emit_heavy): one lane
 // per row would hold its whole wave for the length of the longest merge.
-// (RMAT scale 22, ms per listing: 256 -> 151.1, 384 -> 151.2, 512 -> 154.0,
-// 768 -> 197.1, 1024 -> 263.9.)
+// (RMAT scale 22, ms per listing: 0 -> 59.6, 32 -> 59.7, 64 -> 60.0,
+// 128 -> 62.1, 256 -> 69.2; rows this short waste most of a wave's lanes on
+// sparser graphs.)
 #ifndef SK_TRI_HEAVY
-#define SK_TRI_HEAVY 256
+#define SK_TRI_HEAVY 64
 #endif
 constexpr u64 kTriHeavy = SK_TRI_HEAVY;
 
+// Rows per thread of the light count: a block of kBlock threads takes
+// kTriRows * kBlock consecutive driver rows and reserves its heavy-list slots
+// with one pair of device atomics (same-address device atomics serialize:
+// one pair per wave cost 24 ms over 1 M waves at RMAT scale 22).
+constexpr int kTriRows = 16;
+static_assert(kTriRows * (kBlock / 64) == 64, "one slot count per (row step, wave): a wave scans them");
+
+// Per driver row (u, v): the two lists' lengths from the CSRs.  A light row
+// counts its common values by a merge (or binary searches, for a short list
+// against a long one) and writes cnt; a heavy row is appended to the heavy
+// list (heavy, hshort), in (row step, wave, lane) order within the block.
 template <class K>
-__global__ void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t, const u64* __restrict__ csr2,
-                            const u32* __restrict__ adj2, const u64* __restrict__ csr3, const u32* __restrict__ adj3,
-                            u32* __restrict__ cnt, u32* __restrict__ heavy, u32* __restrict__ hshort,
-                            unsigned long long* __restrict__ hstat) {
-  u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nd) return;
-  K k = d_keys[e];
-  u32 u = col_of<K>(k, 2, t.bits, t.u_col);
-  u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
-  u64 a0 = csr2[v], a1 = csr2[v + 1];
-  u64 b0 = csr3[u], b1 = csr3[u + 1];
-  if ((a1 - a0) + (b1 - b0) > kTriHeavy) {
-    // hstat: {heavy rows, their list values}
-    const u64 h = atomicAdd(&hstat[0], 1ull);
-    heavy[h] = (u32)e;
-    hshort[h] = (u32)min(a1 - a0, b1 - b0);
-    atomicAdd(&hstat[1], (unsigned long long)((a1 - a0) + (b1 - b0)));
-    cnt[e] = 0;  // written by the heavy pass
-    return;
+__global__ __launch_bounds__(kBlock) void k_tri_count(const K* __restrict__ d_keys, u64 nd, TriDesc t,
+                                                      const u64* __restrict__ csr2, const u32* __restrict__ adj2,
+                                                      const u64* __restrict__ csr3, const u32* __restrict__ adj3,
+                                                      u32* __restrict__ cnt, u32* __restrict__ heavy,
+                                                      u32* __restrict__ hshort, unsigned long long* __restrict__ hstat) {
+  __shared__ u32 s_off[kTriRows * (kBlock / 64)];  // per (row step, wave): heavy rows, then first slot
+  __shared__ u64 s_len[kBlock / 64];
+  __shared__ u32 s_hsh[kTriRows][kBlock];  // a heavy row's shorter list length
+  const u32 lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const u64 e0 = (u64)blockIdx.x * (kTriRows * kBlock) + threadIdx.x;
+  u32 hbits = 0;  // bit r: row step r is heavy
+  u64 lens = 0;   // the heavy rows' list values
+#pragma unroll 1
+  for (int r = 0; r < kTriRows; r++) {
+    const u64 e = e0 + (u64)r * kBlock;
+    if (e >= nd) continue;
+    const K k = d_keys[e];
+    const u32 u = col_of<K>(k, 2, t.bits, t.u_col);
+    const u32 v = col_of<K>(k, 2, t.bits, 1 - t.u_col);
+    const u64 a0 = csr2[v], a1 = csr2[v + 1];
+    const u64 b0 = csr3[u], b1 = csr3[u + 1];
+    if ((a1 - a0) + (b1 - b0) > kTriHeavy) {
+      hbits |= 1u << r;
+      s_hsh[r][threadIdx.x] = (u32)min(a1 - a0, b1 - b0);
+      lens += (a1 - a0) + (b1 - b0);
+      cnt[e] = 0;  // written by the heavy pass
+      continue;
+    }
+    u32 c = 0;
+    intersect_each(adj2 + a0, a1 - a0, adj3 + b0, b1 - b0, [&](u32) { c++; });
+    cnt[e] = c;
   }
-  u32 c = 0;
-  intersect_each(adj2 + a0, a1 - a0, adj3 + b0, b1 - b0, [&](u32) { c++; });
-  cnt[e] = c;
+  // the block's heavy rows and list values, then its slots: one wave scans
+  // the (row step, wave) counts
+#pragma unroll
+  for (int r = 0; r < kTriRows; r++) {
+    const u64 m = __ballot((hbits >> r) & 1u);
+    if (lane == 0) s_off[r * (kBlock / 64) + wv] = (u32)__popcll(m);
+  }
+  for (u32 o = 32; o > 0; o >>= 1) lens += __shfl_xor(lens, (int)o, 64);
+  if (lane == 0) s_len[wv] = lens;
+  __syncthreads();
+  if (wv == 0) {
+    const u32 c = s_off[lane];
+    const u32 incl = wave_incl_sum_u32(c);
+    u64 base = 0;
+    if (lane == 63 && incl) {
+      u64 lt = 0;
+      for (int w = 0; w < kBlock / 64; w++) lt += s_len[w];
+      base = atomicAdd(&hstat[0], (unsigned long long)incl);
+      atomicAdd(&hstat[1], (unsigned long long)lt);
+    }
+    base = __shfl(base, 63, 64);
+    s_off[lane] = (u32)base + incl - c;  // heavy-list positions fit u32 (nd < 2^32)
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kTriRows; r++) {
+    const bool hv = (hbits >> r) & 1u;
+    const u64 m = __ballot(hv);
+    if (hv) {
+      const u64 h = (u64)s_off[r * (kBlock / 64) + wv] + (u64)__popcll(m & ((1ull << lane) - 1));
+      heavy[h] = (u32)(e0 + (u64)r * kBlock);
+      hshort[h] = s_hsh[r][threadIdx.x];
+    }
+  }
 }
 
 // The lists of a heavy row, the shorter first.
@@ -1330,7 +1385,8 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
   {
     // bytes credited once the heavy rows' share of the lists is known
     Scope sc(x.timer, "leapfrog_count", KC_LEAPFROG, 0.0, nd);
-    k_tri_count<K><<<gridj(nd), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3, adj3, cnt.get(),
+    k_tri_count<K><<<gridj(nd, (u64)kTriRows * kBlock), kBlock, 0, x.stream>>>(d_keys, nd, t, csr2, adj2, csr3,
+                                                                             adj3, cnt.get(),
                                                       heavy.get(), hshort.get(), (unsigned long long*)hstat.get());
     SK_HIP(hipGetLastError());
   }

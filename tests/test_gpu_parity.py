@@ -158,12 +158,14 @@ def test_triangles_vectors(ctx):
 
 @pytest.mark.parametrize("b", [1500, 2500])
 def test_triangles_long_lists(ctx, b):
-    """Hubs of b leaves each (their rows' two lists hold more than 512
-    values: one wave per row, binary-search membership into the longer list
-    held in LDS up to 2,048 values (b = 1500) or searched in global memory
-    (b = 2500), hit words expanded by the emit pass) beside a random graph
-    whose rows take the one-lane merge; the tri relation equals the oracle's
-    and the count the closed form plus brute force."""
+    """Hubs of b leaves each beside a random graph.  A hub's rows hold more
+    than 64 list values together, so each takes a wave: (hub, hub) rows
+    search a longer list of more than 1,024 values (past the LDS table) in
+    global memory, and their 24 (b = 1500) or 40 (b = 2500) hit words per row
+    span several of the heavy emit's 64-word chunks; (hub, leaf) rows have an
+    empty shorter list (no hit words) and sit between them.  The random
+    graph's rows take the one-lane merge.  The tri relation equals the
+    oracle's and the count the closed form plus brute force."""
     h = 5
     edges = [(i, j) for i in range(h) for j in range(i + 1, h)] + [(hub, h + k) for k in range(b) for hub in range(h)]
     rnd = random.Random(17)
