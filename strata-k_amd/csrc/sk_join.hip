@@ -1377,7 +1377,8 @@ u64 triangles(const Exec& x, const K* d_keys, u64 nd, const TriDesc& t, const u6
     if (!count_only) halloc(hctx, o, 0);
     return 0;
   }
-  if (nd > 0xffffffffull) throw BudgetError{nd};  // heavy rows are listed by u32 index
+  // heavy rows are listed by u32 index and take a one-wave block each
+  if (nd > 0x7ffffff0ull) throw BudgetError{nd};
   double lb = sum_list_len ? *sum_list_len : 0;
   DBuf<u32> heavy(x.pool, nd), hshort(x.pool, nd);
   DBuf<u64> hstat(x.pool, 2);
