@@ -735,7 +735,11 @@ void Engine::load() {
   }
   snap_dict_ = D_;
   snap_bits_.clear();
-  for (auto& R : R_) snap_bits_.push_back(R.bits);
+  snap_version_.clear();
+  for (auto& R : R_) {
+    snap_bits_.push_back(R.bits);
+    snap_version_.push_back(R.version);
+  }
   SK_HIP(hipStreamSynchronize(x.stream));
   loaded = true;
 }
@@ -767,12 +771,20 @@ void Engine::reset() {
       SK_HIP(hipMemcpyAsync(int_vals_.get(), D_.ints.data(), D_.ints.size() * 8, hipMemcpyHostToDevice, x.stream));
     }
   }
+  // a relation no rule writes changes in an execute only when the dictionary
+  // grows under it (grow_dict re-encodes it and bumps its version): one whose
+  // version is still the snapshot's holds the snapshot's content, and keeps
+  // it with its version, so its indexes and column code sets stay cached
+  std::vector<bool> head(R_.size(), false);
+  for (auto& r : P_.rules)
+    if (r.head.rel >= 0) head[r.head.rel] = true;
   for (size_t i = 0; i < R_.size(); i++) {
     DRel& R = R_[i];
     R.delta.release();
     R.delta_order.clear();
     R.dversion++;
     if (!written[i]) continue;
+    if (!head[i] && i < snap_version_.size() && R.version == snap_version_[i]) continue;
     R.bits = snap_bits_[i];
     R.wide = key_bits_of(R.arity, R.bits) > 64;
     const RelData& s = snap_[i];
@@ -793,6 +805,7 @@ void Engine::reset() {
     R.full = std::move(d);
     R.version++;
     R.idx.clear();
+    if (i < snap_version_.size()) snap_version_[i] = R.version;
   }
   SK_HIP(hipStreamSynchronize(x.stream));
 }

@@ -451,6 +451,7 @@ class Engine {
   std::vector<RelData> snap_;  // post-load state of relations written by rules
   Dict snap_dict_;
   std::vector<int> snap_bits_;
+  std::vector<u64> snap_version_;  // each relation's version when its snapshot was (last) equal to it
   u32 nparts_ = 0;
   std::vector<u64> part_universe_;
   std::vector<EvalErr> part_err_;
