@@ -1223,6 +1223,7 @@ __device__ __forceinline__ u32 select_bit64(u64 m, u32 r) {
 // (bits 40-52), and its row's binding (u, v).
 struct TriWord {
   u64 m, ob, ao, uv;
+  u64 klo, khi;  // the word's key without its hit value (when that value is one column): low, high 64 bits
 };
 
 // One wave per 64 heavy rows, over their hit words as one range (rows' words
@@ -1252,6 +1253,15 @@ __global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restr
   TriWord* W = sw[threadIdx.x >> 6];
   u32* mk = smk[threadIdx.x >> 6];
   const u32 nrows = (u32)min((u64)64, nh - h0);
+  // when the hit value w is exactly one column, a key is the word's key
+  // without it (folded once per word) OR w shifted into its column
+  int wcols = 0, wshift = 0;
+  for (int c = 0; c < o.ncols; c++)
+    if (o.kind[c] == 0 && o.idx[c] == 2) {
+      wcols++;
+      wshift = o.bits * (o.ncols - 1 - c);
+    }
+  const bool wone = wcols == 1;
   u64 ra = 0, ruv = 0, rw = 0, ro = 0, rn = 0;  // lane l: row h0 + l
   if (lane < nrows) {
     const u64 h = h0 + lane;
@@ -1293,6 +1303,16 @@ __global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restr
     tw.ob = __shfl(ro, (int)rj, 64) + before - ex;
     tw.ao = (a & (1ull << 63)) | ((u64)ex << 40) | ((a & ((1ull << 40) - 1)) + (wi - rs) * 64);
     tw.uv = __shfl(ruv, (int)rj, 64);
+    if (wone) {
+      KO kp = 0;
+      for (int c = 0; c < o.ncols; c++) {
+        // binding slots: 0 = u, 1 = v, 2 = w (0 here)
+        const u32 val = o.kind[c] != 0 ? o.cnst[c] : o.idx[c] == 0 ? (u32)(tw.uv >> 32) : o.idx[c] == 1 ? (u32)tw.uv : 0u;
+        kp = (kp << o.bits) | (KO)val;
+      }
+      tw.klo = (u64)kp;
+      tw.khi = sizeof(KO) > 8 ? (u64)(kp >> 32 >> 32) : 0ull;
+    }
     W[lane] = tw;
     crow = (u32)__builtin_amdgcn_readlane((int)rj, 63);
     chits = __shfl(before + pc, 63, 64);
@@ -1318,12 +1338,16 @@ __global__ __launch_bounds__(kTriBlock) void k_tri_emit_heavy(const u32* __restr
         const u32 bit = select_bit64(r.m, q - (u32)((r.ao >> 40) & 0x1FFFu));
         const u32* A = (r.ao >> 63) ? adj3 : adj2;
         const u32 x = A[(r.ao & ((1ull << 40) - 1)) + bit];
-        const u32 u = (u32)(r.uv >> 32), v = (u32)r.uv;
         KO key = 0;
-        for (int c = 0; c < o.ncols; c++) {
-          // binding slots: 0 = u, 1 = v, 2 = w
-          const u32 val = o.kind[c] != 0 ? o.cnst[c] : o.idx[c] == 0 ? u : o.idx[c] == 1 ? v : x;
-          key = (key << o.bits) | (KO)val;
+        if (wone) {
+          key = (((KO)r.khi << 32 << 32) | (KO)r.klo) | ((KO)x << wshift);
+        } else {
+          const u32 u = (u32)(r.uv >> 32), v = (u32)r.uv;
+          for (int c = 0; c < o.ncols; c++) {
+            // binding slots: 0 = u, 1 = v, 2 = w
+            const u32 val = o.kind[c] != 0 ? o.cnst[c] : o.idx[c] == 0 ? u : o.idx[c] == 1 ? v : x;
+            key = (key << o.bits) | (KO)val;
+          }
         }
         ((KO*)o.keys)[r.ob + q] = key;
       }
